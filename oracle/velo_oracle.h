@@ -1,0 +1,54 @@
+/* TEST INFRASTRUCTURE — C ABI of the CPU oracle of the VELO reconstruction path.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ * The product library (allen_amd/, include/velo_amd.h) never links or calls it. */
+#ifndef VELO_ORACLE_H
+#define VELO_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Allen event layout (main/src/Transpose.cpp:430-498) plus geometry and properties. */
+typedef struct {
+  const char* raw;            /* per-event blocks: [u32 n_banks][u32 offsets[n+1]][banks] */
+  const uint32_t* offsets;    /* n_events+1 byte offsets of the event blocks */
+  const uint32_t* sizes;      /* [u32 offs in u16 units x (n+1)][u16 bank sizes] */
+  const uint32_t* types;      /* [u32 offs in bytes x (n+1)][u8 bank types] */
+  unsigned n_events;
+  unsigned event_start;
+  int bank_version;           /* 2, 3 or 4 */
+  const char* geometry;       /* velo_geometry.bin contents */
+  size_t geometry_bytes;
+  const float* beamline;      /* beamline.bin contents: x, y */
+  float phi_tolerance;        /* velo_search_by_triplet properties */
+  float max_scatter;
+  unsigned max_skipped_modules;
+  float max_chi2;             /* velo_three_hit_tracks_filter property */
+  unsigned block_dim_x;       /* velo_calculate_number_of_candidates block size (decides grid) */
+  int compute_f64;            /* also run the FP64 Kalman restatement */
+} vo_inputs;
+
+void vo_default_inputs(vo_inputs* a);
+/* Runs the whole sequence; returns a handle holding every intermediate buffer, or NULL. */
+void* vo_run(const vo_inputs* a);
+/* Buffer names are the reference's argument names without the dev_/host_ prefix. */
+int vo_get(void* handle, const char* name, const void** ptr, size_t* nbytes);
+void vo_free(void* handle);
+/* One event per task on n_threads std::threads; returns events/s. */
+double vo_throughput(const vo_inputs* a, unsigned n_threads, unsigned repetitions, double* seconds);
+
+void vo_cache_sp_patterns(uint8_t* patterns, uint8_t* sizes, float* fx, float* fy);
+float vo_fast_atan2f(float y, float x);
+int16_t vo_hit_phi_16(float x, float y);
+int16_t vo_hit_phi_float_to_16(float phi);
+uint32_t vo_prefix_sum(const uint32_t* input, size_t n, uint32_t* output);
+int vo_binary_search_leftmost_i16(const int16_t* array, unsigned n, int16_t value);
+uint64_t vo_make_cluster(uint64_t pixel_map, uint64_t start, unsigned col_lower, unsigned row_lower, uint32_t* xyn);
+float vo_three_hit_chi2(const float* xyz9);
+int vo_load_geometry(const char* bytes, size_t n, float* module_zs, float* local_x, float* x_pitch, float* ltg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
