@@ -1,0 +1,768 @@
+// Super-pixel decoding, masked clustering and phi sort for gfx950.
+//
+//   k_number_of_candidates   VeloCalculateNumberOfCandidates.cu:10-48   one wave per event
+//   k_bank_index             VeloCalculateNumberOfCandidates.cu:35-43   last-writer-wins in CPU order
+//   k_estimate_input_size    EstimateInputSize.cu:10-239                one workgroup per event
+//   k_masked_clustering      MaskedVeloClustering.cu:131-576            one workgroup per event
+//   k_sort_by_phi            SortByPhi.cu:49-153                        one workgroup per module pair
+//   k_cluster_sorted         K2+K3+K4 fused: one workgroup per (event, module pair); the
+//                            module pair's SP words are staged in LDS, clusters are built and
+//                            rank-sorted by (phi, id) in LDS and written once, already sorted.
+//
+// The cluster arithmetic (pattern lookup, 64-bit mask growth, centroid, local->global
+// transform, hit_phi_16) is evaluated exactly as in the reference with fp contraction off.
+#include "velo_common.cuh"
+#include "velo_kernels.h"
+
+namespace velo {
+
+// ---------------------------------------------------------------------------------------
+// Bit tricks of MaskedVeloClustering.cu:21-84.
+__device__ __forceinline__ uint64_t make_8con_mask(uint64_t c)
+{
+  return c | ((c & 0x7FFF7FFF7FFF7FFFull) << 1) | ((c & 0xFFFEFFFEFFFEFFFEull) << 15) | (c << 16) |
+         ((c & 0xFFFCFFFCFFFCFFFCull) << 17) | ((c & 0xFFFEFFFEFFFEFFFEull) >> 1) |
+         ((c & 0x7FFF7FFF7FFF7FFFull) >> 15) | (c >> 16) | ((c & 0x3FFF3FFF3FFF3FFFull) >> 17);
+}
+__device__ __forceinline__ uint64_t mask_east(uint64_t c)
+{
+  const uint64_t m = c >> 48;
+  return m | ((m & 0x7FFF7FFF7FFF7FFFull) << 1) | (m >> 1);
+}
+__device__ __forceinline__ uint64_t mask_west(uint64_t c)
+{
+  const uint64_t m = c << 48;
+  return m | (m << 1) | ((m & 0xFFFEFFFEFFFEFFFEull) >> 1);
+}
+
+struct Grown {
+  uint64_t pixels;
+  uint32_t x, y, n;
+};
+
+__device__ __forceinline__ Grown grow_cluster(uint64_t pixel_map, uint64_t start, unsigned col_lower, unsigned row_lower)
+{
+  uint64_t cur = 0, next = start;
+  while (cur != next) {
+    cur = next;
+    next = pixel_map & make_8con_mask(cur);
+  }
+  const unsigned n = __popcll(cur);
+  const unsigned x = col_lower * n + __popcll(cur & 0x00000000FFFF0000ull) + __popcll(cur & 0x0000FFFF00000000ull) * 2 +
+                     __popcll(cur & 0xFFFF000000000000ull) * 3;
+  unsigned y = row_lower * n;
+#pragma unroll
+  for (unsigned r = 1; r < 15; ++r) y += __popcll(cur & (0x0001000100010001ull << r)) * r;
+  return {cur, x, y, n};
+}
+
+// Global position and LHCbID of a cluster (MaskedVeloClustering.cu:192-225,459-494).
+struct ClusterOut {
+  uint32_t id;
+  float x, y, z;
+  int16_t phi;
+};
+
+__device__ __forceinline__ ClusterOut
+place_cluster(const Geometry& g, const float* ltg, uint32_t lhcb_id, unsigned cx, unsigned cy, float fx, float fy)
+{
+  const float local_x = g.local_x[cx] + fx * g.x_pitch[cx];
+  const float local_y = (cy + 0.5f + fy) * pixel_size;
+  ClusterOut c;
+  c.id = lhcb_id;
+  c.x = ltg[0] * local_x + ltg[1] * local_y + ltg[9];
+  c.y = ltg[3] * local_x + ltg[4] * local_y + ltg[10];
+  c.z = ltg[6] * local_x + ltg[7] * local_y + ltg[11];
+  c.phi = hit_phi_16(c.x, c.y);
+  return c;
+}
+
+// One or two clusters of an isolated SP from the pattern table (MaskedVeloClustering.cu:165-286).
+// Returns the number written into out[].
+template<int version>
+__device__ __forceinline__ unsigned isolated_sp_clusters(
+  const Geometry& g,
+  const uint8_t* sp_patterns,
+  const float* sp_fx,
+  const float* sp_fy,
+  const RawBank& bank,
+  uint32_t sp_word,
+  ClusterOut* out)
+{
+  constexpr bool v23 = (version == 2 || version == 3);
+  const uint32_t sp_addr = (sp_word & 0x007FFF00u) >> 8;
+  const bool sensor_bit = (sp_word & 0x800000u) != 0;
+  const uint32_t pair = bank.sourceID & 0x1FFu;
+  const uint32_t sensor_index = v23 ? bank.sourceID : ((pair << 1) | (sensor_bit ? 1u : 0u));
+  const float* ltg = g.ltg + 12 * (v23 ? pair : ((pair << 1) | (sensor_bit ? 1u : 0u)));
+  const uint32_t sp_row = sp_addr & 0x3Fu;
+  const uint32_t sp_col = sp_addr >> 6;
+  const uint32_t sp = sp_word & 0xFFu;
+  const uint32_t idx = sp_patterns[sp];
+  const uint32_t chip = sp_col >> 7;
+  unsigned n = 0;
+#pragma unroll
+  for (unsigned k = 0; k < 2; ++k) {
+    if (k == 1 && !(idx & 8)) break;
+    const uint32_t row = k == 0 ? (idx & 0x03u) : ((idx >> 4) & 3u);
+    const uint32_t col = k == 0 ? ((idx >> 2) & 1u) : ((idx >> 6) & 1u);
+    const uint32_t cx = sp_col * 2 + col;
+    const uint32_t cy = sp_row * 4 + row;
+    out[n++] = place_cluster(
+      g, ltg, lhcb_id_of(sensor_index, chip, cx & 0xFFu, cy), cx, cy, sp_fx[sp * 2 + k], sp_fy[sp * 2 + k]);
+  }
+  return n;
+}
+
+// Candidate pixels of a non-isolated SP (EstimateInputSize.cu:104-204). `words` are the SP words
+// of its bank. The neighbour scan keeps the reference's behaviour: every word of the bank,
+// of either sensor and isolated or not, contributes (EstimateInputSize.cu:111-141).
+// Returns a 2-bit mask: bit0 = first candidate (pixels 0,1,4,5), bit1 = second; the pixel
+// numbers are returned in *p0, *p1.
+__device__ __forceinline__ unsigned
+sp_candidates(uint32_t sp_word, const uint32_t* words, unsigned n_sp, unsigned* p0, unsigned* p1)
+{
+  const uint32_t sp_addr = (sp_word & 0x007FFF00u) >> 8;
+  const uint32_t sp = sp_word & 0xFFu;
+  uint32_t pixels = (sp & 0x0F) | ((sp & 0xF0) << 2);
+  const uint32_t sp_row = sp_addr & 0x3Fu;
+  const uint32_t sp_col = sp_addr >> 6;
+  for (unsigned k = 0; k < n_sp; ++k) {
+    const uint32_t other = words[k];
+    const uint32_t oaddr = (other & 0x007FFF00u) >> 8;
+    const uint32_t orow = oaddr & 0x3Fu;
+    const uint32_t ocol = oaddr >> 6;
+    const uint32_t osp = other & 0xFFu;
+    const bool top = orow == sp_row + 1, mid = orow == sp_row, bot = orow == sp_row - 1;
+    const bool same_col = ocol == sp_col, right_col = ocol == sp_col + 1;
+    if (top && same_col) pixels |= ((osp & 0x01) | ((osp & 0x10) << 2)) << 4;
+    if (top && right_col) pixels |= (osp & 0x01) << 16;
+    if (mid && right_col) pixels |= (osp & 0x0F) << 12;
+    if (bot && right_col) pixels |= (osp & 0x08) << 8;
+    if (bot && same_col) pixels |= (osp & 0x80) >> 2;
+  }
+  const uint32_t inside = pixels & 0x3CF;
+  const uint32_t mask = (inside << 1) | (inside << 5) | (inside << 6) | (inside << 7);
+  const uint32_t working = mask & (~pixels);
+  const uint32_t cand = ((working >> 1) & (working >> 5) & (working >> 6) & (working >> 7)) & pixels;
+  const uint32_t consolidated = (cand & 0x0F) | ((cand >> 2) & 0xF0);
+  const uint32_t first = consolidated & 0x33;
+  const uint32_t second = consolidated & 0xCC;
+  unsigned m = 0;
+  if (first) {
+    *p0 = __clz(first) - 24;
+    m |= 1;
+  }
+  if (second) {
+    *p1 = __clz(second) - 24;
+    m |= 2;
+  }
+  return m;
+}
+
+__device__ __forceinline__ unsigned isolated_cluster_count(uint32_t sp)
+{
+  const bool pattern_0 = (sp & 0x88) && !(sp & 0x44) && (sp & 0x33);
+  const bool pattern_1 = (sp & 0xCC) && !(sp & 0x22) && (sp & 0x11);
+  return (pattern_0 | pattern_1) ? 2 : 1;
+}
+
+// rest_of_clusters (MaskedVeloClustering.cu:290-496) for one candidate pixel of the SP at
+// `sp_index` of a bank whose words are `words`. Returns true and fills *out if the cluster is kept.
+template<int version>
+__device__ __forceinline__ bool masked_cluster(
+  const Geometry& g,
+  const uint32_t* words,
+  unsigned n_sp,
+  unsigned sp_index,
+  uint32_t sensor_number,
+  uint32_t starting_pixel_location,
+  ClusterOut* out)
+{
+  const float* ltg = g.ltg + 12 * sensor_number;
+  const uint32_t sp_word = words[sp_index];
+  const uint32_t sp_addr = (sp_word & 0x007FFF00u) >> 8;
+  const int32_t sp_row = sp_addr & 0x3Fu;
+  const int32_t sp_col = sp_addr >> 6;
+  uint32_t pa[5] = {0, 0, 0, 0, 0};
+  const int32_t sp_row_lower = sp_row - 2, sp_col_lower = sp_col - 3;
+  const int32_t row_lower_limit = sp_row_lower * 4;
+  const int32_t col_lower_limit = sp_col_lower * 2;
+  for (unsigned k = 0; k < n_sp; ++k) {
+    const uint32_t other = words[k];
+    if constexpr (version > 3) {
+      if (((other >> 23) & 1u) != (sensor_number & 1u)) continue;
+    }
+    if (other & 0x80000000u) continue;
+    const uint32_t oaddr = (other & 0x007FFF00u) >> 8;
+    const int32_t rr = static_cast<int32_t>(oaddr & 0x3Fu) - sp_row_lower;
+    const int32_t rc = static_cast<int32_t>(oaddr >> 6) - sp_col_lower;
+    if (rr >= 0 && rr <= 3 && rc >= 0 && rc <= 4) {
+      const uint32_t osp = other & 0xFFu;
+      const uint32_t bits = ((osp & 0x0Fu) << (4 * rr)) | ((osp & 0xF0u) << (12 + 4 * rr));
+#pragma unroll
+      for (int c = 0; c < 5; ++c)
+        if (rc == c) pa[c] |= bits;
+    }
+  }
+  const uint32_t rel_row = starting_pixel_location & 0x3u;
+  const uint32_t rel_col = starting_pixel_location < 4;
+  const uint32_t col = sp_col * 2 + rel_col;
+  const uint64_t starting_pixel = ((uint64_t)(0x01u << (11 - rel_row)) << (16 * (col & 0x01))) << 32;
+  const uint64_t pixel_map = (((uint64_t) pa[3]) << 32) | pa[2];
+  const Grown cl = grow_cluster(pixel_map, starting_pixel, col_lower_limit + 4, row_lower_limit);
+  const uint64_t precedence = (mask_east(cl.pixels) & pa[4]) | (cl.pixels & (starting_pixel ^ (0ull - starting_pixel)));
+  bool keep = precedence == 0 && !(cl.pixels & 0x8000800080008000ull) && !(cl.pixels & 0x0001000100010001ull);
+  if (!keep) return false;
+  unsigned x = cl.x, y = cl.y, n = cl.n;
+  if (cl.pixels & 0xFFFFull) {
+    const uint64_t west_map = (((uint64_t) pa[1]) << 32) | pa[0];
+    const uint64_t west_start = mask_west(cl.pixels) & west_map;
+    const Grown ext = grow_cluster(west_map, west_start, col_lower_limit, row_lower_limit);
+    x += ext.x;
+    y += ext.y;
+    n += ext.n;
+    keep = !(ext.pixels & 0x8000800080008000ull) && !(ext.pixels & 0xFFFFull);
+  }
+  if (!keep) return false;
+  const unsigned cx = x / n;
+  const unsigned cy = y / n;
+  const float fx = x / static_cast<float>(n) - cx;
+  const float fy = y / static_cast<float>(n) - cy;
+  *out = place_cluster(g, ltg, lhcb_id_of(sensor_number, cx >> 8, cx & 0xFFu, cy), cx, cy, fx, fy);
+  return true;
+}
+
+__device__ __forceinline__ void store_cluster(const Clusters& c, unsigned i, const ClusterOut& o)
+{
+  c.ids()[i] = o.id;
+  c.xyz()[3 * (size_t) i] = o.x;
+  c.xyz()[3 * (size_t) i + 1] = o.y;
+  c.xyz()[3 * (size_t) i + 2] = o.z;
+  c.phis()[i] = o.phi;
+}
+
+// ---------------------------------------------------------------------------------------
+// K1: one wave per event sums the SP words of its VELO banks.
+template<int version>
+__global__ void k_number_of_candidates(
+  const char* raw,
+  const uint32_t* offsets,
+  const uint32_t* sizes,
+  const uint32_t* types,
+  const uint32_t* event_list,
+  unsigned n_list,
+  unsigned event_start,
+  uint32_t* number_of_candidates)
+{
+  const unsigned waves = blockDim.x / wave_size;
+  const unsigned ev_idx = blockIdx.x * waves + threadIdx.x / wave_size;
+  if (ev_idx >= n_list) return;
+  const unsigned event_number = event_list ? event_list[ev_idx] : ev_idx;
+  const RawEvent ev(raw, offsets, sizes, types, event_number + event_start);
+  uint32_t n = 0;
+  for (unsigned b = lane_id(); b < ev.n_banks; b += wave_size) {
+    const RawBank bank = raw_bank<version>(ev, b);
+    if (is_velo_bank(bank.type)) n += bank_n_sp<version>(bank);
+  }
+#pragma unroll
+  for (unsigned d = 32; d > 0; d >>= 1) n += __shfl_xor(n, d, wave_size);
+  if (lane_id() == 0) number_of_candidates[event_number] = 2 * n;
+}
+
+// dev_velo_bank_index as the CPU target leaves it: events of CPU block 0 (ev_idx = 0, G, 2G, ...)
+// write in order and the last write of each entry wins. Walk those events backwards and keep the
+// first value seen per entry; stop once every sensor has one.
+template<int version>
+__global__ void k_bank_index(
+  const char* raw,
+  const uint32_t* offsets,
+  const uint32_t* sizes,
+  const uint32_t* types,
+  const uint32_t* event_list,
+  unsigned n_list,
+  unsigned event_start,
+  unsigned cpu_grid,
+  uint32_t* bank_index)
+{
+  __shared__ int best[n_sensors];
+  __shared__ uint8_t written[n_sensors];
+  __shared__ unsigned n_written;
+  for (unsigned i = threadIdx.x; i < n_sensors; i += blockDim.x) written[i] = 0;
+  if (threadIdx.x == 0) n_written = 0;
+  __syncthreads();
+  if (n_list == 0) return;
+  const unsigned last = ((n_list - 1) / cpu_grid) * cpu_grid;
+  for (int ev_idx = last; ev_idx >= 0; ev_idx -= cpu_grid) {
+    for (unsigned i = threadIdx.x; i < n_sensors; i += blockDim.x) best[i] = -1;
+    __syncthreads();
+    const unsigned event_number = event_list ? event_list[ev_idx] : ev_idx;
+    const RawEvent ev(raw, offsets, sizes, types, event_number + event_start);
+    for (unsigned b = threadIdx.x; b < ev.n_banks; b += blockDim.x) {
+      const RawBank bank = raw_bank<version>(ev, b);
+      const uint32_t pair = bank.sourceID & 0x1FFu;
+      // Entries beyond the 208 sensors would be written out of bounds by the reference;
+      // they are skipped here.
+      if constexpr (version > 3) {
+        if ((pair << 1) < n_sensors) atomicMax(&best[pair << 1], (int) b);
+        if (((pair << 1) | 1u) < n_sensors) atomicMax(&best[(pair << 1) | 1u], (int) b);
+      }
+      else {
+        if (pair < n_sensors) atomicMax(&best[pair], (int) b);
+      }
+    }
+    __syncthreads();
+    for (unsigned i = threadIdx.x; i < n_sensors; i += blockDim.x) {
+      if (best[i] >= 0 && !written[i]) {
+        bank_index[i] = best[i];
+        written[i] = 1;
+        atomicAdd(&n_written, 1u);
+      }
+    }
+    __syncthreads();
+    if (n_written >= n_sensors) break;
+  }
+}
+
+// K2: one workgroup per event; wave w takes banks w, w+W, ...; lanes take SP words.
+template<int version>
+__global__ void k_estimate_input_size(
+  const char* raw,
+  const uint32_t* offsets,
+  const uint32_t* sizes,
+  const uint32_t* types,
+  const uint32_t* event_list,
+  unsigned event_start,
+  const uint32_t* candidates_offsets,
+  uint32_t* estimated_input_size,
+  uint32_t* module_candidate_num,
+  uint32_t* cluster_candidates)
+{
+  __shared__ uint32_t est[n_module_pairs];
+  __shared__ uint32_t n_cand;
+  const unsigned event_number = event_list ? event_list[blockIdx.x] : blockIdx.x;
+  for (unsigned i = threadIdx.x; i < n_module_pairs; i += blockDim.x) est[i] = 0;
+  if (threadIdx.x == 0) n_cand = 0;
+  __syncthreads();
+  uint32_t* cands = cluster_candidates + candidates_offsets[event_number];
+  const RawEvent ev(raw, offsets, sizes, types, event_number + event_start);
+  const unsigned waves = blockDim.x / wave_size;
+  for (unsigned b = threadIdx.x / wave_size; b < ev.n_banks; b += waves) {
+    const RawBank bank = raw_bank<version>(ev, b);
+    if (!is_velo_bank(bank.type)) continue;
+    const unsigned n_sp = bank_n_sp<version>(bank);
+    const uint32_t pair = bank.sourceID & 0x1FFu;
+    const unsigned mp = (version == 2 || version == 3) ? pair / 8 : (pair << 1) / 8;
+    if (mp >= n_module_pairs) continue; // sourceID outside the 208 sensors
+    uint32_t count = 0;
+    for (unsigned s = lane_id(); s < n_sp; s += wave_size) {
+      const uint32_t w = bank.word[s];
+      if (w & 0x80000000u) {
+        count += isolated_cluster_count(w & 0xFFu);
+      }
+      else {
+        unsigned p0 = 0, p1 = 0;
+        const unsigned m = sp_candidates(w, bank.word, n_sp, &p0, &p1);
+        const uint32_t sensor_number =
+          (version == 2 || version == 3) ? pair : ((pair << 1) | ((w >> 23) & 0x1u));
+        if (m & 1) cands[atomicAdd(&n_cand, 1u)] = (s << 11) | (sensor_number << 3) | p0;
+        if (m & 2) cands[atomicAdd(&n_cand, 1u)] = (s << 11) | (sensor_number << 3) | p1;
+        count += __popc(m);
+      }
+    }
+    if (count) atomicAdd(&est[mp], count);
+  }
+  __syncthreads();
+  for (unsigned i = threadIdx.x; i < n_module_pairs; i += blockDim.x)
+    estimated_input_size[event_number * n_module_pairs + i] = est[i];
+  if (threadIdx.x == 0) module_candidate_num[event_number] = n_cand;
+}
+
+// K3: one workgroup per event. Phase 1 takes isolated SPs of all banks, phase 2 the candidates.
+template<int version>
+__global__ void k_masked_clustering(
+  const char* raw,
+  const uint32_t* offsets,
+  const uint32_t* sizes,
+  const uint32_t* types,
+  const uint32_t* event_list,
+  unsigned event_start,
+  const uint32_t* number_of_events,
+  const uint32_t* offsets_estimated_input_size,
+  const uint32_t* module_candidate_num,
+  const uint32_t* cluster_candidates,
+  const uint32_t* candidates_offsets,
+  const uint32_t* bank_index,
+  const Geometry* geometry,
+  const uint8_t* sp_patterns,
+  const float* sp_fx,
+  const float* sp_fy,
+  uint32_t* module_cluster_num,
+  char* cluster_container)
+{
+  __shared__ uint32_t count[n_module_pairs];
+  const unsigned event_number = event_list ? event_list[blockIdx.x] : blockIdx.x;
+  const unsigned n_events = number_of_events[0];
+  const Clusters clusters {cluster_container, offsets_estimated_input_size[n_module_pairs * n_events]};
+  const uint32_t* mp_start = offsets_estimated_input_size + event_number * n_module_pairs;
+  for (unsigned i = threadIdx.x; i < n_module_pairs; i += blockDim.x) count[i] = 0;
+  __syncthreads();
+  const Geometry& g = *geometry;
+  const RawEvent ev(raw, offsets, sizes, types, event_number + event_start);
+  const unsigned waves = blockDim.x / wave_size;
+  for (unsigned b = threadIdx.x / wave_size; b < ev.n_banks; b += waves) {
+    const RawBank bank = raw_bank<version>(ev, b);
+    if (!is_velo_bank(bank.type)) continue;
+    const unsigned n_sp = bank_n_sp<version>(bank);
+    const uint32_t pair = bank.sourceID & 0x1FFu;
+    const unsigned mp = (version == 2 || version == 3) ? pair / 8 : (pair << 1) / 8;
+    if (mp >= n_module_pairs) continue;
+    for (unsigned s = lane_id(); s < n_sp; s += wave_size) {
+      const uint32_t w = bank.word[s];
+      if (!(w & 0x80000000u)) continue;
+      ClusterOut c[2];
+      const unsigned n = isolated_sp_clusters<version>(g, sp_patterns, sp_fx, sp_fy, bank, w, c);
+      const unsigned slot = atomicAdd(&count[mp], n);
+      const unsigned cap = mp_start[mp + 1] - mp_start[mp];
+      for (unsigned k = 0; k < n; ++k)
+        if (slot + k < cap) store_cluster(clusters, mp_start[mp] + slot + k, c[k]);
+    }
+  }
+  __syncthreads();
+  const unsigned n_cand = module_candidate_num[event_number];
+  const uint32_t* cands = cluster_candidates + candidates_offsets[event_number];
+  for (unsigned i = threadIdx.x; i < n_cand; i += blockDim.x) {
+    const uint32_t cand = cands[i];
+    const uint32_t sensor_number = (cand >> 3) & 0xFFu;
+    if (sensor_number >= n_sensors) continue;
+    const RawBank bank = raw_bank<version>(ev, bank_index[sensor_number]);
+    ClusterOut c;
+    if (masked_cluster<version>(g, bank.word, bank_n_sp<version>(bank), cand >> 11, sensor_number, cand & 7u, &c)) {
+      const unsigned mp = sensor_number / 8;
+      const unsigned slot = atomicAdd(&count[mp], 1u);
+      if (mp < n_module_pairs && slot < mp_start[mp + 1] - mp_start[mp]) store_cluster(clusters, mp_start[mp] + slot, c);
+    }
+  }
+  __syncthreads();
+  for (unsigned i = threadIdx.x; i < n_module_pairs; i += blockDim.x)
+    module_cluster_num[event_number * n_module_pairs + i] = min(count[i], mp_start[i + 1] - mp_start[i]);
+}
+
+// (phi, id) rank of element i among n (SortByPhi.cu:90-125, including its tie rule).
+template<typename PhiT, typename IdT>
+__device__ __forceinline__ unsigned phi_rank(const PhiT* phi, const IdT* id, unsigned n, unsigned i)
+{
+  const int16_t pi = phi[i];
+  const uint32_t ii = id[i];
+  unsigned position = 0;
+  for (unsigned j = 0; j < n; ++j) {
+    const int16_t pj = phi[j];
+    position += pi > pj || (i != j && pi == pj && ii > (uint32_t) id[j]);
+  }
+  return position;
+}
+
+// K4: one workgroup per (module pair, event). Keys are staged in LDS when they fit.
+constexpr unsigned sort_lds_cap = 2048;
+__global__ void k_sort_by_phi(
+  const uint32_t* event_list,
+  const uint32_t* number_of_events,
+  const uint32_t* offsets_estimated_input_size,
+  const uint32_t* module_cluster_num,
+  const char* cluster_container,
+  char* sorted_container,
+  uint32_t* hit_permutation)
+{
+  __shared__ int16_t s_phi[sort_lds_cap];
+  __shared__ uint32_t s_id[sort_lds_cap];
+  const unsigned mp = blockIdx.x;
+  const unsigned event_number = event_list ? event_list[blockIdx.y] : blockIdx.y;
+  const unsigned total = offsets_estimated_input_size[n_module_pairs * number_of_events[0]];
+  const Clusters in {const_cast<char*>(cluster_container), total};
+  const Clusters out {sorted_container, total};
+  const unsigned start = offsets_estimated_input_size[event_number * n_module_pairs + mp];
+  const unsigned n = module_cluster_num[event_number * n_module_pairs + mp];
+  const bool lds = n <= sort_lds_cap;
+  if (lds) {
+    for (unsigned i = threadIdx.x; i < n; i += blockDim.x) {
+      s_phi[i] = in.phis()[start + i];
+      s_id[i] = in.ids()[start + i];
+    }
+    __syncthreads();
+  }
+  for (unsigned i = threadIdx.x; i < n; i += blockDim.x) {
+    const unsigned r = lds ? phi_rank(s_phi, s_id, n, i) : phi_rank(in.phis() + start, in.ids() + start, n, i);
+    hit_permutation[start + r] = start + i;
+  }
+  __threadfence_block();
+  __syncthreads();
+  for (unsigned i = threadIdx.x; i < n; i += blockDim.x) {
+    const unsigned src = hit_permutation[start + i];
+    out.ids()[start + i] = in.ids()[src];
+    out.phis()[start + i] = in.phis()[src];
+    out.xyz()[3 * (size_t)(start + i)] = in.xyz()[3 * (size_t) src];
+    out.xyz()[3 * (size_t)(start + i) + 1] = in.xyz()[3 * (size_t) src + 1];
+    out.xyz()[3 * (size_t)(start + i) + 2] = in.xyz()[3 * (size_t) src + 2];
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Fused K2+K3+K4 per (module pair, event), used by the whole-sequence path after the
+// estimate scan. The module pair's SP words are staged in LDS; clusters are built into LDS,
+// ranked by (phi, id) and written to their sorted slot. A module pair that does not fit
+// the LDS budget is processed the same way out of global memory.
+constexpr unsigned fused_word_cap = 1024;
+constexpr unsigned fused_cluster_cap = 1024;
+constexpr unsigned max_banks_per_mp = 8; // v2/v3: 8 sensors per module pair; v4: 4 sensor pairs
+
+template<int version>
+__global__ void __launch_bounds__(256) k_cluster_sorted(
+  const char* raw,
+  const uint32_t* offsets,
+  const uint32_t* sizes,
+  const uint32_t* types,
+  unsigned n_events,
+  const uint32_t* offsets_estimated_input_size,
+  const Geometry* geometry,
+  const uint8_t* sp_patterns,
+  const float* sp_fx,
+  const float* sp_fy,
+  uint32_t* module_cluster_num,
+  char* sorted_container,
+  char* unsorted_scratch,
+  uint32_t* status)
+{
+  __shared__ uint32_t s_words[fused_word_cap];
+  __shared__ int16_t s_phi[fused_cluster_cap];
+  __shared__ uint32_t s_id[fused_cluster_cap];
+  __shared__ float s_xyz[3 * fused_cluster_cap];
+  __shared__ uint32_t s_bank_start[max_banks_per_mp + 1];
+  __shared__ uint32_t s_bank_num[max_banks_per_mp];
+  __shared__ uint32_t s_bank_src[max_banks_per_mp];
+  __shared__ unsigned s_nbanks, s_count;
+
+  constexpr bool v23 = (version == 2 || version == 3);
+  const unsigned mp = blockIdx.x;
+  const unsigned event_number = blockIdx.y;
+  const unsigned total = offsets_estimated_input_size[n_module_pairs * n_events];
+  const unsigned start = offsets_estimated_input_size[event_number * n_module_pairs + mp];
+  const unsigned estimate = offsets_estimated_input_size[event_number * n_module_pairs + mp + 1] - start;
+  const Geometry& g = *geometry;
+  const RawEvent ev(raw, offsets, sizes, types, event_number);
+
+  // Banks of this module pair, in bank order.
+  if (threadIdx.x == 0) s_nbanks = 0;
+  __syncthreads();
+  for (unsigned b = threadIdx.x; b < ev.n_banks; b += blockDim.x) {
+    const RawBank bank = raw_bank<version>(ev, b);
+    if (!is_velo_bank(bank.type)) continue;
+    const uint32_t pair = bank.sourceID & 0x1FFu;
+    const unsigned bmp = v23 ? pair / 8 : (pair << 1) / 8;
+    if (bmp == mp) {
+      const unsigned k = atomicAdd(&s_nbanks, 1u);
+      if (k < max_banks_per_mp) s_bank_num[k] = b;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned nb = min(s_nbanks, max_banks_per_mp);
+    // insertion sort: a handful of banks
+    for (unsigned i = 1; i < nb; ++i) {
+      const uint32_t v = s_bank_num[i];
+      int j = i - 1;
+      while (j >= 0 && s_bank_num[j] > v) {
+        s_bank_num[j + 1] = s_bank_num[j];
+        --j;
+      }
+      s_bank_num[j + 1] = v;
+    }
+    uint32_t acc = 0;
+    for (unsigned i = 0; i < nb; ++i) {
+      const RawBank bank = raw_bank<version>(ev, s_bank_num[i]);
+      s_bank_start[i] = acc;
+      s_bank_src[i] = bank.sourceID;
+      acc += bank_n_sp<version>(bank);
+    }
+    s_bank_start[nb] = acc;
+    s_nbanks = nb;
+    s_count = 0;
+  }
+  __syncthreads();
+  const unsigned nb = s_nbanks;
+  const unsigned n_words = s_bank_start[nb];
+  const bool words_in_lds = n_words <= fused_word_cap;
+  const bool clusters_in_lds = estimate <= fused_cluster_cap;
+  if (!words_in_lds || !clusters_in_lds) {
+    if (threadIdx.x == 0) atomicOr(status, (uint32_t) status_lds_fallback);
+  }
+  if (words_in_lds) {
+    for (unsigned bi = 0; bi < nb; ++bi) {
+      const RawBank bank = raw_bank<version>(ev, s_bank_num[bi]);
+      const unsigned n = s_bank_start[bi + 1] - s_bank_start[bi];
+      for (unsigned s = threadIdx.x; s < n; s += blockDim.x) s_words[s_bank_start[bi] + s] = bank.word[s];
+    }
+  }
+  __syncthreads();
+
+  // Clusters go to LDS, or to the unsorted scratch container (same gapped layout) if too many.
+  const Clusters scratch {unsorted_scratch, total};
+  auto emit = [&](const ClusterOut& c) {
+    const unsigned slot = atomicAdd(&s_count, 1u);
+    if (slot >= estimate) return; // cannot happen for well-formed banks: estimate bounds the count
+    if (clusters_in_lds) {
+      s_phi[slot] = c.phi;
+      s_id[slot] = c.id;
+      s_xyz[3 * slot] = c.x;
+      s_xyz[3 * slot + 1] = c.y;
+      s_xyz[3 * slot + 2] = c.z;
+    }
+    else {
+      store_cluster(scratch, start + slot, c);
+    }
+  };
+
+  for (unsigned w = threadIdx.x; w < n_words; w += blockDim.x) {
+    unsigned bi = 0;
+    while (bi + 1 < nb && s_bank_start[bi + 1] <= w) ++bi;
+    const unsigned s = w - s_bank_start[bi];
+    RawBank bank;
+    bank.sourceID = s_bank_src[bi];
+    const unsigned n_sp = s_bank_start[bi + 1] - s_bank_start[bi];
+    const uint32_t* words;
+    if (words_in_lds) words = s_words + s_bank_start[bi];
+    else words = raw_bank<version>(ev, s_bank_num[bi]).word;
+    const uint32_t sp_word = words[s];
+    if (sp_word & 0x80000000u) {
+      ClusterOut c[2];
+      const unsigned n = isolated_sp_clusters<version>(g, sp_patterns, sp_fx, sp_fy, bank, sp_word, c);
+      for (unsigned k = 0; k < n; ++k) emit(c[k]);
+    }
+    else {
+      unsigned p0 = 0, p1 = 0;
+      const unsigned m = sp_candidates(sp_word, words, n_sp, &p0, &p1);
+      const uint32_t pair = bank.sourceID & 0x1FFu;
+      const uint32_t sensor_number = v23 ? pair : ((pair << 1) | ((sp_word >> 23) & 0x1u));
+      ClusterOut c;
+      if ((m & 1) && masked_cluster<version>(g, words, n_sp, s, sensor_number, p0, &c)) emit(c);
+      if ((m & 2) && masked_cluster<version>(g, words, n_sp, s, sensor_number, p1, &c)) emit(c);
+    }
+  }
+  __threadfence_block();
+  __syncthreads();
+  const unsigned n = min(s_count, estimate);
+  if (threadIdx.x == 0) module_cluster_num[event_number * n_module_pairs + mp] = n;
+  const Clusters out {sorted_container, total};
+  for (unsigned i = threadIdx.x; i < n; i += blockDim.x) {
+    unsigned r;
+    ClusterOut c;
+    if (clusters_in_lds) {
+      r = phi_rank(s_phi, s_id, n, i);
+      c.id = s_id[i];
+      c.phi = s_phi[i];
+      c.x = s_xyz[3 * i];
+      c.y = s_xyz[3 * i + 1];
+      c.z = s_xyz[3 * i + 2];
+    }
+    else {
+      r = phi_rank(scratch.phis() + start, scratch.ids() + start, n, i);
+      c.id = scratch.ids()[start + i];
+      c.phi = scratch.phis()[start + i];
+      c.x = scratch.xyz()[3 * (size_t)(start + i)];
+      c.y = scratch.xyz()[3 * (size_t)(start + i) + 1];
+      c.z = scratch.xyz()[3 * (size_t)(start + i) + 2];
+    }
+    store_cluster(out, start + r, c);
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Host launchers.
+template<int version>
+static hipError_t launch_candidates_v(const CandidatesLaunch& a, hipStream_t s)
+{
+  const unsigned waves = 4;
+  const unsigned grid = (a.n_list + waves - 1) / waves;
+  if (grid) {
+    hipLaunchKernelGGL(
+      k_number_of_candidates<version>, dim3(grid), dim3(waves * wave_size), 0, s, a.raw, a.offsets, a.sizes, a.types,
+      a.event_list, a.n_list, a.event_start, a.number_of_candidates);
+  }
+  const unsigned cpu_grid = (a.n_list + a.block_dim_x - 1) / a.block_dim_x;
+  if (a.n_list) {
+    hipLaunchKernelGGL(
+      k_bank_index<version>, dim3(1), dim3(256), 0, s, a.raw, a.offsets, a.sizes, a.types, a.event_list, a.n_list,
+      a.event_start, cpu_grid, a.bank_index);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_number_of_candidates(const CandidatesLaunch& a, hipStream_t s)
+{
+  switch (a.version) {
+  case 2: return launch_candidates_v<2>(a, s);
+  case 3: return launch_candidates_v<3>(a, s);
+  default: return launch_candidates_v<4>(a, s);
+  }
+}
+
+hipError_t launch_estimate_input_size(const EstimateLaunch& a, hipStream_t s)
+{
+  if (!a.n_list) return hipSuccess;
+#define VELO_LAUNCH_EST(V)                                                                                          \
+  hipLaunchKernelGGL(                                                                                               \
+    k_estimate_input_size<V>, dim3(a.n_list), dim3(256), 0, s, a.raw, a.offsets, a.sizes, a.types, a.event_list,    \
+    a.event_start, a.candidates_offsets, a.estimated_input_size, a.module_candidate_num, a.cluster_candidates)
+  switch (a.version) {
+  case 2: VELO_LAUNCH_EST(2); break;
+  case 3: VELO_LAUNCH_EST(3); break;
+  default: VELO_LAUNCH_EST(4); break;
+  }
+#undef VELO_LAUNCH_EST
+  return hipGetLastError();
+}
+
+hipError_t launch_masked_clustering(const MaskedClusteringLaunch& a, hipStream_t s)
+{
+  if (!a.n_list) return hipSuccess;
+#define VELO_LAUNCH_MC(V)                                                                                           \
+  hipLaunchKernelGGL(                                                                                               \
+    k_masked_clustering<V>, dim3(a.n_list), dim3(256), 0, s, a.raw, a.offsets, a.sizes, a.types, a.event_list,      \
+    a.event_start, a.number_of_events, a.offsets_estimated_input_size, a.module_candidate_num, a.cluster_candidates, \
+    a.candidates_offsets, a.bank_index, reinterpret_cast<const Geometry*>(a.geometry), a.sp_patterns, a.sp_fx,       \
+    a.sp_fy, a.module_cluster_num, a.cluster_container)
+  switch (a.version) {
+  case 2: VELO_LAUNCH_MC(2); break;
+  case 3: VELO_LAUNCH_MC(3); break;
+  default: VELO_LAUNCH_MC(4); break;
+  }
+#undef VELO_LAUNCH_MC
+  return hipGetLastError();
+}
+
+hipError_t launch_sort_by_phi(const SortLaunch& a, hipStream_t s)
+{
+  if (!a.n_list) return hipSuccess;
+  hipLaunchKernelGGL(
+    k_sort_by_phi, dim3(n_module_pairs, a.n_list), dim3(256), 0, s, a.event_list, a.number_of_events,
+    a.offsets_estimated_input_size, a.module_cluster_num, a.cluster_container, a.sorted_container, a.hit_permutation);
+  return hipGetLastError();
+}
+
+hipError_t launch_cluster_sorted(const FusedClusterLaunch& a, hipStream_t s)
+{
+  if (!a.n_events) return hipSuccess;
+#define VELO_LAUNCH_FUSED(V)                                                                                        \
+  hipLaunchKernelGGL(                                                                                               \
+    k_cluster_sorted<V>, dim3(n_module_pairs, a.n_events), dim3(256), 0, s, a.raw, a.offsets, a.sizes, a.types,     \
+    a.n_events, a.offsets_estimated_input_size, reinterpret_cast<const Geometry*>(a.geometry), a.sp_patterns,        \
+    a.sp_fx, a.sp_fy, a.module_cluster_num, a.sorted_container, a.unsorted_scratch, a.status)
+  switch (a.version) {
+  case 2: VELO_LAUNCH_FUSED(2); break;
+  case 3: VELO_LAUNCH_FUSED(3); break;
+  default: VELO_LAUNCH_FUSED(4); break;
+  }
+#undef VELO_LAUNCH_FUSED
+  return hipGetLastError();
+}
+
+} // namespace velo

@@ -1,0 +1,384 @@
+// Track consolidation, the simplified VELO Kalman fit, the device prefix sum and fills.
+//
+//   k_copy_track_hit_number   VeloCopyTrackHitNumber.cu:49-78
+//   k_consolidate_tracks      VeloConsolidateTracks.cu:120-192 (one thread per track hit)
+//   k_kalman_filter           VeloKalmanFilter.cu:335-418, VeloKalmanFilter.cuh:25-158 (one thread per track)
+//   k_scan_*                  replaces host_prefix_sum_t (HostPrefixSum.cpp:76-107) by a device scan
+#include "velo_common.cuh"
+#include "velo_kernels.h"
+
+#include <algorithm>
+
+namespace velo {
+
+// ---------------------------------------------------------------------------------------
+// Prefix sum: out[0] = 0, out[i+1] = in[0] + ... + in[i] (u32 wrap), in three passes:
+// per-tile sums, a scan of the tile sums, then per-tile scans seeded with their offset.
+// Sizes up to one tile run as a single workgroup.
+constexpr unsigned scan_block = 1024;
+constexpr unsigned scan_items = 4;
+constexpr unsigned scan_tile = scan_block * scan_items;
+
+__device__ __forceinline__ uint32_t block_scan_1024(uint32_t v, uint32_t* scratch, uint32_t* total)
+{
+  return block_exclusive_scan(v, scratch, total);
+}
+
+__device__ __forceinline__ unsigned scan_length(const uint32_t* n_dev, unsigned n_cap)
+{
+  return n_dev ? min(n_dev[0], n_cap) : n_cap;
+}
+
+__global__ void __launch_bounds__(scan_block)
+  k_scan_tile_sums(const uint32_t* in, const uint32_t* n_dev, unsigned n_cap, uint32_t* tile_sums)
+{
+  __shared__ uint32_t scratch[scan_block / wave_size + 1];
+  const unsigned n = scan_length(n_dev, n_cap);
+  const size_t base = (size_t) blockIdx.x * scan_tile;
+  uint32_t s = 0;
+#pragma unroll
+  for (unsigned k = 0; k < scan_items; ++k) {
+    const size_t i = base + (size_t) k * scan_block + threadIdx.x;
+    if (i < n) s += in[i];
+  }
+  uint32_t total;
+  block_scan_1024(s, scratch, &total);
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+// Exclusive scan of `n` values in place by one workgroup (n_tiles is small).
+__global__ void __launch_bounds__(scan_block) k_scan_small_inplace(uint32_t* v, unsigned n)
+{
+  __shared__ uint32_t scratch[scan_block / wave_size + 1];
+  uint32_t carry = 0;
+  for (unsigned c = 0; c < n; c += scan_block) {
+    const unsigned i = c + threadIdx.x;
+    const uint32_t x = i < n ? v[i] : 0u;
+    uint32_t total;
+    const uint32_t e = block_scan_1024(x, scratch, &total);
+    if (i < n) v[i] = carry + e;
+    carry += total;
+  }
+}
+
+// Per-tile scan: thread t owns items [t*items, (t+1)*items) of its tile (contiguous, so the
+// prefix within a thread is a short serial sum).
+__global__ void __launch_bounds__(scan_block)
+  k_scan_tiles(const uint32_t* in, const uint32_t* n_dev, unsigned n_cap, uint32_t* out, const uint32_t* tile_offsets)
+{
+  __shared__ uint32_t scratch[scan_block / wave_size + 1];
+  const unsigned n = scan_length(n_dev, n_cap);
+  const size_t base = (size_t) blockIdx.x * scan_tile + (size_t) threadIdx.x * scan_items;
+  uint32_t v[scan_items];
+  uint32_t s = 0;
+#pragma unroll
+  for (unsigned k = 0; k < scan_items; ++k) {
+    const size_t i = base + k;
+    v[k] = i < n ? in[i] : 0u;
+    s += v[k];
+  }
+  uint32_t total;
+  uint32_t run = block_scan_1024(s, scratch, &total) + (tile_offsets ? tile_offsets[blockIdx.x] : 0u);
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = 0;
+#pragma unroll
+  for (unsigned k = 0; k < scan_items; ++k) {
+    const size_t i = base + k;
+    run += v[k];
+    if (i < n) out[i + 1] = run;
+  }
+}
+
+size_t prefix_sum_scratch_words(unsigned n) { return (n + scan_tile - 1) / scan_tile; }
+
+hipError_t launch_prefix_sum_dyn(
+  const uint32_t* in,
+  const uint32_t* n_dev,
+  unsigned n_cap,
+  uint32_t* out,
+  uint32_t* scratch,
+  hipStream_t s)
+{
+  const unsigned tiles = (n_cap + scan_tile - 1) / scan_tile;
+  if (tiles <= 1) {
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(scan_block), 0, s, in, n_dev, n_cap, out, nullptr);
+    return hipGetLastError();
+  }
+  if (!scratch) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_scan_tile_sums, dim3(tiles), dim3(scan_block), 0, s, in, n_dev, n_cap, scratch);
+  hipLaunchKernelGGL(k_scan_small_inplace, dim3(1), dim3(scan_block), 0, s, scratch, tiles);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(scan_block), 0, s, in, n_dev, n_cap, out, scratch);
+  return hipGetLastError();
+}
+
+hipError_t launch_prefix_sum(const uint32_t* in, unsigned n, uint32_t* out, uint32_t* scratch, hipStream_t s)
+{
+  return launch_prefix_sum_dyn(in, nullptr, n, out, scratch, s);
+}
+
+__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t value)
+{
+  for (size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t) gridDim.x * blockDim.x)
+    p[i] = value;
+}
+__global__ void k_fill_u8(uint8_t* p, size_t n, uint8_t value)
+{
+  for (size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t) gridDim.x * blockDim.x)
+    p[i] = value;
+}
+
+hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t value, hipStream_t s)
+{
+  if (!n) return hipSuccess;
+  const unsigned grid = (unsigned) std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid), dim3(256), 0, s, p, n, value);
+  return hipGetLastError();
+}
+hipError_t launch_fill_u8(uint8_t* p, size_t n, uint8_t value, hipStream_t s)
+{
+  if (!n) return hipSuccess;
+  const unsigned grid = (unsigned) std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_u8, dim3(grid), dim3(256), 0, s, p, n, value);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------
+// K7 (grid = number of events).
+__global__ void k_copy_track_hit_number(
+  const TrackHits* tracks_all,
+  const uint32_t* offsets_velo_tracks,
+  const uint32_t* offsets_estimated_input_size,
+  const uint32_t* offsets_three_hit,
+  uint32_t* velo_track_hit_number,
+  uint32_t* offsets_all_velo_tracks)
+{
+  const unsigned e = blockIdx.x;
+  const TrackHits* tracks = tracks_all + track_offset(offsets_estimated_input_size, e);
+  const unsigned n_tracks = offsets_velo_tracks[e + 1] - offsets_velo_tracks[e];
+  const unsigned n_three = offsets_three_hit[e + 1] - offsets_three_hit[e];
+  uint32_t* out = velo_track_hit_number + offsets_velo_tracks[e] + offsets_three_hit[e];
+  for (unsigned i = threadIdx.x; i < n_tracks; i += blockDim.x) out[i] = tracks[i].hitsNum;
+  for (unsigned i = threadIdx.x; i < n_three; i += blockDim.x) out[n_tracks + i] = 3;
+  if (threadIdx.x == 0) {
+    offsets_all_velo_tracks[e + 1] = offsets_velo_tracks[e + 1] + offsets_three_hit[e + 1];
+    if (e == 0) offsets_all_velo_tracks[0] = 0;
+  }
+}
+
+// K8: one thread per (track, hit); the event's tracks are [main tracks][filtered 3-hit tracks].
+__global__ void k_consolidate_tracks(
+  const uint32_t* event_list,
+  const uint32_t* number_of_events,
+  const uint32_t* offsets_all_velo_tracks,
+  const TrackHits* tracks_all,
+  const uint32_t* offsets_velo_track_hit_number,
+  const char* sorted_container,
+  const uint32_t* offsets_estimated_input_size,
+  const TrackletHits* three_hit_output_all,
+  const uint32_t* offsets_three_hit,
+  char* velo_track_hits,
+  uint8_t* accepted)
+{
+  const unsigned e = event_list ? event_list[blockIdx.x] : blockIdx.x;
+  const unsigned n_events = number_of_events[0];
+  const unsigned total_clusters = offsets_estimated_input_size[n_module_pairs * n_events];
+  const unsigned total_tracks = offsets_all_velo_tracks[n_events];
+  const unsigned total_hits = offsets_velo_track_hit_number[total_tracks];
+  const Clusters in {const_cast<char*>(sorted_container), total_clusters};
+  const Clusters out {velo_track_hits, total_hits};
+  const unsigned hit_offset = offsets_estimated_input_size[e * n_module_pairs];
+  const unsigned toff = track_offset(offsets_estimated_input_size, e);
+  const unsigned t0 = offsets_all_velo_tracks[e];
+  const unsigned n_total = offsets_all_velo_tracks[e + 1] - t0;
+  const unsigned n_main = n_total - (offsets_three_hit[e + 1] - offsets_three_hit[e]);
+  const unsigned h_begin = offsets_velo_track_hit_number[t0];
+  const unsigned h_end = offsets_velo_track_hit_number[t0 + n_total];
+  for (unsigned i = threadIdx.x; i < n_total; i += blockDim.x)
+    if (accepted) accepted[t0 + i] = 1;
+  for (unsigned h = h_begin + threadIdx.x; h < h_end; h += blockDim.x) {
+    // track of hit h: binary search in the event's slice of the hit-number offsets
+    unsigned lo = 0, hi = n_total;
+    while (hi - lo > 1) {
+      const unsigned mid = (lo + hi) / 2;
+      if (offsets_velo_track_hit_number[t0 + mid] <= h) lo = mid;
+      else hi = mid;
+    }
+    const unsigned k = h - offsets_velo_track_hit_number[t0 + lo];
+    const unsigned hit = lo < n_main ? tracks_all[toff + lo].hits[k] : three_hit_output_all[toff + lo - n_main].hits[k];
+    const size_t src = (size_t) hit_offset + hit;
+    out.ids()[h] = in.ids()[src];
+    out.xyz()[3 * (size_t) h] = in.xyz()[3 * src];
+    out.xyz()[3 * (size_t) h + 1] = in.xyz()[3 * src + 1];
+    out.xyz()[3 * (size_t) h + 2] = in.xyz()[3 * src + 2];
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// K9
+struct KState {
+  float x, y, z, tx, ty;
+  float c00, c20, c22, c11, c31, c33;
+};
+
+__device__ __forceinline__ void
+filter_step(float z, float zhit, float xhit, float whit, float& x, float& tx, float& covXX, float& covXTx, float& covTxTx)
+{
+  const float dz = zhit - z;
+  const float predx = x + dz * tx;
+  const float dz_t_covTxTx = dz * covTxTx;
+  const float predcovXTx = covXTx + dz_t_covTxTx;
+  const float dx_t_covXTx = dz * covXTx;
+  const float predcovXX = covXX + 2 * dx_t_covXTx + dz * dz_t_covTxTx;
+  const float predcovTxTx = covTxTx;
+  const float R = 1.0f / ((1.0f / whit) + predcovXX);
+  const float Kx = predcovXX * R;
+  const float KTx = predcovXTx * R;
+  const float r = xhit - predx;
+  x = predx + Kx * r;
+  tx = tx + KTx * r;
+  covXX = (1 - Kx) * predcovXX;
+  covXTx = (1 - Kx) * predcovXTx;
+  covTxTx = predcovTxTx - KTx * predcovXTx;
+}
+
+template<bool upstream>
+__device__ __forceinline__ KState
+simplified_fit(const float* xyz, unsigned nh, float at_tx, float at_ty, const float* beamline, bool backward)
+{
+  const int direction = (backward ? 1 : -1) * (upstream ? 1 : -1);
+  const float noise2PerLayer = 1e-8f + 7e-6f * (at_tx * at_tx + at_ty * at_ty);
+  int firsthit = 0;
+  int lasthit = static_cast<int>(nh) - 1;
+  int dhit = 1;
+  if ((xyz[3 * lasthit + 2] - xyz[3 * firsthit + 2]) * direction < 0) {
+    const int t = firsthit;
+    firsthit = lasthit;
+    lasthit = t;
+    dhit = -1;
+  }
+  KState s;
+  s.x = xyz[3 * firsthit];
+  s.y = xyz[3 * firsthit + 1];
+  s.z = xyz[3 * firsthit + 2];
+  s.tx = at_tx;
+  s.ty = at_ty;
+  s.c00 = param_w_inverted;
+  s.c11 = param_w_inverted;
+  s.c20 = 0.f;
+  s.c31 = 0.f;
+  s.c22 = 1.f;
+  s.c33 = 1.f;
+  for (int i = firsthit + dhit; i != lasthit + dhit; i += dhit) {
+    const float hx = xyz[3 * i], hy = xyz[3 * i + 1], hz = xyz[3 * i + 2];
+    s.c22 += noise2PerLayer;
+    s.c33 += noise2PerLayer;
+    filter_step(s.z, hz, hx, param_w, s.x, s.tx, s.c00, s.c20, s.c22);
+    filter_step(s.z, hz, hy, param_w, s.y, s.ty, s.c11, s.c31, s.c33);
+    s.z = hz;
+  }
+  s.c22 += noise2PerLayer;
+  s.c33 += noise2PerLayer;
+  float delta_z = 0.f;
+  if constexpr (upstream) {
+    delta_z = (s.tx * (beamline[0] - s.x) + s.ty * (beamline[1] - s.y)) / (s.tx * s.tx + s.ty * s.ty);
+  }
+  else {
+    delta_z = z_end_velo - s.z;
+  }
+  s.x = s.x + s.tx * delta_z;
+  s.y = s.y + s.ty * delta_z;
+  s.z = s.z + delta_z;
+  const float dz2 = delta_z * delta_z;
+  s.c00 += dz2 * s.c22 + 2.f * delta_z * s.c20;
+  s.c11 += dz2 * s.c33 + 2.f * delta_z * s.c31;
+  s.c20 += s.c22 * delta_z;
+  s.c31 += s.c33 * delta_z;
+  return s;
+}
+
+__device__ __forceinline__ void store_state(float* base, unsigned total_tracks, unsigned i, const KState& s)
+{
+  float* st = base + 6 * (size_t) i;
+  st[0] = s.x;
+  st[1] = s.y;
+  st[2] = s.z;
+  st[3] = s.tx;
+  st[4] = s.ty;
+  float* cv = base + 6 * (size_t) total_tracks + 8 * (size_t) i;
+  cv[0] = s.c00;
+  cv[1] = s.c20;
+  cv[2] = s.c22;
+  cv[3] = s.c11;
+  cv[4] = s.c31;
+  cv[5] = s.c33;
+}
+
+__global__ void k_kalman_filter(
+  const uint32_t* event_list,
+  const uint32_t* number_of_events,
+  const uint32_t* offsets_all_velo_tracks,
+  const uint32_t* offsets_velo_track_hit_number,
+  const char* velo_track_hits,
+  const float* beamline,
+  uint8_t* is_backward,
+  float* beamline_states,
+  float* endvelo_states)
+{
+  const unsigned e = event_list ? event_list[blockIdx.x] : blockIdx.x;
+  const unsigned n_events = number_of_events[0];
+  const unsigned total_tracks = offsets_all_velo_tracks[n_events];
+  const unsigned total_hits = offsets_velo_track_hit_number[total_tracks];
+  const float* xyz_all = reinterpret_cast<const float*>(velo_track_hits) + total_hits;
+  const float bl[2] = {beamline[0], beamline[1]};
+  for (unsigned t = offsets_all_velo_tracks[e] + threadIdx.x; t < offsets_all_velo_tracks[e + 1]; t += blockDim.x) {
+    const unsigned h0 = offsets_velo_track_hit_number[t];
+    const unsigned nh = offsets_velo_track_hit_number[t + 1] - h0;
+    if (nh < 2) continue; // every VELO track has >= 3 hits
+    const float* xyz = xyz_all + 3 * (size_t) h0;
+    // linear_fit (VeloKalmanFilter.cu:335-355)
+    const float fx = xyz[0], fy = xyz[1], fz = xyz[2];
+    const float lx = xyz[3 * (nh - 1)], ly = xyz[3 * (nh - 1) + 1], lz = xyz[3 * (nh - 1) + 2];
+    const float ltx = (lx - fx) / (lz - fz);
+    const float lty = (ly - fy) / (lz - fz);
+    const float ldz = (ltx * (bl[0] - lx) + lty * (bl[1] - ly)) / (ltx * ltx + lty * lty);
+    const float lin_z = lz + ldz;
+    const bool backward = lin_z > fz;
+    is_backward[t] = backward;
+    const KState b = simplified_fit<true>(xyz, nh, ltx, lty, bl, backward);
+    const KState v = simplified_fit<false>(xyz, nh, b.tx, b.ty, bl, backward);
+    store_state(beamline_states, total_tracks, t, b);
+    store_state(endvelo_states, total_tracks, t, v);
+  }
+}
+
+hipError_t launch_copy_track_hit_number(const CopyHitNumberLaunch& a, hipStream_t s)
+{
+  if (!a.n_events) return hipSuccess;
+  hipLaunchKernelGGL(
+    k_copy_track_hit_number, dim3(a.n_events), dim3(256), 0, s, reinterpret_cast<const TrackHits*>(a.tracks),
+    a.offsets_velo_tracks, a.offsets_estimated_input_size, a.offsets_three_hit, a.velo_track_hit_number,
+    a.offsets_all_velo_tracks);
+  return hipGetLastError();
+}
+
+hipError_t launch_consolidate_tracks(const ConsolidateLaunch& a, hipStream_t s)
+{
+  if (!a.n_list) return hipSuccess;
+  hipLaunchKernelGGL(
+    k_consolidate_tracks, dim3(a.n_list), dim3(256), 0, s, a.event_list, a.number_of_events, a.offsets_all_velo_tracks,
+    reinterpret_cast<const TrackHits*>(a.tracks), a.offsets_velo_track_hit_number, a.sorted_container,
+    a.offsets_estimated_input_size, reinterpret_cast<const TrackletHits*>(a.three_hit_tracks_output),
+    a.offsets_three_hit, a.velo_track_hits, a.accepted_velo_tracks);
+  return hipGetLastError();
+}
+
+hipError_t launch_kalman_filter(const KalmanLaunch& a, hipStream_t s)
+{
+  if (!a.n_list) return hipSuccess;
+  hipLaunchKernelGGL(
+    k_kalman_filter, dim3(a.n_list), dim3(256), 0, s, a.event_list, a.number_of_events, a.offsets_all_velo_tracks,
+    a.offsets_velo_track_hit_number, a.velo_track_hits, a.beamline, a.is_backward,
+    reinterpret_cast<float*>(a.beamline_states), reinterpret_cast<float*>(a.endvelo_states));
+  return hipGetLastError();
+}
+
+} // namespace velo

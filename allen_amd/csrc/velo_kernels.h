@@ -1,0 +1,193 @@
+// Internal launch descriptors between the C ABI (velo_api.hip) and the kernel files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+
+namespace velo {
+
+struct CandidatesLaunch {
+  int version;
+  const char* raw;
+  const uint32_t* offsets;
+  const uint32_t* sizes;
+  const uint32_t* types;
+  const uint32_t* event_list; // NULL = identity
+  unsigned n_list;
+  unsigned event_start;
+  unsigned block_dim_x;
+  uint32_t* number_of_candidates;
+  uint32_t* bank_index;
+};
+
+struct EstimateLaunch {
+  int version;
+  const char* raw;
+  const uint32_t* offsets;
+  const uint32_t* sizes;
+  const uint32_t* types;
+  const uint32_t* event_list;
+  unsigned n_list;
+  unsigned event_start;
+  const uint32_t* candidates_offsets;
+  uint32_t* estimated_input_size;
+  uint32_t* module_candidate_num;
+  uint32_t* cluster_candidates;
+};
+
+struct MaskedClusteringLaunch {
+  int version;
+  const char* raw;
+  const uint32_t* offsets;
+  const uint32_t* sizes;
+  const uint32_t* types;
+  const uint32_t* event_list;
+  unsigned n_list;
+  unsigned event_start;
+  const uint32_t* number_of_events;
+  const uint32_t* offsets_estimated_input_size;
+  const uint32_t* module_candidate_num;
+  const uint32_t* cluster_candidates;
+  const uint32_t* candidates_offsets;
+  const uint32_t* bank_index;
+  const void* geometry;
+  const uint8_t* sp_patterns;
+  const float* sp_fx;
+  const float* sp_fy;
+  uint32_t* module_cluster_num;
+  char* cluster_container;
+};
+
+struct SortLaunch {
+  const uint32_t* event_list;
+  unsigned n_list;
+  const uint32_t* number_of_events;
+  const uint32_t* offsets_estimated_input_size;
+  const uint32_t* module_cluster_num;
+  const char* cluster_container;
+  char* sorted_container;
+  uint32_t* hit_permutation;
+};
+
+struct FusedClusterLaunch {
+  int version;
+  const char* raw;
+  const uint32_t* offsets;
+  const uint32_t* sizes;
+  const uint32_t* types;
+  unsigned n_events;
+  const uint32_t* offsets_estimated_input_size;
+  const void* geometry;
+  const uint8_t* sp_patterns;
+  const float* sp_fx;
+  const float* sp_fy;
+  uint32_t* module_cluster_num;
+  char* sorted_container;
+  char* unsorted_scratch;
+  uint32_t* status;
+};
+
+struct SearchLaunch {
+  const uint32_t* event_list;
+  unsigned n_list;
+  const uint32_t* number_of_events;
+  const char* sorted_container;
+  const uint32_t* offsets_estimated_input_size;
+  const uint32_t* module_cluster_num;
+  const void* geometry;
+  void* tracks;
+  void* tracklets;
+  uint32_t* tracks_to_follow;
+  void* three_hit_tracks;
+  uint8_t* hit_used;
+  uint32_t* atomics_velo;
+  uint16_t* rel_indices;
+  uint32_t* number_of_velo_tracks;
+  uint32_t* status;
+  float phi_tolerance;
+  float max_scatter;
+  unsigned max_skipped_modules;
+  unsigned block_dim;
+  unsigned long long* work_stats; // optional: seeding evals, forwarding evals, extrapolations, search steps
+};
+
+struct ThreeHitFilterLaunch {
+  const uint32_t* event_list;
+  unsigned n_list;
+  const uint32_t* number_of_events;
+  const char* sorted_container;
+  const uint32_t* offsets_estimated_input_size;
+  const void* three_hit_tracks_input;
+  const uint32_t* atomics_velo;
+  const uint8_t* hit_used;
+  void* three_hit_tracks_output;
+  uint32_t* number_of_three_hit_tracks_output;
+  float max_chi2;
+};
+
+struct CopyHitNumberLaunch {
+  unsigned n_events;
+  const void* tracks;
+  const uint32_t* offsets_velo_tracks;
+  const uint32_t* offsets_estimated_input_size;
+  const uint32_t* offsets_three_hit;
+  uint32_t* velo_track_hit_number;
+  uint32_t* offsets_all_velo_tracks;
+};
+
+struct ConsolidateLaunch {
+  const uint32_t* event_list;
+  unsigned n_list;
+  const uint32_t* number_of_events;
+  const uint32_t* offsets_all_velo_tracks;
+  const void* tracks;
+  const uint32_t* offsets_velo_track_hit_number;
+  const char* sorted_container;
+  const uint32_t* offsets_estimated_input_size;
+  const void* three_hit_tracks_output;
+  const uint32_t* offsets_three_hit;
+  char* velo_track_hits;
+  uint8_t* accepted_velo_tracks; // optional
+};
+
+struct KalmanLaunch {
+  const uint32_t* event_list;
+  unsigned n_list;
+  const uint32_t* number_of_events;
+  const uint32_t* offsets_all_velo_tracks;
+  const uint32_t* offsets_velo_track_hit_number;
+  const char* velo_track_hits;
+  const float* beamline;
+  uint8_t* is_backward;
+  char* beamline_states;
+  char* endvelo_states;
+};
+
+hipError_t launch_number_of_candidates(const CandidatesLaunch& a, hipStream_t s);
+hipError_t launch_estimate_input_size(const EstimateLaunch& a, hipStream_t s);
+hipError_t launch_masked_clustering(const MaskedClusteringLaunch& a, hipStream_t s);
+hipError_t launch_sort_by_phi(const SortLaunch& a, hipStream_t s);
+hipError_t launch_cluster_sorted(const FusedClusterLaunch& a, hipStream_t s);
+hipError_t launch_search_by_triplet(const SearchLaunch& a, hipStream_t s);
+hipError_t launch_three_hit_tracks_filter(const ThreeHitFilterLaunch& a, hipStream_t s);
+hipError_t launch_copy_track_hit_number(const CopyHitNumberLaunch& a, hipStream_t s);
+hipError_t launch_consolidate_tracks(const ConsolidateLaunch& a, hipStream_t s);
+hipError_t launch_kalman_filter(const KalmanLaunch& a, hipStream_t s);
+
+// Device exclusive scan with a leading zero: out[0] = 0, out[i+1] = sum(in[0..i]).
+size_t prefix_sum_scratch_words(unsigned n);
+hipError_t launch_prefix_sum(const uint32_t* in, unsigned n, uint32_t* out, uint32_t* scratch, hipStream_t s);
+// Same, with the length read on the device: min(*n_dev, n_cap).
+hipError_t launch_prefix_sum_dyn(
+  const uint32_t* in,
+  const uint32_t* n_dev,
+  unsigned n_cap,
+  uint32_t* out,
+  uint32_t* scratch,
+  hipStream_t s);
+// Fill `n` u32 with `value` (a kernel, so the work can be captured in a graph).
+hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t value, hipStream_t s);
+hipError_t launch_fill_u8(uint8_t* p, size_t n, uint8_t value, hipStream_t s);
+
+} // namespace velo

@@ -1,0 +1,723 @@
+// search_by_triplet and the three-hit filter for gfx950.
+//
+//   k_search_by_triplet     SearchByTriplet.cu:110-609   one workgroup per event
+//   k_three_hit_filter      ThreeHitTracksFilter.cu:48-169
+//
+// The reference appends seeds, tracks to follow, promoted tracks and three-hit tracks with
+// atomicAdd, so its GPU output order depends on scheduling. Here every append is an ordered
+// compaction (wave ballot + popcount, then a prefix over the waves of the block) over the
+// items in the order the CPU target visits them (h1 index order in seeding, ring order in
+// forwarding). Ring slots, track numbers and three-hit slots therefore land exactly where the
+// CPU target puts them, and everything downstream is bit-identical.
+//
+// The event's hits (x, y, z, phi, used) live in LDS when they fit the per-launch budget;
+// a larger event runs the same code on the global containers.
+#include "velo_common.cuh"
+#include "velo_kernels.h"
+
+namespace velo {
+
+// Three 0/1 flags scanned at once over the block. Exclusive ranks in r[], totals in t[].
+struct Scan3 {
+  uint32_t r[3];
+  uint32_t t[3];
+};
+__device__ __forceinline__ Scan3 block_flag_scan3(bool f0, bool f1, bool f2, uint32_t* scratch)
+{
+  const unsigned wave = threadIdx.x / wave_size;
+  const unsigned n_waves = blockDim.x / wave_size;
+  const uint64_t below = lanes_below_mask();
+  const uint64_t b0 = __ballot(f0), b1 = __ballot(f1), b2 = __ballot(f2);
+  if (lane_id() == 0) {
+    scratch[3 * wave] = __popcll(b0);
+    scratch[3 * wave + 1] = __popcll(b1);
+    scratch[3 * wave + 2] = __popcll(b2);
+  }
+  __syncthreads();
+  Scan3 s;
+  s.r[0] = __popcll(b0 & below);
+  s.r[1] = __popcll(b1 & below);
+  s.r[2] = __popcll(b2 & below);
+  s.t[0] = s.t[1] = s.t[2] = 0;
+  for (unsigned w = 0; w < n_waves; ++w) {
+    const uint32_t c0 = scratch[3 * w], c1 = scratch[3 * w + 1], c2 = scratch[3 * w + 2];
+    if (w < wave) {
+      s.r[0] += c0;
+      s.r[1] += c1;
+      s.r[2] += c2;
+    }
+    s.t[0] += c0;
+    s.t[1] += c1;
+    s.t[2] += c2;
+  }
+  __syncthreads();
+  return s;
+}
+
+struct ModulePairData {
+  unsigned hit_start;
+  unsigned hit_num;
+  float z[2];
+};
+
+// Hits of one event, structure-of-arrays in LDS.
+struct LdsHits {
+  const float* xs;
+  const float* ys;
+  const float* zs;
+  const int16_t* ph;
+  uint8_t* used;
+  __device__ __forceinline__ float x(unsigned i) const { return xs[i]; }
+  __device__ __forceinline__ float y(unsigned i) const { return ys[i]; }
+  __device__ __forceinline__ float z(unsigned i) const { return zs[i]; }
+  __device__ __forceinline__ int16_t phi(unsigned i) const { return ph[i]; }
+  __device__ __forceinline__ const int16_t* phi_begin() const { return ph; }
+};
+
+// Hits of one event read from the sorted cluster container in global memory.
+struct GlobalHits {
+  const float* xyz;
+  const int16_t* ph;
+  uint8_t* used;
+  __device__ __forceinline__ float x(unsigned i) const { return xyz[3 * i]; }
+  __device__ __forceinline__ float y(unsigned i) const { return xyz[3 * i + 1]; }
+  __device__ __forceinline__ float z(unsigned i) const { return xyz[3 * i + 2]; }
+  __device__ __forceinline__ int16_t phi(unsigned i) const { return ph[i]; }
+  __device__ __forceinline__ const int16_t* phi_begin() const { return ph; }
+};
+
+// Work counters of one thread (seeding evaluations, forwarding evaluations, extrapolations,
+// binary-search iterations); only kept by the counting instantiation of the kernel.
+struct WorkCount {
+  uint32_t seed_eval = 0, fwd_eval = 0, extrap = 0, steps = 0;
+};
+
+template<bool count>
+__device__ __forceinline__ int counted_search(const int16_t* array, unsigned size, int16_t value, WorkCount& wc)
+{
+  int l = 0;
+  int r = size;
+  while (l < r) {
+    if constexpr (count) ++wc.steps;
+    const int m = (l + r) / 2;
+    if (value > array[m]) l = m + 1;
+    else r = m;
+  }
+  return l;
+}
+
+struct SearchShared {
+  ModulePairData mpd[3];
+  uint32_t ttf_counter;   // atomics::tracks_to_follow
+  uint32_t three_counter; // atomics::number_of_three_hit_tracks
+  uint32_t local_hits;    // atomics::local_number_of_hits
+  uint32_t n_tracks;      // dev_number_of_velo_tracks[event]
+  uint32_t status;
+  uint32_t scan[3 * 16 + 4];
+};
+
+struct EventOutputs {
+  TrackHits* tracks;
+  TrackletHits* three_hit_tracks;
+  TrackletHits* tracklets;
+  uint32_t* ttf;
+  uint16_t* h1_indices;
+  const uint32_t* ids; // event-relative LHCbIDs of the sorted container
+  unsigned track_capacity;
+};
+
+// SearchByTriplet.cu:272-292
+template<bool count, typename H>
+__device__ __forceinline__ void find_forward_candidate(
+  WorkCount& wc,
+  const H& hits,
+  const ModulePairData& mp,
+  float h0x,
+  float h0y,
+  float tx,
+  float ty,
+  float dz,
+  int16_t phi_tolerance,
+  int16_t& candidate,
+  int16_t& extrapolated_phi)
+{
+  const float predx = tx * dz;
+  const float predy = ty * dz;
+  const float xp = h0x + predx;
+  const float yp = h0y + predy;
+  const int16_t phi = hit_phi_16(xp, yp);
+  if constexpr (count) ++wc.extrap;
+  candidate = static_cast<int16_t>(counted_search<count>(
+    hits.phi_begin() + mp.hit_start, mp.hit_num, static_cast<int16_t>(phi - phi_tolerance), wc));
+  extrapolated_phi = phi;
+}
+
+// SearchByTriplet.cu:298-446, ordered.
+template<bool count, typename H>
+__device__ void track_seeding(
+  WorkCount& wc,
+  const H& hits,
+  SearchShared& sh,
+  const EventOutputs& o,
+  float max_scatter,
+  int16_t phi_tolerance,
+  unsigned h0_candidates_to_consider)
+{
+  const ModulePairData prev = sh.mpd[0];
+  const ModulePairData cur = sh.mpd[1];
+  const ModulePairData next = sh.mpd[2];
+
+  // Unused hits of the current module pair, compacted in index order.
+  uint32_t base = 0;
+  for (unsigned c = 0; c < cur.hit_num; c += blockDim.x) {
+    const unsigned r = c + threadIdx.x;
+    const bool act = r < cur.hit_num && !hits.used[cur.hit_start + r];
+    uint32_t total;
+    const uint32_t pos = block_flag_scan(act, sh.scan, &total);
+    if (act) o.h1_indices[base + pos] = cur.hit_start + r;
+    base += total;
+  }
+  if (threadIdx.x == 0) sh.local_hits = base;
+  __threadfence_block();
+  __syncthreads();
+  const unsigned n_h1 = base;
+
+  for (unsigned c = 0; c < n_h1; c += blockDim.x) {
+    const unsigned rel = c + threadIdx.x;
+    bool found = false;
+    uint16_t best_h0 = 0, best_h2 = 0, h1_index = 0;
+    if (rel < n_h1) {
+      h1_index = o.h1_indices[rel];
+      float best_fit = max_scatter;
+      const float h1x = hits.x(h1_index), h1y = hits.y(h1_index), h1z = hits.z(h1_index);
+      const int16_t h1_phi = hits.phi(h1_index);
+      unsigned best_h0s[max_h0_candidates];
+      int phi_index = counted_search<count>(hits.phi_begin() + prev.hit_start, prev.hit_num, h1_phi, wc);
+      unsigned n_found = 0;
+      for (unsigned i = 0; i < prev.hit_num && n_found < h0_candidates_to_consider; ++i) {
+        const int index_diff = (i & 1u) ? static_cast<int>(i) : -static_cast<int>(i);
+        phi_index += index_diff;
+        const int n = static_cast<int>(prev.hit_num);
+        const int in_bounds = phi_index < 0 ? phi_index + n : (phi_index >= n ? phi_index - n : phi_index);
+        const unsigned h0_index = prev.hit_start + in_bounds;
+        if (!hits.used[h0_index]) {
+#pragma unroll
+          for (unsigned k = 0; k < max_h0_candidates; ++k)
+            if (k == n_found) best_h0s[k] = h0_index;
+          ++n_found;
+        }
+      }
+      for (unsigned i = 0; i < n_found; ++i) {
+        unsigned h0_index = 0;
+#pragma unroll
+        for (unsigned k = 0; k < max_h0_candidates; ++k)
+          if (k == i) h0_index = best_h0s[k];
+        const float h0x = hits.x(h0_index), h0y = hits.y(h0_index), h0z = hits.z(h0_index);
+        const float td = 1.0f / (h1z - h0z);
+        const float txn = h1x - h0x;
+        const float tyn = h1y - h0y;
+        const float tx = txn * td;
+        const float ty = tyn * td;
+        int16_t cand, ephi;
+        find_forward_candidate<count>(
+          wc, hits, next, h0x, h0y, tx, ty, next.z[0] - prev.z[0], phi_tolerance, cand, ephi);
+        for (unsigned j = 0; j < next.hit_num; ++j) {
+          const unsigned in_bounds = (static_cast<unsigned>(static_cast<int>(cand)) + j) % next.hit_num;
+          const unsigned h2_index = next.hit_start + in_bounds;
+          const int16_t phi_diff = hits.phi(h2_index) - ephi;
+          const int16_t abs_phi_diff = phi_diff < 0 ? -phi_diff : phi_diff;
+          if (abs_phi_diff > phi_tolerance) break;
+          if (!hits.used[h2_index]) {
+            if constexpr (count) ++wc.seed_eval;
+            const float h2x = hits.x(h2_index), h2y = hits.y(h2_index), h2z = hits.z(h2_index);
+            const float dz = h2z - h0z;
+            const float predx = h0x + tx * dz;
+            const float predy = h0y + ty * dz;
+            const float dx = predx - h2x;
+            const float dy = predy - h2y;
+            const float scatter = dx * dx + dy * dy;
+            if (scatter < best_fit) {
+              best_fit = scatter;
+              best_h0 = h0_index;
+              best_h2 = h2_index;
+            }
+          }
+        }
+      }
+      found = best_fit < max_scatter;
+    }
+    const Scan3 s = block_flag_scan3(found, false, false, sh.scan);
+    if (found) {
+      const unsigned tn = (sh.ttf_counter + s.r[0]) % max_tracks_to_follow;
+      TrackletHits t;
+      t.hits[0] = best_h0;
+      t.hits[1] = h1_index;
+      t.hits[2] = best_h2;
+      o.tracklets[tn] = t;
+      o.ttf[tn] = bit_seed | tn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) sh.ttf_counter += s.t[0];
+    __syncthreads();
+  }
+}
+
+// SearchByTriplet.cu:451-609, ordered.
+template<bool count, typename H>
+__device__ void track_forwarding(
+  WorkCount& wc,
+  const H& hits,
+  SearchShared& sh,
+  const EventOutputs& o,
+  unsigned prev_ttf,
+  unsigned diff_ttf,
+  int16_t phi_tolerance,
+  float max_scatter,
+  unsigned max_skipped_modules)
+{
+  const ModulePairData next = sh.mpd[2];
+  for (unsigned c = 0; c < diff_ttf; c += blockDim.x) {
+    const unsigned e = c + threadIdx.x;
+    bool promote = false, reinsert = false, three = false;
+    uint32_t full = 0;
+    unsigned number_of_hits = 0;
+    int best_h2 = -1;
+    uint16_t th[3] = {0, 0, 0};
+    unsigned track_number = 0;
+    bool valid = false;
+    const uint16_t* t_hits = nullptr;
+    if (e < diff_ttf) {
+      full = o.ttf[(prev_ttf + e) % max_tracks_to_follow];
+      track_number = full & bit_track_number;
+      if ((full & bit_seed) == bit_seed) {
+        valid = track_number < max_tracks_to_follow;
+        if (valid) t_hits = o.tracklets[track_number].hits;
+        number_of_hits = 3;
+      }
+      else if (track_number < o.track_capacity) {
+        // Guards that only matter once the slots of an event have overflowed (status raised).
+        t_hits = o.tracks[track_number].hits;
+        number_of_hits = o.tracks[track_number].hitsNum;
+        valid = number_of_hits >= 3 && number_of_hits < max_track_size;
+      }
+    }
+    if (valid) {
+      const unsigned skipped = (full & bit_skipped) >> skipped_pos;
+      const unsigned h0_num = t_hits[number_of_hits - 2];
+      const unsigned h1_num = t_hits[number_of_hits - 1];
+      th[0] = t_hits[0];
+      th[1] = t_hits[1];
+      th[2] = t_hits[2];
+      const float h0x = hits.x(h0_num), h0y = hits.y(h0_num), h0z = hits.z(h0_num);
+      const unsigned h0_module = ((o.ids[h0_num] & 0x3fc0000u) >> 18) / 4;
+      const float h1x = hits.x(h1_num), h1y = hits.y(h1_num), h1z = hits.z(h1_num);
+      const float td = 1.0f / (h1z - h0z);
+      const float txn = h1x - h0x;
+      const float tyn = h1y - h0y;
+      const float tx = txn * td;
+      const float ty = tyn * td;
+      float best_fit = max_scatter;
+      int16_t cand, ephi;
+      find_forward_candidate<count>(
+        wc, hits, next, h0x, h0y, tx, ty, (h0_module % 2 ? next.z[1] : next.z[0]) - h0z, phi_tolerance, cand, ephi);
+      for (unsigned j = 0; j < next.hit_num; ++j) {
+        const unsigned in_bounds = (static_cast<unsigned>(static_cast<int>(cand)) + j) % next.hit_num;
+        const unsigned h2_index = next.hit_start + in_bounds;
+        const int16_t phi_diff = hits.phi(h2_index) - ephi;
+        const int16_t abs_phi_diff = phi_diff < 0 ? -phi_diff : phi_diff;
+        if (abs_phi_diff > phi_tolerance) break;
+        if constexpr (count) ++wc.fwd_eval;
+        const float h2x = hits.x(h2_index), h2y = hits.y(h2_index), h2z = hits.z(h2_index);
+        const float dz = h2z - h0z;
+        const float predx = h0x + tx * dz;
+        const float predy = h0y + ty * dz;
+        const float dx = predx - h2x;
+        const float dy = predy - h2y;
+        const float scatter = dx * dx + dy * dy;
+        if (scatter < best_fit) {
+          best_fit = scatter;
+          best_h2 = h2_index;
+        }
+      }
+      if (best_h2 != -1) {
+        promote = number_of_hits == 3;
+        reinsert = number_of_hits + 1 < max_track_size;
+      }
+      else if (skipped < max_skipped_modules) {
+        reinsert = true;
+        track_number = ((skipped + 1) << skipped_pos) | (full & (bit_seed | bit_track_number));
+      }
+      else if (number_of_hits == 3) {
+        three = true;
+      }
+    }
+    const Scan3 s = block_flag_scan3(promote, reinsert, three, sh.scan);
+    if (valid) {
+      if (best_h2 != -1) {
+        hits.used[best_h2] = 1;
+        if (promote) {
+          hits.used[th[0]] = 1;
+          hits.used[th[1]] = 1;
+          hits.used[th[2]] = 1;
+          track_number = sh.n_tracks + s.r[0];
+          if (track_number < o.track_capacity) {
+            TrackHits& t = o.tracks[track_number];
+            t.hits[0] = th[0];
+            t.hits[1] = th[1];
+            t.hits[2] = th[2];
+            t.hits[3] = best_h2;
+            t.hitsNum = 4;
+          }
+        }
+        else {
+          TrackHits& t = o.tracks[track_number];
+          t.hits[number_of_hits] = best_h2;
+          t.hitsNum = number_of_hits + 1;
+        }
+      }
+      if (reinsert) {
+        const unsigned p = (sh.ttf_counter + s.r[1]) % max_tracks_to_follow;
+        // A promoted track beyond the slots of its event is skipped when read back.
+        o.ttf[p] = track_number;
+      }
+      if (three) {
+        const unsigned p = sh.three_counter + s.r[2];
+        if (p < o.track_capacity) {
+          TrackletHits t;
+          t.hits[0] = th[0];
+          t.hits[1] = th[1];
+          t.hits[2] = th[2];
+          o.three_hit_tracks[p] = t;
+        }
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      sh.n_tracks += s.t[0];
+      sh.ttf_counter += s.t[1];
+      sh.three_counter += s.t[2];
+      if (sh.n_tracks > o.track_capacity) sh.status |= status_track_slots_overflow;
+      if (sh.three_counter > o.track_capacity) sh.status |= status_three_hit_slots_overflow;
+    }
+    __syncthreads();
+  }
+}
+
+template<bool count, typename H>
+__device__ void search_event(
+  WorkCount& wc,
+  const H& hits,
+  SearchShared& sh,
+  const EventOutputs& o,
+  const uint32_t* module_hit_start,
+  const uint32_t* module_hit_num,
+  const Geometry& g,
+  float phi_tolerance_f,
+  float max_scatter,
+  unsigned max_skipped_modules)
+{
+  const unsigned hit_offset = module_hit_start[0];
+  auto load = [&](unsigned first) {
+    if (threadIdx.x < 3) {
+      const unsigned m = first - threadIdx.x;
+      sh.mpd[threadIdx.x].hit_start = module_hit_start[m] - hit_offset;
+      sh.mpd[threadIdx.x].hit_num = module_hit_num[m];
+      sh.mpd[threadIdx.x].z[0] = g.module_zs[2 * m];
+      sh.mpd[threadIdx.x].z[1] = g.module_zs[2 * m + 1];
+    }
+    __syncthreads();
+  };
+  const int16_t phi_tolerance = f32_to_i16(phi_tolerance_f * convert_factor_i16);
+  unsigned first_module_pair = n_module_pairs - 1;
+  load(first_module_pair);
+  track_seeding<count>(wc, hits, sh, o, max_scatter, phi_tolerance, initial_seeding_h0_candidates);
+  unsigned last_ttf = 0;
+  --first_module_pair;
+  while (first_module_pair > 1) {
+    load(first_module_pair);
+    const unsigned prev_ttf = last_ttf;
+    last_ttf = sh.ttf_counter;
+    const unsigned diff_ttf = last_ttf - prev_ttf;
+    if (threadIdx.x == 0 && diff_ttf >= max_tracks_to_follow) sh.status |= status_ttf_ring_overflow;
+    track_forwarding<count>(wc, hits, sh, o, prev_ttf, diff_ttf, phi_tolerance, max_scatter, max_skipped_modules);
+    __threadfence_block();
+    __syncthreads();
+    track_seeding<count>(wc, hits, sh, o, max_scatter, phi_tolerance, seeding_h0_candidates);
+    --first_module_pair;
+  }
+  __syncthreads();
+  const unsigned prev_ttf = last_ttf;
+  last_ttf = sh.ttf_counter;
+  const unsigned diff_ttf = last_ttf - prev_ttf;
+  if (threadIdx.x == 0 && diff_ttf >= max_tracks_to_follow) sh.status |= status_ttf_ring_overflow;
+  for (unsigned c = 0; c < diff_ttf; c += blockDim.x) {
+    const unsigned e = c + threadIdx.x;
+    bool seed = false;
+    uint32_t full = 0;
+    if (e < diff_ttf) {
+      full = o.ttf[(prev_ttf + e) % max_tracks_to_follow];
+      seed = (full & bit_seed) == bit_seed;
+    }
+    const Scan3 s = block_flag_scan3(seed, false, false, sh.scan);
+    if (seed) {
+      const unsigned p = sh.three_counter + s.r[0];
+      if (p < o.track_capacity) o.three_hit_tracks[p] = o.tracklets[full & bit_track_number];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      sh.three_counter += s.t[0];
+      if (sh.three_counter > o.track_capacity) sh.status |= status_three_hit_slots_overflow;
+    }
+    __syncthreads();
+  }
+}
+
+// One workgroup per event. Dynamic LDS holds `lds_hits` hits as x[], y[], z[] (f32),
+// phi[] (i16), used[] (u8). The `count` instantiation also accumulates the work counters
+// into work_stats[4] (one atomic per workgroup).
+template<bool count>
+__global__ void k_search_by_triplet(
+  const uint32_t* event_list,
+  const uint32_t* number_of_events,
+  const char* sorted_container,
+  const uint32_t* offsets_estimated_input_size,
+  const uint32_t* module_cluster_num,
+  const Geometry* geometry,
+  TrackHits* tracks_all,
+  TrackletHits* tracklets_all,
+  uint32_t* ttf_all,
+  TrackletHits* three_hit_all,
+  uint8_t* hit_used_all,
+  uint32_t* atomics_all,
+  uint16_t* rel_indices_all,
+  uint32_t* number_of_velo_tracks,
+  uint32_t* status,
+  float phi_tolerance,
+  float max_scatter,
+  unsigned max_skipped_modules,
+  unsigned lds_hits,
+  unsigned long long* work_stats)
+{
+  extern __shared__ __align__(16) char dyn[];
+  WorkCount wc;
+  __shared__ SearchShared sh;
+  const unsigned event_number = event_list ? event_list[blockIdx.x] : blockIdx.x;
+  const unsigned n_events = number_of_events[0];
+  const unsigned total = offsets_estimated_input_size[n_module_pairs * n_events];
+  const uint32_t* mhs = offsets_estimated_input_size + event_number * n_module_pairs;
+  const uint32_t* mhn = module_cluster_num + event_number * n_module_pairs;
+  const unsigned hit_offset = mhs[0];
+  const unsigned n_hits = mhs[n_module_pairs] - hit_offset;
+  const Clusters cc {const_cast<char*>(sorted_container), total};
+
+  const unsigned toff = track_offset(offsets_estimated_input_size, event_number);
+  const unsigned container = track_container_size(total, n_events);
+  const unsigned next_off =
+    event_number + 1 < n_events ? track_offset(offsets_estimated_input_size, event_number + 1) : container;
+  EventOutputs o;
+  o.tracks = tracks_all + toff;
+  o.three_hit_tracks = three_hit_all + toff;
+  o.tracklets = tracklets_all + (size_t) event_number * max_tracks_to_follow;
+  o.ttf = ttf_all + (size_t) event_number * max_tracks_to_follow;
+  o.h1_indices = rel_indices_all + hit_offset;
+  o.ids = cc.ids() + hit_offset;
+  // Slots of this event; the reference lets a busy event run into its neighbour's slots, here
+  // nothing is written past the container and the overflow is reported.
+  o.track_capacity = min(next_off > toff ? next_off - toff : 0u, container - min(toff, container));
+
+  if (threadIdx.x == 0) {
+    sh.ttf_counter = 0;
+    sh.three_counter = 0;
+    sh.local_hits = 0;
+    sh.n_tracks = 0;
+    sh.status = n_hits > 65536u ? status_hit_index_overflow : 0u;
+  }
+  const Geometry& g = *geometry;
+  if (n_hits <= lds_hits) {
+    float* xs = reinterpret_cast<float*>(dyn);
+    float* ys = xs + lds_hits;
+    float* zs = ys + lds_hits;
+    int16_t* ph = reinterpret_cast<int16_t*>(zs + lds_hits);
+    uint8_t* used = reinterpret_cast<uint8_t*>(ph + lds_hits);
+    const float* gxyz = cc.xyz() + 3 * (size_t) hit_offset;
+    const int16_t* gph = cc.phis() + hit_offset;
+    for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) {
+      xs[i] = gxyz[3 * i];
+      ys[i] = gxyz[3 * i + 1];
+      zs[i] = gxyz[3 * i + 2];
+      ph[i] = gph[i];
+      used[i] = 0;
+    }
+    __syncthreads();
+    const LdsHits hits {xs, ys, zs, ph, used};
+    search_event<count>(wc, hits, sh, o, mhs, mhn, g, phi_tolerance, max_scatter, max_skipped_modules);
+    __syncthreads();
+    uint8_t* gused = hit_used_all + hit_offset;
+    for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) gused[i] = used[i];
+  }
+  else {
+    uint8_t* gused = hit_used_all + hit_offset;
+    for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) gused[i] = 0;
+    if (threadIdx.x == 0) sh.status |= status_lds_fallback;
+    __threadfence_block();
+    __syncthreads();
+    const GlobalHits hits {cc.xyz() + 3 * (size_t) hit_offset, cc.phis() + hit_offset, gused};
+    search_event<count>(wc, hits, sh, o, mhs, mhn, g, phi_tolerance, max_scatter, max_skipped_modules);
+  }
+  if constexpr (count) {
+    // Wave reduction, then one atomic per wave.
+    unsigned long long v[4] = {wc.seed_eval, wc.fwd_eval, wc.extrap, wc.steps};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+#pragma unroll
+      for (unsigned d = 32; d > 0; d >>= 1) v[k] += __shfl_xor(v[k], d, wave_size);
+      if (lane_id() == 0 && v[k]) atomicAdd(work_stats + k, v[k]);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t* at = atomics_all + event_number * num_atomics;
+    at[atomic_three_hit] = sh.three_counter;
+    at[atomic_ttf] = sh.ttf_counter;
+    at[atomic_local_hits] = sh.local_hits;
+    number_of_velo_tracks[event_number] = min(sh.n_tracks, o.track_capacity);
+    if (sh.status && status) atomicOr(status, sh.status);
+  }
+}
+
+// ThreeHitTracksFilter.cu:48-107
+__device__ __forceinline__ float means_square_fit_chi2(const Clusters& cc, unsigned hit_offset, const TrackletHits& t)
+{
+  float s0, sx, sz, sxz, sz2, u0, uy, uz, uyz, uz2;
+  s0 = sx = sz = sxz = sz2 = 0.0f;
+  u0 = uy = uz = uyz = uz2 = 0.0f;
+  float xs[3], ys[3], zs[3];
+#pragma unroll
+  for (unsigned h = 0; h < 3; ++h) {
+    const size_t i = (size_t) hit_offset + t.hits[h];
+    xs[h] = cc.xyz()[3 * i];
+    ys[h] = cc.xyz()[3 * i + 1];
+    zs[h] = cc.xyz()[3 * i + 2];
+  }
+#pragma unroll
+  for (unsigned h = 0; h < 3; ++h) {
+    const float x = xs[h], y = ys[h], z = zs[h];
+    const float wx = param_w;
+    const float wx_t_x = wx * x;
+    const float wx_t_z = wx * z;
+    s0 += wx;
+    sx += wx_t_x;
+    sz += wx_t_z;
+    sxz += wx_t_x * z;
+    sz2 += wx_t_z * z;
+    const float wy = param_w;
+    const float wy_t_y = wy * y;
+    const float wy_t_z = wy * z;
+    u0 += wy;
+    uy += wy_t_y;
+    uz += wy_t_z;
+    uyz += wy_t_y * z;
+    uz2 += wy_t_z * z;
+  }
+  const float dens = 1.0f / (sz2 * s0 - sz * sz);
+  const float state_tx = (sxz * s0 - sx * sz) * dens;
+  const float state_x = (sx * sz2 - sxz * sz) * dens;
+  const float denu = 1.0f / (uz2 * u0 - uz * uz);
+  const float state_ty = (uyz * u0 - uy * uz) * denu;
+  const float state_y = (uy * uz2 - uyz * uz) * denu;
+  float chi2 = 0.0f;
+#pragma unroll
+  for (unsigned h = 0; h < 3; ++h) {
+    const float z = zs[h];
+    const float x = state_x + state_tx * z;
+    const float y = state_y + state_ty * z;
+    const float dx = x - xs[h];
+    const float dy = y - ys[h];
+    chi2 += dx * dx * param_w + dy * dy * param_w;
+  }
+  return chi2 * 0.5f;
+}
+
+// ThreeHitTracksFilter.cu:109-169, ordered output.
+__global__ void k_three_hit_filter(
+  const uint32_t* event_list,
+  const uint32_t* number_of_events,
+  const char* sorted_container,
+  const uint32_t* offsets_estimated_input_size,
+  const TrackletHits* input_all,
+  const uint32_t* atomics_all,
+  const uint8_t* hit_used_all,
+  TrackletHits* output_all,
+  uint32_t* number_of_output,
+  float max_chi2)
+{
+  __shared__ uint32_t scan[3 * 16 + 4];
+  const unsigned event_number = event_list ? event_list[blockIdx.x] : blockIdx.x;
+  const unsigned n_events = number_of_events[0];
+  const unsigned total = offsets_estimated_input_size[n_module_pairs * n_events];
+  const unsigned hit_offset = offsets_estimated_input_size[event_number * n_module_pairs];
+  const uint8_t* used = hit_used_all + hit_offset;
+  const Clusters cc {const_cast<char*>(sorted_container), total};
+  const unsigned toff = track_offset(offsets_estimated_input_size, event_number);
+  const TrackletHits* in = input_all + toff;
+  const unsigned container = track_container_size(total, n_events);
+  const unsigned next_off =
+    event_number + 1 < n_events ? track_offset(offsets_estimated_input_size, event_number + 1) : container;
+  const unsigned capacity = min(next_off > toff ? next_off - toff : 0u, container - min(toff, container));
+  // The count can only exceed the event's slots once they overflowed (a status bit is set then).
+  const unsigned n_in = min(atomics_all[event_number * num_atomics + atomic_three_hit], capacity);
+  TrackletHits* out = output_all + toff;
+  unsigned base = 0;
+  for (unsigned c = 0; c < n_in; c += blockDim.x) {
+    const unsigned i = c + threadIdx.x;
+    bool keep = false;
+    TrackletHits t;
+    if (i < n_in) {
+      t = in[i];
+      const bool any_used = used[t.hits[0]] || used[t.hits[1]] || used[t.hits[2]];
+      const float chi2 = means_square_fit_chi2(cc, hit_offset, t);
+      keep = !any_used && chi2 < max_chi2;
+    }
+    const Scan3 s = block_flag_scan3(keep, false, false, scan);
+    if (keep) out[base + s.r[0]] = t;
+    base += s.t[0];
+  }
+  if (threadIdx.x == 0) number_of_output[event_number] = base;
+}
+
+// Launch geometry: LDS budget for the event hits (bytes per hit: 3 x 4 + 2 + 1).
+constexpr unsigned search_bytes_per_hit = 15;
+
+hipError_t launch_search_by_triplet(const SearchLaunch& a, hipStream_t s)
+{
+  if (!a.n_list) return hipSuccess;
+  const unsigned block = a.block_dim ? a.block_dim : 256;
+  const unsigned lds_hits = 4096;
+  const size_t dyn = (size_t) lds_hits * search_bytes_per_hit;
+#define VELO_LAUNCH_SEARCH(C)                                                                                       \
+  hipLaunchKernelGGL(                                                                                               \
+    k_search_by_triplet<C>, dim3(a.n_list), dim3(block), dyn, s, a.event_list, a.number_of_events,                  \
+    a.sorted_container, a.offsets_estimated_input_size, a.module_cluster_num,                                       \
+    reinterpret_cast<const Geometry*>(a.geometry), reinterpret_cast<TrackHits*>(a.tracks),                          \
+    reinterpret_cast<TrackletHits*>(a.tracklets), a.tracks_to_follow,                                               \
+    reinterpret_cast<TrackletHits*>(a.three_hit_tracks), a.hit_used, a.atomics_velo, a.rel_indices,                \
+    a.number_of_velo_tracks, a.status, a.phi_tolerance, a.max_scatter, a.max_skipped_modules, lds_hits,             \
+    a.work_stats)
+  if (a.work_stats) VELO_LAUNCH_SEARCH(true);
+  else VELO_LAUNCH_SEARCH(false);
+#undef VELO_LAUNCH_SEARCH
+  return hipGetLastError();
+}
+
+hipError_t launch_three_hit_tracks_filter(const ThreeHitFilterLaunch& a, hipStream_t s)
+{
+  if (!a.n_list) return hipSuccess;
+  hipLaunchKernelGGL(
+    k_three_hit_filter, dim3(a.n_list), dim3(256), 0, s, a.event_list, a.number_of_events, a.sorted_container,
+    a.offsets_estimated_input_size, reinterpret_cast<const TrackletHits*>(a.three_hit_tracks_input), a.atomics_velo,
+    a.hit_used, reinterpret_cast<TrackletHits*>(a.three_hit_tracks_output), a.number_of_three_hit_tracks_output,
+    a.max_chi2);
+  return hipGetLastError();
+}
+
+} // namespace velo

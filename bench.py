@@ -1,0 +1,200 @@
+"""VELO reconstruction throughput on one MI355X.
+
+One step = the whole VELO path (super-pixel decoding + masked clustering, phi sort,
+search_by_triplet, three-hit filter, consolidation, simplified Kalman fit) over one slice
+of 1000 synthetic events with the raw banks already resident in HBM.
+
+Prints ONE JSON line (rank 0). Progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+WORKLOADS = {
+    # One slice of 1000 events; nu ~ Poisson(7.6) collisions x NegBinomial(16, k=1.5) tracks.
+    "pileup_1000": dict(n_events=1000, pileup=1, nu=7.6, seed=1),
+    # 200 events of Poisson(120) tracks from one PV (the reference's CPU-runnable case).
+    "single_pv_200": dict(n_events=200, pileup=0, mean_tracks=120.0, seed=1),
+    # 20k events, as pileup_1000 plus 5 % of events at nu = 20.
+    "pileup_20k": dict(n_events=20000, pileup=1, nu=7.6, high_occupancy_fraction=0.05, high_nu=20.0, seed=1),
+    # 10k events at nu ~ Poisson(15), 5 % noise pixels, 1 % dead ASICs.
+    "stress_10k": dict(n_events=10000, pileup=1, nu=15.0, noise_fraction=0.05, dead_asic_fraction=0.01, seed=1),
+}
+
+HBM_PEAK_GBS = 8000.0                    # MI355X HBM3E
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 lanes x 2.4 GHz, non-packed lane ops/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(ev, counts: dict) -> dict:
+    """Algorithmic HBM bytes of each stage for one step (definitions in DESIGN.md)."""
+    n = ev.n_events
+    sizes16 = ev.sizes.view("<u2")
+    n_banks = int(ev.sizes[n]) - int(ev.sizes[0])
+    raw_read = int(ev.raw.size) + 2 * n_banks + n_banks + 3 * 4 * (n + 1)
+    n_c = counts["clusters"]
+    cl = raw_read + 18 * n_c + 4 * 26 * n
+    search_read = 18 * n_c + 2 * 4 * 26 * n
+    search_write = n_c + counts["track_hit_bytes"] + 6 * counts["three_hit_raw"] + 10 * counts["seeds"] + 16 * n
+    return {"clustering": cl, "search": search_read + search_write}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="pileup_1000", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores (capped at 16)")
+    ap.add_argument("--search-block", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the VELO path shards by events "
+              f"and a multi-GPU figure is not part of this build.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from allen_amd import synth, velo
+
+    assert torch.cuda.is_available(), "no HIP device"
+    wl = WORKLOADS[args.workload]
+    t0 = time.time()
+    ev = synth.generate(**wl, keep_truth=False, n_threads=min(16, os.cpu_count() or 1))
+    log(f"[bench] generated {ev.n_events} events ({ev.raw.size / 1e6:.2f} MB raw) in {time.time() - t0:.1f}s")
+    geometry, beamline = synth.read_geometry(), synth.read_beamline()
+    constants = velo.VeloConstants(geometry, beamline)
+    dev = velo.DeviceEvents(ev)
+    seq = velo.VeloSequence(constants, ev.n_events, ev.raw.size)
+    if args.search_block:
+        seq.set_tuning("search_block", args.search_block)
+    stream = torch.cuda.current_stream()
+
+    for _ in range(args.warmup):
+        seq.run(dev)
+    torch.cuda.synchronize()
+    status = seq.status()
+    if status & velo.STATUS_OVERFLOW_MASK:
+        log(f"[bench] WARNING status {status:#x}: a fixed-capacity structure overflowed")
+
+    # Per-stage times (diagnostic, outside the timed region) to find the dominant kernel.
+    stage_ms = {}
+    for stage in ["candidates", "estimate", "clustering", "search", "three_hit_filter", "consolidate", "kalman"]:
+        seq.time_stage(stage)
+        for _ in range(2):
+            seq.run(dev)
+        ms, runs = seq.stage_ms()
+        stage_ms[stage] = ms / runs
+    dominant = max(stage_ms, key=stage_ms.get)
+    log(f"[bench] stage ms: " + ", ".join(f"{k}={v:.3f}" for k, v in stage_ms.items()) + f" -> {dominant}")
+
+    # Work counts and output sizes of one step.
+    seq.time_stage(None)
+    seq.set_tuning("count_work", 1)
+    seq.run(dev)
+    work = seq.fetch("search_work", np.uint64, 4)
+    seq.set_tuning("count_work", 0)
+    n = ev.n_events
+    mcn = seq.fetch("module_cluster_num", np.uint32, 26 * n)
+    off_all = seq.fetch("offsets_all_velo_tracks", np.uint32, n + 1)
+    atomics = seq.fetch("atomics_velo", np.uint32, 3 * n)
+    n_tracks = int(off_all[-1])
+    hit_n = seq.fetch("velo_track_hit_number", np.uint32, n_tracks)
+    counts = {
+        "clusters": int(mcn.sum()),
+        "tracks": n_tracks,
+        "track_hit_bytes": int((2 * hit_n + 1).sum()),
+        "three_hit_raw": int(atomics[0::3].sum()),
+        "seeds": int(atomics[1::3].sum()),
+    }
+    log(f"[bench] {counts}")
+
+    # Timed region: K steps, the dominant stage bracketed by HIP events on the run's stream.
+    seq.time_stage(dominant)
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(args.steps):
+        seq.run(dev)
+    end.record(stream)
+    torch.cuda.synchronize()
+    total_ms = start.elapsed_time(end)
+    kern_ms, kern_runs = seq.stage_ms()
+    kern_avg_ms = kern_ms / kern_runs
+    ms_per_step = total_ms / args.steps
+    events_per_s = n * args.steps / (total_ms / 1e3)
+    status = seq.status()
+
+    alg = algorithmic_bytes(ev, counts)
+    dom_bytes = alg.get(dominant)
+    if dom_bytes is None:
+        dom_bytes = alg["clustering"]
+    achieved = dom_bytes / (kern_avg_ms / 1e3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "algorithmic_bytes": dom_bytes, "avg_launch_ms": round(kern_avg_ms, 4)}
+    # search_by_triplet is bounded by lane arithmetic, not bytes: 9 VALU ops per scored candidate,
+    # ~12 per extrapolation (hit_phi_16), 3 per binary-search iteration.
+    ops = 9 * (int(work[0]) + int(work[1])) + 12 * int(work[2]) + 3 * int(work[3])
+    search_tops = ops / (stage_ms["search"] / 1e3) / 1e12
+    extra = {"search_valu": {"ops_per_step": ops, "achieved_tops": round(search_tops, 4),
+                             "peak_tops": round(VALU_PEAK_TOPS, 2), "frac": round(search_tops / VALU_PEAK_TOPS, 5),
+                             "work": [int(x) for x in work]},
+             "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()}}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        sample = synth.generate(**dict(wl, n_events=min(n, 1000)), keep_truth=False)
+        reps = 4
+        eps, secs = oracle.throughput(sample, geometry, beamline, threads, reps)
+        eps1, secs1 = oracle.throughput(synth.generate(**dict(wl, n_events=200), keep_truth=False), geometry,
+                                        beamline, 1, 1)
+        cpu = {"value": round(eps, 1), "unit": "events/s", "cores": threads, "kind": "port",
+               "sample": f"{sample.n_events} events x {reps} repetitions of the same workload, one event per task "
+                         f"on {threads} std::threads ({secs:.1f} s); single thread: {eps1:.1f} events/s on 200 "
+                         f"events ({secs1:.1f} s)",
+               "single_thread_events_per_s": round(eps1, 1)}
+
+    out = {
+        "metric": "VELO events/s on 1 MI355X, plain vs NN-pruned, at fixed long-track efficiency",
+        "value": round(events_per_s, 1),
+        "unit": "events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator, real VELO geometry, v4 super-pixel banks)",
+        "config": {"workload": args.workload, "events_per_step": n, "path": "plain",
+                   "raw_bytes_per_step": int(ev.raw.size), **{k: v for k, v in wl.items() if k != "n_events"}},
+        "status": status,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        **extra,
+    }
+    print(json.dumps(out), flush=True)
+    seq.close()
+    constants.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,310 @@
+/* velo_amd — MI355X (gfx950) VELO reconstruction path behind a C ABI.
+ *
+ * Each entry point replaces one algorithm of Allen's VELO-SP sequence
+ * (configuration/python/AllenConf/velo_reconstruction.py:72-321) and takes a POD whose
+ * fields mirror, name for name, that algorithm's `Parameters` struct: HOST_INPUTs become
+ * scalars, DEVICE_INPUT/OUTPUTs and MASK_INPUTs become device pointers, PROPERTYs become
+ * scalars.  Every entry point enqueues work on the given HIP stream and returns without
+ * synchronising.  Entry points never allocate or free argument buffers; the only device
+ * memory the library owns lives in a velo_constants (geometry, super-pixel tables,
+ * beamline) or a velo_sequence (scratch and the intermediate buffers of a whole run).
+ *
+ * Data layouts are those of the reference with half_t == float (the CPU-target layout):
+ *   raw input      per-event blocks [u32 n_banks][u32 bank_offsets[n_banks+1]][banks],
+ *                  bank = [u32 sourceID][(v2/v3: u32 count)][u32 SP words] (Transpose.cpp:430-498)
+ *   raw sizes      [u32 offsets_in_u16_units x (n+1)][u16 bank sizes]   (MEPTools.h:35-44)
+ *   raw types      [u32 offsets_in_bytes x (n+1)][u8 bank types]        (MEPTools.h:46-55)
+ *   clusters       [u32 LHCbID x T][f32 x,y,z x T][i16 phi x T], 18 B each (VeloEventModel.cuh:92-183)
+ *   track hits     Velo::TrackHits (u16 hits[26], u8 hitsNum; 54 B), Velo::TrackletHits (3 x u16)
+ *   states         [x,y,z,tx,ty,qop x N][c00,c20,c22,c11,c31,c33,chi2,ndof x N] (VeloConsolidated.cuh:386-640)
+ *
+ * Return codes: VELO_OK, or one of the errors below; a HIP launch error is reported as
+ * VELO_ERR_HIP (query it with velo_last_hip_error). */
+#ifndef VELO_AMD_H
+#define VELO_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  VELO_OK = 0,
+  VELO_ERR_INVALID_ARGUMENT = 1,
+  VELO_ERR_BANK_VERSION = 2,    /* like StrException in EstimateInputSize.cu:265-268 */
+  VELO_ERR_CAPACITY = 3,        /* a run exceeds what a velo_sequence was created for */
+  VELO_ERR_HIP = 4,
+  VELO_ERR_UNSUPPORTED_LAYOUT = 5 /* mep_layout != 0 is not supported */
+};
+
+/* Device status bits written by the kernels when a fixed-capacity structure of the
+ * reference overflows (its results are then no longer defined by the reference). */
+enum {
+  VELO_STATUS_TTF_RING_OVERFLOW = 1u << 0,  /* > 2048 live tracks to follow (VeloDefinitions.cuh:26) */
+  VELO_STATUS_TRACK_SLOTS_OVERFLOW = 1u << 1,/* more tracks than 0.34 x clusters slots (VeloEventModel.cuh:192-196) */
+  VELO_STATUS_HIT_INDEX_OVERFLOW = 1u << 2,  /* > 65536 clusters in an event (uint16 hit indices) */
+  VELO_STATUS_THREE_HIT_SLOTS_OVERFLOW = 1u << 3,
+  VELO_STATUS_LDS_FALLBACK = 1u << 8         /* informational: an event took the global-memory path */
+};
+
+int velo_last_hip_error(void);
+const char* velo_error_string(int code);
+
+/* ---- Constants: replaces Constants::dev_velo_geometry / dev_velo_sp_patterns / sp_fx / sp_fy /
+ * dev_beamline filled by integration/non_event_data/src/VPGeometry.cpp:21-63 and Beamline.cpp:17-31. */
+typedef struct velo_constants velo_constants;
+int velo_constants_create(const char* velo_geometry_bin, size_t nbytes, const float* beamline_xy, velo_constants** out);
+int velo_constants_destroy(velo_constants* c);
+/* Device pointers held by the constants (for use in the *_params structs below). */
+const void* velo_constants_dev_geometry(const velo_constants* c);
+const uint8_t* velo_constants_dev_sp_patterns(const velo_constants* c);
+const float* velo_constants_dev_sp_fx(const velo_constants* c);
+const float* velo_constants_dev_sp_fy(const velo_constants* c);
+const float* velo_constants_dev_beamline(const velo_constants* c);
+/* cache_sp_patterns (host/velo/clustering/src/ClusteringCommon.cpp:6-104): the cluster table of
+ * every 8-pixel super-pixel pattern, computed on the host (no device needed). */
+void velo_cache_sp_patterns(uint8_t* patterns, uint8_t* sizes, float* fx, float* fy);
+/* Host copy of the super-pixel tables (ClusteringCommon.cpp:6-104): 256 + 256 B, 2 x 512 floats. */
+int velo_constants_sp_tables(const velo_constants* c, uint8_t* patterns, uint8_t* sizes, float* fx, float* fy);
+
+/* ---- host_prefix_sum_t (host/prefix_sum/src/HostPrefixSum.cpp:76-107), as a device scan:
+ * dev_output_buffer[0] = 0, dev_output_buffer[i+1] = sum(dev_input_buffer[0..i]) (u32 wrap).
+ * If host_total_sum_holder is not NULL the total is copied there (the one host<->device sync point
+ * the reference has here); with NULL nothing waits on the host. */
+typedef struct {
+  const unsigned* dev_input_buffer;
+  unsigned input_size;
+  unsigned* dev_output_buffer;     /* input_size + 1 */
+  unsigned* dev_scratch;           /* >= velo_prefix_sum_scratch_size(input_size) u32, or NULL for sizes <= 4096 */
+  unsigned* host_total_sum_holder; /* optional */
+} velo_prefix_sum_params;
+size_t velo_prefix_sum_scratch_size(unsigned input_size);
+int velo_prefix_sum(const velo_prefix_sum_params* p, hipStream_t stream);
+
+/* ---- velo_calculate_number_of_candidates_t
+ * (device/velo/mask_clustering/include/VeloCalculateNumberOfCandidates.cuh:10-35) */
+typedef struct {
+  unsigned host_number_of_events;
+  int host_raw_bank_version;
+  const unsigned* dev_event_list; unsigned event_list_size; /* MASK_INPUT */
+  const char* dev_velo_raw_input;
+  const unsigned* dev_velo_raw_input_offsets;
+  const unsigned* dev_velo_raw_input_sizes;
+  const unsigned* dev_velo_raw_input_types;
+  unsigned* dev_number_of_candidates;  /* host_number_of_events */
+  unsigned* dev_velo_bank_index;       /* 208 */
+  unsigned block_dim_x;                /* property, default 256 */
+  unsigned event_start;                /* RuntimeOptions::event_interval first */
+  int mep_layout;                      /* RuntimeOptions::mep_layout (only 0 is supported) */
+} velo_calculate_number_of_candidates_params;
+int velo_calculate_number_of_candidates(const velo_calculate_number_of_candidates_params* p, hipStream_t stream);
+
+/* ---- velo_estimate_input_size_t (device/velo/mask_clustering/include/EstimateInputSize.cuh:10-36) */
+typedef struct {
+  unsigned host_number_of_events;
+  unsigned host_number_of_cluster_candidates;
+  int host_raw_bank_version;
+  const unsigned* dev_event_list; unsigned event_list_size;
+  const unsigned* dev_candidates_offsets;
+  const char* dev_velo_raw_input;
+  const unsigned* dev_velo_raw_input_offsets;
+  const unsigned* dev_velo_raw_input_sizes;
+  const unsigned* dev_velo_raw_input_types;
+  unsigned* dev_estimated_input_size;  /* 26 x events */
+  unsigned* dev_module_candidate_num;  /* events */
+  unsigned* dev_cluster_candidates;    /* host_number_of_cluster_candidates */
+  unsigned event_start;
+  int mep_layout;
+} velo_estimate_input_size_params;
+int velo_estimate_input_size(const velo_estimate_input_size_params* p, hipStream_t stream);
+
+/* ---- velo_masked_clustering_t (device/velo/mask_clustering/include/MaskedVeloClustering.cuh:14-37) */
+typedef struct {
+  unsigned host_total_number_of_velo_clusters;
+  unsigned host_number_of_events;
+  int host_raw_bank_version;
+  const char* dev_velo_raw_input;
+  const unsigned* dev_velo_raw_input_offsets;
+  const unsigned* dev_velo_raw_input_sizes;
+  const unsigned* dev_velo_raw_input_types;
+  const unsigned* dev_offsets_estimated_input_size;
+  const unsigned* dev_module_candidate_num;
+  const unsigned* dev_cluster_candidates;
+  const unsigned* dev_event_list; unsigned event_list_size;
+  const unsigned* dev_candidates_offsets;
+  const unsigned* dev_number_of_events;
+  const unsigned* dev_velo_bank_index;
+  unsigned* dev_module_cluster_num;        /* 26 x events */
+  char* dev_velo_cluster_container;        /* 18 x host_total_number_of_velo_clusters bytes */
+  const velo_constants* constants;
+  unsigned event_start;
+  int mep_layout;
+} velo_masked_clustering_params;
+int velo_masked_clustering(const velo_masked_clustering_params* p, hipStream_t stream);
+
+/* ---- velo_sort_by_phi_t (device/velo/search_by_triplet/include/SortByPhi.cuh:14-28) */
+typedef struct {
+  unsigned host_number_of_events;
+  unsigned host_total_number_of_velo_clusters;
+  const unsigned* dev_event_list; unsigned event_list_size;
+  const unsigned* dev_offsets_estimated_input_size;
+  const unsigned* dev_module_cluster_num;
+  const char* dev_velo_cluster_container;
+  const unsigned* dev_number_of_events;
+  char* dev_sorted_velo_cluster_container;
+  unsigned* dev_hit_permutation;           /* host_total_number_of_velo_clusters */
+} velo_sort_by_phi_params;
+int velo_sort_by_phi(const velo_sort_by_phi_params* p, hipStream_t stream);
+
+/* ---- velo_search_by_triplet_t (device/velo/search_by_triplet/include/SearchByTriplet.cuh:15-45) */
+typedef struct {
+  unsigned host_number_of_events;
+  unsigned host_total_number_of_velo_clusters;
+  const unsigned* dev_event_list; unsigned event_list_size;
+  const unsigned* dev_number_of_events;
+  const char* dev_sorted_velo_cluster_container;
+  const unsigned* dev_offsets_estimated_input_size;
+  const unsigned* dev_module_cluster_num;
+  void* dev_tracks;                 /* Velo::TrackHits x track_container_size */
+  void* dev_tracklets;              /* Velo::TrackletHits x 2048 x events */
+  unsigned* dev_tracks_to_follow;   /* 2048 x events */
+  void* dev_three_hit_tracks;       /* Velo::TrackletHits x track_container_size */
+  uint8_t* dev_hit_used;            /* bool x clusters */
+  unsigned* dev_atomics_velo;       /* 3 x events */
+  uint16_t* dev_rel_indices;        /* clusters */
+  unsigned* dev_number_of_velo_tracks; /* events */
+  unsigned* dev_status;             /* optional: VELO_STATUS_* bits are OR-ed in */
+  const velo_constants* constants;
+  float phi_tolerance;              /* 0.045 */
+  float max_scatter;                /* 0.08 */
+  unsigned max_skipped_modules;     /* 1 */
+  unsigned block_dim_x;             /* 0 = library default */
+} velo_search_by_triplet_params;
+int velo_search_by_triplet(const velo_search_by_triplet_params* p, hipStream_t stream);
+/* track_container_size of SearchByTriplet.cu:18-28 */
+unsigned velo_track_container_size(unsigned total_clusters, unsigned number_of_events);
+
+/* ---- velo_three_hit_tracks_filter_t (device/velo/search_by_triplet/include/ThreeHitTracksFilter.cuh:10-31) */
+typedef struct {
+  unsigned host_number_of_events;
+  unsigned host_total_number_of_velo_clusters;
+  const unsigned* dev_event_list; unsigned event_list_size;
+  const char* dev_sorted_velo_cluster_container;
+  const unsigned* dev_offsets_estimated_input_size;
+  const void* dev_three_hit_tracks_input;
+  const unsigned* dev_atomics_velo;
+  const uint8_t* dev_hit_used;
+  const unsigned* dev_number_of_events;
+  void* dev_three_hit_tracks_output;
+  unsigned* dev_number_of_three_hit_tracks_output;
+  float max_chi2;                   /* 20 */
+} velo_three_hit_tracks_filter_params;
+int velo_three_hit_tracks_filter(const velo_three_hit_tracks_filter_params* p, hipStream_t stream);
+
+/* ---- velo_copy_track_hit_number_t (device/velo/consolidate_tracks/include/VeloCopyTrackHitNumber.cuh) */
+typedef struct {
+  unsigned host_number_of_events;
+  unsigned host_number_of_velo_tracks_at_least_four_hits;
+  unsigned host_number_of_three_hit_tracks_filtered;
+  const void* dev_tracks;
+  const unsigned* dev_offsets_velo_tracks;
+  const unsigned* dev_offsets_estimated_input_size;
+  const unsigned* dev_offsets_number_of_three_hit_tracks_filtered;
+  unsigned* dev_velo_track_hit_number;
+  unsigned* dev_offsets_all_velo_tracks;   /* events + 1 */
+  unsigned* host_number_of_reconstructed_velo_tracks; /* optional host copy (D2H + wait) */
+} velo_copy_track_hit_number_params;
+int velo_copy_track_hit_number(const velo_copy_track_hit_number_params* p, hipStream_t stream);
+
+/* ---- velo_consolidate_tracks_t (device/velo/consolidate_tracks/include/VeloConsolidateTracks.cuh) */
+typedef struct {
+  unsigned host_accumulated_number_of_hits_in_velo_tracks;
+  unsigned host_number_of_reconstructed_velo_tracks;
+  unsigned host_number_of_three_hit_tracks_filtered;
+  unsigned host_number_of_events;
+  const unsigned* dev_event_list; unsigned event_list_size;
+  const unsigned* dev_offsets_all_velo_tracks;
+  const void* dev_tracks;
+  const unsigned* dev_offsets_velo_track_hit_number;
+  const char* dev_sorted_velo_cluster_container;
+  const unsigned* dev_offsets_estimated_input_size;
+  const void* dev_three_hit_tracks_output;
+  const unsigned* dev_offsets_number_of_three_hit_tracks_filtered;
+  const unsigned* dev_number_of_events;
+  char* dev_velo_track_hits;                 /* 18 x accumulated hits bytes */
+  uint8_t* dev_accepted_velo_tracks;         /* tracks; set to 1 */
+} velo_consolidate_tracks_params;
+int velo_consolidate_tracks(const velo_consolidate_tracks_params* p, hipStream_t stream);
+
+/* ---- velo_kalman_filter_t (device/velo/simplified_kalman_filter/include/VeloKalmanFilter.cuh:160-284) */
+typedef struct {
+  unsigned host_number_of_reconstructed_velo_tracks;
+  unsigned host_number_of_events;
+  const unsigned* dev_event_list; unsigned event_list_size;
+  const unsigned* dev_number_of_events;
+  const unsigned* dev_offsets_all_velo_tracks;
+  const unsigned* dev_offsets_velo_track_hit_number; /* replaces dev_velo_tracks_view */
+  const char* dev_velo_track_hits;
+  uint8_t* dev_is_backward;
+  char* dev_velo_kalman_beamline_states;     /* 56 x tracks bytes */
+  char* dev_velo_kalman_endvelo_states;      /* 56 x tracks bytes */
+  const velo_constants* constants;
+} velo_kalman_filter_params;
+int velo_kalman_filter(const velo_kalman_filter_params* p, hipStream_t stream);
+
+/* ---- Whole sequence: the nine algorithms and five scans above, enqueued back to back on one
+ * stream with every size read on the device (no host synchronisation inside a run), so the run
+ * can be captured in a hipGraph.  Clustering and the phi sort are fused per module pair.
+ * Buffers are sized at creation for at most max_events events and max_raw_bytes of raw input. */
+typedef struct velo_sequence velo_sequence;
+typedef struct {
+  const char* dev_velo_raw_input;
+  const unsigned* dev_velo_raw_input_offsets;
+  const unsigned* dev_velo_raw_input_sizes;
+  const unsigned* dev_velo_raw_input_types;
+  size_t raw_input_bytes;          /* host-known size of dev_velo_raw_input */
+  unsigned number_of_events;
+  int bank_version;
+  float phi_tolerance, max_scatter, max_chi2;
+  unsigned max_skipped_modules;
+} velo_sequence_input;
+int velo_sequence_create(const velo_constants* c, unsigned max_events, size_t max_raw_bytes, velo_sequence** out);
+int velo_sequence_destroy(velo_sequence* s);
+void velo_sequence_default_input(velo_sequence_input* in);
+int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream_t stream);
+/* Device pointer and capacity (bytes) of a named buffer of the last run; names are the
+ * reference's argument names without the dev_ prefix (e.g. "offsets_all_velo_tracks"). */
+int velo_sequence_buffer(velo_sequence* s, const char* name, void** dev_ptr, size_t* capacity_bytes);
+/* Waits for the stream and returns the OR of the VELO_STATUS_* bits of the last run. */
+int velo_sequence_status(velo_sequence* s, hipStream_t stream, unsigned* status);
+/* Knobs applied to subsequent runs:
+ *   "search_block"      workgroup size of search_by_triplet (multiple of 64, <= 1024; 0 = 256)
+ *   "fused_clustering"  1: clustering and phi sort fused per module pair (default); 0: the four
+ *                       reference algorithms one after the other
+ *   "timed_stage"       VELO_STAGE_* to bracket with HIP events on the run's stream (0 = none)
+ *   "count_work"        1: search_by_triplet also counts its work into the buffer "search_work"
+ *                       (u64 x 4: seeding evaluations, forwarding evaluations, extrapolations,
+ *                       binary-search iterations) */
+int velo_sequence_set_tuning(velo_sequence* s, const char* knob, unsigned value);
+enum {
+  VELO_STAGE_ALL = 1,              /* the whole run */
+  VELO_STAGE_CANDIDATES = 2,       /* velo_calculate_number_of_candidates */
+  VELO_STAGE_ESTIMATE = 3,         /* velo_estimate_input_size */
+  VELO_STAGE_CLUSTERING = 4,       /* masked clustering + sort by phi */
+  VELO_STAGE_SEARCH = 5,           /* velo_search_by_triplet */
+  VELO_STAGE_THREE_HIT_FILTER = 6, /* velo_three_hit_tracks_filter */
+  VELO_STAGE_CONSOLIDATE = 7,      /* velo_consolidate_tracks */
+  VELO_STAGE_KALMAN = 8,           /* velo_kalman_filter */
+  VELO_STAGE_COUNT = 8
+};
+/* Sum of the elapsed milliseconds of the timed stage over the runs since "timed_stage" was
+ * last set, and the number of those runs (waits for them). */
+int velo_sequence_stage_ms(velo_sequence* s, float* total_ms, unsigned* runs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

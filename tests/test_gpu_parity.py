@@ -1,0 +1,85 @@
+"""The HIP path against the CPU oracle on seeded synthetic events (runs on an MI355X)."""
+import numpy as np
+import pytest
+
+import oracle
+from allen_amd import synth, velo
+from tests import parity
+
+pytestmark = pytest.mark.gpu
+
+# Small, seeded datasets that the oracle finishes in seconds.
+DATASETS = {
+    "single_pv": dict(n_events=30, pileup=0, mean_tracks=120.0, seed=11),
+    "pileup": dict(n_events=120, pileup=1, nu=7.6, seed=12),
+    "shuffled_backward": dict(n_events=30, pileup=0, mean_tracks=150.0, backward_fraction=0.3, shuffle_sp=1,
+                              seed=13),
+    "v2_banks": dict(n_events=20, pileup=0, mean_tracks=100.0, bank_version=2, seed=14),
+    "empty_events": dict(n_events=6, pileup=0, mean_tracks=0.0, noise_fraction=0.0, seed=15),
+    "stress": dict(n_events=12, pileup=1, nu=15.0, noise_fraction=0.05, dead_asic_fraction=0.01, seed=16),
+}
+
+
+@pytest.fixture(scope="module")
+def constants(geometry, beamline):
+    c = velo.VeloConstants(geometry, beamline)
+    yield c
+    c.close()
+
+
+def run_both(name, geometry, beamline, constants, fused):
+    ev = synth.generate(**DATASETS[name])
+    print(f"[parity] dataset={name} events={ev.n_events} raw_bytes={ev.raw.size} fused={fused}", flush=True)
+    ref = oracle.run(ev, geometry, beamline)
+    dev = velo.DeviceEvents(ev)
+    seq = velo.VeloSequence(constants, ev.n_events, max(ev.raw.size, 4096))
+    seq.set_tuning("fused_clustering", int(fused))
+    seq.run(dev)
+    status = seq.status()
+    gpu = parity.gpu_buffers(seq, ref)
+    seq.close()
+    return ref, gpu, status
+
+
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "per_algorithm"])
+@pytest.mark.parametrize("name", list(DATASETS))
+def test_sequence_matches_oracle(name, fused, geometry, beamline, constants):
+    ref, gpu, status = run_both(name, geometry, beamline, constants, fused)
+    assert int(ref["status"][0]) == 0, "the oracle flags a capacity overflow; pick a smaller dataset"
+    assert status & velo.STATUS_OVERFLOW_MASK == 0
+    errs = parity.compare(gpu, ref, fused)
+    assert not errs, "\n".join(errs[:20])
+    if name != "empty_events":
+        assert ref["offsets_all_velo_tracks"][-1] > 0
+
+
+def test_prefix_sum_iota(constants):
+    """The reference's own prefix-sum test shape (TestPrefixSum.cu:9-33): iota 1..1000."""
+    import torch
+    x = torch.arange(1, 1001, dtype=torch.int32, device="cuda")
+    out = torch.empty(1001, dtype=torch.int32, device="cuda")
+    total = velo.prefix_sum(x, 1000, out, want_total=True)
+    expect = np.concatenate([[0], np.cumsum(np.arange(1, 1001))])
+    assert total == expect[-1]
+    np.testing.assert_array_equal(out.cpu().numpy(), expect)
+
+
+@pytest.mark.parametrize("n", [0, 1, 4095, 4096, 4097, 26 * 1000, 26 * 20000 + 3])
+def test_prefix_sum_sizes(n, constants):
+    import torch
+    rng = np.random.default_rng(n)
+    host = rng.integers(0, 1 << 20, size=n, dtype=np.int64).astype(np.uint32)
+    x = torch.from_numpy(host.view(np.int32)).cuda()
+    out = torch.full((n + 1,), -1, dtype=torch.int32, device="cuda")
+    total = velo.prefix_sum(x, n, out, want_total=True)
+    expect = np.zeros(n + 1, np.uint32)
+    oracle.lib().vo_prefix_sum(host.ctypes.data, n, expect.ctypes.data)
+    np.testing.assert_array_equal(out.cpu().numpy().view(np.uint32), expect)
+    assert total == int(expect[-1])
+
+
+def test_constants_tables_match_oracle(constants):
+    mine = constants.sp_tables()
+    ref = oracle.sp_tables()
+    for a, b in zip(mine, ref):
+        np.testing.assert_array_equal(a, b)
