@@ -338,6 +338,7 @@ int velo_search_by_triplet(const velo_search_by_triplet_params* p, hipStream_t s
   a.max_scatter = p->max_scatter;
   a.max_skipped_modules = p->max_skipped_modules;
   a.block_dim = p->block_dim_x;
+  a.lds_hits = 0;
   a.work_stats = nullptr;
   // The kernel writes every per-event output, so the reference's memsets are not needed.
   return hip_status(velo::launch_search_by_triplet(a, stream));
@@ -436,6 +437,7 @@ struct velo_sequence {
   size_t sp_cap = 0, cluster_cap = 0, track_cap = 0, hits_cap = 0;
   std::map<std::string, std::pair<void*, size_t>> bufs;
   unsigned search_block = 256;
+  unsigned search_lds_hits = 2048;
   bool fused_clustering = true;
   bool count_work = false;
   // HIP events bracketing one stage of the run (0 = none), see velo_sequence_stage_ms.
@@ -504,6 +506,7 @@ int velo_sequence_create(const velo_constants* c, unsigned max_events, size_t ma
     {"velo_kalman_endvelo_states", 56 * 2 * s->track_cap},
     {"scan_scratch", 4 * scan_words},
     {"search_work", 4 * sizeof(unsigned long long)},
+    {"sp_flags", s->sp_cap},
   };
   for (const auto& kv : layout) {
     void* p = nullptr;
@@ -558,6 +561,11 @@ int velo_sequence_set_tuning(velo_sequence* s, const char* knob, unsigned value)
   if (k == "search_block") {
     if (value && (value % 64 || value > 1024)) return VELO_ERR_INVALID_ARGUMENT;
     s->search_block = value ? value : 256;
+    return VELO_OK;
+  }
+  if (k == "search_lds_hits") {
+    if (value > 4096) return VELO_ERR_INVALID_ARGUMENT;
+    s->search_lds_hits = value ? value : 2048;
     return VELO_OK;
   }
   if (k == "fused_clustering") {
@@ -653,8 +661,24 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream
   k2.estimated_input_size = s->get<uint32_t>("estimated_input_size");
   k2.module_candidate_num = s->get<uint32_t>("module_candidate_num");
   k2.cluster_candidates = s->get<uint32_t>("cluster_candidates");
+  auto* sp_flags = s->get<uint8_t>("sp_flags");
   VELO_MARK(mark_begin(VELO_STAGE_ESTIMATE));
-  VELO_TRY(velo::launch_estimate_input_size(k2, stream));
+  if (s->fused_clustering) {
+    velo::EstimateMpLaunch e;
+    e.version = in->bank_version;
+    e.raw = k1.raw;
+    e.offsets = k1.offsets;
+    e.sizes = k1.sizes;
+    e.types = k1.types;
+    e.n_events = E;
+    e.estimated_input_size = k2.estimated_input_size;
+    e.sp_flags = sp_flags;
+    VELO_TRY(velo::launch_estimate_mp(e, stream));
+  }
+  else {
+    VELO_TRY(velo::launch_fill_u32(k2.module_candidate_num, E, 0, stream));
+    VELO_TRY(velo::launch_estimate_input_size(k2, stream));
+  }
   VELO_MARK(mark_end(VELO_STAGE_ESTIMATE));
   auto* est_off = s->get<uint32_t>("offsets_estimated_input_size");
   VELO_TRY(velo::launch_prefix_sum(k2.estimated_input_size, 26 * E, est_off, scratch, stream));
@@ -663,7 +687,7 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream
   auto* sorted = s->get<char>("sorted_velo_cluster_container");
   VELO_MARK(mark_begin(VELO_STAGE_CLUSTERING));
   if (s->fused_clustering) {
-    velo::FusedClusterLaunch f;
+    velo::ClusterMpLaunch f;
     f.version = in->bank_version;
     f.raw = k1.raw;
     f.offsets = k1.offsets;
@@ -671,6 +695,7 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream
     f.types = k1.types;
     f.n_events = E;
     f.offsets_estimated_input_size = est_off;
+    f.sp_flags = sp_flags;
     f.geometry = s->constants->dev_geometry;
     f.sp_patterns = s->constants->dev_sp_patterns;
     f.sp_fx = s->constants->dev_sp_fx;
@@ -678,8 +703,7 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream
     f.module_cluster_num = mcn;
     f.sorted_container = sorted;
     f.unsorted_scratch = s->get<char>("velo_cluster_container");
-    f.status = status;
-    VELO_TRY(velo::launch_cluster_sorted(f, stream));
+    VELO_TRY(velo::launch_cluster_mp(f, stream));
   }
   else {
     velo::MaskedClusteringLaunch k3;
@@ -738,6 +762,7 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream
   k5.max_scatter = in->max_scatter;
   k5.max_skipped_modules = in->max_skipped_modules;
   k5.block_dim = s->search_block;
+  k5.lds_hits = s->search_lds_hits;
   k5.work_stats = s->count_work ? s->get<unsigned long long>("search_work") : nullptr;
   if (k5.work_stats) VELO_TRY(velo::launch_fill_u32(reinterpret_cast<uint32_t*>(k5.work_stats), 8, 0, stream));
   VELO_MARK(mark_begin(VELO_STAGE_SEARCH));

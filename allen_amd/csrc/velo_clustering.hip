@@ -87,7 +87,8 @@ __device__ __forceinline__ unsigned isolated_sp_clusters(
   const float* sp_fy,
   const RawBank& bank,
   uint32_t sp_word,
-  ClusterOut* out)
+  ClusterOut& out0,
+  ClusterOut& out1)
 {
   constexpr bool v23 = (version == 2 || version == 3);
   const uint32_t sp_addr = (sp_word & 0x007FFF00u) >> 8;
@@ -108,39 +109,45 @@ __device__ __forceinline__ unsigned isolated_sp_clusters(
     const uint32_t col = k == 0 ? ((idx >> 2) & 1u) : ((idx >> 6) & 1u);
     const uint32_t cx = sp_col * 2 + col;
     const uint32_t cy = sp_row * 4 + row;
-    out[n++] = place_cluster(
-      g, ltg, lhcb_id_of(sensor_index, chip, cx & 0xFFu, cy), cx, cy, sp_fx[sp * 2 + k], sp_fy[sp * 2 + k]);
+    const ClusterOut c =
+      place_cluster(g, ltg, lhcb_id_of(sensor_index, chip, cx & 0xFFu, cy), cx, cy, sp_fx[sp * 2 + k], sp_fy[sp * 2 + k]);
+    if (k == 0) out0 = c;
+    else out1 = c;
+    ++n;
   }
   return n;
 }
 
-// Candidate pixels of a non-isolated SP (EstimateInputSize.cu:104-204). `words` are the SP words
-// of its bank. The neighbour scan keeps the reference's behaviour: every word of the bank,
-// of either sensor and isolated or not, contributes (EstimateInputSize.cu:111-141).
+// Candidate pixels of a non-isolated SP (EstimateInputSize.cu:104-204), in two steps:
+// neighbour_acc() ORs into the 18-bit neighbourhood `pixels` what one other SP word of the bank
+// contributes, candidates_final() applies the candidate mask. The neighbour scan keeps the
+// reference's behaviour: every word of the bank, of either sensor and isolated or not,
+// contributes (EstimateInputSize.cu:111-141).
+__device__ __forceinline__ uint32_t neighbourhood_init(uint32_t sp_word)
+{
+  const uint32_t sp = sp_word & 0xFFu;
+  return (sp & 0x0F) | ((sp & 0xF0) << 2);
+}
+
+__device__ __forceinline__ void neighbour_acc(uint32_t sp_row, uint32_t sp_col, uint32_t other, uint32_t& pixels)
+{
+  const uint32_t oaddr = (other & 0x007FFF00u) >> 8;
+  const uint32_t orow = oaddr & 0x3Fu;
+  const uint32_t ocol = oaddr >> 6;
+  const uint32_t osp = other & 0xFFu;
+  const bool top = orow == sp_row + 1, mid = orow == sp_row, bot = orow == sp_row - 1;
+  const bool same_col = ocol == sp_col, right_col = ocol == sp_col + 1;
+  if (top && same_col) pixels |= ((osp & 0x01) | ((osp & 0x10) << 2)) << 4;
+  if (top && right_col) pixels |= (osp & 0x01) << 16;
+  if (mid && right_col) pixels |= (osp & 0x0F) << 12;
+  if (bot && right_col) pixels |= (osp & 0x08) << 8;
+  if (bot && same_col) pixels |= (osp & 0x80) >> 2;
+}
+
 // Returns a 2-bit mask: bit0 = first candidate (pixels 0,1,4,5), bit1 = second; the pixel
 // numbers are returned in *p0, *p1.
-__device__ __forceinline__ unsigned
-sp_candidates(uint32_t sp_word, const uint32_t* words, unsigned n_sp, unsigned* p0, unsigned* p1)
+__device__ __forceinline__ unsigned candidates_final(uint32_t pixels, unsigned* p0, unsigned* p1)
 {
-  const uint32_t sp_addr = (sp_word & 0x007FFF00u) >> 8;
-  const uint32_t sp = sp_word & 0xFFu;
-  uint32_t pixels = (sp & 0x0F) | ((sp & 0xF0) << 2);
-  const uint32_t sp_row = sp_addr & 0x3Fu;
-  const uint32_t sp_col = sp_addr >> 6;
-  for (unsigned k = 0; k < n_sp; ++k) {
-    const uint32_t other = words[k];
-    const uint32_t oaddr = (other & 0x007FFF00u) >> 8;
-    const uint32_t orow = oaddr & 0x3Fu;
-    const uint32_t ocol = oaddr >> 6;
-    const uint32_t osp = other & 0xFFu;
-    const bool top = orow == sp_row + 1, mid = orow == sp_row, bot = orow == sp_row - 1;
-    const bool same_col = ocol == sp_col, right_col = ocol == sp_col + 1;
-    if (top && same_col) pixels |= ((osp & 0x01) | ((osp & 0x10) << 2)) << 4;
-    if (top && right_col) pixels |= (osp & 0x01) << 16;
-    if (mid && right_col) pixels |= (osp & 0x0F) << 12;
-    if (bot && right_col) pixels |= (osp & 0x08) << 8;
-    if (bot && same_col) pixels |= (osp & 0x80) >> 2;
-  }
   const uint32_t inside = pixels & 0x3CF;
   const uint32_t mask = (inside << 1) | (inside << 5) | (inside << 6) | (inside << 7);
   const uint32_t working = mask & (~pixels);
@@ -160,6 +167,16 @@ sp_candidates(uint32_t sp_word, const uint32_t* words, unsigned n_sp, unsigned* 
   return m;
 }
 
+__device__ __forceinline__ unsigned
+sp_candidates(uint32_t sp_word, const uint32_t* words, unsigned n_sp, unsigned* p0, unsigned* p1)
+{
+  const uint32_t sp_addr = (sp_word & 0x007FFF00u) >> 8;
+  const uint32_t sp_row = sp_addr & 0x3Fu, sp_col = sp_addr >> 6;
+  uint32_t pixels = neighbourhood_init(sp_word);
+  for (unsigned k = 0; k < n_sp; ++k) neighbour_acc(sp_row, sp_col, words[k], pixels);
+  return candidates_final(pixels, p0, p1);
+}
+
 __device__ __forceinline__ unsigned isolated_cluster_count(uint32_t sp)
 {
   const bool pattern_0 = (sp & 0x88) && !(sp & 0x44) && (sp & 0x33);
@@ -167,56 +184,63 @@ __device__ __forceinline__ unsigned isolated_cluster_count(uint32_t sp)
   return (pattern_0 | pattern_1) ? 2 : 1;
 }
 
-// rest_of_clusters (MaskedVeloClustering.cu:290-496) for one candidate pixel of the SP at
-// `sp_index` of a bank whose words are `words`. Returns true and fills *out if the cluster is kept.
+// rest_of_clusters (MaskedVeloClustering.cu:290-496) for one candidate pixel, in two steps:
+// window_acc() ORs one other SP word of the bank into the 5-column x 4-row SP window around the
+// candidate's SP (pa[c]: one SP column, rows bottom to top, 16 bits per pixel column), and
+// masked_cluster_final() grows the cluster, applies the precedence and window-edge rules and
+// places it.
+struct Window {
+  uint32_t pa0, pa1, pa2, pa3, pa4;
+};
+
 template<int version>
-__device__ __forceinline__ bool masked_cluster(
+__device__ __forceinline__ void
+window_acc(int32_t sp_row_lower, int32_t sp_col_lower, uint32_t sensor_number, uint32_t other, Window& w)
+{
+  if constexpr (version > 3) {
+    if (((other >> 23) & 1u) != (sensor_number & 1u)) return;
+  }
+  if (other & 0x80000000u) return;
+  const uint32_t oaddr = (other & 0x007FFF00u) >> 8;
+  const int32_t rr = static_cast<int32_t>(oaddr & 0x3Fu) - sp_row_lower;
+  const int32_t rc = static_cast<int32_t>(oaddr >> 6) - sp_col_lower;
+  if (rr >= 0 && rr <= 3 && rc >= 0 && rc <= 4) {
+    const uint32_t osp = other & 0xFFu;
+    const uint32_t bits = ((osp & 0x0Fu) << (4 * rr)) | ((osp & 0xF0u) << (12 + 4 * rr));
+    w.pa0 |= rc == 0 ? bits : 0u;
+    w.pa1 |= rc == 1 ? bits : 0u;
+    w.pa2 |= rc == 2 ? bits : 0u;
+    w.pa3 |= rc == 3 ? bits : 0u;
+    w.pa4 |= rc == 4 ? bits : 0u;
+  }
+}
+
+__device__ __forceinline__ bool masked_cluster_final(
   const Geometry& g,
-  const uint32_t* words,
-  unsigned n_sp,
-  unsigned sp_index,
+  uint32_t sp_word,
   uint32_t sensor_number,
   uint32_t starting_pixel_location,
+  const Window& pa,
   ClusterOut* out)
 {
   const float* ltg = g.ltg + 12 * sensor_number;
-  const uint32_t sp_word = words[sp_index];
   const uint32_t sp_addr = (sp_word & 0x007FFF00u) >> 8;
   const int32_t sp_row = sp_addr & 0x3Fu;
   const int32_t sp_col = sp_addr >> 6;
-  uint32_t pa[5] = {0, 0, 0, 0, 0};
-  const int32_t sp_row_lower = sp_row - 2, sp_col_lower = sp_col - 3;
-  const int32_t row_lower_limit = sp_row_lower * 4;
-  const int32_t col_lower_limit = sp_col_lower * 2;
-  for (unsigned k = 0; k < n_sp; ++k) {
-    const uint32_t other = words[k];
-    if constexpr (version > 3) {
-      if (((other >> 23) & 1u) != (sensor_number & 1u)) continue;
-    }
-    if (other & 0x80000000u) continue;
-    const uint32_t oaddr = (other & 0x007FFF00u) >> 8;
-    const int32_t rr = static_cast<int32_t>(oaddr & 0x3Fu) - sp_row_lower;
-    const int32_t rc = static_cast<int32_t>(oaddr >> 6) - sp_col_lower;
-    if (rr >= 0 && rr <= 3 && rc >= 0 && rc <= 4) {
-      const uint32_t osp = other & 0xFFu;
-      const uint32_t bits = ((osp & 0x0Fu) << (4 * rr)) | ((osp & 0xF0u) << (12 + 4 * rr));
-#pragma unroll
-      for (int c = 0; c < 5; ++c)
-        if (rc == c) pa[c] |= bits;
-    }
-  }
+  const int32_t row_lower_limit = (sp_row - 2) * 4;
+  const int32_t col_lower_limit = (sp_col - 3) * 2;
   const uint32_t rel_row = starting_pixel_location & 0x3u;
   const uint32_t rel_col = starting_pixel_location < 4;
   const uint32_t col = sp_col * 2 + rel_col;
   const uint64_t starting_pixel = ((uint64_t)(0x01u << (11 - rel_row)) << (16 * (col & 0x01))) << 32;
-  const uint64_t pixel_map = (((uint64_t) pa[3]) << 32) | pa[2];
+  const uint64_t pixel_map = (((uint64_t) pa.pa3) << 32) | pa.pa2;
   const Grown cl = grow_cluster(pixel_map, starting_pixel, col_lower_limit + 4, row_lower_limit);
-  const uint64_t precedence = (mask_east(cl.pixels) & pa[4]) | (cl.pixels & (starting_pixel ^ (0ull - starting_pixel)));
+  const uint64_t precedence = (mask_east(cl.pixels) & pa.pa4) | (cl.pixels & (starting_pixel ^ (0ull - starting_pixel)));
   bool keep = precedence == 0 && !(cl.pixels & 0x8000800080008000ull) && !(cl.pixels & 0x0001000100010001ull);
   if (!keep) return false;
   unsigned x = cl.x, y = cl.y, n = cl.n;
   if (cl.pixels & 0xFFFFull) {
-    const uint64_t west_map = (((uint64_t) pa[1]) << 32) | pa[0];
+    const uint64_t west_map = (((uint64_t) pa.pa1) << 32) | pa.pa0;
     const uint64_t west_start = mask_west(cl.pixels) & west_map;
     const Grown ext = grow_cluster(west_map, west_start, col_lower_limit, row_lower_limit);
     x += ext.x;
@@ -231,6 +255,26 @@ __device__ __forceinline__ bool masked_cluster(
   const float fy = y / static_cast<float>(n) - cy;
   *out = place_cluster(g, ltg, lhcb_id_of(sensor_number, cx >> 8, cx & 0xFFu, cy), cx, cy, fx, fy);
   return true;
+}
+
+// The whole of rest_of_clusters for the SP at `sp_index` of a bank whose words are `words`.
+template<int version>
+__device__ __forceinline__ bool masked_cluster(
+  const Geometry& g,
+  const uint32_t* words,
+  unsigned n_sp,
+  unsigned sp_index,
+  uint32_t sensor_number,
+  uint32_t starting_pixel_location,
+  ClusterOut* out)
+{
+  const uint32_t sp_word = words[sp_index];
+  const uint32_t sp_addr = (sp_word & 0x007FFF00u) >> 8;
+  const int32_t sp_row_lower = static_cast<int32_t>(sp_addr & 0x3Fu) - 2;
+  const int32_t sp_col_lower = static_cast<int32_t>(sp_addr >> 6) - 3;
+  Window w {0, 0, 0, 0, 0};
+  for (unsigned k = 0; k < n_sp; ++k) window_acc<version>(sp_row_lower, sp_col_lower, sensor_number, words[k], w);
+  return masked_cluster_final(g, sp_word, sensor_number, starting_pixel_location, w, out);
 }
 
 __device__ __forceinline__ void store_cluster(const Clusters& c, unsigned i, const ClusterOut& o)
@@ -420,12 +464,12 @@ __global__ void k_masked_clustering(
     for (unsigned s = lane_id(); s < n_sp; s += wave_size) {
       const uint32_t w = bank.word[s];
       if (!(w & 0x80000000u)) continue;
-      ClusterOut c[2];
-      const unsigned n = isolated_sp_clusters<version>(g, sp_patterns, sp_fx, sp_fy, bank, w, c);
+      ClusterOut c0, c1;
+      const unsigned n = isolated_sp_clusters<version>(g, sp_patterns, sp_fx, sp_fy, bank, w, c0, c1);
       const unsigned slot = atomicAdd(&count[mp], n);
       const unsigned cap = mp_start[mp + 1] - mp_start[mp];
-      for (unsigned k = 0; k < n; ++k)
-        if (slot + k < cap) store_cluster(clusters, mp_start[mp] + slot + k, c[k]);
+      if (slot < cap) store_cluster(clusters, mp_start[mp] + slot, c0);
+      if (n > 1 && slot + 1 < cap) store_cluster(clusters, mp_start[mp] + slot + 1, c1);
     }
   }
   __syncthreads();
@@ -633,9 +677,10 @@ __global__ void __launch_bounds__(256) k_cluster_sorted(
     else words = raw_bank<version>(ev, s_bank_num[bi]).word;
     const uint32_t sp_word = words[s];
     if (sp_word & 0x80000000u) {
-      ClusterOut c[2];
-      const unsigned n = isolated_sp_clusters<version>(g, sp_patterns, sp_fx, sp_fy, bank, sp_word, c);
-      for (unsigned k = 0; k < n; ++k) emit(c[k]);
+      ClusterOut c0, c1;
+      const unsigned n = isolated_sp_clusters<version>(g, sp_patterns, sp_fx, sp_fy, bank, sp_word, c0, c1);
+      emit(c0);
+      if (n > 1) emit(c1);
     }
     else {
       unsigned p0 = 0, p1 = 0;
@@ -762,6 +807,380 @@ hipError_t launch_cluster_sorted(const FusedClusterLaunch& a, hipStream_t s)
   default: VELO_LAUNCH_FUSED(4); break;
   }
 #undef VELO_LAUNCH_FUSED
+  return hipGetLastError();
+}
+
+
+// ---------------------------------------------------------------------------------------
+// Whole-sequence clustering, one wave per (event, module pair), no block barriers.
+//
+//   k_estimate_mp   the K2 estimate of one module pair; also records, for every non-isolated
+//                   SP word, its candidate pixels in one byte indexed by the word's position in
+//                   the raw buffer (bit0/bit1: first/second candidate, bits 2-4 and 5-7: pixels)
+//   k_cluster_mp    K3 + K4 of one module pair: isolated SPs through the pattern table, the
+//                   recorded candidates through the masked clustering window, clusters appended
+//                   in SP order with wave ballots, then ranked by (phi, id) and written sorted
+//
+// Each wave stages the SP words of its module pair's banks in its own LDS slice; a module pair
+// larger than the slice runs the same code on global memory.
+namespace mp {
+constexpr unsigned waves_per_block = 4;
+constexpr unsigned word_cap = 448;
+constexpr unsigned cluster_cap = 320;
+
+// Banks of one module pair (at most 8), kept in the wave's LDS slice.
+struct Banks {
+  unsigned n;
+  uint32_t start[max_banks_per_mp + 1];
+  uint32_t source[max_banks_per_mp];
+  const uint32_t* word[max_banks_per_mp];
+};
+
+__device__ __forceinline__ void wave_sync_lds()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The banks of module pair `mp` in bank order; lanes scan the bank headers of the event.
+template<int version>
+__device__ __forceinline__ void find_banks(const RawEvent& ev, unsigned mp, Banks& B)
+{
+  constexpr bool v23 = (version == 2 || version == 3);
+  unsigned nb = 0, acc = 0;
+  for (unsigned base = 0; base < ev.n_banks; base += wave_size) {
+    const unsigned b = base + lane_id();
+    bool mine = false;
+    uint32_t n_sp = 0;
+    if (b < ev.n_banks) {
+      const RawBank bank = raw_bank<version>(ev, b);
+      const uint32_t pair = bank.sourceID & 0x1FFu;
+      mine = is_velo_bank(bank.type) && (v23 ? pair / 8 : (pair << 1) / 8) == mp;
+      n_sp = bank_n_sp<version>(bank);
+    }
+    const uint64_t m = __ballot(mine);
+    const unsigned rank = nb + __popcll(m & lanes_below_mask());
+    const uint32_t inc = wave_inclusive_scan(mine ? n_sp : 0u);
+    if (mine && rank < max_banks_per_mp) {
+      const RawBank bank = raw_bank<version>(ev, b);
+      B.start[rank] = acc + inc - n_sp;
+      B.source[rank] = bank.sourceID;
+      B.word[rank] = bank.word;
+    }
+    nb += __popcll(m);
+    acc += __shfl(inc, wave_size - 1, wave_size);
+  }
+  nb = min(nb, max_banks_per_mp);
+  if (lane_id() == 0) {
+    B.n = nb;
+    B.start[nb] = acc;
+  }
+  wave_sync_lds();
+}
+
+__device__ __forceinline__ unsigned bank_of(const Banks& B, unsigned w)
+{
+  unsigned bi = 0;
+  const unsigned n = B.n;
+  for (unsigned k = 1; k < n; ++k)
+    if (B.start[k] <= w) bi = k;
+  return bi;
+}
+} // namespace mp
+
+namespace mp {
+// Word `w` of the module pair (flattened over its banks), from global memory.
+__device__ __forceinline__ uint32_t word_at(const Banks& B, unsigned w)
+{
+  const unsigned bi = bank_of(B, w);
+  return B.word[bi][w - B.start[bi]];
+}
+
+// Copies words [t0, t0 + n) of the module pair into the wave's LDS slice.
+__device__ __forceinline__ void load_words(const Banks& B, unsigned t0, unsigned n, uint32_t* lw)
+{
+  wave_sync_lds();
+  for (unsigned i = lane_id(); i < n; i += wave_size) lw[i] = word_at(B, t0 + i);
+  wave_sync_lds();
+}
+
+// Calls f(word) for every word of the flattened range [lo, hi): straight from the resident
+// slice, or tile by tile through it (wave-uniform tile loop, per-lane ranges).
+template<typename F>
+__device__ __forceinline__ void
+for_words(const Banks& B, unsigned n_words, bool resident, uint32_t* lw, bool active, unsigned lo, unsigned hi, F&& f)
+{
+  if (resident) {
+    if (active)
+      for (unsigned k = lo; k < hi; ++k) f(lw[k]);
+    return;
+  }
+  for (unsigned t0 = 0; t0 < n_words; t0 += word_cap) {
+    const unsigned tn = min(word_cap, n_words - t0);
+    load_words(B, t0, tn, lw);
+    if (active) {
+      const unsigned a = max(lo, t0), b = min(hi, t0 + tn);
+      for (unsigned k = a; k < b; ++k) f(lw[k - t0]);
+    }
+  }
+}
+} // namespace mp
+
+template<int version>
+__global__ void __launch_bounds__(mp::waves_per_block* wave_size) k_estimate_mp(
+  const char* raw,
+  const uint32_t* offsets,
+  const uint32_t* sizes,
+  const uint32_t* types,
+  unsigned n_events,
+  uint32_t* estimated_input_size,
+  uint8_t* sp_flags)
+{
+  __shared__ uint32_t s_words[mp::waves_per_block][mp::word_cap];
+  __shared__ mp::Banks s_banks[mp::waves_per_block];
+  const unsigned wave = threadIdx.x / wave_size;
+  const unsigned idx = blockIdx.x * mp::waves_per_block + wave;
+  if (idx >= n_events * n_module_pairs) return;
+  const unsigned event_number = idx / n_module_pairs;
+  const unsigned m = idx % n_module_pairs;
+  const RawEvent ev(raw, offsets, sizes, types, event_number);
+  mp::Banks& B = s_banks[wave];
+  mp::find_banks<version>(ev, m, B);
+  const unsigned n_words = B.start[B.n];
+  const bool resident = n_words <= mp::word_cap;
+  uint32_t* lw = s_words[wave];
+  if (resident) mp::load_words(B, 0, n_words, lw);
+  uint32_t count = 0;
+  for (unsigned base = 0; base < n_words; base += wave_size) {
+    const unsigned w = base + lane_id();
+    const bool active = w < n_words;
+    const unsigned bi = active ? mp::bank_of(B, w) : 0;
+    const uint32_t sp_word = active ? (resident ? lw[w] : B.word[bi][w - B.start[bi]]) : 0u;
+    const bool isolated = (sp_word & 0x80000000u) != 0;
+    const uint32_t sp_addr = (sp_word & 0x007FFF00u) >> 8;
+    const uint32_t sp_row = sp_addr & 0x3Fu, sp_col = sp_addr >> 6;
+    uint32_t pixels = neighbourhood_init(sp_word);
+    mp::for_words(B, n_words, resident, lw, active && !isolated, B.start[bi], B.start[bi + 1],
+                  [&](uint32_t other) { neighbour_acc(sp_row, sp_col, other, pixels); });
+    if (active) {
+      if (isolated) {
+        count += isolated_cluster_count(sp_word & 0xFFu);
+      }
+      else {
+        unsigned p0 = 0, p1 = 0;
+        const unsigned mk = candidates_final(pixels, &p0, &p1);
+        count += __popc(mk);
+        const uint32_t* wp = B.word[bi] + (w - B.start[bi]);
+        sp_flags[static_cast<size_t>(reinterpret_cast<const char*>(wp) - raw) / 4] =
+          static_cast<uint8_t>(mk | (p0 << 2) | (p1 << 5));
+      }
+    }
+  }
+#pragma unroll
+  for (unsigned d = 32; d > 0; d >>= 1) count += __shfl_xor(count, d, wave_size);
+  if (lane_id() == 0) estimated_input_size[event_number * n_module_pairs + m] = count;
+}
+
+template<int version>
+__global__ void __launch_bounds__(mp::waves_per_block* wave_size) k_cluster_mp(
+  const char* raw,
+  const uint32_t* offsets,
+  const uint32_t* sizes,
+  const uint32_t* types,
+  unsigned n_events,
+  const uint32_t* offsets_estimated_input_size,
+  const uint8_t* sp_flags,
+  const Geometry* geometry,
+  const uint8_t* sp_patterns,
+  const float* sp_fx,
+  const float* sp_fy,
+  uint32_t* module_cluster_num,
+  char* sorted_container,
+  char* unsorted_scratch)
+{
+  __shared__ uint32_t s_words[mp::waves_per_block][mp::word_cap];
+  __shared__ int16_t s_phi[mp::waves_per_block][mp::cluster_cap];
+  __shared__ uint32_t s_id[mp::waves_per_block][mp::cluster_cap];
+  __shared__ float s_xyz[mp::waves_per_block][3 * mp::cluster_cap];
+  __shared__ mp::Banks s_banks[mp::waves_per_block];
+  constexpr bool v23 = (version == 2 || version == 3);
+  const unsigned wave = threadIdx.x / wave_size;
+  const unsigned idx = blockIdx.x * mp::waves_per_block + wave;
+  if (idx >= n_events * n_module_pairs) return;
+  const unsigned event_number = idx / n_module_pairs;
+  const unsigned m = idx % n_module_pairs;
+  const unsigned total = offsets_estimated_input_size[n_module_pairs * n_events];
+  const unsigned start = offsets_estimated_input_size[idx];
+  const unsigned estimate = offsets_estimated_input_size[idx + 1] - start;
+  const Geometry& g = *geometry;
+  const RawEvent ev(raw, offsets, sizes, types, event_number);
+  mp::Banks& B = s_banks[wave];
+  mp::find_banks<version>(ev, m, B);
+  const unsigned n_words = B.start[B.n];
+  const bool resident = n_words <= mp::word_cap;
+  const bool clusters_in_lds = estimate <= mp::cluster_cap;
+  uint32_t* lw = s_words[wave];
+  if (resident) mp::load_words(B, 0, n_words, lw);
+  const Clusters scratch {unsorted_scratch, total};
+  int16_t* lphi = s_phi[wave];
+  uint32_t* lid = s_id[wave];
+  float* lxyz = s_xyz[wave];
+  unsigned n = 0; // clusters so far (wave-uniform)
+  // Appends up to two clusters per lane in lane order (wave scan of the per-lane counts).
+  auto emit = [&](unsigned k, const ClusterOut& c0, const ClusterOut& c1) {
+    const unsigned inc = wave_inclusive_scan(k);
+    const unsigned slot0 = n + inc - k;
+    for (unsigned j = 0; j < k; ++j) {
+      const unsigned slot = slot0 + j;
+      const ClusterOut& c = j == 0 ? c0 : c1;
+      if (slot < estimate) {
+        if (clusters_in_lds) {
+          lphi[slot] = c.phi;
+          lid[slot] = c.id;
+          lxyz[3 * slot] = c.x;
+          lxyz[3 * slot + 1] = c.y;
+          lxyz[3 * slot + 2] = c.z;
+        }
+        else {
+          store_cluster(scratch, start + slot, c);
+        }
+      }
+    }
+    n += __shfl(inc, wave_size - 1, wave_size);
+  };
+  // Isolated SPs first, then the candidates, each in SP order: the CPU target's order
+  // (MaskedVeloClustering.cu:544-575).
+  for (unsigned base = 0; base < n_words; base += wave_size) {
+    const unsigned w = base + lane_id();
+    ClusterOut c0, c1;
+    unsigned k = 0;
+    if (w < n_words) {
+      const unsigned bi = mp::bank_of(B, w);
+      const uint32_t sp_word = resident ? lw[w] : B.word[bi][w - B.start[bi]];
+      if (sp_word & 0x80000000u) {
+        RawBank bank;
+        bank.sourceID = B.source[bi];
+        k = isolated_sp_clusters<version>(g, sp_patterns, sp_fx, sp_fy, bank, sp_word, c0, c1);
+      }
+    }
+    emit(k, c0, c1);
+  }
+  for (unsigned base = 0; base < n_words; base += wave_size) {
+    const unsigned w = base + lane_id();
+    const bool active = w < n_words;
+    const unsigned bi = active ? mp::bank_of(B, w) : 0;
+    const uint32_t sp_word = active ? (resident ? lw[w] : B.word[bi][w - B.start[bi]]) : 0x80000000u;
+    uint32_t f = 0;
+    if (active && !(sp_word & 0x80000000u)) {
+      const uint32_t* wp = B.word[bi] + (w - B.start[bi]);
+      f = sp_flags[static_cast<size_t>(reinterpret_cast<const char*>(wp) - raw) / 4];
+    }
+    const uint32_t pair = B.source[bi] & 0x1FFu;
+    const uint32_t sensor_number = v23 ? pair : ((pair << 1) | ((sp_word >> 23) & 0x1u));
+    const uint32_t sp_addr = (sp_word & 0x007FFF00u) >> 8;
+    const int32_t sp_row_lower = static_cast<int32_t>(sp_addr & 0x3Fu) - 2;
+    const int32_t sp_col_lower = static_cast<int32_t>(sp_addr >> 6) - 3;
+    Window win {0, 0, 0, 0, 0};
+    // Both candidates of an SP share its window.
+    mp::for_words(B, n_words, resident, lw, (f & 3) != 0, B.start[bi], B.start[bi + 1], [&](uint32_t other) {
+      window_acc<version>(sp_row_lower, sp_col_lower, sensor_number, other, win);
+    });
+    ClusterOut c0, c1, t;
+    unsigned k = 0;
+    if ((f & 1) && masked_cluster_final(g, sp_word, sensor_number, (f >> 2) & 7u, win, &t)) {
+      c0 = t;
+      k = 1;
+    }
+    if ((f & 2) && masked_cluster_final(g, sp_word, sensor_number, (f >> 5) & 7u, win, &t)) {
+      if (k == 0) c0 = t;
+      else c1 = t;
+      ++k;
+    }
+    emit(k, c0, c1);
+  }
+  n = min(n, estimate);
+  mp::wave_sync_lds();
+  if (lane_id() == 0) module_cluster_num[idx] = n;
+  const Clusters out {sorted_container, total};
+  if (clusters_in_lds) {
+    for (unsigned i = lane_id(); i < n; i += wave_size) {
+      ClusterOut c;
+      c.id = lid[i];
+      c.phi = lphi[i];
+      c.x = lxyz[3 * i];
+      c.y = lxyz[3 * i + 1];
+      c.z = lxyz[3 * i + 2];
+      store_cluster(out, start + phi_rank(lphi, lid, n, i), c);
+    }
+    return;
+  }
+  // Too many clusters for the slice: they sit in the scratch container; rank them against
+  // tiles of (phi, id) staged through the slice.
+  __threadfence_block();
+  for (unsigned i0 = 0; i0 < n; i0 += wave_size) {
+    const unsigned i = i0 + lane_id();
+    const bool mine = i < n;
+    ClusterOut c;
+    if (mine) {
+      c.id = scratch.ids()[start + i];
+      c.phi = scratch.phis()[start + i];
+      c.x = scratch.xyz()[3 * (size_t)(start + i)];
+      c.y = scratch.xyz()[3 * (size_t)(start + i) + 1];
+      c.z = scratch.xyz()[3 * (size_t)(start + i) + 2];
+    }
+    unsigned rank = 0;
+    for (unsigned t0 = 0; t0 < n; t0 += mp::cluster_cap) {
+      const unsigned tn = min(mp::cluster_cap, n - t0);
+      mp::wave_sync_lds();
+      for (unsigned j = lane_id(); j < tn; j += wave_size) {
+        lphi[j] = scratch.phis()[start + t0 + j];
+        lid[j] = scratch.ids()[start + t0 + j];
+      }
+      mp::wave_sync_lds();
+      if (mine) {
+        for (unsigned j = 0; j < tn; ++j) {
+          const int16_t pj = lphi[j];
+          rank += c.phi > pj || (i != t0 + j && c.phi == pj && c.id > lid[j]);
+        }
+      }
+    }
+    if (mine) store_cluster(out, start + rank, c);
+  }
+}
+
+hipError_t launch_estimate_mp(const EstimateMpLaunch& a, hipStream_t s)
+{
+  if (!a.n_events) return hipSuccess;
+  const unsigned grid = (a.n_events * n_module_pairs + mp::waves_per_block - 1) / mp::waves_per_block;
+#define VELO_LAUNCH_EMP(V)                                                                                          \
+  hipLaunchKernelGGL(                                                                                               \
+    k_estimate_mp<V>, dim3(grid), dim3(mp::waves_per_block * wave_size), 0, s, a.raw, a.offsets, a.sizes, a.types,  \
+    a.n_events, a.estimated_input_size, a.sp_flags)
+  switch (a.version) {
+  case 2: VELO_LAUNCH_EMP(2); break;
+  case 3: VELO_LAUNCH_EMP(3); break;
+  default: VELO_LAUNCH_EMP(4); break;
+  }
+#undef VELO_LAUNCH_EMP
+  return hipGetLastError();
+}
+
+hipError_t launch_cluster_mp(const ClusterMpLaunch& a, hipStream_t s)
+{
+  if (!a.n_events) return hipSuccess;
+  const unsigned grid = (a.n_events * n_module_pairs + mp::waves_per_block - 1) / mp::waves_per_block;
+#define VELO_LAUNCH_CMP(V)                                                                                          \
+  hipLaunchKernelGGL(                                                                                               \
+    k_cluster_mp<V>, dim3(grid), dim3(mp::waves_per_block * wave_size), 0, s, a.raw, a.offsets, a.sizes, a.types,   \
+    a.n_events, a.offsets_estimated_input_size, a.sp_flags, reinterpret_cast<const Geometry*>(a.geometry),           \
+    a.sp_patterns, a.sp_fx, a.sp_fy, a.module_cluster_num, a.sorted_container, a.unsorted_scratch)
+  switch (a.version) {
+  case 2: VELO_LAUNCH_CMP(2); break;
+  case 3: VELO_LAUNCH_CMP(3); break;
+  default: VELO_LAUNCH_CMP(4); break;
+  }
+#undef VELO_LAUNCH_CMP
   return hipGetLastError();
 }
 

@@ -88,6 +88,35 @@ struct FusedClusterLaunch {
   uint32_t* status;
 };
 
+struct EstimateMpLaunch {
+  int version;
+  const char* raw;
+  const uint32_t* offsets;
+  const uint32_t* sizes;
+  const uint32_t* types;
+  unsigned n_events;
+  uint32_t* estimated_input_size;
+  uint8_t* sp_flags; // one byte per 32-bit word of the raw buffer
+};
+
+struct ClusterMpLaunch {
+  int version;
+  const char* raw;
+  const uint32_t* offsets;
+  const uint32_t* sizes;
+  const uint32_t* types;
+  unsigned n_events;
+  const uint32_t* offsets_estimated_input_size;
+  const uint8_t* sp_flags;
+  const void* geometry;
+  const uint8_t* sp_patterns;
+  const float* sp_fx;
+  const float* sp_fy;
+  uint32_t* module_cluster_num;
+  char* sorted_container;
+  char* unsorted_scratch;
+};
+
 struct SearchLaunch {
   const uint32_t* event_list;
   unsigned n_list;
@@ -108,7 +137,8 @@ struct SearchLaunch {
   float phi_tolerance;
   float max_scatter;
   unsigned max_skipped_modules;
-  unsigned block_dim;
+  unsigned block_dim;          // 0 = 256
+  unsigned lds_hits;           // event hits staged in LDS (15 B each); 0 = 2048, at most 4096
   unsigned long long* work_stats; // optional: seeding evals, forwarding evals, extrapolations, search steps
 };
 
@@ -169,6 +199,8 @@ hipError_t launch_estimate_input_size(const EstimateLaunch& a, hipStream_t s);
 hipError_t launch_masked_clustering(const MaskedClusteringLaunch& a, hipStream_t s);
 hipError_t launch_sort_by_phi(const SortLaunch& a, hipStream_t s);
 hipError_t launch_cluster_sorted(const FusedClusterLaunch& a, hipStream_t s);
+hipError_t launch_estimate_mp(const EstimateMpLaunch& a, hipStream_t s);
+hipError_t launch_cluster_mp(const ClusterMpLaunch& a, hipStream_t s);
 hipError_t launch_search_by_triplet(const SearchLaunch& a, hipStream_t s);
 hipError_t launch_three_hit_tracks_filter(const ThreeHitFilterLaunch& a, hipStream_t s);
 hipError_t launch_copy_track_hit_number(const CopyHitNumberLaunch& a, hipStream_t s);

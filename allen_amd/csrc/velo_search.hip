@@ -24,6 +24,19 @@ struct Scan3 {
 };
 __device__ __forceinline__ Scan3 block_flag_scan3(bool f0, bool f1, bool f2, uint32_t* scratch)
 {
+  if (blockDim.x == wave_size) {
+    // One wave: ranks straight from the ballots.
+    const uint64_t below = lanes_below_mask();
+    const uint64_t b0 = __ballot(f0), b1 = __ballot(f1), b2 = __ballot(f2);
+    Scan3 s;
+    s.r[0] = __popcll(b0 & below);
+    s.r[1] = __popcll(b1 & below);
+    s.r[2] = __popcll(b2 & below);
+    s.t[0] = __popcll(b0);
+    s.t[1] = __popcll(b1);
+    s.t[2] = __popcll(b2);
+    return s;
+  }
   const unsigned wave = threadIdx.x / wave_size;
   const unsigned n_waves = blockDim.x / wave_size;
   const uint64_t below = lanes_below_mask();
@@ -106,6 +119,19 @@ __device__ __forceinline__ int counted_search(const int16_t* array, unsigned siz
   return l;
 }
 
+// Hits of one event: phi and used flags in LDS, coordinates in global memory (events too large
+// for the full LDS layout).
+struct MixedHits {
+  const float* xyz;
+  const int16_t* ph;
+  uint8_t* used;
+  __device__ __forceinline__ float x(unsigned i) const { return xyz[3 * i]; }
+  __device__ __forceinline__ float y(unsigned i) const { return xyz[3 * i + 1]; }
+  __device__ __forceinline__ float z(unsigned i) const { return xyz[3 * i + 2]; }
+  __device__ __forceinline__ int16_t phi(unsigned i) const { return ph[i]; }
+  __device__ __forceinline__ const int16_t* phi_begin() const { return ph; }
+};
+
 struct SearchShared {
   ModulePairData mpd[3];
   uint32_t ttf_counter;   // atomics::tracks_to_follow
@@ -114,6 +140,15 @@ struct SearchShared {
   uint32_t n_tracks;      // dev_number_of_velo_tracks[event]
   uint32_t status;
   uint32_t scan[3 * 16 + 4];
+};
+
+// Seeding scratch for one chunk of h1 hits, blockDim entries each, in dynamic LDS.
+struct SeedScratch {
+  float* pair_scatter;
+  uint16_t* h1;
+  uint16_t* h0;
+  uint16_t* pair_h2;
+  uint8_t* n_h0;
 };
 
 struct EventOutputs {
@@ -152,11 +187,19 @@ __device__ __forceinline__ void find_forward_candidate(
   extrapolated_phi = phi;
 }
 
-// SearchByTriplet.cu:298-446, ordered.
+// SearchByTriplet.cu:298-446, ordered, with the (h1, h0) pairs spread over lanes.
+//
+// The reference walks, per h1, its h0 candidates in pendulum order and, per h0, the h2 window
+// in phi order, keeping the first strictly smaller scatter. Here a chunk of C = blockDim / K
+// h1 hits is processed at once (K = h0 candidates considered): lanes first collect the h0
+// candidates of one h1 each, then every lane scores one (h1, h0) pair over its h2 window
+// (first strict minimum), and finally one lane per h1 keeps the first strict minimum over its
+// h0 candidates in pendulum order: the same pair the sequential double loop selects.
 template<bool count, typename H>
 __device__ void track_seeding(
   WorkCount& wc,
   const H& hits,
+  const SeedScratch& ss,
   SearchShared& sh,
   const EventOutputs& o,
   float max_scatter,
@@ -172,46 +215,51 @@ __device__ void track_seeding(
   for (unsigned c = 0; c < cur.hit_num; c += blockDim.x) {
     const unsigned r = c + threadIdx.x;
     const bool act = r < cur.hit_num && !hits.used[cur.hit_start + r];
-    uint32_t total;
-    const uint32_t pos = block_flag_scan(act, sh.scan, &total);
-    if (act) o.h1_indices[base + pos] = cur.hit_start + r;
-    base += total;
+    const Scan3 sc = block_flag_scan3(act, false, false, sh.scan);
+    if (act) o.h1_indices[base + sc.r[0]] = cur.hit_start + r;
+    base += sc.t[0];
   }
   if (threadIdx.x == 0) sh.local_hits = base;
   __threadfence_block();
   __syncthreads();
   const unsigned n_h1 = base;
+  const unsigned K = h0_candidates_to_consider;
+  const unsigned C = blockDim.x / K;
+  const float dz_next = next.z[0] - prev.z[0];
 
-  for (unsigned c = 0; c < n_h1; c += blockDim.x) {
-    const unsigned rel = c + threadIdx.x;
-    bool found = false;
-    uint16_t best_h0 = 0, best_h2 = 0, h1_index = 0;
-    if (rel < n_h1) {
-      h1_index = o.h1_indices[rel];
-      float best_fit = max_scatter;
-      const float h1x = hits.x(h1_index), h1y = hits.y(h1_index), h1z = hits.z(h1_index);
-      const int16_t h1_phi = hits.phi(h1_index);
-      unsigned best_h0s[max_h0_candidates];
-      int phi_index = counted_search<count>(hits.phi_begin() + prev.hit_start, prev.hit_num, h1_phi, wc);
+  for (unsigned c = 0; c < n_h1; c += C) {
+    // 1. h0 candidates of h1 number c + t (lanes t < C), pendulum order.
+    if (threadIdx.x < C) {
+      const unsigned rel = c + threadIdx.x;
       unsigned n_found = 0;
-      for (unsigned i = 0; i < prev.hit_num && n_found < h0_candidates_to_consider; ++i) {
-        const int index_diff = (i & 1u) ? static_cast<int>(i) : -static_cast<int>(i);
-        phi_index += index_diff;
-        const int n = static_cast<int>(prev.hit_num);
-        const int in_bounds = phi_index < 0 ? phi_index + n : (phi_index >= n ? phi_index - n : phi_index);
-        const unsigned h0_index = prev.hit_start + in_bounds;
-        if (!hits.used[h0_index]) {
-#pragma unroll
-          for (unsigned k = 0; k < max_h0_candidates; ++k)
-            if (k == n_found) best_h0s[k] = h0_index;
-          ++n_found;
+      if (rel < n_h1) {
+        const unsigned h1_index = o.h1_indices[rel];
+        const int16_t h1_phi = hits.phi(h1_index);
+        int phi_index = counted_search<count>(hits.phi_begin() + prev.hit_start, prev.hit_num, h1_phi, wc);
+        for (unsigned i = 0; i < prev.hit_num && n_found < K; ++i) {
+          const int index_diff = (i & 1u) ? static_cast<int>(i) : -static_cast<int>(i);
+          phi_index += index_diff;
+          const int n = static_cast<int>(prev.hit_num);
+          const int in_bounds = phi_index < 0 ? phi_index + n : (phi_index >= n ? phi_index - n : phi_index);
+          const unsigned h0_index = prev.hit_start + in_bounds;
+          if (!hits.used[h0_index]) ss.h0[threadIdx.x * K + n_found++] = h0_index;
         }
+        ss.h1[threadIdx.x] = h1_index;
       }
-      for (unsigned i = 0; i < n_found; ++i) {
-        unsigned h0_index = 0;
-#pragma unroll
-        for (unsigned k = 0; k < max_h0_candidates; ++k)
-          if (k == i) h0_index = best_h0s[k];
+      ss.n_h0[threadIdx.x] = n_found;
+    }
+    __syncthreads();
+
+    // 2. One (h1, h0) pair per lane: best h2 of this h0 (first strict minimum).
+    {
+      const unsigned t = threadIdx.x / K;
+      const unsigned k = threadIdx.x % K;
+      float best = max_scatter;
+      uint16_t best_h2 = 0;
+      if (t < C && c + t < n_h1 && k < ss.n_h0[t]) {
+        const unsigned h1_index = ss.h1[t];
+        const unsigned h0_index = ss.h0[t * K + k];
+        const float h1x = hits.x(h1_index), h1y = hits.y(h1_index), h1z = hits.z(h1_index);
         const float h0x = hits.x(h0_index), h0y = hits.y(h0_index), h0z = hits.z(h0_index);
         const float td = 1.0f / (h1z - h0z);
         const float txn = h1x - h0x;
@@ -219,8 +267,7 @@ __device__ void track_seeding(
         const float tx = txn * td;
         const float ty = tyn * td;
         int16_t cand, ephi;
-        find_forward_candidate<count>(
-          wc, hits, next, h0x, h0y, tx, ty, next.z[0] - prev.z[0], phi_tolerance, cand, ephi);
+        find_forward_candidate<count>(wc, hits, next, h0x, h0y, tx, ty, dz_next, phi_tolerance, cand, ephi);
         for (unsigned j = 0; j < next.hit_num; ++j) {
           const unsigned in_bounds = (static_cast<unsigned>(static_cast<int>(cand)) + j) % next.hit_num;
           const unsigned h2_index = next.hit_start + in_bounds;
@@ -236,14 +283,33 @@ __device__ void track_seeding(
             const float dx = predx - h2x;
             const float dy = predy - h2y;
             const float scatter = dx * dx + dy * dy;
-            if (scatter < best_fit) {
-              best_fit = scatter;
-              best_h0 = h0_index;
+            if (scatter < best) {
+              best = scatter;
               best_h2 = h2_index;
             }
           }
         }
       }
+      ss.pair_scatter[threadIdx.x] = best;
+      ss.pair_h2[threadIdx.x] = best_h2;
+    }
+    __syncthreads();
+
+    // 3. One lane per h1: first strict minimum over its h0 candidates, then ordered seeds.
+    bool found = false;
+    uint16_t best_h0 = 0, best_h2 = 0, h1_index = 0;
+    if (threadIdx.x < C && c + threadIdx.x < n_h1) {
+      const unsigned t = threadIdx.x;
+      float best_fit = max_scatter;
+      for (unsigned k = 0; k < ss.n_h0[t]; ++k) {
+        const float sc = ss.pair_scatter[t * K + k];
+        if (sc < best_fit) {
+          best_fit = sc;
+          best_h0 = ss.h0[t * K + k];
+          best_h2 = ss.pair_h2[t * K + k];
+        }
+      }
+      h1_index = ss.h1[t];
       found = best_fit < max_scatter;
     }
     const Scan3 s = block_flag_scan3(found, false, false, sh.scan);
@@ -407,6 +473,7 @@ template<bool count, typename H>
 __device__ void search_event(
   WorkCount& wc,
   const H& hits,
+  const SeedScratch& ss,
   SearchShared& sh,
   const EventOutputs& o,
   const uint32_t* module_hit_start,
@@ -430,7 +497,7 @@ __device__ void search_event(
   const int16_t phi_tolerance = f32_to_i16(phi_tolerance_f * convert_factor_i16);
   unsigned first_module_pair = n_module_pairs - 1;
   load(first_module_pair);
-  track_seeding<count>(wc, hits, sh, o, max_scatter, phi_tolerance, initial_seeding_h0_candidates);
+  track_seeding<count>(wc, hits, ss, sh, o, max_scatter, phi_tolerance, initial_seeding_h0_candidates);
   unsigned last_ttf = 0;
   --first_module_pair;
   while (first_module_pair > 1) {
@@ -442,7 +509,7 @@ __device__ void search_event(
     track_forwarding<count>(wc, hits, sh, o, prev_ttf, diff_ttf, phi_tolerance, max_scatter, max_skipped_modules);
     __threadfence_block();
     __syncthreads();
-    track_seeding<count>(wc, hits, sh, o, max_scatter, phi_tolerance, seeding_h0_candidates);
+    track_seeding<count>(wc, hits, ss, sh, o, max_scatter, phi_tolerance, seeding_h0_candidates);
     --first_module_pair;
   }
   __syncthreads();
@@ -498,8 +565,16 @@ __global__ void k_search_by_triplet(
   unsigned lds_hits,
   unsigned long long* work_stats)
 {
-  extern __shared__ __align__(16) char dyn[];
+  extern __shared__ __align__(16) char dyn_all[];
   WorkCount wc;
+  // Dynamic LDS: seeding scratch (11 B per lane), then the event hits.
+  SeedScratch ss;
+  ss.pair_scatter = reinterpret_cast<float*>(dyn_all);
+  ss.h1 = reinterpret_cast<uint16_t*>(ss.pair_scatter + blockDim.x);
+  ss.h0 = ss.h1 + blockDim.x;
+  ss.pair_h2 = ss.h0 + blockDim.x;
+  ss.n_h0 = reinterpret_cast<uint8_t*>(ss.pair_h2 + blockDim.x);
+  char* dyn = dyn_all + ((11 * blockDim.x + 15) & ~15u);
   __shared__ SearchShared sh;
   const unsigned event_number = event_list ? event_list[blockIdx.x] : blockIdx.x;
   const unsigned n_events = number_of_events[0];
@@ -550,7 +625,23 @@ __global__ void k_search_by_triplet(
     }
     __syncthreads();
     const LdsHits hits {xs, ys, zs, ph, used};
-    search_event<count>(wc, hits, sh, o, mhs, mhn, g, phi_tolerance, max_scatter, max_skipped_modules);
+    search_event<count>(wc, hits, ss, sh, o, mhs, mhn, g, phi_tolerance, max_scatter, max_skipped_modules);
+    __syncthreads();
+    uint8_t* gused = hit_used_all + hit_offset;
+    for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) gused[i] = used[i];
+  }
+  else if (3 * n_hits <= 15 * lds_hits) {
+    int16_t* ph = reinterpret_cast<int16_t*>(dyn);
+    uint8_t* used = reinterpret_cast<uint8_t*>(ph + n_hits);
+    const int16_t* gph = cc.phis() + hit_offset;
+    for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) {
+      ph[i] = gph[i];
+      used[i] = 0;
+    }
+    if (threadIdx.x == 0) sh.status |= status_lds_fallback;
+    __syncthreads();
+    const MixedHits hits {cc.xyz() + 3 * (size_t) hit_offset, ph, used};
+    search_event<count>(wc, hits, ss, sh, o, mhs, mhn, g, phi_tolerance, max_scatter, max_skipped_modules);
     __syncthreads();
     uint8_t* gused = hit_used_all + hit_offset;
     for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) gused[i] = used[i];
@@ -562,7 +653,7 @@ __global__ void k_search_by_triplet(
     __threadfence_block();
     __syncthreads();
     const GlobalHits hits {cc.xyz() + 3 * (size_t) hit_offset, cc.phis() + hit_offset, gused};
-    search_event<count>(wc, hits, sh, o, mhs, mhn, g, phi_tolerance, max_scatter, max_skipped_modules);
+    search_event<count>(wc, hits, ss, sh, o, mhs, mhn, g, phi_tolerance, max_scatter, max_skipped_modules);
   }
   if constexpr (count) {
     // Wave reduction, then one atomic per wave.
@@ -692,8 +783,11 @@ hipError_t launch_search_by_triplet(const SearchLaunch& a, hipStream_t s)
 {
   if (!a.n_list) return hipSuccess;
   const unsigned block = a.block_dim ? a.block_dim : 256;
-  const unsigned lds_hits = 4096;
-  const size_t dyn = (size_t) lds_hits * search_bytes_per_hit;
+  // Dynamic LDS stays within 64 KiB per workgroup (with the static part).
+  const size_t seed_bytes = (11 * block + 15) & ~15u;
+  const unsigned max_hits = static_cast<unsigned>((65536 - 1024 - seed_bytes) / search_bytes_per_hit);
+  const unsigned lds_hits = min(a.lds_hits ? a.lds_hits : 2048u, max_hits);
+  const size_t dyn = seed_bytes + (size_t) lds_hits * search_bytes_per_hit;
 #define VELO_LAUNCH_SEARCH(C)                                                                                       \
   hipLaunchKernelGGL(                                                                                               \
     k_search_by_triplet<C>, dim3(a.n_list), dim3(block), dyn, s, a.event_list, a.number_of_events,                  \

@@ -60,6 +60,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores (capped at 16)")
     ap.add_argument("--search-block", type=int, default=0)
+    ap.add_argument("--search-lds-hits", type=int, default=0)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -81,6 +82,8 @@ def main():
     seq = velo.VeloSequence(constants, ev.n_events, ev.raw.size)
     if args.search_block:
         seq.set_tuning("search_block", args.search_block)
+    if args.search_lds_hits:
+        seq.set_tuning("search_lds_hits", args.search_lds_hits)
     stream = torch.cuda.current_stream()
 
     for _ in range(args.warmup):

@@ -181,7 +181,7 @@ typedef struct {
   float phi_tolerance;              /* 0.045 */
   float max_scatter;                /* 0.08 */
   unsigned max_skipped_modules;     /* 1 */
-  unsigned block_dim_x;             /* 0 = library default */
+  unsigned block_dim_x;             /* 0 = library default (256) */
 } velo_search_by_triplet_params;
 int velo_search_by_triplet(const velo_search_by_triplet_params* p, hipStream_t stream);
 /* track_container_size of SearchByTriplet.cu:18-28 */
@@ -282,6 +282,8 @@ int velo_sequence_buffer(velo_sequence* s, const char* name, void** dev_ptr, siz
 int velo_sequence_status(velo_sequence* s, hipStream_t stream, unsigned* status);
 /* Knobs applied to subsequent runs:
  *   "search_block"      workgroup size of search_by_triplet (multiple of 64, <= 1024; 0 = 256)
+ *   "search_lds_hits"   hits of an event staged in LDS by search_by_triplet (<= 4096; 0 = 2048);
+ *                       larger events run on global memory
  *   "fused_clustering"  1: clustering and phi sort fused per module pair (default); 0: the four
  *                       reference algorithms one after the other
  *   "timed_stage"       VELO_STAGE_* to bracket with HIP events on the run's stream (0 = none)

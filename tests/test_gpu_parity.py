@@ -27,13 +27,16 @@ def constants(geometry, beamline):
     c.close()
 
 
-def run_both(name, geometry, beamline, constants, fused):
+def run_both(name, geometry, beamline, constants, fused, tuning=None):
     ev = synth.generate(**DATASETS[name])
-    print(f"[parity] dataset={name} events={ev.n_events} raw_bytes={ev.raw.size} fused={fused}", flush=True)
+    print(f"[parity] dataset={name} events={ev.n_events} raw_bytes={ev.raw.size} fused={fused} tuning={tuning}",
+          flush=True)
     ref = oracle.run(ev, geometry, beamline)
     dev = velo.DeviceEvents(ev)
     seq = velo.VeloSequence(constants, ev.n_events, max(ev.raw.size, 4096))
     seq.set_tuning("fused_clustering", int(fused))
+    for k, v in (tuning or {}).items():
+        seq.set_tuning(k, v)
     seq.run(dev)
     status = seq.status()
     gpu = parity.gpu_buffers(seq, ref)
@@ -51,6 +54,38 @@ def test_sequence_matches_oracle(name, fused, geometry, beamline, constants):
     assert not errs, "\n".join(errs[:20])
     if name != "empty_events":
         assert ref["offsets_all_velo_tracks"][-1] > 0
+
+
+TUNINGS = {
+    "block64": {"search_block": 64},
+    "block1024": {"search_block": 1024, "search_lds_hits": 2048},
+    "mixed_lds": {"search_block": 256, "search_lds_hits": 600},   # phi + used in LDS, xyz global
+    "global_hits": {"search_block": 128, "search_lds_hits": 64},  # everything in global memory
+}
+
+
+@pytest.mark.parametrize("tuning", list(TUNINGS))
+def test_sequence_tuning_variants(tuning, geometry, beamline, constants):
+    """Launch geometries and LDS layouts do not change a single bit of the result."""
+    ref, gpu, status = run_both("pileup", geometry, beamline, constants, True, TUNINGS[tuning])
+    assert status & velo.STATUS_OVERFLOW_MASK == 0
+    errs = parity.compare(gpu, ref, True)
+    assert not errs, "\n".join(errs[:20])
+
+
+def test_dense_module_pairs_take_the_tiled_paths(geometry, beamline, constants):
+    """Events dense enough that module pairs exceed the per-wave LDS slices of clustering."""
+    DATASETS["dense"] = dict(n_events=4, pileup=1, nu=60.0, noise_fraction=0.3, seed=17)
+    ref, gpu, status = run_both("dense", geometry, beamline, constants, True)
+    est = ref["offsets_estimated_input_size"]
+    assert np.diff(est).max() > 320  # some module pair is above the cluster slice
+    if int(ref["status"][0]) == 0:
+        errs = parity.compare(gpu, ref, True)
+        assert not errs, "\n".join(errs[:20])
+    else:  # capacity overflow in the search: compare the clustering stages only
+        errs = [e for e in parity.compare(gpu, ref, True) if e.split(":")[0].startswith(("sorted", "module_cluster",
+                                                                                          "estimated", "offsets_est"))]
+        assert not errs, "\n".join(errs[:20])
 
 
 def test_prefix_sum_iota(constants):
