@@ -115,6 +115,16 @@ extern "C" {
 
 int velo_last_hip_error(void) { return g_last_hip_error; }
 
+int velo_wall_clock_khz(unsigned* khz)
+{
+  if (!khz) return VELO_ERR_INVALID_ARGUMENT;
+  int dev = 0, rate = 0;
+  int rc = hip_status(hipGetDevice(&dev));
+  if (rc == VELO_OK) rc = hip_status(hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, dev));
+  *khz = rc == VELO_OK ? static_cast<unsigned>(rate) : 0u;
+  return rc;
+}
+
 const char* velo_error_string(int code)
 {
   switch (code) {
@@ -437,7 +447,8 @@ struct velo_sequence {
   size_t sp_cap = 0, cluster_cap = 0, track_cap = 0, hits_cap = 0;
   std::map<std::string, std::pair<void*, size_t>> bufs;
   unsigned search_block = 256;
-  unsigned search_lds_hits = 2048;
+  unsigned search_lds_hits = velo::search_default_lds_hits;
+  bool search_largest_first = true;
   bool fused_clustering = true;
   bool count_work = false;
   // HIP events bracketing one stage of the run (0 = none), see velo_sequence_stage_ms.
@@ -506,7 +517,11 @@ int velo_sequence_create(const velo_constants* c, unsigned max_events, size_t ma
     {"velo_kalman_endvelo_states", 56 * 2 * s->track_cap},
     {"scan_scratch", 4 * scan_words},
     {"search_work", 4 * sizeof(unsigned long long)},
+    {"search_event_clock", velo::event_clock_words * sizeof(unsigned long long) * E},
+    {"search_order", 4 * E},
     {"sp_flags", s->sp_cap},
+    {"bank_directory", 16 * 8 * 26 * E},
+    {"bank_directory_count", 4 * 26 * E},
   };
   for (const auto& kv : layout) {
     void* p = nullptr;
@@ -564,8 +579,12 @@ int velo_sequence_set_tuning(velo_sequence* s, const char* knob, unsigned value)
     return VELO_OK;
   }
   if (k == "search_lds_hits") {
-    if (value > 4096) return VELO_ERR_INVALID_ARGUMENT;
-    s->search_lds_hits = value ? value : 2048;
+    if (value > 8192) return VELO_ERR_INVALID_ARGUMENT;
+    s->search_lds_hits = value ? value : velo::search_default_lds_hits;
+    return VELO_OK;
+  }
+  if (k == "search_largest_first") {
+    s->search_largest_first = value != 0;
     return VELO_OK;
   }
   if (k == "fused_clustering") {
@@ -643,6 +662,10 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream
   k1.block_dim_x = 256;
   k1.number_of_candidates = s->get<uint32_t>("number_of_candidates");
   k1.bank_index = s->get<uint32_t>("velo_bank_index");
+  if (s->fused_clustering) {
+    k1.directory = s->get<void>("bank_directory");
+    k1.directory_count = s->get<uint32_t>("bank_directory_count");
+  }
   VELO_MARK(mark_begin(VELO_STAGE_CANDIDATES));
   VELO_TRY(velo::launch_number_of_candidates(k1, stream));
   VELO_MARK(mark_end(VELO_STAGE_CANDIDATES));
@@ -667,9 +690,8 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream
     velo::EstimateMpLaunch e;
     e.version = in->bank_version;
     e.raw = k1.raw;
-    e.offsets = k1.offsets;
-    e.sizes = k1.sizes;
-    e.types = k1.types;
+    e.directory = k1.directory;
+    e.directory_count = k1.directory_count;
     e.n_events = E;
     e.estimated_input_size = k2.estimated_input_size;
     e.sp_flags = sp_flags;
@@ -690,9 +712,8 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream
     velo::ClusterMpLaunch f;
     f.version = in->bank_version;
     f.raw = k1.raw;
-    f.offsets = k1.offsets;
-    f.sizes = k1.sizes;
-    f.types = k1.types;
+    f.directory = k1.directory;
+    f.directory_count = k1.directory_count;
     f.n_events = E;
     f.offsets_estimated_input_size = est_off;
     f.sp_flags = sp_flags;
@@ -764,8 +785,13 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream
   k5.block_dim = s->search_block;
   k5.lds_hits = s->search_lds_hits;
   k5.work_stats = s->count_work ? s->get<unsigned long long>("search_work") : nullptr;
+  k5.event_clock = s->count_work ? s->get<unsigned long long>("search_event_clock") : nullptr;
   if (k5.work_stats) VELO_TRY(velo::launch_fill_u32(reinterpret_cast<uint32_t*>(k5.work_stats), 8, 0, stream));
   VELO_MARK(mark_begin(VELO_STAGE_SEARCH));
+  if (s->search_largest_first) {
+    k5.event_list = s->get<uint32_t>("search_order");
+    VELO_TRY(velo::launch_order_events_by_hits(est_off, n_events_dev, const_cast<uint32_t*>(k5.event_list), stream));
+  }
   VELO_TRY(velo::launch_search_by_triplet(k5, stream));
   VELO_MARK(mark_end(VELO_STAGE_SEARCH));
   auto* off_tracks = s->get<uint32_t>("offsets_velo_tracks");

@@ -16,6 +16,8 @@
 
 namespace velo {
 
+constexpr unsigned max_banks_per_mp = 8; // v2/v3: 8 sensors per module pair; v4: 4 sensor pairs
+
 // ---------------------------------------------------------------------------------------
 // Bit tricks of MaskedVeloClustering.cu:21-84.
 __device__ __forceinline__ uint64_t make_8con_mask(uint64_t c)
@@ -314,6 +316,67 @@ __global__ void k_number_of_candidates(
   if (lane_id() == 0) number_of_candidates[event_number] = 2 * n;
 }
 
+// Bank directory of the whole-sequence path: one wave per event reads the bank headers once
+// and writes, per (event, module pair), the module pair's banks in bank order (at most 8) as
+// {first SP word (index into the raw buffer in 32-bit words), SP words, sourceID}, plus their
+// number. Also writes dev_number_of_candidates (2 x SP words of the VELO banks).
+template<int version>
+__global__ void k_bank_directory(
+  const char* raw,
+  const uint32_t* offsets,
+  const uint32_t* sizes,
+  const uint32_t* types,
+  unsigned n_events,
+  uint32_t* number_of_candidates,
+  BankEntry* directory,
+  uint32_t* directory_count)
+{
+  constexpr bool v23 = (version == 2 || version == 3);
+  const unsigned waves = blockDim.x / wave_size;
+  const unsigned e = blockIdx.x * waves + threadIdx.x / wave_size;
+  if (e >= n_events) return;
+  const RawEvent ev(raw, offsets, sizes, types, e);
+  uint32_t total = 0;
+  unsigned my_count = 0;
+  unsigned cnt[n_module_pairs];
+#pragma unroll
+  for (unsigned m = 0; m < n_module_pairs; ++m) cnt[m] = 0;
+  for (unsigned base = 0; base < ev.n_banks; base += wave_size) {
+    const unsigned b = base + lane_id();
+    unsigned mp = n_module_pairs;
+    RawBank bank;
+    if (b < ev.n_banks) {
+      bank = raw_bank<version>(ev, b);
+      if (is_velo_bank(bank.type)) {
+        total += bank_n_sp<version>(bank);
+        const uint32_t pair = bank.sourceID & 0x1FFu;
+        mp = v23 ? pair / 8 : (pair << 1) / 8;
+      }
+    }
+#pragma unroll
+    for (unsigned m = 0; m < n_module_pairs; ++m) {
+      const uint64_t bal = __ballot(mp == m);
+      if (mp == m) {
+        const unsigned rank = cnt[m] + __popcll(bal & lanes_below_mask());
+        if (rank < max_banks_per_mp) {
+          BankEntry en;
+          en.word_offset = static_cast<uint32_t>((reinterpret_cast<const char*>(bank.word) - raw) / 4);
+          en.n_sp = bank_n_sp<version>(bank);
+          en.source = bank.sourceID;
+          en.pad = 0;
+          directory[(static_cast<size_t>(e) * n_module_pairs + m) * max_banks_per_mp + rank] = en;
+        }
+      }
+      cnt[m] += __popcll(bal);
+      if (lane_id() == m) my_count = cnt[m];
+    }
+  }
+#pragma unroll
+  for (unsigned d = 32; d > 0; d >>= 1) total += __shfl_xor(total, d, wave_size);
+  if (lane_id() < n_module_pairs) directory_count[e * n_module_pairs + lane_id()] = min(my_count, max_banks_per_mp);
+  if (lane_id() == 0) number_of_candidates[e] = 2 * total;
+}
+
 // dev_velo_bank_index as the CPU target leaves it: events of CPU block 0 (ev_idx = 0, G, 2G, ...)
 // write in order and the last write of each entry wins. Walk those events backwards and keep the
 // first value seen per entry; stop once every sensor has one.
@@ -557,7 +620,6 @@ __global__ void k_sort_by_phi(
 // the LDS budget is processed the same way out of global memory.
 constexpr unsigned fused_word_cap = 1024;
 constexpr unsigned fused_cluster_cap = 1024;
-constexpr unsigned max_banks_per_mp = 8; // v2/v3: 8 sensors per module pair; v4: 4 sensor pairs
 
 template<int version>
 __global__ void __launch_bounds__(256) k_cluster_sorted(
@@ -727,7 +789,13 @@ static hipError_t launch_candidates_v(const CandidatesLaunch& a, hipStream_t s)
 {
   const unsigned waves = 4;
   const unsigned grid = (a.n_list + waves - 1) / waves;
-  if (grid) {
+  if (grid && a.directory) {
+    if (a.event_list || a.event_start) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(
+      k_bank_directory<version>, dim3(grid), dim3(waves * wave_size), 0, s, a.raw, a.offsets, a.sizes, a.types,
+      a.n_list, a.number_of_candidates, static_cast<BankEntry*>(a.directory), a.directory_count);
+  }
+  else if (grid) {
     hipLaunchKernelGGL(
       k_number_of_candidates<version>, dim3(grid), dim3(waves * wave_size), 0, s, a.raw, a.offsets, a.sizes, a.types,
       a.event_list, a.n_list, a.event_start, a.number_of_candidates);
@@ -843,39 +911,22 @@ __device__ __forceinline__ void wave_sync_lds()
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// The banks of module pair `mp` in bank order; lanes scan the bank headers of the event.
-template<int version>
-__device__ __forceinline__ void find_banks(const RawEvent& ev, unsigned mp, Banks& B)
+// The banks of module pair `idx` (= event * 26 + module pair) from the bank directory.
+__device__ __forceinline__ void
+load_banks(const char* raw, const BankEntry* directory, const uint32_t* directory_count, unsigned idx, Banks& B)
 {
-  constexpr bool v23 = (version == 2 || version == 3);
-  unsigned nb = 0, acc = 0;
-  for (unsigned base = 0; base < ev.n_banks; base += wave_size) {
-    const unsigned b = base + lane_id();
-    bool mine = false;
-    uint32_t n_sp = 0;
-    if (b < ev.n_banks) {
-      const RawBank bank = raw_bank<version>(ev, b);
-      const uint32_t pair = bank.sourceID & 0x1FFu;
-      mine = is_velo_bank(bank.type) && (v23 ? pair / 8 : (pair << 1) / 8) == mp;
-      n_sp = bank_n_sp<version>(bank);
-    }
-    const uint64_t m = __ballot(mine);
-    const unsigned rank = nb + __popcll(m & lanes_below_mask());
-    const uint32_t inc = wave_inclusive_scan(mine ? n_sp : 0u);
-    if (mine && rank < max_banks_per_mp) {
-      const RawBank bank = raw_bank<version>(ev, b);
-      B.start[rank] = acc + inc - n_sp;
-      B.source[rank] = bank.sourceID;
-      B.word[rank] = bank.word;
-    }
-    nb += __popcll(m);
-    acc += __shfl(inc, wave_size - 1, wave_size);
+  const unsigned nb = directory_count[idx];
+  const unsigned l = lane_id();
+  BankEntry en {0, 0, 0, 0};
+  if (l < nb) en = directory[static_cast<size_t>(idx) * max_banks_per_mp + l];
+  const uint32_t inc = wave_inclusive_scan(l < nb ? en.n_sp : 0u);
+  if (l < nb) {
+    B.start[l] = inc - en.n_sp;
+    B.source[l] = en.source;
+    B.word[l] = reinterpret_cast<const uint32_t*>(raw) + en.word_offset;
   }
-  nb = min(nb, max_banks_per_mp);
-  if (lane_id() == 0) {
-    B.n = nb;
-    B.start[nb] = acc;
-  }
+  if (l == 0) B.n = nb;
+  if (l == nb) B.start[nb] = inc - (l < nb ? en.n_sp : 0u);
   wave_sync_lds();
 }
 
@@ -930,9 +981,8 @@ for_words(const Banks& B, unsigned n_words, bool resident, uint32_t* lw, bool ac
 template<int version>
 __global__ void __launch_bounds__(mp::waves_per_block* wave_size) k_estimate_mp(
   const char* raw,
-  const uint32_t* offsets,
-  const uint32_t* sizes,
-  const uint32_t* types,
+  const BankEntry* directory,
+  const uint32_t* directory_count,
   unsigned n_events,
   uint32_t* estimated_input_size,
   uint8_t* sp_flags)
@@ -942,11 +992,8 @@ __global__ void __launch_bounds__(mp::waves_per_block* wave_size) k_estimate_mp(
   const unsigned wave = threadIdx.x / wave_size;
   const unsigned idx = blockIdx.x * mp::waves_per_block + wave;
   if (idx >= n_events * n_module_pairs) return;
-  const unsigned event_number = idx / n_module_pairs;
-  const unsigned m = idx % n_module_pairs;
-  const RawEvent ev(raw, offsets, sizes, types, event_number);
   mp::Banks& B = s_banks[wave];
-  mp::find_banks<version>(ev, m, B);
+  mp::load_banks(raw, directory, directory_count, idx, B);
   const unsigned n_words = B.start[B.n];
   const bool resident = n_words <= mp::word_cap;
   uint32_t* lw = s_words[wave];
@@ -979,15 +1026,14 @@ __global__ void __launch_bounds__(mp::waves_per_block* wave_size) k_estimate_mp(
   }
 #pragma unroll
   for (unsigned d = 32; d > 0; d >>= 1) count += __shfl_xor(count, d, wave_size);
-  if (lane_id() == 0) estimated_input_size[event_number * n_module_pairs + m] = count;
+  if (lane_id() == 0) estimated_input_size[idx] = count;
 }
 
 template<int version>
 __global__ void __launch_bounds__(mp::waves_per_block* wave_size) k_cluster_mp(
   const char* raw,
-  const uint32_t* offsets,
-  const uint32_t* sizes,
-  const uint32_t* types,
+  const BankEntry* directory,
+  const uint32_t* directory_count,
   unsigned n_events,
   const uint32_t* offsets_estimated_input_size,
   const uint8_t* sp_flags,
@@ -1008,15 +1054,12 @@ __global__ void __launch_bounds__(mp::waves_per_block* wave_size) k_cluster_mp(
   const unsigned wave = threadIdx.x / wave_size;
   const unsigned idx = blockIdx.x * mp::waves_per_block + wave;
   if (idx >= n_events * n_module_pairs) return;
-  const unsigned event_number = idx / n_module_pairs;
-  const unsigned m = idx % n_module_pairs;
   const unsigned total = offsets_estimated_input_size[n_module_pairs * n_events];
   const unsigned start = offsets_estimated_input_size[idx];
   const unsigned estimate = offsets_estimated_input_size[idx + 1] - start;
   const Geometry& g = *geometry;
-  const RawEvent ev(raw, offsets, sizes, types, event_number);
   mp::Banks& B = s_banks[wave];
-  mp::find_banks<version>(ev, m, B);
+  mp::load_banks(raw, directory, directory_count, idx, B);
   const unsigned n_words = B.start[B.n];
   const bool resident = n_words <= mp::word_cap;
   const bool clusters_in_lds = estimate <= mp::cluster_cap;
@@ -1155,8 +1198,8 @@ hipError_t launch_estimate_mp(const EstimateMpLaunch& a, hipStream_t s)
   const unsigned grid = (a.n_events * n_module_pairs + mp::waves_per_block - 1) / mp::waves_per_block;
 #define VELO_LAUNCH_EMP(V)                                                                                          \
   hipLaunchKernelGGL(                                                                                               \
-    k_estimate_mp<V>, dim3(grid), dim3(mp::waves_per_block * wave_size), 0, s, a.raw, a.offsets, a.sizes, a.types,  \
-    a.n_events, a.estimated_input_size, a.sp_flags)
+    k_estimate_mp<V>, dim3(grid), dim3(mp::waves_per_block * wave_size), 0, s, a.raw,                               \
+    static_cast<const BankEntry*>(a.directory), a.directory_count, a.n_events, a.estimated_input_size, a.sp_flags)
   switch (a.version) {
   case 2: VELO_LAUNCH_EMP(2); break;
   case 3: VELO_LAUNCH_EMP(3); break;
@@ -1172,8 +1215,8 @@ hipError_t launch_cluster_mp(const ClusterMpLaunch& a, hipStream_t s)
   const unsigned grid = (a.n_events * n_module_pairs + mp::waves_per_block - 1) / mp::waves_per_block;
 #define VELO_LAUNCH_CMP(V)                                                                                          \
   hipLaunchKernelGGL(                                                                                               \
-    k_cluster_mp<V>, dim3(grid), dim3(mp::waves_per_block * wave_size), 0, s, a.raw, a.offsets, a.sizes, a.types,   \
-    a.n_events, a.offsets_estimated_input_size, a.sp_flags, reinterpret_cast<const Geometry*>(a.geometry),           \
+    k_cluster_mp<V>, dim3(grid), dim3(mp::waves_per_block * wave_size), 0, s, a.raw,                                \
+    static_cast<const BankEntry*>(a.directory), a.directory_count, a.n_events, a.offsets_estimated_input_size, a.sp_flags, reinterpret_cast<const Geometry*>(a.geometry),           \
     a.sp_patterns, a.sp_fx, a.sp_fy, a.module_cluster_num, a.sorted_container, a.unsorted_scratch)
   switch (a.version) {
   case 2: VELO_LAUNCH_CMP(2); break;

@@ -100,6 +100,14 @@ struct RawEvent {
   }
 };
 
+// One bank of a module pair in the bank directory (16 bytes, one 128-bit load).
+struct BankEntry {
+  uint32_t word_offset; // first SP word, as an index into the raw buffer in 32-bit words
+  uint32_t n_sp;
+  uint32_t source;
+  uint32_t pad;
+};
+
 struct RawBank {
   uint32_t sourceID;
   uint32_t count;

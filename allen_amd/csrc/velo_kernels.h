@@ -19,6 +19,10 @@ struct CandidatesLaunch {
   unsigned block_dim_x;
   uint32_t* number_of_candidates;
   uint32_t* bank_index;
+  // Whole-sequence path (identity event list only): also write the bank directory, per
+  // (event, module pair) up to 8 BankEntry records in bank order and their number.
+  void* directory = nullptr;
+  uint32_t* directory_count = nullptr;
 };
 
 struct EstimateLaunch {
@@ -91,9 +95,8 @@ struct FusedClusterLaunch {
 struct EstimateMpLaunch {
   int version;
   const char* raw;
-  const uint32_t* offsets;
-  const uint32_t* sizes;
-  const uint32_t* types;
+  const void* directory;
+  const uint32_t* directory_count;
   unsigned n_events;
   uint32_t* estimated_input_size;
   uint8_t* sp_flags; // one byte per 32-bit word of the raw buffer
@@ -102,9 +105,8 @@ struct EstimateMpLaunch {
 struct ClusterMpLaunch {
   int version;
   const char* raw;
-  const uint32_t* offsets;
-  const uint32_t* sizes;
-  const uint32_t* types;
+  const void* directory;
+  const uint32_t* directory_count;
   unsigned n_events;
   const uint32_t* offsets_estimated_input_size;
   const uint8_t* sp_flags;
@@ -138,8 +140,9 @@ struct SearchLaunch {
   float max_scatter;
   unsigned max_skipped_modules;
   unsigned block_dim;          // 0 = 256
-  unsigned lds_hits;           // event hits staged in LDS (15 B each); 0 = 2048, at most 4096
+  unsigned lds_hits;           // event hits staged in LDS (16 B each); 0 = search_default_lds_hits
   unsigned long long* work_stats; // optional: seeding evals, forwarding evals, extrapolations, search steps
+  unsigned long long* event_clock = nullptr; // optional: per event, event_clock_words u64 of wall clock ticks
 };
 
 struct ThreeHitFilterLaunch {
@@ -202,6 +205,16 @@ hipError_t launch_cluster_sorted(const FusedClusterLaunch& a, hipStream_t s);
 hipError_t launch_estimate_mp(const EstimateMpLaunch& a, hipStream_t s);
 hipError_t launch_cluster_mp(const ClusterMpLaunch& a, hipStream_t s);
 hipError_t launch_search_by_triplet(const SearchLaunch& a, hipStream_t s);
+constexpr unsigned event_clock_words = 8; // start, end, ticks in 6 phases
+// Event hits staged in LDS by default: four workgroups of 256 lanes fit one CU's 160 KiB.
+constexpr unsigned search_default_lds_hits = 1760;
+size_t search_dynamic_lds(unsigned block, unsigned lds_hits);
+// order[0..n) = events by descending hit count (n read on the device).
+hipError_t launch_order_events_by_hits(
+  const uint32_t* offsets_estimated_input_size,
+  const uint32_t* number_of_events,
+  uint32_t* order,
+  hipStream_t s);
 hipError_t launch_three_hit_tracks_filter(const ThreeHitFilterLaunch& a, hipStream_t s);
 hipError_t launch_copy_track_hit_number(const CopyHitNumberLaunch& a, hipStream_t s);
 hipError_t launch_consolidate_tracks(const ConsolidateLaunch& a, hipStream_t s);

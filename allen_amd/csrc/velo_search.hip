@@ -18,6 +18,8 @@
 namespace velo {
 
 // Three 0/1 flags scanned at once over the block. Exclusive ranks in r[], totals in t[].
+// One barrier: consecutive scans must alternate between two scratch buffers (a scan's reads of
+// its buffer are ordered before the next scan's barrier, hence before the reuse two scans on).
 struct Scan3 {
   uint32_t r[3];
   uint32_t t[3];
@@ -63,7 +65,6 @@ __device__ __forceinline__ Scan3 block_flag_scan3(bool f0, bool f1, bool f2, uin
     s.t[1] += c1;
     s.t[2] += c2;
   }
-  __syncthreads();
   return s;
 }
 
@@ -80,43 +81,86 @@ struct LdsHits {
   const float* zs;
   const int16_t* ph;
   uint8_t* used;
+  const uint8_t* odd; // module parity of the hit's sensor, ((id >> 18) & 0xFF) / 4 % 2
   __device__ __forceinline__ float x(unsigned i) const { return xs[i]; }
   __device__ __forceinline__ float y(unsigned i) const { return ys[i]; }
   __device__ __forceinline__ float z(unsigned i) const { return zs[i]; }
   __device__ __forceinline__ int16_t phi(unsigned i) const { return ph[i]; }
   __device__ __forceinline__ const int16_t* phi_begin() const { return ph; }
+  __device__ __forceinline__ bool module_odd(unsigned i) const { return odd[i] != 0; }
 };
+
+// Module parity of a hit from its LHCbID (SearchByTriplet.cu:498-499).
+__device__ __forceinline__ uint8_t module_parity(uint32_t id) { return static_cast<uint8_t>(((id & 0x3fc0000u) >> 18) / 4 % 2); }
 
 // Hits of one event read from the sorted cluster container in global memory.
 struct GlobalHits {
   const float* xyz;
   const int16_t* ph;
   uint8_t* used;
+  const uint32_t* ids;
   __device__ __forceinline__ float x(unsigned i) const { return xyz[3 * i]; }
   __device__ __forceinline__ float y(unsigned i) const { return xyz[3 * i + 1]; }
   __device__ __forceinline__ float z(unsigned i) const { return xyz[3 * i + 2]; }
   __device__ __forceinline__ int16_t phi(unsigned i) const { return ph[i]; }
   __device__ __forceinline__ const int16_t* phi_begin() const { return ph; }
+  __device__ __forceinline__ bool module_odd(unsigned i) const { return module_parity(ids[i]) != 0; }
 };
 
 // Work counters of one thread (seeding evaluations, forwarding evaluations, extrapolations,
-// binary-search iterations); only kept by the counting instantiation of the kernel.
+// search probes); only kept by the counting instantiation of the kernel.
 struct WorkCount {
   uint32_t seed_eval = 0, fwd_eval = 0, extrap = 0, steps = 0;
+  // Wall-clock ticks spent by the workgroup in each phase (kept by thread 0 right after a
+  // barrier): forwarding, compaction of the h1 hits, seeding steps 1, 2 and 3, the rest.
+  unsigned long long phase[6] = {0, 0, 0, 0, 0, 0};
+  unsigned long long last = 0;
 };
 
+enum Phase { phase_forwarding, phase_compaction, phase_seed1, phase_seed2, phase_seed3, phase_other };
+
+template<bool count>
+__device__ __forceinline__ void phase_mark(WorkCount& wc, Phase p)
+{
+  if constexpr (count) {
+    if (threadIdx.x == 0) {
+      const unsigned long long now = wall_clock64();
+      wc.phase[p] += now - wc.last;
+      wc.last = now;
+    }
+  }
+}
+
+// First index i of the sorted array with array[i] >= value (the leftmost binary search of
+// BinarySearch.cuh, same result on every input). Eight-way: each round probes seven split
+// points at once (independent LDS reads) and keeps the eighth of the range that holds the
+// answer, so a module pair of n hits takes about log8(n) dependent round trips instead of
+// log2(n). `steps` counts the probes.
 template<bool count>
 __device__ __forceinline__ int counted_search(const int16_t* array, unsigned size, int16_t value, WorkCount& wc)
 {
-  int l = 0;
-  int r = size;
-  while (l < r) {
-    if constexpr (count) ++wc.steps;
-    const int m = (l + r) / 2;
-    if (value > array[m]) l = m + 1;
-    else r = m;
+  unsigned lo = 0, hi = size; // answer in [lo, hi]; array[< lo] < value <= array[>= hi]
+  while (lo < hi) {
+    const unsigned span = hi - lo;
+    unsigned p[7];
+    bool below[7];
+#pragma unroll
+    for (unsigned j = 0; j < 7; ++j) {
+      p[j] = lo + ((j + 1) * span) / 8; // lo <= p[j] < hi, non-decreasing in j
+      below[j] = array[p[j]] < value;   // true for a prefix of j (sorted array)
+    }
+    if constexpr (count) wc.steps += 7;
+    unsigned nlo = lo, nhi = hi;
+#pragma unroll
+    for (unsigned j = 0; j < 7; ++j)
+      if (below[j]) nlo = p[j] + 1;
+#pragma unroll
+    for (int j = 6; j >= 0; --j)
+      if (!below[j]) nhi = p[j];
+    lo = nlo;
+    hi = nhi;
   }
-  return l;
+  return static_cast<int>(lo);
 }
 
 // Hits of one event: phi and used flags in LDS, coordinates in global memory (events too large
@@ -125,22 +169,37 @@ struct MixedHits {
   const float* xyz;
   const int16_t* ph;
   uint8_t* used;
+  const uint8_t* odd;
   __device__ __forceinline__ float x(unsigned i) const { return xyz[3 * i]; }
   __device__ __forceinline__ float y(unsigned i) const { return xyz[3 * i + 1]; }
   __device__ __forceinline__ float z(unsigned i) const { return xyz[3 * i + 2]; }
   __device__ __forceinline__ int16_t phi(unsigned i) const { return ph[i]; }
   __device__ __forceinline__ const int16_t* phi_begin() const { return ph; }
+  __device__ __forceinline__ bool module_odd(unsigned i) const { return odd[i] != 0; }
 };
 
 struct SearchShared {
-  ModulePairData mpd[3];
-  uint32_t ttf_counter;   // atomics::tracks_to_follow
-  uint32_t three_counter; // atomics::number_of_three_hit_tracks
-  uint32_t local_hits;    // atomics::local_number_of_hits
-  uint32_t n_tracks;      // dev_number_of_velo_tracks[event]
-  uint32_t status;
-  uint32_t scan[3 * 16 + 4];
+  ModulePairData mp[n_module_pairs]; // hit range (event-relative) and module z of every module pair
+  uint32_t scan[2][3 * 16 + 4];      // block_flag_scan3 scratch, used alternately
 };
+
+// The event's counters, identical in every lane of the workgroup (updated from block-wide
+// scan totals), so no lane has to publish them through LDS.
+struct Counters {
+  uint32_t ttf = 0;        // atomics::tracks_to_follow
+  uint32_t three = 0;      // atomics::number_of_three_hit_tracks
+  uint32_t local_hits = 0; // atomics::local_number_of_hits
+  uint32_t n_tracks = 0;   // dev_number_of_velo_tracks[event]
+  uint32_t status = 0;
+  unsigned scan_buffer = 0;
+};
+
+__device__ __forceinline__ Scan3 scan3(SearchShared& sh, Counters& c, bool f0, bool f1, bool f2)
+{
+  const Scan3 s = block_flag_scan3(f0, f1, f2, sh.scan[c.scan_buffer]);
+  c.scan_buffer ^= 1u;
+  return s;
+}
 
 // Seeding scratch for one chunk of h1 hits, blockDim entries each, in dynamic LDS.
 struct SeedScratch {
@@ -149,16 +208,44 @@ struct SeedScratch {
   uint16_t* h0;
   uint16_t* pair_h2;
   uint8_t* n_h0;
+  uint16_t* h1_chunk; // unused hits of the current module pair, one compaction chunk
 };
 
 struct EventOutputs {
   TrackHits* tracks;
   TrackletHits* three_hit_tracks;
   TrackletHits* tracklets;
-  uint32_t* ttf;
+  uint32_t* ttf;        // the ring of tracks to follow, in LDS
   uint16_t* h1_indices;
-  const uint32_t* ids; // event-relative LHCbIDs of the sorted container
   unsigned track_capacity;
+};
+
+// Four consecutive positions of an h2 window, read at once: position j of the window is hit
+// (c + j) % n of the module pair (SearchByTriplet.cu:393-395, 520-522); positions past the
+// module pair read the window's first hit and are not used.
+struct Window4 {
+  unsigned index[4];
+  int16_t phi[4];
+  uint8_t used[4];
+  float x[4], y[4], z[4];
+  template<typename H>
+  __device__ __forceinline__ void
+  load(const H& hits, unsigned start, unsigned n, unsigned c, unsigned j0, bool with_used)
+  {
+#pragma unroll
+    for (unsigned k = 0; k < 4; ++k) {
+      const unsigned j = j0 + k < n ? j0 + k : j0;
+      index[k] = start + (c + j) % n;
+    }
+#pragma unroll
+    for (unsigned k = 0; k < 4; ++k) {
+      phi[k] = hits.phi(index[k]);
+      used[k] = with_used ? hits.used[index[k]] : 0;
+      x[k] = hits.x(index[k]);
+      y[k] = hits.y(index[k]);
+      z[k] = hits.z(index[k]);
+    }
+  }
 };
 
 // SearchByTriplet.cu:272-292
@@ -201,131 +288,163 @@ __device__ void track_seeding(
   const H& hits,
   const SeedScratch& ss,
   SearchShared& sh,
+  Counters& cn,
   const EventOutputs& o,
+  const ModulePairData prev,
+  const ModulePairData cur,
+  const ModulePairData next,
   float max_scatter,
   int16_t phi_tolerance,
   unsigned h0_candidates_to_consider)
 {
-  const ModulePairData prev = sh.mpd[0];
-  const ModulePairData cur = sh.mpd[1];
-  const ModulePairData next = sh.mpd[2];
-
-  // Unused hits of the current module pair, compacted in index order.
-  uint32_t base = 0;
-  for (unsigned c = 0; c < cur.hit_num; c += blockDim.x) {
-    const unsigned r = c + threadIdx.x;
-    const bool act = r < cur.hit_num && !hits.used[cur.hit_start + r];
-    const Scan3 sc = block_flag_scan3(act, false, false, sh.scan);
-    if (act) o.h1_indices[base + sc.r[0]] = cur.hit_start + r;
-    base += sc.t[0];
-  }
-  if (threadIdx.x == 0) sh.local_hits = base;
-  __threadfence_block();
-  __syncthreads();
-  const unsigned n_h1 = base;
   const unsigned K = h0_candidates_to_consider;
   const unsigned C = blockDim.x / K;
   const float dz_next = next.z[0] - prev.z[0];
 
-  for (unsigned c = 0; c < n_h1; c += C) {
-    // 1. h0 candidates of h1 number c + t (lanes t < C), pendulum order.
-    if (threadIdx.x < C) {
-      const unsigned rel = c + threadIdx.x;
-      unsigned n_found = 0;
-      if (rel < n_h1) {
-        const unsigned h1_index = o.h1_indices[rel];
-        const int16_t h1_phi = hits.phi(h1_index);
-        int phi_index = counted_search<count>(hits.phi_begin() + prev.hit_start, prev.hit_num, h1_phi, wc);
-        for (unsigned i = 0; i < prev.hit_num && n_found < K; ++i) {
-          const int index_diff = (i & 1u) ? static_cast<int>(i) : -static_cast<int>(i);
-          phi_index += index_diff;
-          const int n = static_cast<int>(prev.hit_num);
-          const int in_bounds = phi_index < 0 ? phi_index + n : (phi_index >= n ? phi_index - n : phi_index);
-          const unsigned h0_index = prev.hit_start + in_bounds;
-          if (!hits.used[h0_index]) ss.h0[threadIdx.x * K + n_found++] = h0_index;
-        }
-        ss.h1[threadIdx.x] = h1_index;
-      }
-      ss.n_h0[threadIdx.x] = n_found;
+  // Unused hits of the current module pair, compacted in index order one chunk of blockDim
+  // hits at a time (dev_rel_indices in global memory, the chunk's list in LDS); the seeds of a
+  // chunk are made before the next chunk, which keeps the h1 order of the reference.
+  uint32_t base = 0;
+  for (unsigned c0 = 0; c0 < cur.hit_num; c0 += blockDim.x) {
+    const unsigned r = c0 + threadIdx.x;
+    const bool act = r < cur.hit_num && !hits.used[cur.hit_start + r];
+    const Scan3 sc = scan3(sh, cn, act, false, false);
+    if (act) {
+      o.h1_indices[base + sc.r[0]] = cur.hit_start + r;
+      ss.h1_chunk[sc.r[0]] = cur.hit_start + r;
     }
+    base += sc.t[0];
+    const unsigned n_h1 = sc.t[0];
     __syncthreads();
+    phase_mark<count>(wc, phase_compaction);
 
-    // 2. One (h1, h0) pair per lane: best h2 of this h0 (first strict minimum).
-    {
-      const unsigned t = threadIdx.x / K;
-      const unsigned k = threadIdx.x % K;
-      float best = max_scatter;
-      uint16_t best_h2 = 0;
-      if (t < C && c + t < n_h1 && k < ss.n_h0[t]) {
-        const unsigned h1_index = ss.h1[t];
-        const unsigned h0_index = ss.h0[t * K + k];
-        const float h1x = hits.x(h1_index), h1y = hits.y(h1_index), h1z = hits.z(h1_index);
-        const float h0x = hits.x(h0_index), h0y = hits.y(h0_index), h0z = hits.z(h0_index);
-        const float td = 1.0f / (h1z - h0z);
-        const float txn = h1x - h0x;
-        const float tyn = h1y - h0y;
-        const float tx = txn * td;
-        const float ty = tyn * td;
-        int16_t cand, ephi;
-        find_forward_candidate<count>(wc, hits, next, h0x, h0y, tx, ty, dz_next, phi_tolerance, cand, ephi);
-        for (unsigned j = 0; j < next.hit_num; ++j) {
-          const unsigned in_bounds = (static_cast<unsigned>(static_cast<int>(cand)) + j) % next.hit_num;
-          const unsigned h2_index = next.hit_start + in_bounds;
-          const int16_t phi_diff = hits.phi(h2_index) - ephi;
-          const int16_t abs_phi_diff = phi_diff < 0 ? -phi_diff : phi_diff;
-          if (abs_phi_diff > phi_tolerance) break;
-          if (!hits.used[h2_index]) {
-            if constexpr (count) ++wc.seed_eval;
-            const float h2x = hits.x(h2_index), h2y = hits.y(h2_index), h2z = hits.z(h2_index);
-            const float dz = h2z - h0z;
-            const float predx = h0x + tx * dz;
-            const float predy = h0y + ty * dz;
-            const float dx = predx - h2x;
-            const float dy = predy - h2y;
-            const float scatter = dx * dx + dy * dy;
-            if (scatter < best) {
-              best = scatter;
-              best_h2 = h2_index;
+    for (unsigned c = 0; c < n_h1; c += C) {
+      // 1. h0 candidates of h1 number c + t (lanes t < C), pendulum order.
+      if (threadIdx.x < C) {
+        const unsigned rel = c + threadIdx.x;
+        unsigned n_found = 0;
+        if (rel < n_h1) {
+          const unsigned h1_index = ss.h1_chunk[rel];
+          const int16_t h1_phi = hits.phi(h1_index);
+          const int start = counted_search<count>(hits.phi_begin() + prev.hit_start, prev.hit_num, h1_phi, wc);
+          // Pendulum order around `start`: start, +1, -1, +2, -2, ... (each wrapped once into
+          // the module pair). The used flags of 16 positions are read per round, all at once
+          // (positions past the module pair read its first hit and are ignored).
+          const int n = static_cast<int>(prev.hit_num);
+          constexpr unsigned R = 16;
+          for (unsigned i0 = 0; i0 < prev.hit_num && n_found < K; i0 += R) {
+            unsigned pos[R];
+            uint8_t flag[R];
+#pragma unroll
+            for (unsigned k = 0; k < R; ++k) {
+              const unsigned i = i0 + k;
+              const int off = (i & 1u) ? static_cast<int>((i + 1) / 2) : -static_cast<int>(i / 2);
+              const int p = start + off;
+              const int in_bounds = p < 0 ? p + n : (p >= n ? p - n : p);
+              pos[k] = prev.hit_start + (i < prev.hit_num ? static_cast<unsigned>(in_bounds) : 0u);
+              flag[k] = hits.used[pos[k]];
+            }
+#pragma unroll
+            for (unsigned k = 0; k < R; ++k) {
+              if (i0 + k < prev.hit_num && !flag[k] && n_found < K) ss.h0[threadIdx.x * K + n_found++] = pos[k];
             }
           }
+          ss.h1[threadIdx.x] = h1_index;
         }
+        ss.n_h0[threadIdx.x] = n_found;
       }
-      ss.pair_scatter[threadIdx.x] = best;
-      ss.pair_h2[threadIdx.x] = best_h2;
-    }
-    __syncthreads();
+      __syncthreads();
+      phase_mark<count>(wc, phase_seed1);
 
-    // 3. One lane per h1: first strict minimum over its h0 candidates, then ordered seeds.
-    bool found = false;
-    uint16_t best_h0 = 0, best_h2 = 0, h1_index = 0;
-    if (threadIdx.x < C && c + threadIdx.x < n_h1) {
-      const unsigned t = threadIdx.x;
-      float best_fit = max_scatter;
-      for (unsigned k = 0; k < ss.n_h0[t]; ++k) {
-        const float sc = ss.pair_scatter[t * K + k];
-        if (sc < best_fit) {
-          best_fit = sc;
-          best_h0 = ss.h0[t * K + k];
-          best_h2 = ss.pair_h2[t * K + k];
+      // 2. One (h1, h0) pair per lane: best h2 of this h0 (first strict minimum).
+      {
+        const unsigned t = threadIdx.x / K;
+        const unsigned k = threadIdx.x % K;
+        float best = max_scatter;
+        uint16_t best_h2 = 0;
+        if (t < C && c + t < n_h1 && k < ss.n_h0[t]) {
+          const unsigned h1_index = ss.h1[t];
+          const unsigned h0_index = ss.h0[t * K + k];
+          const float h1x = hits.x(h1_index), h1y = hits.y(h1_index), h1z = hits.z(h1_index);
+          const float h0x = hits.x(h0_index), h0y = hits.y(h0_index), h0z = hits.z(h0_index);
+          const float td = 1.0f / (h1z - h0z);
+          const float txn = h1x - h0x;
+          const float tyn = h1y - h0y;
+          const float tx = txn * td;
+          const float ty = tyn * td;
+          int16_t cand, ephi;
+          find_forward_candidate<count>(wc, hits, next, h0x, h0y, tx, ty, dz_next, phi_tolerance, cand, ephi);
+          // The h2 window in phi order from the candidate, four positions read per round.
+          const unsigned n2 = next.hit_num;
+          const unsigned c2 = static_cast<unsigned>(static_cast<int>(cand));
+          for (unsigned j0 = 0; j0 < n2; j0 += 4) {
+            Window4 w;
+            w.load(hits, next.hit_start, n2, c2, j0, true);
+            bool stop = false;
+#pragma unroll
+            for (unsigned k = 0; k < 4; ++k) {
+              if (stop || j0 + k >= n2) continue;
+              const int16_t phi_diff = w.phi[k] - ephi;
+              const int16_t abs_phi_diff = phi_diff < 0 ? -phi_diff : phi_diff;
+              if (abs_phi_diff > phi_tolerance) {
+                stop = true;
+                continue;
+              }
+              if (!w.used[k]) {
+                if constexpr (count) ++wc.seed_eval;
+                const float dz = w.z[k] - h0z;
+                const float predx = h0x + tx * dz;
+                const float predy = h0y + ty * dz;
+                const float dx = predx - w.x[k];
+                const float dy = predy - w.y[k];
+                const float scatter = dx * dx + dy * dy;
+                if (scatter < best) {
+                  best = scatter;
+                  best_h2 = w.index[k];
+                }
+              }
+            }
+            if (stop) break;
+          }
         }
+        ss.pair_scatter[threadIdx.x] = best;
+        ss.pair_h2[threadIdx.x] = best_h2;
       }
-      h1_index = ss.h1[t];
-      found = best_fit < max_scatter;
+      __syncthreads();
+      phase_mark<count>(wc, phase_seed2);
+
+      // 3. One lane per h1: first strict minimum over its h0 candidates, then ordered seeds.
+      bool found = false;
+      uint16_t best_h0 = 0, best_h2 = 0, h1_index = 0;
+      if (threadIdx.x < C && c + threadIdx.x < n_h1) {
+        const unsigned t = threadIdx.x;
+        float best_fit = max_scatter;
+        for (unsigned k = 0; k < ss.n_h0[t]; ++k) {
+          const float sc = ss.pair_scatter[t * K + k];
+          if (sc < best_fit) {
+            best_fit = sc;
+            best_h0 = ss.h0[t * K + k];
+            best_h2 = ss.pair_h2[t * K + k];
+          }
+        }
+        h1_index = ss.h1[t];
+        found = best_fit < max_scatter;
+      }
+      const Scan3 s = scan3(sh, cn, found, false, false);
+      if (found) {
+        const unsigned tn = (cn.ttf + s.r[0]) % max_tracks_to_follow;
+        TrackletHits t;
+        t.hits[0] = best_h0;
+        t.hits[1] = h1_index;
+        t.hits[2] = best_h2;
+        o.tracklets[tn] = t;
+        o.ttf[tn] = bit_seed | tn;
+      }
+      cn.ttf += s.t[0];
+      phase_mark<count>(wc, phase_seed3);
     }
-    const Scan3 s = block_flag_scan3(found, false, false, sh.scan);
-    if (found) {
-      const unsigned tn = (sh.ttf_counter + s.r[0]) % max_tracks_to_follow;
-      TrackletHits t;
-      t.hits[0] = best_h0;
-      t.hits[1] = h1_index;
-      t.hits[2] = best_h2;
-      o.tracklets[tn] = t;
-      o.ttf[tn] = bit_seed | tn;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) sh.ttf_counter += s.t[0];
-    __syncthreads();
   }
+  cn.local_hits = base;
 }
 
 // SearchByTriplet.cu:451-609, ordered.
@@ -334,14 +453,15 @@ __device__ void track_forwarding(
   WorkCount& wc,
   const H& hits,
   SearchShared& sh,
+  Counters& cn,
   const EventOutputs& o,
+  const ModulePairData next,
   unsigned prev_ttf,
   unsigned diff_ttf,
   int16_t phi_tolerance,
   float max_scatter,
   unsigned max_skipped_modules)
 {
-  const ModulePairData next = sh.mpd[2];
   for (unsigned c = 0; c < diff_ttf; c += blockDim.x) {
     const unsigned e = c + threadIdx.x;
     bool promote = false, reinsert = false, three = false;
@@ -375,7 +495,6 @@ __device__ void track_forwarding(
       th[1] = t_hits[1];
       th[2] = t_hits[2];
       const float h0x = hits.x(h0_num), h0y = hits.y(h0_num), h0z = hits.z(h0_num);
-      const unsigned h0_module = ((o.ids[h0_num] & 0x3fc0000u) >> 18) / 4;
       const float h1x = hits.x(h1_num), h1y = hits.y(h1_num), h1z = hits.z(h1_num);
       const float td = 1.0f / (h1z - h0z);
       const float txn = h1x - h0x;
@@ -385,25 +504,35 @@ __device__ void track_forwarding(
       float best_fit = max_scatter;
       int16_t cand, ephi;
       find_forward_candidate<count>(
-        wc, hits, next, h0x, h0y, tx, ty, (h0_module % 2 ? next.z[1] : next.z[0]) - h0z, phi_tolerance, cand, ephi);
-      for (unsigned j = 0; j < next.hit_num; ++j) {
-        const unsigned in_bounds = (static_cast<unsigned>(static_cast<int>(cand)) + j) % next.hit_num;
-        const unsigned h2_index = next.hit_start + in_bounds;
-        const int16_t phi_diff = hits.phi(h2_index) - ephi;
-        const int16_t abs_phi_diff = phi_diff < 0 ? -phi_diff : phi_diff;
-        if (abs_phi_diff > phi_tolerance) break;
-        if constexpr (count) ++wc.fwd_eval;
-        const float h2x = hits.x(h2_index), h2y = hits.y(h2_index), h2z = hits.z(h2_index);
-        const float dz = h2z - h0z;
-        const float predx = h0x + tx * dz;
-        const float predy = h0y + ty * dz;
-        const float dx = predx - h2x;
-        const float dy = predy - h2y;
-        const float scatter = dx * dx + dy * dy;
-        if (scatter < best_fit) {
-          best_fit = scatter;
-          best_h2 = h2_index;
+        wc, hits, next, h0x, h0y, tx, ty, (hits.module_odd(h0_num) ? next.z[1] : next.z[0]) - h0z, phi_tolerance, cand, ephi);
+      const unsigned n2 = next.hit_num;
+      const unsigned c2 = static_cast<unsigned>(static_cast<int>(cand));
+      for (unsigned j0 = 0; j0 < n2; j0 += 4) {
+        Window4 w;
+        w.load(hits, next.hit_start, n2, c2, j0, false);
+        bool stop = false;
+#pragma unroll
+        for (unsigned k = 0; k < 4; ++k) {
+          if (stop || j0 + k >= n2) continue;
+          const int16_t phi_diff = w.phi[k] - ephi;
+          const int16_t abs_phi_diff = phi_diff < 0 ? -phi_diff : phi_diff;
+          if (abs_phi_diff > phi_tolerance) {
+            stop = true;
+            continue;
+          }
+          if constexpr (count) ++wc.fwd_eval;
+          const float dz = w.z[k] - h0z;
+          const float predx = h0x + tx * dz;
+          const float predy = h0y + ty * dz;
+          const float dx = predx - w.x[k];
+          const float dy = predy - w.y[k];
+          const float scatter = dx * dx + dy * dy;
+          if (scatter < best_fit) {
+            best_fit = scatter;
+            best_h2 = w.index[k];
+          }
         }
+        if (stop) break;
       }
       if (best_h2 != -1) {
         promote = number_of_hits == 3;
@@ -417,7 +546,7 @@ __device__ void track_forwarding(
         three = true;
       }
     }
-    const Scan3 s = block_flag_scan3(promote, reinsert, three, sh.scan);
+    const Scan3 s = scan3(sh, cn, promote, reinsert, three);
     if (valid) {
       if (best_h2 != -1) {
         hits.used[best_h2] = 1;
@@ -425,7 +554,7 @@ __device__ void track_forwarding(
           hits.used[th[0]] = 1;
           hits.used[th[1]] = 1;
           hits.used[th[2]] = 1;
-          track_number = sh.n_tracks + s.r[0];
+          track_number = cn.n_tracks + s.r[0];
           if (track_number < o.track_capacity) {
             TrackHits& t = o.tracks[track_number];
             t.hits[0] = th[0];
@@ -442,12 +571,12 @@ __device__ void track_forwarding(
         }
       }
       if (reinsert) {
-        const unsigned p = (sh.ttf_counter + s.r[1]) % max_tracks_to_follow;
+        const unsigned p = (cn.ttf + s.r[1]) % max_tracks_to_follow;
         // A promoted track beyond the slots of its event is skipped when read back.
         o.ttf[p] = track_number;
       }
       if (three) {
-        const unsigned p = sh.three_counter + s.r[2];
+        const unsigned p = cn.three + s.r[2];
         if (p < o.track_capacity) {
           TrackletHits t;
           t.hits[0] = th[0];
@@ -457,15 +586,14 @@ __device__ void track_forwarding(
         }
       }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      sh.n_tracks += s.t[0];
-      sh.ttf_counter += s.t[1];
-      sh.three_counter += s.t[2];
-      if (sh.n_tracks > o.track_capacity) sh.status |= status_track_slots_overflow;
-      if (sh.three_counter > o.track_capacity) sh.status |= status_three_hit_slots_overflow;
-    }
-    __syncthreads();
+    // Chunks need no barrier between them: a chunk writes ring slots at and beyond the
+    // counter, reads only the slots before it (distinct unless the ring overflows, which is
+    // reported), and the used flags it sets are read by seeding after the caller's barrier.
+    cn.n_tracks += s.t[0];
+    cn.ttf += s.t[1];
+    cn.three += s.t[2];
+    if (cn.n_tracks > o.track_capacity) cn.status |= status_track_slots_overflow;
+    if (cn.three > o.track_capacity) cn.status |= status_three_hit_slots_overflow;
   }
 }
 
@@ -475,48 +603,45 @@ __device__ void search_event(
   const H& hits,
   const SeedScratch& ss,
   SearchShared& sh,
+  Counters& cn,
   const EventOutputs& o,
-  const uint32_t* module_hit_start,
-  const uint32_t* module_hit_num,
-  const Geometry& g,
   float phi_tolerance_f,
   float max_scatter,
   unsigned max_skipped_modules)
 {
-  const unsigned hit_offset = module_hit_start[0];
-  auto load = [&](unsigned first) {
-    if (threadIdx.x < 3) {
-      const unsigned m = first - threadIdx.x;
-      sh.mpd[threadIdx.x].hit_start = module_hit_start[m] - hit_offset;
-      sh.mpd[threadIdx.x].hit_num = module_hit_num[m];
-      sh.mpd[threadIdx.x].z[0] = g.module_zs[2 * m];
-      sh.mpd[threadIdx.x].z[1] = g.module_zs[2 * m + 1];
-    }
-    __syncthreads();
-  };
+  // Module pairs are visited from the last one upstream: seeding on (first, first - 1,
+  // first - 2), forwarding into first - 2 (SearchByTriplet.cu:160-236).
+  phase_mark<count>(wc, phase_other);
   const int16_t phi_tolerance = f32_to_i16(phi_tolerance_f * convert_factor_i16);
   unsigned first_module_pair = n_module_pairs - 1;
-  load(first_module_pair);
-  track_seeding<count>(wc, hits, ss, sh, o, max_scatter, phi_tolerance, initial_seeding_h0_candidates);
+  track_seeding<count>(
+    wc, hits, ss, sh, cn, o, sh.mp[first_module_pair], sh.mp[first_module_pair - 1], sh.mp[first_module_pair - 2],
+    max_scatter, phi_tolerance, initial_seeding_h0_candidates);
   unsigned last_ttf = 0;
   --first_module_pair;
   while (first_module_pair > 1) {
-    load(first_module_pair);
     const unsigned prev_ttf = last_ttf;
-    last_ttf = sh.ttf_counter;
+    last_ttf = cn.ttf;
     const unsigned diff_ttf = last_ttf - prev_ttf;
-    if (threadIdx.x == 0 && diff_ttf >= max_tracks_to_follow) sh.status |= status_ttf_ring_overflow;
-    track_forwarding<count>(wc, hits, sh, o, prev_ttf, diff_ttf, phi_tolerance, max_scatter, max_skipped_modules);
-    __threadfence_block();
+    if (diff_ttf >= max_tracks_to_follow) cn.status |= status_ttf_ring_overflow;
+    // Orders the ring slots and tracklets written by the last seeding chunk before they are
+    // read (seeding chunks end right after their writes, without a barrier).
     __syncthreads();
-    track_seeding<count>(wc, hits, ss, sh, o, max_scatter, phi_tolerance, seeding_h0_candidates);
+    track_forwarding<count>(
+      wc, hits, sh, cn, o, sh.mp[first_module_pair - 2], prev_ttf, diff_ttf, phi_tolerance, max_scatter,
+      max_skipped_modules);
+    __syncthreads();
+    phase_mark<count>(wc, phase_forwarding);
+    track_seeding<count>(
+      wc, hits, ss, sh, cn, o, sh.mp[first_module_pair], sh.mp[first_module_pair - 1], sh.mp[first_module_pair - 2],
+      max_scatter, phi_tolerance, seeding_h0_candidates);
     --first_module_pair;
   }
   __syncthreads();
   const unsigned prev_ttf = last_ttf;
-  last_ttf = sh.ttf_counter;
+  last_ttf = cn.ttf;
   const unsigned diff_ttf = last_ttf - prev_ttf;
-  if (threadIdx.x == 0 && diff_ttf >= max_tracks_to_follow) sh.status |= status_ttf_ring_overflow;
+  if (diff_ttf >= max_tracks_to_follow) cn.status |= status_ttf_ring_overflow;
   for (unsigned c = 0; c < diff_ttf; c += blockDim.x) {
     const unsigned e = c + threadIdx.x;
     bool seed = false;
@@ -525,23 +650,24 @@ __device__ void search_event(
       full = o.ttf[(prev_ttf + e) % max_tracks_to_follow];
       seed = (full & bit_seed) == bit_seed;
     }
-    const Scan3 s = block_flag_scan3(seed, false, false, sh.scan);
+    const Scan3 s = scan3(sh, cn, seed, false, false);
     if (seed) {
-      const unsigned p = sh.three_counter + s.r[0];
+      const unsigned p = cn.three + s.r[0];
       if (p < o.track_capacity) o.three_hit_tracks[p] = o.tracklets[full & bit_track_number];
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      sh.three_counter += s.t[0];
-      if (sh.three_counter > o.track_capacity) sh.status |= status_three_hit_slots_overflow;
-    }
-    __syncthreads();
+    cn.three += s.t[0];
+    if (cn.three > o.track_capacity) cn.status |= status_three_hit_slots_overflow;
   }
 }
 
-// One workgroup per event. Dynamic LDS holds `lds_hits` hits as x[], y[], z[] (f32),
-// phi[] (i16), used[] (u8). The `count` instantiation also accumulates the work counters
-// into work_stats[4] (one atomic per workgroup).
+// One workgroup per event. Dynamic LDS: the seeding scratch (13 B per lane), the ring of
+// tracks to follow (2048 x u32), then `lds_hits` hits as x[], y[], z[] (f32), phi[] (i16),
+// used[] (u8), module parity[] (u8). Events with more hits keep phi, used and parity in LDS
+// and read coordinates from global memory; beyond 4 x lds_hits everything stays global.
+// The `count` instantiation also accumulates the work counters into work_stats[4] (one
+// atomic per wave). With event_clock, each workgroup stores at event_clock[8 * event_number]
+// its event's start and end time (wall clock ticks) and, in the counting instantiation, the
+// ticks spent in each Phase.
 template<bool count>
 __global__ void k_search_by_triplet(
   const uint32_t* event_list,
@@ -563,18 +689,22 @@ __global__ void k_search_by_triplet(
   float max_scatter,
   unsigned max_skipped_modules,
   unsigned lds_hits,
-  unsigned long long* work_stats)
+  unsigned long long* work_stats,
+  unsigned long long* event_clock)
 {
+  const unsigned long long t_start = event_clock ? wall_clock64() : 0ull;
   extern __shared__ __align__(16) char dyn_all[];
   WorkCount wc;
-  // Dynamic LDS: seeding scratch (11 B per lane), then the event hits.
+  wc.last = t_start;
   SeedScratch ss;
   ss.pair_scatter = reinterpret_cast<float*>(dyn_all);
   ss.h1 = reinterpret_cast<uint16_t*>(ss.pair_scatter + blockDim.x);
   ss.h0 = ss.h1 + blockDim.x;
   ss.pair_h2 = ss.h0 + blockDim.x;
-  ss.n_h0 = reinterpret_cast<uint8_t*>(ss.pair_h2 + blockDim.x);
-  char* dyn = dyn_all + ((11 * blockDim.x + 15) & ~15u);
+  ss.h1_chunk = ss.pair_h2 + blockDim.x;
+  ss.n_h0 = reinterpret_cast<uint8_t*>(ss.h1_chunk + blockDim.x);
+  uint32_t* ring = reinterpret_cast<uint32_t*>(dyn_all + ((13 * blockDim.x + 15) & ~15u));
+  char* dyn = reinterpret_cast<char*>(ring + max_tracks_to_follow);
   __shared__ SearchShared sh;
   const unsigned event_number = event_list ? event_list[blockIdx.x] : blockIdx.x;
   const unsigned n_events = number_of_events[0];
@@ -593,68 +723,75 @@ __global__ void k_search_by_triplet(
   o.tracks = tracks_all + toff;
   o.three_hit_tracks = three_hit_all + toff;
   o.tracklets = tracklets_all + (size_t) event_number * max_tracks_to_follow;
-  o.ttf = ttf_all + (size_t) event_number * max_tracks_to_follow;
+  o.ttf = ring;
   o.h1_indices = rel_indices_all + hit_offset;
-  o.ids = cc.ids() + hit_offset;
   // Slots of this event; the reference lets a busy event run into its neighbour's slots, here
   // nothing is written past the container and the overflow is reported.
   o.track_capacity = min(next_off > toff ? next_off - toff : 0u, container - min(toff, container));
 
-  if (threadIdx.x == 0) {
-    sh.ttf_counter = 0;
-    sh.three_counter = 0;
-    sh.local_hits = 0;
-    sh.n_tracks = 0;
-    sh.status = n_hits > 65536u ? status_hit_index_overflow : 0u;
+  Counters cn;
+  cn.status = n_hits > 65536u ? status_hit_index_overflow : 0u;
+  if (threadIdx.x < n_module_pairs) {
+    const unsigned m = threadIdx.x;
+    sh.mp[m].hit_start = mhs[m] - hit_offset;
+    sh.mp[m].hit_num = mhn[m];
+    sh.mp[m].z[0] = geometry->module_zs[2 * m];
+    sh.mp[m].z[1] = geometry->module_zs[2 * m + 1];
   }
-  const Geometry& g = *geometry;
+  const uint32_t* gids = cc.ids() + hit_offset;
+  const int16_t* gph = cc.phis() + hit_offset;
+  const float* gxyz = cc.xyz() + 3 * (size_t) hit_offset;
+  uint8_t* gused = hit_used_all + hit_offset;
   if (n_hits <= lds_hits) {
     float* xs = reinterpret_cast<float*>(dyn);
     float* ys = xs + lds_hits;
     float* zs = ys + lds_hits;
     int16_t* ph = reinterpret_cast<int16_t*>(zs + lds_hits);
     uint8_t* used = reinterpret_cast<uint8_t*>(ph + lds_hits);
-    const float* gxyz = cc.xyz() + 3 * (size_t) hit_offset;
-    const int16_t* gph = cc.phis() + hit_offset;
+    uint8_t* odd = used + lds_hits;
     for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) {
       xs[i] = gxyz[3 * i];
       ys[i] = gxyz[3 * i + 1];
       zs[i] = gxyz[3 * i + 2];
       ph[i] = gph[i];
       used[i] = 0;
+      odd[i] = module_parity(gids[i]);
     }
     __syncthreads();
-    const LdsHits hits {xs, ys, zs, ph, used};
-    search_event<count>(wc, hits, ss, sh, o, mhs, mhn, g, phi_tolerance, max_scatter, max_skipped_modules);
+    const LdsHits hits {xs, ys, zs, ph, used, odd};
+    search_event<count>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
     __syncthreads();
-    uint8_t* gused = hit_used_all + hit_offset;
     for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) gused[i] = used[i];
   }
-  else if (3 * n_hits <= 15 * lds_hits) {
+  else if (n_hits <= 4 * lds_hits) {
     int16_t* ph = reinterpret_cast<int16_t*>(dyn);
     uint8_t* used = reinterpret_cast<uint8_t*>(ph + n_hits);
-    const int16_t* gph = cc.phis() + hit_offset;
+    uint8_t* odd = used + n_hits;
     for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) {
       ph[i] = gph[i];
       used[i] = 0;
+      odd[i] = module_parity(gids[i]);
     }
-    if (threadIdx.x == 0) sh.status |= status_lds_fallback;
+    cn.status |= status_lds_fallback;
     __syncthreads();
-    const MixedHits hits {cc.xyz() + 3 * (size_t) hit_offset, ph, used};
-    search_event<count>(wc, hits, ss, sh, o, mhs, mhn, g, phi_tolerance, max_scatter, max_skipped_modules);
+    const MixedHits hits {gxyz, ph, used, odd};
+    search_event<count>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
     __syncthreads();
-    uint8_t* gused = hit_used_all + hit_offset;
     for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) gused[i] = used[i];
   }
   else {
-    uint8_t* gused = hit_used_all + hit_offset;
     for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) gused[i] = 0;
-    if (threadIdx.x == 0) sh.status |= status_lds_fallback;
+    cn.status |= status_lds_fallback;
     __threadfence_block();
     __syncthreads();
-    const GlobalHits hits {cc.xyz() + 3 * (size_t) hit_offset, cc.phis() + hit_offset, gused};
-    search_event<count>(wc, hits, ss, sh, o, mhs, mhn, g, phi_tolerance, max_scatter, max_skipped_modules);
+    const GlobalHits hits {gxyz, gph, gused, gids};
+    search_event<count>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
+    __syncthreads();
   }
+  // The ring of tracks to follow as far as it was written (dev_tracks_to_follow).
+  uint32_t* gttf = ttf_all + (size_t) event_number * max_tracks_to_follow;
+  const unsigned ring_written = min(cn.ttf, max_tracks_to_follow);
+  for (unsigned i = threadIdx.x; i < ring_written; i += blockDim.x) gttf[i] = ring[i];
   if constexpr (count) {
     // Wave reduction, then one atomic per wave.
     unsigned long long v[4] = {wc.seed_eval, wc.fwd_eval, wc.extrap, wc.steps};
@@ -668,12 +805,68 @@ __global__ void k_search_by_triplet(
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t* at = atomics_all + event_number * num_atomics;
-    at[atomic_three_hit] = sh.three_counter;
-    at[atomic_ttf] = sh.ttf_counter;
-    at[atomic_local_hits] = sh.local_hits;
-    number_of_velo_tracks[event_number] = min(sh.n_tracks, o.track_capacity);
-    if (sh.status && status) atomicOr(status, sh.status);
+    at[atomic_three_hit] = cn.three;
+    at[atomic_ttf] = cn.ttf;
+    at[atomic_local_hits] = cn.local_hits;
+    number_of_velo_tracks[event_number] = min(cn.n_tracks, o.track_capacity);
+    if (cn.status && status) atomicOr(status, cn.status);
+    if (event_clock) {
+      unsigned long long* ec = event_clock + event_clock_words * event_number;
+      const unsigned long long t_end = wall_clock64();
+      ec[0] = t_start;
+      ec[1] = t_end;
+      if constexpr (count) {
+        phase_mark<count>(wc, phase_other);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) ec[2 + k] = wc.phase[k];
+      }
+    }
   }
+}
+
+// Events in descending order of their hit count (buckets of 32 hits, order within a bucket
+// unspecified), for launching search_by_triplet largest events first: with every event in
+// flight at once, the largest event sets the duration of the launch. Results do not depend on
+// the order. One workgroup of 1024 threads.
+constexpr unsigned order_buckets = 1024;
+__global__ void __launch_bounds__(1024) k_order_events_by_hits(
+  const uint32_t* offsets_estimated_input_size,
+  const uint32_t* number_of_events,
+  uint32_t* order)
+{
+  __shared__ uint32_t bucket[order_buckets];
+  __shared__ uint32_t scan_scratch[1024 / wave_size + 1];
+  const unsigned n = number_of_events[0];
+  for (unsigned b = threadIdx.x; b < order_buckets; b += blockDim.x) bucket[b] = 0;
+  __syncthreads();
+  auto key = [&](unsigned e) {
+    const unsigned hits =
+      offsets_estimated_input_size[(e + 1) * n_module_pairs] - offsets_estimated_input_size[e * n_module_pairs];
+    return order_buckets - 1 - min(hits >> 5, order_buckets - 1); // descending
+  };
+  for (unsigned e = threadIdx.x; e < n; e += blockDim.x) atomicAdd(&bucket[key(e)], 1u);
+  __syncthreads();
+  for (unsigned b0 = 0, carry = 0; b0 < order_buckets; b0 += blockDim.x) {
+    const unsigned b = b0 + threadIdx.x;
+    const uint32_t v = b < order_buckets ? bucket[b] : 0u;
+    uint32_t tot = 0;
+    const uint32_t ex = block_exclusive_scan(v, scan_scratch, &tot);
+    if (b < order_buckets) bucket[b] = carry + ex;
+    carry += tot;
+    __syncthreads();
+  }
+  for (unsigned e = threadIdx.x; e < n; e += blockDim.x) order[atomicAdd(&bucket[key(e)], 1u)] = e;
+}
+
+hipError_t launch_order_events_by_hits(
+  const uint32_t* offsets_estimated_input_size,
+  const uint32_t* number_of_events,
+  uint32_t* order,
+  hipStream_t s)
+{
+  hipLaunchKernelGGL(
+    k_order_events_by_hits, dim3(1), dim3(1024), 0, s, offsets_estimated_input_size, number_of_events, order);
+  return hipGetLastError();
 }
 
 // ThreeHitTracksFilter.cu:48-107
@@ -776,18 +969,32 @@ __global__ void k_three_hit_filter(
   if (threadIdx.x == 0) number_of_output[event_number] = base;
 }
 
-// Launch geometry: LDS budget for the event hits (bytes per hit: 3 x 4 + 2 + 1).
-constexpr unsigned search_bytes_per_hit = 15;
+// Launch geometry: LDS bytes per hit of the full layout (x, y, z, phi, used, parity).
+constexpr unsigned search_bytes_per_hit = 16;
+
+size_t search_dynamic_lds(unsigned block, unsigned lds_hits)
+{
+  return ((13 * (size_t) block + 15) & ~size_t(15)) + 4 * (size_t) max_tracks_to_follow +
+         (size_t) lds_hits * search_bytes_per_hit;
+}
 
 hipError_t launch_search_by_triplet(const SearchLaunch& a, hipStream_t s)
 {
   if (!a.n_list) return hipSuccess;
   const unsigned block = a.block_dim ? a.block_dim : 256;
-  // Dynamic LDS stays within 64 KiB per workgroup (with the static part).
-  const size_t seed_bytes = (11 * block + 15) & ~15u;
-  const unsigned max_hits = static_cast<unsigned>((65536 - 1024 - seed_bytes) / search_bytes_per_hit);
-  const unsigned lds_hits = min(a.lds_hits ? a.lds_hits : 2048u, max_hits);
-  const size_t dyn = seed_bytes + (size_t) lds_hits * search_bytes_per_hit;
+  // Dynamic LDS stays within the device's per-workgroup limit (less the static part).
+  static int max_lds = 0;
+  if (!max_lds) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      return hipErrorInvalidDevice;
+  }
+  const size_t fixed = search_dynamic_lds(block, 0) + 1024;
+  if ((size_t) max_lds <= fixed) return hipErrorInvalidConfiguration;
+  const unsigned max_hits = static_cast<unsigned>(((size_t) max_lds - fixed) / search_bytes_per_hit);
+  const unsigned lds_hits = min(a.lds_hits ? a.lds_hits : search_default_lds_hits, max_hits);
+  const size_t dyn = search_dynamic_lds(block, lds_hits);
 #define VELO_LAUNCH_SEARCH(C)                                                                                       \
   hipLaunchKernelGGL(                                                                                               \
     k_search_by_triplet<C>, dim3(a.n_list), dim3(block), dyn, s, a.event_list, a.number_of_events,                  \
@@ -796,7 +1003,7 @@ hipError_t launch_search_by_triplet(const SearchLaunch& a, hipStream_t s)
     reinterpret_cast<TrackletHits*>(a.tracklets), a.tracks_to_follow,                                               \
     reinterpret_cast<TrackletHits*>(a.three_hit_tracks), a.hit_used, a.atomics_velo, a.rel_indices,                \
     a.number_of_velo_tracks, a.status, a.phi_tolerance, a.max_scatter, a.max_skipped_modules, lds_hits,             \
-    a.work_stats)
+    a.work_stats, a.event_clock)
   if (a.work_stats) VELO_LAUNCH_SEARCH(true);
   else VELO_LAUNCH_SEARCH(false);
 #undef VELO_LAUNCH_SEARCH

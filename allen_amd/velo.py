@@ -117,6 +117,7 @@ SequenceInput = _params("SequenceInput", [
 SIGNATURES = {
     "velo_last_hip_error": (I, []),
     "velo_error_string": (ctypes.c_char_p, [I]),
+    "velo_wall_clock_khz": (I, [ctypes.POINTER(ctypes.c_uint)]),
     "velo_constants_create": (I, [ctypes.c_char_p, ctypes.c_size_t, P, ctypes.POINTER(P)]),
     "velo_constants_destroy": (I, [P]),
     "velo_constants_dev_geometry": (P, [P]),
@@ -174,6 +175,13 @@ def check(rc: int, what: str = ""):
         msg = lib().velo_error_string(rc).decode()
         hip = lib().velo_last_hip_error()
         raise VeloError(f"{what}: {msg} (code {rc}, hip error {hip})")
+
+
+def wall_clock_khz() -> int:
+    """Tick rate of the device wall clock (kHz), for the "search_event_clock" buffer."""
+    khz = ctypes.c_uint(0)
+    check(lib().velo_wall_clock_khz(ctypes.byref(khz)), "wall clock rate")
+    return int(khz.value)
 
 
 def cache_sp_patterns():

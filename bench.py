@@ -61,6 +61,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores (capped at 16)")
     ap.add_argument("--search-block", type=int, default=0)
     ap.add_argument("--search-lds-hits", type=int, default=0)
+    ap.add_argument("--search-event-order", choices=["largest_first", "natural"], default="largest_first")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB five times (the reference kernel for "
+                         "scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -72,6 +76,15 @@ def main():
     from allen_amd import synth, velo
 
     assert torch.cuda.is_available(), "no HIP device"
+    if args.copy_calibration:
+        n_bytes = args.copy_calibration << 20
+        src = torch.ones(n_bytes // 4, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n_bytes, "launches": 5}), flush=True)
+        return
     wl = WORKLOADS[args.workload]
     t0 = time.time()
     ev = synth.generate(**wl, keep_truth=False, n_threads=min(16, os.cpu_count() or 1))
@@ -84,6 +97,7 @@ def main():
         seq.set_tuning("search_block", args.search_block)
     if args.search_lds_hits:
         seq.set_tuning("search_lds_hits", args.search_lds_hits)
+    seq.set_tuning("search_largest_first", int(args.search_event_order == "largest_first"))
     stream = torch.cuda.current_stream()
 
     for _ in range(args.warmup):
@@ -111,6 +125,28 @@ def main():
     work = seq.fetch("search_work", np.uint64, 4)
     seq.set_tuning("count_work", 0)
     n = ev.n_events
+    # Per-event workgroup durations of search_by_triplet (counting run, wall clock).
+    tick_us = 1e3 / velo.wall_clock_khz()
+    clk = seq.fetch("search_event_clock", np.uint64, 8 * n).reshape(n, 8).astype(np.int64)
+    us = (clk[:, 1] - clk[:, 0]) * tick_us
+    span_us = (clk[:, 1].max() - clk[:, 0].min()) * tick_us
+    last_start_us = (clk[:, 0].max() - clk[:, 0].min()) * tick_us
+    event_hits = np.diff(seq.fetch("offsets_estimated_input_size", np.uint32, 26 * n + 1)[::26].astype(np.int64))
+    slope, intercept = (float(v) for v in np.polyfit(event_hits, us, 1)) if n > 1 else (0.0, 0.0)
+    big = int(np.argmax(event_hits))
+    med = int(np.argsort(event_hits)[n // 2])
+    phases = ["forwarding", "compaction", "seed_step1", "seed_step2", "seed_step3", "other"]
+    search_events = {"p50_us": round(float(np.median(us)), 2), "p90_us": round(float(np.percentile(us, 90)), 2),
+                     "max_us": round(float(us.max()), 2), "span_us": round(float(span_us), 2),
+                     "last_start_us": round(float(last_start_us), 2), "us_per_hit": round(slope, 5),
+                     "us_fixed": round(intercept, 2), "largest_event_hits": int(event_hits[big]),
+                     "largest_event_us": round(float(us[big]), 2),
+                     "largest_event_phase_us": {k: round(float(clk[big, 2 + i] * tick_us), 2)
+                                                for i, k in enumerate(phases)},
+                     "median_event_hits": int(event_hits[med]),
+                     "median_event_phase_us": {k: round(float(clk[med, 2 + i] * tick_us), 2)
+                                               for i, k in enumerate(phases)}}
+    log(f"[bench] search workgroups (counting run): {search_events}")
     mcn = seq.fetch("module_cluster_num", np.uint32, 26 * n)
     off_all = seq.fetch("offsets_all_velo_tracks", np.uint32, n + 1)
     atomics = seq.fetch("atomics_velo", np.uint32, 3 * n)
@@ -157,7 +193,8 @@ def main():
     extra = {"search_valu": {"ops_per_step": ops, "achieved_tops": round(search_tops, 4),
                              "peak_tops": round(VALU_PEAK_TOPS, 2), "frac": round(search_tops / VALU_PEAK_TOPS, 5),
                              "work": [int(x) for x in work]},
-             "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()}}
+             "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+             "search_events": search_events}
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -52,6 +52,8 @@ enum {
 
 int velo_last_hip_error(void);
 const char* velo_error_string(int code);
+/* Tick rate (kHz) of the device wall clock used by the "search_event_clock" buffer. */
+int velo_wall_clock_khz(unsigned* khz);
 
 /* ---- Constants: replaces Constants::dev_velo_geometry / dev_velo_sp_patterns / sp_fx / sp_fy /
  * dev_beamline filled by integration/non_event_data/src/VPGeometry.cpp:21-63 and Beamline.cpp:17-31. */
@@ -282,14 +284,21 @@ int velo_sequence_buffer(velo_sequence* s, const char* name, void** dev_ptr, siz
 int velo_sequence_status(velo_sequence* s, hipStream_t stream, unsigned* status);
 /* Knobs applied to subsequent runs:
  *   "search_block"      workgroup size of search_by_triplet (multiple of 64, <= 1024; 0 = 256)
- *   "search_lds_hits"   hits of an event staged in LDS by search_by_triplet (<= 4096; 0 = 2048);
- *                       larger events run on global memory
+ *   "search_lds_hits"   hits of an event staged in LDS by search_by_triplet (<= 8192, within the
+ *                       device's LDS per workgroup; 0 = 1760); up to 4x that many hits keep phi
+ *                       and flags in LDS and coordinates in global memory, larger events run on
+ *                       global memory
+ *   "search_largest_first"  1 (default): search_by_triplet takes the events in descending order
+ *                       of their hit count; 0: in event order. Outputs do not depend on it
  *   "fused_clustering"  1: clustering and phi sort fused per module pair (default); 0: the four
  *                       reference algorithms one after the other
  *   "timed_stage"       VELO_STAGE_* to bracket with HIP events on the run's stream (0 = none)
  *   "count_work"        1: search_by_triplet also counts its work into the buffer "search_work"
  *                       (u64 x 4: seeding evaluations, forwarding evaluations, extrapolations,
- *                       binary-search iterations) */
+ *                       binary-search iterations) and, per event, the start and end wall-clock
+ *                       ticks of its workgroup and the ticks spent in forwarding, h1 compaction,
+ *                       seeding steps 1-3 and the rest into "search_event_clock" (u64 x 8 per
+ *                       event) */
 int velo_sequence_set_tuning(velo_sequence* s, const char* knob, unsigned value);
 enum {
   VELO_STAGE_ALL = 1,              /* the whole run */

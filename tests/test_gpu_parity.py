@@ -61,6 +61,7 @@ TUNINGS = {
     "block1024": {"search_block": 1024, "search_lds_hits": 2048},
     "mixed_lds": {"search_block": 256, "search_lds_hits": 600},   # phi + used in LDS, xyz global
     "global_hits": {"search_block": 128, "search_lds_hits": 64},  # everything in global memory
+    "event_order": {"search_largest_first": 0},                   # events in their own order
 }
 
 
