@@ -205,7 +205,7 @@ hipError_t launch_cluster_sorted(const FusedClusterLaunch& a, hipStream_t s);
 hipError_t launch_estimate_mp(const EstimateMpLaunch& a, hipStream_t s);
 hipError_t launch_cluster_mp(const ClusterMpLaunch& a, hipStream_t s);
 hipError_t launch_search_by_triplet(const SearchLaunch& a, hipStream_t s);
-constexpr unsigned event_clock_words = 8; // start, end, ticks in 6 phases
+constexpr unsigned event_clock_words = 10; // start, end, ticks in 8 phases
 // Event hits staged in LDS by default: four workgroups of 256 lanes fit one CU's 160 KiB.
 constexpr unsigned search_default_lds_hits = 1760;
 size_t search_dynamic_lds(unsigned block, unsigned lds_hits);

@@ -111,13 +111,26 @@ struct GlobalHits {
 // search probes); only kept by the counting instantiation of the kernel.
 struct WorkCount {
   uint32_t seed_eval = 0, fwd_eval = 0, extrap = 0, steps = 0;
-  // Wall-clock ticks spent by the workgroup in each phase (kept by thread 0 right after a
-  // barrier): forwarding, compaction of the h1 hits, seeding steps 1, 2 and 3, the rest.
-  unsigned long long phase[6] = {0, 0, 0, 0, 0, 0};
+  // Wall-clock ticks spent by the workgroup in each Phase, kept by thread 0 (which takes part
+  // in every phase, so its marks follow the slowest wave only at barriers).
+  unsigned long long phase[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   unsigned long long last = 0;
 };
 
-enum Phase { phase_forwarding, phase_compaction, phase_seed1, phase_seed2, phase_seed3, phase_other };
+// forwarding; h1 compaction; seeding step 1 split into the h0 search (h1 and phi reads, then
+// the search in the previous module pair), the pendulum walk, and the wait at the barrier that
+// ends the step; seeding steps 2 and 3; the rest.
+enum Phase {
+  phase_forwarding,
+  phase_compaction,
+  phase_seed1_search,
+  phase_seed1_pendulum,
+  phase_seed1_wait,
+  phase_seed2,
+  phase_seed3,
+  phase_other,
+  n_phases
+};
 
 template<bool count>
 __device__ __forceinline__ void phase_mark(WorkCount& wc, Phase p)
@@ -327,6 +340,7 @@ __device__ void track_seeding(
           const unsigned h1_index = ss.h1_chunk[rel];
           const int16_t h1_phi = hits.phi(h1_index);
           const int start = counted_search<count>(hits.phi_begin() + prev.hit_start, prev.hit_num, h1_phi, wc);
+          phase_mark<count>(wc, phase_seed1_search);
           // Pendulum order around `start`: start, +1, -1, +2, -2, ... (each wrapped once into
           // the module pair). The used flags of 16 positions are read per round, all at once
           // (positions past the module pair read its first hit and are ignored).
@@ -350,11 +364,12 @@ __device__ void track_seeding(
             }
           }
           ss.h1[threadIdx.x] = h1_index;
+          phase_mark<count>(wc, phase_seed1_pendulum);
         }
         ss.n_h0[threadIdx.x] = n_found;
       }
       __syncthreads();
-      phase_mark<count>(wc, phase_seed1);
+      phase_mark<count>(wc, phase_seed1_wait);
 
       // 2. One (h1, h0) pair per lane: best h2 of this h0 (first strict minimum).
       {
@@ -665,7 +680,7 @@ __device__ void search_event(
 // used[] (u8), module parity[] (u8). Events with more hits keep phi, used and parity in LDS
 // and read coordinates from global memory; beyond 4 x lds_hits everything stays global.
 // The `count` instantiation also accumulates the work counters into work_stats[4] (one
-// atomic per wave). With event_clock, each workgroup stores at event_clock[8 * event_number]
+// atomic per wave). With event_clock, each workgroup stores at event_clock[10 * event_number]
 // its event's start and end time (wall clock ticks) and, in the counting instantiation, the
 // ticks spent in each Phase.
 template<bool count>
@@ -818,7 +833,7 @@ __global__ void k_search_by_triplet(
       if constexpr (count) {
         phase_mark<count>(wc, phase_other);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) ec[2 + k] = wc.phase[k];
+        for (int k = 0; k < n_phases; ++k) ec[2 + k] = wc.phase[k];
       }
     }
   }

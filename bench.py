@@ -127,7 +127,7 @@ def main():
     n = ev.n_events
     # Per-event workgroup durations of search_by_triplet (counting run, wall clock).
     tick_us = 1e3 / velo.wall_clock_khz()
-    clk = seq.fetch("search_event_clock", np.uint64, 8 * n).reshape(n, 8).astype(np.int64)
+    clk = seq.fetch("search_event_clock", np.uint64, 10 * n).reshape(n, 10).astype(np.int64)
     us = (clk[:, 1] - clk[:, 0]) * tick_us
     span_us = (clk[:, 1].max() - clk[:, 0].min()) * tick_us
     last_start_us = (clk[:, 0].max() - clk[:, 0].min()) * tick_us
@@ -135,7 +135,8 @@ def main():
     slope, intercept = (float(v) for v in np.polyfit(event_hits, us, 1)) if n > 1 else (0.0, 0.0)
     big = int(np.argmax(event_hits))
     med = int(np.argsort(event_hits)[n // 2])
-    phases = ["forwarding", "compaction", "seed_step1", "seed_step2", "seed_step3", "other"]
+    phases = ["forwarding", "compaction", "seed1_search", "seed1_pendulum", "seed1_wait", "seed_step2", "seed_step3",
+              "other"]
     search_events = {"p50_us": round(float(np.median(us)), 2), "p90_us": round(float(np.percentile(us, 90)), 2),
                      "max_us": round(float(us.max()), 2), "span_us": round(float(span_us), 2),
                      "last_start_us": round(float(last_start_us), 2), "us_per_hit": round(slope, 5),

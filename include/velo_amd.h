@@ -295,10 +295,10 @@ int velo_sequence_status(velo_sequence* s, hipStream_t stream, unsigned* status)
  *   "timed_stage"       VELO_STAGE_* to bracket with HIP events on the run's stream (0 = none)
  *   "count_work"        1: search_by_triplet also counts its work into the buffer "search_work"
  *                       (u64 x 4: seeding evaluations, forwarding evaluations, extrapolations,
- *                       binary-search iterations) and, per event, the start and end wall-clock
+ *                       phi-search probes) and, per event, the start and end wall-clock
  *                       ticks of its workgroup and the ticks spent in forwarding, h1 compaction,
- *                       seeding steps 1-3 and the rest into "search_event_clock" (u64 x 8 per
- *                       event) */
+ *                       seeding step 1 (h0 search, pendulum, barrier wait), seeding steps 2 and 3
+ *                       and the rest into "search_event_clock" (u64 x 10 per event) */
 int velo_sequence_set_tuning(velo_sequence* s, const char* knob, unsigned value);
 enum {
   VELO_STAGE_ALL = 1,              /* the whole run */
