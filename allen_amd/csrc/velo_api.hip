@@ -433,6 +433,7 @@ int velo_kalman_filter(const velo_kalman_filter_params* p, hipStream_t stream)
   a.is_backward = p->dev_is_backward;
   a.beamline_states = p->dev_velo_kalman_beamline_states;
   a.endvelo_states = p->dev_velo_kalman_endvelo_states;
+  a.fp64_fallback = p->covariance_fp64_fallback != 0;
   return hip_status(velo::launch_kalman_filter(a, stream));
 }
 
@@ -449,6 +450,7 @@ struct velo_sequence {
   unsigned search_block = 256;
   unsigned search_lds_hits = velo::search_default_lds_hits;
   bool search_largest_first = true;
+  unsigned kalman_fp64_fallback = 0;
   bool fused_clustering = true;
   bool count_work = false;
   // HIP events bracketing one stage of the run (0 = none), see velo_sequence_stage_ms.
@@ -581,6 +583,10 @@ int velo_sequence_set_tuning(velo_sequence* s, const char* knob, unsigned value)
   if (k == "search_lds_hits") {
     if (value > 8192) return VELO_ERR_INVALID_ARGUMENT;
     s->search_lds_hits = value ? value : velo::search_default_lds_hits;
+    return VELO_OK;
+  }
+  if (k == "kalman_fp64_fallback") {
+    s->kalman_fp64_fallback = value != 0;
     return VELO_OK;
   }
   if (k == "search_largest_first") {
@@ -855,6 +861,7 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream
   k9.beamline = s->constants->dev_beamline;
   k9.is_backward = s->get<uint8_t>("is_backward");
   k9.beamline_states = s->get<char>("velo_kalman_beamline_states");
+  k9.fp64_fallback = s->kalman_fp64_fallback;
   k9.endvelo_states = s->get<char>("velo_kalman_endvelo_states");
   VELO_MARK(mark_begin(VELO_STAGE_KALMAN));
   VELO_TRY(velo::launch_kalman_filter(k9, stream));

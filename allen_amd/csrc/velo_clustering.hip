@@ -911,14 +911,30 @@ __device__ __forceinline__ void wave_sync_lds()
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// The banks of module pair `idx` (= event * 26 + module pair) from the bank directory.
-__device__ __forceinline__ void
-load_banks(const char* raw, const BankEntry* directory, const uint32_t* directory_count, unsigned idx, Banks& B)
+// The bank directory record of one (event, module pair) as this lane holds it: lanes 0-7 load
+// one entry each, every lane the number of entries; both loads are issued together, without
+// waiting, so a wave can fetch the next item's record while it works on the current one.
+struct DirFetch {
+  BankEntry en;
+  unsigned nb;
+};
+
+__device__ __forceinline__ DirFetch fetch_dir(const BankEntry* directory, const uint32_t* directory_count, unsigned idx)
 {
-  const unsigned nb = directory_count[idx];
+  DirFetch f;
+  f.en = BankEntry {0, 0, 0, 0};
   const unsigned l = lane_id();
-  BankEntry en {0, 0, 0, 0};
-  if (l < nb) en = directory[static_cast<size_t>(idx) * max_banks_per_mp + l];
+  if (l < max_banks_per_mp) f.en = directory[static_cast<size_t>(idx) * max_banks_per_mp + l];
+  f.nb = min(directory_count[idx], max_banks_per_mp);
+  return f;
+}
+
+// The banks of one module pair from its directory record.
+__device__ __forceinline__ void load_banks(const char* raw, const DirFetch& f, Banks& B)
+{
+  const unsigned nb = f.nb;
+  const unsigned l = lane_id();
+  const BankEntry en = l < nb ? f.en : BankEntry {0, 0, 0, 0};
   const uint32_t inc = wave_inclusive_scan(l < nb ? en.n_sp : 0u);
   if (l < nb) {
     B.start[l] = inc - en.n_sp;
@@ -978,25 +994,19 @@ for_words(const Banks& B, unsigned n_words, bool resident, uint32_t* lw, bool ac
 }
 } // namespace mp
 
-template<int version>
-__global__ void __launch_bounds__(mp::waves_per_block* wave_size) k_estimate_mp(
+// K2 of one module pair (idx = event * 26 + module pair) by one wave.
+__device__ __forceinline__ void estimate_item(
   const char* raw,
-  const BankEntry* directory,
-  const uint32_t* directory_count,
-  unsigned n_events,
+  const mp::DirFetch& dir,
+  unsigned idx,
+  mp::Banks& B,
+  uint32_t* lw,
   uint32_t* estimated_input_size,
   uint8_t* sp_flags)
 {
-  __shared__ uint32_t s_words[mp::waves_per_block][mp::word_cap];
-  __shared__ mp::Banks s_banks[mp::waves_per_block];
-  const unsigned wave = threadIdx.x / wave_size;
-  const unsigned idx = blockIdx.x * mp::waves_per_block + wave;
-  if (idx >= n_events * n_module_pairs) return;
-  mp::Banks& B = s_banks[wave];
-  mp::load_banks(raw, directory, directory_count, idx, B);
+  mp::load_banks(raw, dir, B);
   const unsigned n_words = B.start[B.n];
   const bool resident = n_words <= mp::word_cap;
-  uint32_t* lw = s_words[wave];
   if (resident) mp::load_words(B, 0, n_words, lw);
   uint32_t count = 0;
   for (unsigned base = 0; base < n_words; base += wave_size) {
@@ -1029,46 +1039,73 @@ __global__ void __launch_bounds__(mp::waves_per_block* wave_size) k_estimate_mp(
   if (lane_id() == 0) estimated_input_size[idx] = count;
 }
 
+// Resident grid: each wave takes items idx, idx + stride, ... and fetches the next item's
+// bank directory record before working on the current one.
 template<int version>
-__global__ void __launch_bounds__(mp::waves_per_block* wave_size) k_cluster_mp(
+__global__ void __launch_bounds__(mp::waves_per_block* wave_size) k_estimate_mp(
   const char* raw,
   const BankEntry* directory,
   const uint32_t* directory_count,
   unsigned n_events,
+  uint32_t* estimated_input_size,
+  uint8_t* sp_flags)
+{
+  __shared__ uint32_t s_words[mp::waves_per_block][mp::word_cap];
+  __shared__ mp::Banks s_banks[mp::waves_per_block];
+  const unsigned wave = threadIdx.x / wave_size;
+  const unsigned n_items = n_events * n_module_pairs;
+  const unsigned stride = gridDim.x * mp::waves_per_block;
+  unsigned idx = blockIdx.x * mp::waves_per_block + wave;
+  if (idx >= n_items) return;
+  mp::DirFetch cur = mp::fetch_dir(directory, directory_count, idx);
+  for (; idx < n_items; idx += stride) {
+    mp::DirFetch next = cur;
+    if (idx + stride < n_items) next = mp::fetch_dir(directory, directory_count, idx + stride);
+    estimate_item(raw, cur, idx, s_banks[wave], s_words[wave], estimated_input_size, sp_flags);
+    cur = next;
+  }
+}
+
+// The wave-private LDS slices of k_cluster_mp.
+struct ClusterSlices {
+  uint32_t* lw;
+  int16_t* lphi;
+  uint32_t* lid;
+  float* lxyz;
+};
+
+// K3 + K4 of one module pair (idx = event * 26 + module pair) by one wave.
+template<int version>
+__device__ __forceinline__ void cluster_item(
+  const char* raw,
+  const mp::DirFetch& dir,
+  unsigned idx,
+  unsigned total,
   const uint32_t* offsets_estimated_input_size,
   const uint8_t* sp_flags,
-  const Geometry* geometry,
+  const Geometry& g,
   const uint8_t* sp_patterns,
   const float* sp_fx,
   const float* sp_fy,
   uint32_t* module_cluster_num,
   char* sorted_container,
-  char* unsorted_scratch)
+  char* unsorted_scratch,
+  mp::Banks& B,
+  const ClusterSlices& sl)
 {
-  __shared__ uint32_t s_words[mp::waves_per_block][mp::word_cap];
-  __shared__ int16_t s_phi[mp::waves_per_block][mp::cluster_cap];
-  __shared__ uint32_t s_id[mp::waves_per_block][mp::cluster_cap];
-  __shared__ float s_xyz[mp::waves_per_block][3 * mp::cluster_cap];
-  __shared__ mp::Banks s_banks[mp::waves_per_block];
   constexpr bool v23 = (version == 2 || version == 3);
-  const unsigned wave = threadIdx.x / wave_size;
-  const unsigned idx = blockIdx.x * mp::waves_per_block + wave;
-  if (idx >= n_events * n_module_pairs) return;
-  const unsigned total = offsets_estimated_input_size[n_module_pairs * n_events];
   const unsigned start = offsets_estimated_input_size[idx];
   const unsigned estimate = offsets_estimated_input_size[idx + 1] - start;
-  const Geometry& g = *geometry;
-  mp::Banks& B = s_banks[wave];
-  mp::load_banks(raw, directory, directory_count, idx, B);
+  mp::load_banks(raw, dir, B);
   const unsigned n_words = B.start[B.n];
   const bool resident = n_words <= mp::word_cap;
   const bool clusters_in_lds = estimate <= mp::cluster_cap;
-  uint32_t* lw = s_words[wave];
+  uint32_t* lw = sl.lw;
   if (resident) mp::load_words(B, 0, n_words, lw);
   const Clusters scratch {unsorted_scratch, total};
-  int16_t* lphi = s_phi[wave];
-  uint32_t* lid = s_id[wave];
-  float* lxyz = s_xyz[wave];
+  int16_t* lphi = sl.lphi;
+  uint32_t* lid = sl.lid;
+  float* lxyz = sl.lxyz;
   unsigned n = 0; // clusters so far (wave-uniform)
   // Appends up to two clusters per lane in lane order (wave scan of the per-lane counts).
   auto emit = [&](unsigned k, const ClusterOut& c0, const ClusterOut& c1) {
@@ -1192,10 +1229,75 @@ __global__ void __launch_bounds__(mp::waves_per_block* wave_size) k_cluster_mp(
   }
 }
 
+// Resident grid: each wave takes items idx, idx + stride, ... and fetches the next item's
+// bank directory record before working on the current one.
+template<int version>
+__global__ void __launch_bounds__(mp::waves_per_block* wave_size) k_cluster_mp(
+  const char* raw,
+  const BankEntry* directory,
+  const uint32_t* directory_count,
+  unsigned n_events,
+  const uint32_t* offsets_estimated_input_size,
+  const uint8_t* sp_flags,
+  const Geometry* geometry,
+  const uint8_t* sp_patterns,
+  const float* sp_fx,
+  const float* sp_fy,
+  uint32_t* module_cluster_num,
+  char* sorted_container,
+  char* unsorted_scratch)
+{
+  __shared__ uint32_t s_words[mp::waves_per_block][mp::word_cap];
+  __shared__ int16_t s_phi[mp::waves_per_block][mp::cluster_cap];
+  __shared__ uint32_t s_id[mp::waves_per_block][mp::cluster_cap];
+  __shared__ float s_xyz[mp::waves_per_block][3 * mp::cluster_cap];
+  __shared__ mp::Banks s_banks[mp::waves_per_block];
+  const unsigned wave = threadIdx.x / wave_size;
+  const unsigned n_items = n_events * n_module_pairs;
+  const unsigned stride = gridDim.x * mp::waves_per_block;
+  unsigned idx = blockIdx.x * mp::waves_per_block + wave;
+  if (idx >= n_items) return;
+  const unsigned total = offsets_estimated_input_size[n_items];
+  const ClusterSlices sl {s_words[wave], s_phi[wave], s_id[wave], s_xyz[wave]};
+  mp::DirFetch cur = mp::fetch_dir(directory, directory_count, idx);
+  for (; idx < n_items; idx += stride) {
+    mp::DirFetch next = cur;
+    if (idx + stride < n_items) next = mp::fetch_dir(directory, directory_count, idx + stride);
+    cluster_item<version>(
+      raw, cur, idx, total, offsets_estimated_input_size, sp_flags, *geometry, sp_patterns, sp_fx, sp_fy,
+      module_cluster_num, sorted_container, unsorted_scratch, s_banks[wave], sl);
+    cur = next;
+  }
+}
+
+// Blocks of `kernel` that stay resident on the whole device (0 if the runtime cannot tell).
+static unsigned resident_blocks(const void* kernel, unsigned block)
+{
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, static_cast<int>(block), 0) != hipSuccess)
+    return 0;
+  return static_cast<unsigned>(cus) * static_cast<unsigned>(per_cu);
+}
+
+// One wave per item up to the resident capacity; beyond it the waves loop.
+static unsigned mp_grid(const void* kernel, unsigned n_items, unsigned* cache)
+{
+  if (!*cache) *cache = resident_blocks(kernel, mp::waves_per_block * wave_size);
+  const unsigned need = (n_items + mp::waves_per_block - 1) / mp::waves_per_block;
+  return *cache ? min(need, *cache) : need;
+}
+
 hipError_t launch_estimate_mp(const EstimateMpLaunch& a, hipStream_t s)
 {
   if (!a.n_events) return hipSuccess;
-  const unsigned grid = (a.n_events * n_module_pairs + mp::waves_per_block - 1) / mp::waves_per_block;
+  static unsigned cache[5] = {0, 0, 0, 0, 0};
+  const unsigned v = a.version == 2 || a.version == 3 ? a.version : 4;
+  const void* kernel = v == 2 ? reinterpret_cast<const void*>(&k_estimate_mp<2>)
+                     : v == 3 ? reinterpret_cast<const void*>(&k_estimate_mp<3>)
+                              : reinterpret_cast<const void*>(&k_estimate_mp<4>);
+  const unsigned grid = mp_grid(kernel, a.n_events * n_module_pairs, &cache[v]);
 #define VELO_LAUNCH_EMP(V)                                                                                          \
   hipLaunchKernelGGL(                                                                                               \
     k_estimate_mp<V>, dim3(grid), dim3(mp::waves_per_block * wave_size), 0, s, a.raw,                               \
@@ -1212,7 +1314,12 @@ hipError_t launch_estimate_mp(const EstimateMpLaunch& a, hipStream_t s)
 hipError_t launch_cluster_mp(const ClusterMpLaunch& a, hipStream_t s)
 {
   if (!a.n_events) return hipSuccess;
-  const unsigned grid = (a.n_events * n_module_pairs + mp::waves_per_block - 1) / mp::waves_per_block;
+  static unsigned cache[5] = {0, 0, 0, 0, 0};
+  const unsigned v = a.version == 2 || a.version == 3 ? a.version : 4;
+  const void* kernel = v == 2 ? reinterpret_cast<const void*>(&k_cluster_mp<2>)
+                     : v == 3 ? reinterpret_cast<const void*>(&k_cluster_mp<3>)
+                              : reinterpret_cast<const void*>(&k_cluster_mp<4>);
+  const unsigned grid = mp_grid(kernel, a.n_events * n_module_pairs, &cache[v]);
 #define VELO_LAUNCH_CMP(V)                                                                                          \
   hipLaunchKernelGGL(                                                                                               \
     k_cluster_mp<V>, dim3(grid), dim3(mp::waves_per_block * wave_size), 0, s, a.raw,                                \

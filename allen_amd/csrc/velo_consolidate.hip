@@ -219,6 +219,14 @@ struct KState {
   float c00, c20, c22, c11, c31, c33;
 };
 
+// FP64 covariance fallback (no reference counterpart): the FP32 update (1 - Kx) * predcovXX
+// loses about kappa * 2^-24 of relative accuracy, kappa = 1 + whit * predcovXX (about 1e7 on a
+// track's first step). Above kappa = 100 the step's covariance prediction and update run in
+// FP64 from the stored FP32 covariance, the update as (1 - Kx) = R / whit, and the three terms
+// are rounded to FP32; the state update keeps the reference's FP32 arithmetic.
+constexpr float kalman_kappa_limit_minus_one = 99.f;
+
+template<bool fp64_fallback>
 __device__ __forceinline__ void
 filter_step(float z, float zhit, float xhit, float whit, float& x, float& tx, float& covXX, float& covXTx, float& covTxTx)
 {
@@ -235,12 +243,24 @@ filter_step(float z, float zhit, float xhit, float whit, float& x, float& tx, fl
   const float r = xhit - predx;
   x = predx + Kx * r;
   tx = tx + KTx * r;
+  if (fp64_fallback && predcovXX * whit > kalman_kappa_limit_minus_one) {
+    const double dzd = dz, c00 = covXX, c20 = covXTx, c22 = covTxTx;
+    const double pXTx = c20 + dzd * c22;
+    const double pXX = c00 + 2.0 * (dzd * c20) + dzd * (dzd * c22);
+    const double pTxTx = c22;
+    const double iw = 1.0 / static_cast<double>(whit);
+    const double Rd = 1.0 / (iw + pXX);
+    covXX = static_cast<float>(pXX * iw * Rd);
+    covXTx = static_cast<float>(pXTx * iw * Rd);
+    covTxTx = static_cast<float>(pTxTx - pXTx * pXTx * Rd);
+    return;
+  }
   covXX = (1 - Kx) * predcovXX;
   covXTx = (1 - Kx) * predcovXTx;
   covTxTx = predcovTxTx - KTx * predcovXTx;
 }
 
-template<bool upstream>
+template<bool upstream, bool fp64_fallback>
 __device__ __forceinline__ KState
 simplified_fit(const float* xyz, unsigned nh, float at_tx, float at_ty, const float* beamline, bool backward)
 {
@@ -271,8 +291,8 @@ simplified_fit(const float* xyz, unsigned nh, float at_tx, float at_ty, const fl
     const float hx = xyz[3 * i], hy = xyz[3 * i + 1], hz = xyz[3 * i + 2];
     s.c22 += noise2PerLayer;
     s.c33 += noise2PerLayer;
-    filter_step(s.z, hz, hx, param_w, s.x, s.tx, s.c00, s.c20, s.c22);
-    filter_step(s.z, hz, hy, param_w, s.y, s.ty, s.c11, s.c31, s.c33);
+    filter_step<fp64_fallback>(s.z, hz, hx, param_w, s.x, s.tx, s.c00, s.c20, s.c22);
+    filter_step<fp64_fallback>(s.z, hz, hy, param_w, s.y, s.ty, s.c11, s.c31, s.c33);
     s.z = hz;
   }
   s.c22 += noise2PerLayer;
@@ -321,7 +341,8 @@ __global__ void k_kalman_filter(
   const float* beamline,
   uint8_t* is_backward,
   float* beamline_states,
-  float* endvelo_states)
+  float* endvelo_states,
+  unsigned fp64_fallback)
 {
   const unsigned e = event_list ? event_list[blockIdx.x] : blockIdx.x;
   const unsigned n_events = number_of_events[0];
@@ -343,10 +364,18 @@ __global__ void k_kalman_filter(
     const float lin_z = lz + ldz;
     const bool backward = lin_z > fz;
     is_backward[t] = backward;
-    const KState b = simplified_fit<true>(xyz, nh, ltx, lty, bl, backward);
-    const KState v = simplified_fit<false>(xyz, nh, b.tx, b.ty, bl, backward);
-    store_state(beamline_states, total_tracks, t, b);
-    store_state(endvelo_states, total_tracks, t, v);
+    if (fp64_fallback) {
+      const KState b = simplified_fit<true, true>(xyz, nh, ltx, lty, bl, backward);
+      const KState v = simplified_fit<false, true>(xyz, nh, b.tx, b.ty, bl, backward);
+      store_state(beamline_states, total_tracks, t, b);
+      store_state(endvelo_states, total_tracks, t, v);
+    }
+    else {
+      const KState b = simplified_fit<true, false>(xyz, nh, ltx, lty, bl, backward);
+      const KState v = simplified_fit<false, false>(xyz, nh, b.tx, b.ty, bl, backward);
+      store_state(beamline_states, total_tracks, t, b);
+      store_state(endvelo_states, total_tracks, t, v);
+    }
   }
 }
 
@@ -377,7 +406,7 @@ hipError_t launch_kalman_filter(const KalmanLaunch& a, hipStream_t s)
   hipLaunchKernelGGL(
     k_kalman_filter, dim3(a.n_list), dim3(256), 0, s, a.event_list, a.number_of_events, a.offsets_all_velo_tracks,
     a.offsets_velo_track_hit_number, a.velo_track_hits, a.beamline, a.is_backward,
-    reinterpret_cast<float*>(a.beamline_states), reinterpret_cast<float*>(a.endvelo_states));
+    reinterpret_cast<float*>(a.beamline_states), reinterpret_cast<float*>(a.endvelo_states), a.fp64_fallback);
   return hipGetLastError();
 }
 

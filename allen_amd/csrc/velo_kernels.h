@@ -195,6 +195,7 @@ struct KalmanLaunch {
   uint8_t* is_backward;
   char* beamline_states;
   char* endvelo_states;
+  unsigned fp64_fallback = 0; // FP64 covariance step where the FP32 update is ill-conditioned
 };
 
 hipError_t launch_number_of_candidates(const CandidatesLaunch& a, hipStream_t s);

@@ -102,7 +102,8 @@ KalmanParams = _params("KalmanParams", [
     ("host_number_of_reconstructed_velo_tracks", U), ("host_number_of_events", U), ("dev_event_list", P),
     ("event_list_size", U), ("dev_number_of_events", P), ("dev_offsets_all_velo_tracks", P),
     ("dev_offsets_velo_track_hit_number", P), ("dev_velo_track_hits", P), ("dev_is_backward", P),
-    ("dev_velo_kalman_beamline_states", P), ("dev_velo_kalman_endvelo_states", P), ("constants", P)])
+    ("dev_velo_kalman_beamline_states", P), ("dev_velo_kalman_endvelo_states", P), ("constants", P),
+    ("covariance_fp64_fallback", U)])
 
 PrefixSumParams = _params("PrefixSumParams", [
     ("dev_input_buffer", P), ("input_size", U), ("dev_output_buffer", P), ("dev_scratch", P),

@@ -254,6 +254,11 @@ typedef struct {
   char* dev_velo_kalman_beamline_states;     /* 56 x tracks bytes */
   char* dev_velo_kalman_endvelo_states;      /* 56 x tracks bytes */
   const velo_constants* constants;
+  /* 0: the reference's FP32 arithmetic. 1: a filter step whose FP32 covariance update is
+   * ill-conditioned (kappa = 1 + 6400 * predcovXX > 100; every track's first step) has its
+   * covariance prediction and update computed in FP64 and rounded to FP32 (no reference
+   * counterpart; covariance within ~1e-6 of an FP64 fit instead of up to ~50 %). */
+  unsigned covariance_fp64_fallback;
 } velo_kalman_filter_params;
 int velo_kalman_filter(const velo_kalman_filter_params* p, hipStream_t stream);
 
@@ -293,6 +298,7 @@ int velo_sequence_status(velo_sequence* s, hipStream_t stream, unsigned* status)
  *   "fused_clustering"  1: clustering and phi sort fused per module pair (default); 0: the four
  *                       reference algorithms one after the other
  *   "timed_stage"       VELO_STAGE_* to bracket with HIP events on the run's stream (0 = none)
+ *   "kalman_fp64_fallback"  1: velo_kalman_filter's covariance_fp64_fallback (default 0)
  *   "count_work"        1: search_by_triplet also counts its work into the buffer "search_work"
  *                       (u64 x 4: seeding evaluations, forwarding evaluations, extrapolations,
  *                       phi-search probes) and, per event, the start and end wall-clock

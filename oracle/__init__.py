@@ -3,6 +3,8 @@
 Only tests/, ``__graft_entry__.smoke()`` and bench.py's cpu_baseline leg may import this
 package.  It loads ``oracle/build/libvelo_oracle.so`` (built by ``oracle/Makefile``), a
 from-scratch C++ restatement of Allen's VELO-SP sequence with CPU-target semantics.
+What pins each part, and which parts are "parity unpinned", is listed in the header of
+``oracle/common.hpp``.
 """
 from __future__ import annotations
 
@@ -33,6 +35,7 @@ class VoInputs(ctypes.Structure):
         ("max_chi2", ctypes.c_float),
         ("block_dim_x", ctypes.c_uint),
         ("compute_f64", ctypes.c_int),
+        ("kalman_fp64_fallback", ctypes.c_int),
     ]
 
 

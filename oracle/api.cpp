@@ -66,6 +66,7 @@ struct Params {
   float max_chi2 = 20.f;
   unsigned block_dim_x = 256;
   bool f64 = true;
+  bool kalman_fp64_fallback = false;
 };
 
 void run_sequence(
@@ -153,7 +154,7 @@ void run_sequence(
   auto* ev = s.make<float>("velo_kalman_endvelo_states", (size_t) nt * 14);
   double* bl64 = prm.f64 ? s.make<double>("velo_kalman_beamline_states_f64", (size_t) nt * 14) : nullptr;
   double* ev64 = prm.f64 ? s.make<double>("velo_kalman_endvelo_states_f64", (size_t) nt * 14) : nullptr;
-  kalman_filter(el, nl, n_events, off_all, off_hits, track_hits, beamline, bwd, bl, ev, bl64, ev64);
+  kalman_filter(el, nl, n_events, off_all, off_hits, track_hits, beamline, bwd, bl, ev, bl64, ev64, prm.kalman_fp64_fallback);
 
   auto* st = s.make<uint64_t>("search_stats", 4);
   st[0] = s.stats.seed_evaluations;
@@ -185,6 +186,7 @@ bool make_inputs(const vo_inputs* a, vo::RawInput& in, vo::Geometry& g, vo::SpTa
   p.max_chi2 = a->max_chi2;
   p.block_dim_x = a->block_dim_x ? a->block_dim_x : 256;
   p.f64 = a->compute_f64 != 0;
+  p.kalman_fp64_fallback = a->kalman_fp64_fallback != 0;
   return true;
 }
 } // namespace

@@ -10,6 +10,19 @@
 //
 // Execution semantics followed: backend/include/CPUBackend.h:99-145,
 // stream/gear/include/Invoke.cuh:40-52.
+//
+// What pins each part (tests/test_oracle_pins.py): the super-pixel tables by a textbook
+// labelling; LHCbIDs, cluster membership and positions by a textbook 8-connected flood fill
+// (float64 positions) on the bundled MDF's real banks and on synthetic banks; fast_atan2f /
+// hit_phi_16 by special angles and the closed-form error bound; the binary search by bisect;
+// the prefix sum by the reference's own test shape (TestPrefixSum.cu:9-33); the phi sort and
+// search by the reference's contracts plus known-answer single tracks; the three-hit chi2 by
+// the closed-form 3-point fit; the FP64 Kalman by a textbook matrix filter.
+// Parity unpinned (the restatement follows the reference's code and nothing independent
+// fixes the result): K2's estimate for non-isolated SPs (EstimateInputSize.cu:104-204),
+// rest_of_clusters' decisions at the edges of its window, seeding tie-breaks of the CPU
+// target's umesimd track_seeding_vectorized (SearchByTriplet.cu:616-818; the scalar
+// track_seeding is followed), and the ring of tracks to follow beyond 2048 live entries.
 #pragma once
 
 #include <cstdint>

@@ -106,8 +106,25 @@ struct Mini {
 };
 
 // VeloKalmanFilter.cuh:25-60
+// Conditioning limit of the FP64 covariance fallback: the FP32 update (1 - Kx) * predcovXX loses
+// about kappa * 2^-24 of relative accuracy, kappa = 1 + whit * predcovXX; above kappa = 100 the
+// fallback recomputes the step's covariance prediction and update in FP64, the update in a form
+// without that cancellation ((1 - Kx) = R / whit), and rounds the three terms to FP32. The state
+// update keeps the reference's FP32 arithmetic (no reference counterpart).
+constexpr float kalman_kappa_limit_minus_one = 99.f;
+
 template<typename F>
-inline void filter_step(F z, F zhit, F xhit, F whit, F& x, F& tx, F& covXX, F& covXTx, F& covTxTx)
+inline void filter_step(
+  F z,
+  F zhit,
+  F xhit,
+  F whit,
+  F& x,
+  F& tx,
+  F& covXX,
+  F& covXTx,
+  F& covTxTx,
+  bool fp64_fallback = false)
 {
   const F dz = zhit - z;
   const F predx = x + dz * tx;
@@ -122,6 +139,20 @@ inline void filter_step(F z, F zhit, F xhit, F whit, F& x, F& tx, F& covXX, F& c
   const F r = xhit - predx;
   x = predx + Kx * r;
   tx = tx + KTx * r;
+  if (fp64_fallback && predcovXX * whit > F(kalman_kappa_limit_minus_one)) {
+    // The prediction is ill-conditioned too (covXX is a few ulps of dz^2 * covTxTx), so the
+    // whole covariance part of the step runs in FP64 from the stored FP32 covariance.
+    const double dzd = dz, c00 = covXX, c20 = covXTx, c22 = covTxTx;
+    const double pXTx = c20 + dzd * c22;
+    const double pXX = c00 + 2.0 * (dzd * c20) + dzd * (dzd * c22);
+    const double pTxTx = c22;
+    const double iw = 1.0 / static_cast<double>(whit);
+    const double Rd = 1.0 / (iw + pXX);
+    covXX = static_cast<F>(pXX * iw * Rd);
+    covXTx = static_cast<F>(pXTx * iw * Rd);
+    covTxTx = static_cast<F>(pTxTx - pXTx * pXTx * Rd);
+    return;
+  }
   covXX = (1 - Kx) * predcovXX;
   covXTx = (1 - Kx) * predcovXTx;
   covTxTx = predcovTxTx - KTx * predcovXTx;
@@ -129,7 +160,13 @@ inline void filter_step(F z, F zhit, F xhit, F whit, F& x, F& tx, F& covXX, F& c
 
 // VeloKalmanFilter.cuh:67-158
 template<bool upstream, typename F>
-State<F> simplified_fit(const float* xyz, unsigned nh, const Mini<F>& at_beamline, const F* beamline, bool backward)
+State<F> simplified_fit(
+  const float* xyz,
+  unsigned nh,
+  const Mini<F>& at_beamline,
+  const F* beamline,
+  bool backward,
+  bool fp64_fallback)
 {
   const int direction = (backward ? 1 : -1) * (upstream ? 1 : -1);
   const F noise2PerLayer = F(1e-8f) + F(7e-6f) * (at_beamline.tx * at_beamline.tx + at_beamline.ty * at_beamline.ty);
@@ -156,8 +193,8 @@ State<F> simplified_fit(const float* xyz, unsigned nh, const Mini<F>& at_beamlin
     const F hx = xyz[3 * i], hy = xyz[3 * i + 1], hz = xyz[3 * i + 2];
     s.c22 += noise2PerLayer;
     s.c33 += noise2PerLayer;
-    filter_step<F>(s.z, hz, hx, F(param_w), s.x, s.tx, s.c00, s.c20, s.c22);
-    filter_step<F>(s.z, hz, hy, F(param_w), s.y, s.ty, s.c11, s.c31, s.c33);
+    filter_step<F>(s.z, hz, hx, F(param_w), s.x, s.tx, s.c00, s.c20, s.c22, fp64_fallback);
+    filter_step<F>(s.z, hz, hy, F(param_w), s.y, s.ty, s.c11, s.c31, s.c33, fp64_fallback);
     s.z = hz;
   }
   s.c22 += noise2PerLayer;
@@ -218,13 +255,20 @@ inline void store_state(Out* base, unsigned total_tracks, unsigned i, const Stat
 
 // Per-track work of the kernel (VeloKalmanFilter.cu:386-417), in precision F.
 template<typename F>
-void fit_track(const float* xyz, unsigned nh, const F* beamline, bool& backward, State<F>& bl, State<F>& ev)
+void fit_track(
+  const float* xyz,
+  unsigned nh,
+  const F* beamline,
+  bool& backward,
+  State<F>& bl,
+  State<F>& ev,
+  bool fp64_fallback = false)
 {
   const Mini<F> lin = linear_fit<F>(xyz, nh, beamline);
   backward = lin.z > F(xyz[2]);
-  bl = simplified_fit<true, F>(xyz, nh, lin, beamline, backward);
+  bl = simplified_fit<true, F>(xyz, nh, lin, beamline, backward, fp64_fallback);
   const Mini<F> blm {bl.x, bl.y, bl.z, bl.tx, bl.ty};
-  ev = simplified_fit<false, F>(xyz, nh, blm, beamline, backward);
+  ev = simplified_fit<false, F>(xyz, nh, blm, beamline, backward, fp64_fallback);
 }
 } // namespace
 
@@ -242,7 +286,8 @@ void kalman_filter(
   float* beamline_states,
   float* endvelo_states,
   double* beamline_states_f64,
-  double* endvelo_states_f64)
+  double* endvelo_states_f64,
+  bool fp64_fallback)
 {
   const unsigned total_tracks = offsets_all_velo_tracks[n_events];
   const unsigned total_hits = offsets_velo_track_hit_number[total_tracks];
@@ -256,7 +301,7 @@ void kalman_filter(
       const float* xyz = xyz_all + 3 * (size_t) h0;
       bool backward;
       State<float> bl, ev;
-      fit_track<float>(xyz, nh, beamline, backward, bl, ev);
+      fit_track<float>(xyz, nh, beamline, backward, bl, ev, fp64_fallback);
       is_backward[t] = backward;
       store_state(beamline_states, total_tracks, t, bl);
       store_state(endvelo_states, total_tracks, t, ev);

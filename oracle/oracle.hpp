@@ -152,7 +152,8 @@ void kalman_filter(
   float* beamline_states,
   float* endvelo_states,
   double* beamline_states_f64,
-  double* endvelo_states_f64);
+  double* endvelo_states_f64,
+  bool fp64_fallback);
 
 bool load_geometry(const char* bytes, size_t n, Geometry& g);
 

@@ -27,6 +27,7 @@ typedef struct {
   float max_chi2;             /* velo_three_hit_tracks_filter property */
   unsigned block_dim_x;       /* velo_calculate_number_of_candidates block size (decides grid) */
   int compute_f64;            /* also run the FP64 Kalman restatement */
+  int kalman_fp64_fallback;   /* FP32 Kalman with the FP64 covariance update on ill-conditioned steps */
 } vo_inputs;
 
 void vo_default_inputs(vo_inputs* a);

@@ -27,11 +27,11 @@ def constants(geometry, beamline):
     c.close()
 
 
-def run_both(name, geometry, beamline, constants, fused, tuning=None):
+def run_both(name, geometry, beamline, constants, fused, tuning=None, oracle_props=None):
     ev = synth.generate(**DATASETS[name])
     print(f"[parity] dataset={name} events={ev.n_events} raw_bytes={ev.raw.size} fused={fused} tuning={tuning}",
           flush=True)
-    ref = oracle.run(ev, geometry, beamline)
+    ref = oracle.run(ev, geometry, beamline, **(oracle_props or {}))
     dev = velo.DeviceEvents(ev)
     seq = velo.VeloSequence(constants, ev.n_events, max(ev.raw.size, 4096))
     seq.set_tuning("fused_clustering", int(fused))
@@ -69,6 +69,16 @@ TUNINGS = {
 def test_sequence_tuning_variants(tuning, geometry, beamline, constants):
     """Launch geometries and LDS layouts do not change a single bit of the result."""
     ref, gpu, status = run_both("pileup", geometry, beamline, constants, True, TUNINGS[tuning])
+    assert status & velo.STATUS_OVERFLOW_MASK == 0
+    errs = parity.compare(gpu, ref, True)
+    assert not errs, "\n".join(errs[:20])
+
+
+def test_kalman_fp64_covariance_fallback(geometry, beamline, constants):
+    """With the FP64 covariance fallback on, the GPU states and covariances equal the oracle's
+    fallback bit for bit (same FP64 operations, no contraction on either side)."""
+    ref, gpu, status = run_both("pileup", geometry, beamline, constants, True, {"kalman_fp64_fallback": 1},
+                                {"kalman_fp64_fallback": 1})
     assert status & velo.STATUS_OVERFLOW_MASK == 0
     errs = parity.compare(gpu, ref, True)
     assert not errs, "\n".join(errs[:20])

@@ -438,6 +438,42 @@ def test_kalman_fp32_error_and_truth(geometry, beamline):
     assert len(dtx) > 1000 and abs(np.mean(dtx)) < 1e-4
 
 
+def _normalised_cov_error(c32, c64):
+    """|c32 - c64| per term, diagonal terms relative, off-diagonal terms over sqrt(c_ii c_jj)."""
+    pairs = [(0, 0), (0, 2), (2, 2), (3, 3), (3, 5), (5, 5)]
+    e = np.empty_like(c64)
+    for k, (i, j) in enumerate(pairs):
+        e[:, k] = np.abs(c32[:, k] - c64[:, k]) / np.sqrt(np.abs(c64[:, i] * c64[:, j]))
+    return e
+
+
+def test_kalman_fp64_covariance_fallback_against_fp64(geometry, beamline):
+    """The FP32 fit's covariance cancels on ill-conditioned steps (kappa = 1 + 6400 predcovXX up to
+    ~1e7 on the first step): up to tens of percent against the FP64 fit. With the FP64 covariance
+    fallback the FP32 fit stays within FP32 storage precision of the FP64 fit, and so do the states
+    (the gains no longer carry the covariance error)."""
+    ev = synth.generate(n_events=40, pileup=1, nu=7.6, seed=31)
+    plain = oracle.run(ev, geometry, beamline)
+    fb = oracle.run(ev, geometry, beamline, kalman_fp64_fallback=1)
+    n = int(plain["offsets_all_velo_tracks"][-1])
+    assert n > 1000
+    for name in ["velo_kalman_beamline_states", "velo_kalman_endvelo_states"]:
+        f64 = fb[name + "_f64"][:14 * n]
+        c64 = f64[6 * n:].reshape(-1, 8)[:, :6]
+        s64 = f64[:6 * n].reshape(-1, 6)[:, :5]
+        e_plain = _normalised_cov_error(plain[name][6 * n:14 * n].reshape(-1, 8)[:, :6].astype(np.float64), c64)
+        c_fb = fb[name][6 * n:14 * n].reshape(-1, 8)[:, :6].astype(np.float64)
+        e_fb = _normalised_cov_error(c_fb, c64)
+        assert np.quantile(e_plain, 0.99) > 1e-2          # the reference's FP32 arithmetic
+        assert e_fb.max() < 1e-5, e_fb.max()              # with the fallback
+        d = np.abs(fb[name][:6 * n].reshape(-1, 6)[:, :5].astype(np.float64) - s64)
+        assert d[:, 3:].max() < 1e-6                      # slopes
+        assert d[:, :2].max() < 2e-4                      # x, y (mm)
+    # The fallback leaves the FP64 restatement and the track list untouched.
+    np.testing.assert_array_equal(plain["velo_kalman_beamline_states_f64"], fb["velo_kalman_beamline_states_f64"])
+    np.testing.assert_array_equal(plain["is_backward"], fb["is_backward"])
+
+
 def test_kalman_backward_flag(geometry, beamline):
     ev = synth.generate(n_events=10, mean_tracks=100.0, backward_fraction=0.5, seed=26)
     r = oracle.run(ev, geometry, beamline)
