@@ -1,0 +1,138 @@
+"""Autotuner for the launch knobs of the VELO sequence.
+
+It sweeps the knobs of ``search_by_triplet``, the dominant kernel, on the caller's own events:
+- workgroup size (64-1024);
+- the LDS hit budget per event (256-4096 hits);
+- persistent (work-stealing) mode;
+- event order.
+
+Each candidate is timed with the sequence's own stage events (HIP events on the run's stream). The fastest configuration is kept only if its outputs are byte-identical to those of the default configuration: a tuning must never change results.
+
+Results are cached in a JSON file keyed by the device name and a signature of the workload (event count and hit-count distribution), so a later process with a similar workload reuses them.
+
+Usage::
+
+    from allen_amd import autotune
+    best = autotune.tune(seq, dev_events)          # sweeps, applies the best knobs to `seq`
+    autotune.apply(seq, best)                       # or apply a stored result
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import velo
+
+BLOCKS = (64, 128, 256, 512, 1024)
+LDS_HITS = (256, 512, 1024, 1536, 1760, 2048, 3072, 4096)
+# Outputs that must not change with any knob (the consolidated tracks and their states).
+CHECKED_BUFFERS = ("offsets_all_velo_tracks", "velo_track_hit_number", "velo_track_hits",
+                   "velo_kalman_beamline_states", "velo_kalman_endvelo_states", "atomics_velo")
+
+
+@dataclass
+class Result:
+    knobs: dict
+    search_ms: float
+    default_search_ms: float
+    trials: list = field(default_factory=list)
+
+    def to_json(self) -> dict:
+        return {"knobs": self.knobs, "search_ms": self.search_ms, "default_search_ms": self.default_search_ms,
+                "trials": self.trials}
+
+
+def default_knobs() -> dict:
+    return {"search_block": 256, "search_lds_hits": 0, "search_persistent": 0, "search_largest_first": 1}
+
+
+def apply(seq: velo.VeloSequence, knobs: dict) -> None:
+    for k, v in knobs.items():
+        seq.set_tuning(k, int(v))
+
+
+def _outputs_digest(seq: velo.VeloSequence, n_events: int) -> str:
+    """SHA-256 over the checked output buffers of the last run (sizes read from the run itself)."""
+    h = hashlib.sha256()
+    off = seq.fetch("offsets_all_velo_tracks", np.uint32, n_events + 1)
+    n_tracks = int(off[-1])
+    hits = seq.fetch("offsets_velo_track_hit_number", np.uint32, n_tracks + 1)
+    n_hits = int(hits[-1])
+    sizes = {"offsets_all_velo_tracks": 4 * (n_events + 1), "velo_track_hit_number": 4 * n_tracks,
+             "velo_track_hits": 18 * n_hits, "velo_kalman_beamline_states": 56 * n_tracks,
+             "velo_kalman_endvelo_states": 56 * n_tracks, "atomics_velo": 12 * n_events}
+    for name in CHECKED_BUFFERS:
+        h.update(seq.fetch(name, np.uint8, sizes[name]).tobytes())
+    return h.hexdigest()
+
+
+def _time_search(seq: velo.VeloSequence, dev: velo.DeviceEvents, reps: int) -> float:
+    seq.time_stage("search")
+    for _ in range(reps):
+        seq.run(dev)
+    ms, runs = seq.stage_ms()
+    seq.time_stage(None)
+    return ms / max(runs, 1)
+
+
+def workload_signature(dev: velo.DeviceEvents, seq: velo.VeloSequence) -> str:
+    """Event count and a coarse histogram of hits per event (after one run of `seq`)."""
+    n = dev.n_events
+    est = seq.fetch("offsets_estimated_input_size", np.uint32, 26 * n + 1)[::26].astype(np.int64)
+    hist = np.histogram(np.diff(est), bins=[0, 500, 1000, 1500, 2000, 3000, 4000, 1 << 30])[0]
+    frac = np.round(hist / max(n, 1), 2)
+    return f"n{n}:" + ",".join(f"{f:.2f}" for f in frac)
+
+
+def _device_name() -> str:
+    import torch
+    return torch.cuda.get_device_name(0)
+
+
+def tune(seq: velo.VeloSequence, dev: velo.DeviceEvents, reps: int = 3, cache: str | None = None,
+         blocks=BLOCKS, lds_hits=LDS_HITS, log=None) -> Result:
+    """Sweeps the knobs on `dev`'s events, applies the fastest result-preserving set to `seq`."""
+    apply(seq, default_knobs())
+    seq.run(dev)
+    reference = _outputs_digest(seq, dev.n_events)
+    key = f"{_device_name()}|{workload_signature(dev, seq)}"
+    if cache and os.path.exists(cache):
+        stored = json.load(open(cache)).get(key)
+        if stored:
+            apply(seq, stored["knobs"])
+            return Result(stored["knobs"], stored["search_ms"], stored["default_search_ms"], stored.get("trials", []))
+    default_ms = _time_search(seq, dev, reps)
+    best_knobs, best_ms = default_knobs(), default_ms
+    trials = []
+    candidates = []
+    for persistent in (0, 1):
+        for order in (1, 0):
+            for b in blocks:
+                for h in lds_hits:
+                    candidates.append({"search_block": b, "search_lds_hits": h, "search_persistent": persistent,
+                                       "search_largest_first": order})
+    for knobs in candidates:
+        apply(seq, knobs)
+        seq.run(dev)
+        if _outputs_digest(seq, dev.n_events) != reference:
+            raise velo.VeloError(f"knobs {knobs} changed the outputs of the sequence")
+        ms = _time_search(seq, dev, reps)
+        trials.append({**knobs, "search_ms": round(ms, 4)})
+        if log:
+            log(f"[autotune] {knobs} -> {ms:.4f} ms")
+        if ms < best_ms:
+            best_knobs, best_ms = knobs, ms
+    apply(seq, best_knobs)
+    result = Result(best_knobs, best_ms, default_ms, trials)
+    if cache:
+        data = json.load(open(cache)) if os.path.exists(cache) else {}
+        data[key] = result.to_json()
+        tmp = cache + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(data, f, indent=1)
+        os.replace(tmp, cache)
+    return result

@@ -450,6 +450,7 @@ struct velo_sequence {
   unsigned search_block = 256;
   unsigned search_lds_hits = velo::search_default_lds_hits;
   bool search_largest_first = true;
+  bool search_persistent = false;
   unsigned kalman_fp64_fallback = 0;
   bool fused_clustering = true;
   bool count_work = false;
@@ -458,6 +459,17 @@ struct velo_sequence {
   unsigned timed_stage = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pairs;
   size_t n_timed = 0;
+  // One captured run (velo_sequence_capture / velo_sequence_replay).
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+
+  void release_graph()
+  {
+    if (graph_exec) (void) hipGraphExecDestroy(graph_exec);
+    if (graph) (void) hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+  }
 
   template<typename T>
   T* get(const char* name)
@@ -521,6 +533,7 @@ int velo_sequence_create(const velo_constants* c, unsigned max_events, size_t ma
     {"search_work", 4 * sizeof(unsigned long long)},
     {"search_event_clock", velo::event_clock_words * sizeof(unsigned long long) * E},
     {"search_order", 4 * E},
+    {"search_counter", 4},
     {"sp_flags", s->sp_cap},
     {"bank_directory", 16 * 8 * 26 * E},
     {"bank_directory_count", 4 * 26 * E},
@@ -542,6 +555,7 @@ int velo_sequence_create(const velo_constants* c, unsigned max_events, size_t ma
 int velo_sequence_destroy(velo_sequence* s)
 {
   if (!s) return VELO_ERR_INVALID_ARGUMENT;
+  s->release_graph();
   for (auto& e : s->ev_pairs) {
     (void) hipEventDestroy(e.first);
     (void) hipEventDestroy(e.second);
@@ -587,6 +601,10 @@ int velo_sequence_set_tuning(velo_sequence* s, const char* knob, unsigned value)
   }
   if (k == "kalman_fp64_fallback") {
     s->kalman_fp64_fallback = value != 0;
+    return VELO_OK;
+  }
+  if (k == "search_persistent") {
+    s->search_persistent = value != 0;
     return VELO_OK;
   }
   if (k == "search_largest_first") {
@@ -794,6 +812,10 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream
   k5.event_clock = s->count_work ? s->get<unsigned long long>("search_event_clock") : nullptr;
   if (k5.work_stats) VELO_TRY(velo::launch_fill_u32(reinterpret_cast<uint32_t*>(k5.work_stats), 8, 0, stream));
   VELO_MARK(mark_begin(VELO_STAGE_SEARCH));
+  if (s->search_persistent) {
+    k5.work_counter = s->get<uint32_t>("search_counter");
+    VELO_TRY(velo::launch_fill_u32(k5.work_counter, 1, 0, stream));
+  }
   if (s->search_largest_first) {
     k5.event_list = s->get<uint32_t>("search_order");
     VELO_TRY(velo::launch_order_events_by_hits(est_off, n_events_dev, const_cast<uint32_t*>(k5.event_list), stream));
@@ -870,6 +892,33 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream
 #undef VELO_MARK
 #undef VELO_TRY
   return VELO_OK;
+}
+
+int velo_sequence_capture(velo_sequence* s, const velo_sequence_input* in, hipStream_t stream)
+{
+  if (!s || !in || !stream) return VELO_ERR_INVALID_ARGUMENT;
+  s->release_graph();
+  // Stage timing records events on the host's schedule; it is off inside a captured run.
+  const unsigned timed = s->timed_stage;
+  s->timed_stage = 0;
+  int rc = hip_status(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+  if (rc == VELO_OK) {
+    const int run_rc = velo_sequence_run(s, in, stream);
+    hipGraph_t g = nullptr;
+    const int end_rc = hip_status(hipStreamEndCapture(stream, &g));
+    rc = run_rc != VELO_OK ? run_rc : end_rc;
+    if (rc == VELO_OK) rc = hip_status(hipGraphInstantiate(&s->graph_exec, g, nullptr, nullptr, 0));
+    if (rc == VELO_OK) s->graph = g;
+    else if (g) (void) hipGraphDestroy(g);
+  }
+  s->timed_stage = timed;
+  return rc;
+}
+
+int velo_sequence_replay(velo_sequence* s, hipStream_t stream)
+{
+  if (!s || !s->graph_exec) return VELO_ERR_INVALID_ARGUMENT;
+  return hip_status(hipGraphLaunch(s->graph_exec, stream));
 }
 
 int velo_sequence_stage_ms(velo_sequence* s, float* total_ms, unsigned* runs)

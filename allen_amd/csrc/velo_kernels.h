@@ -143,6 +143,9 @@ struct SearchLaunch {
   unsigned lds_hits;           // event hits staged in LDS (16 B each); 0 = search_default_lds_hits
   unsigned long long* work_stats; // optional: seeding evals, forwarding evals, extrapolations, search steps
   unsigned long long* event_clock = nullptr; // optional: per event, event_clock_words u64 of wall clock ticks
+  // Optional: persistent mode, a resident grid whose workgroups take event slots from this
+  // counter (zero at launch) instead of one workgroup per event.
+  uint32_t* work_counter = nullptr;
 };
 
 struct ThreeHitFilterLaunch {

@@ -675,7 +675,8 @@ __device__ void search_event(
   }
 }
 
-// One workgroup per event. Dynamic LDS: the seeding scratch (13 B per lane), the ring of
+// One workgroup per event, or (work_counter set) a resident grid whose workgroups take the
+// next event slot from the counter until the slots run out. Dynamic LDS: the seeding scratch (13 B per lane), the ring of
 // tracks to follow (2048 x u32), then `lds_hits` hits as x[], y[], z[] (f32), phi[] (i16),
 // used[] (u8), module parity[] (u8). Events with more hits keep phi, used and parity in LDS
 // and read coordinates from global memory; beyond 4 x lds_hits everything stays global.
@@ -705,12 +706,14 @@ __global__ void k_search_by_triplet(
   unsigned max_skipped_modules,
   unsigned lds_hits,
   unsigned long long* work_stats,
-  unsigned long long* event_clock)
+  unsigned long long* event_clock,
+  unsigned n_list,
+  uint32_t* work_counter)
 {
-  const unsigned long long t_start = event_clock ? wall_clock64() : 0ull;
   extern __shared__ __align__(16) char dyn_all[];
+  __shared__ SearchShared sh;
+  __shared__ unsigned s_slot;
   WorkCount wc;
-  wc.last = t_start;
   SeedScratch ss;
   ss.pair_scatter = reinterpret_cast<float*>(dyn_all);
   ss.h1 = reinterpret_cast<uint16_t*>(ss.pair_scatter + blockDim.x);
@@ -720,122 +723,143 @@ __global__ void k_search_by_triplet(
   ss.n_h0 = reinterpret_cast<uint8_t*>(ss.h1_chunk + blockDim.x);
   uint32_t* ring = reinterpret_cast<uint32_t*>(dyn_all + ((13 * blockDim.x + 15) & ~15u));
   char* dyn = reinterpret_cast<char*>(ring + max_tracks_to_follow);
-  __shared__ SearchShared sh;
-  const unsigned event_number = event_list ? event_list[blockIdx.x] : blockIdx.x;
-  const unsigned n_events = number_of_events[0];
-  const unsigned total = offsets_estimated_input_size[n_module_pairs * n_events];
-  const uint32_t* mhs = offsets_estimated_input_size + event_number * n_module_pairs;
-  const uint32_t* mhn = module_cluster_num + event_number * n_module_pairs;
-  const unsigned hit_offset = mhs[0];
-  const unsigned n_hits = mhs[n_module_pairs] - hit_offset;
-  const Clusters cc {const_cast<char*>(sorted_container), total};
+  // One event per workgroup (work_counter == NULL), or a resident grid whose workgroups take
+  // event slots from the counter until they run out (the counter starts at 0).
+  for (unsigned round = 0;; ++round) {
+    unsigned slot = blockIdx.x;
+    if (work_counter) {
+      if (threadIdx.x == 0) s_slot = atomicAdd(work_counter, 1u);
+      __syncthreads();
+      slot = s_slot;
+      __syncthreads();
+    }
+    else if (round > 0) {
+      break;
+    }
+    if (slot >= n_list) break;
+    const unsigned long long t_start = event_clock ? wall_clock64() : 0ull;
+    wc.last = t_start;
+    const unsigned event_number = event_list ? event_list[slot] : slot;
+    const unsigned n_events = number_of_events[0];
+    const unsigned total = offsets_estimated_input_size[n_module_pairs * n_events];
+    const uint32_t* mhs = offsets_estimated_input_size + event_number * n_module_pairs;
+    const uint32_t* mhn = module_cluster_num + event_number * n_module_pairs;
+    const unsigned hit_offset = mhs[0];
+    const unsigned n_hits = mhs[n_module_pairs] - hit_offset;
+    const Clusters cc {const_cast<char*>(sorted_container), total};
 
-  const unsigned toff = track_offset(offsets_estimated_input_size, event_number);
-  const unsigned container = track_container_size(total, n_events);
-  const unsigned next_off =
-    event_number + 1 < n_events ? track_offset(offsets_estimated_input_size, event_number + 1) : container;
-  EventOutputs o;
-  o.tracks = tracks_all + toff;
-  o.three_hit_tracks = three_hit_all + toff;
-  o.tracklets = tracklets_all + (size_t) event_number * max_tracks_to_follow;
-  o.ttf = ring;
-  o.h1_indices = rel_indices_all + hit_offset;
-  // Slots of this event; the reference lets a busy event run into its neighbour's slots, here
-  // nothing is written past the container and the overflow is reported.
-  o.track_capacity = min(next_off > toff ? next_off - toff : 0u, container - min(toff, container));
+    const unsigned toff = track_offset(offsets_estimated_input_size, event_number);
+    const unsigned container = track_container_size(total, n_events);
+    const unsigned next_off =
+      event_number + 1 < n_events ? track_offset(offsets_estimated_input_size, event_number + 1) : container;
+    EventOutputs o;
+    o.tracks = tracks_all + toff;
+    o.three_hit_tracks = three_hit_all + toff;
+    o.tracklets = tracklets_all + (size_t) event_number * max_tracks_to_follow;
+    o.ttf = ring;
+    o.h1_indices = rel_indices_all + hit_offset;
+    // Slots of this event; the reference lets a busy event run into its neighbour's slots, here
+    // nothing is written past the container and the overflow is reported.
+    o.track_capacity = min(next_off > toff ? next_off - toff : 0u, container - min(toff, container));
 
-  Counters cn;
-  cn.status = n_hits > 65536u ? status_hit_index_overflow : 0u;
-  if (threadIdx.x < n_module_pairs) {
-    const unsigned m = threadIdx.x;
-    sh.mp[m].hit_start = mhs[m] - hit_offset;
-    sh.mp[m].hit_num = mhn[m];
-    sh.mp[m].z[0] = geometry->module_zs[2 * m];
-    sh.mp[m].z[1] = geometry->module_zs[2 * m + 1];
-  }
-  const uint32_t* gids = cc.ids() + hit_offset;
-  const int16_t* gph = cc.phis() + hit_offset;
-  const float* gxyz = cc.xyz() + 3 * (size_t) hit_offset;
-  uint8_t* gused = hit_used_all + hit_offset;
-  if (n_hits <= lds_hits) {
-    float* xs = reinterpret_cast<float*>(dyn);
-    float* ys = xs + lds_hits;
-    float* zs = ys + lds_hits;
-    int16_t* ph = reinterpret_cast<int16_t*>(zs + lds_hits);
-    uint8_t* used = reinterpret_cast<uint8_t*>(ph + lds_hits);
-    uint8_t* odd = used + lds_hits;
-    for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) {
-      xs[i] = gxyz[3 * i];
-      ys[i] = gxyz[3 * i + 1];
-      zs[i] = gxyz[3 * i + 2];
-      ph[i] = gph[i];
-      used[i] = 0;
-      odd[i] = module_parity(gids[i]);
+    Counters cn;
+    cn.status = n_hits > 65536u ? status_hit_index_overflow : 0u;
+    if (threadIdx.x < n_module_pairs) {
+      const unsigned m = threadIdx.x;
+      sh.mp[m].hit_start = mhs[m] - hit_offset;
+      sh.mp[m].hit_num = mhn[m];
+      sh.mp[m].z[0] = geometry->module_zs[2 * m];
+      sh.mp[m].z[1] = geometry->module_zs[2 * m + 1];
+    }
+    const uint32_t* gids = cc.ids() + hit_offset;
+    const int16_t* gph = cc.phis() + hit_offset;
+    const float* gxyz = cc.xyz() + 3 * (size_t) hit_offset;
+    uint8_t* gused = hit_used_all + hit_offset;
+    if (n_hits <= lds_hits) {
+      float* xs = reinterpret_cast<float*>(dyn);
+      float* ys = xs + lds_hits;
+      float* zs = ys + lds_hits;
+      int16_t* ph = reinterpret_cast<int16_t*>(zs + lds_hits);
+      uint8_t* used = reinterpret_cast<uint8_t*>(ph + lds_hits);
+      uint8_t* odd = used + lds_hits;
+      for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) {
+        xs[i] = gxyz[3 * i];
+        ys[i] = gxyz[3 * i + 1];
+        zs[i] = gxyz[3 * i + 2];
+        ph[i] = gph[i];
+        used[i] = 0;
+        odd[i] = module_parity(gids[i]);
+      }
+      __syncthreads();
+      const LdsHits hits {xs, ys, zs, ph, used, odd};
+      search_event<count>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
+      __syncthreads();
+      for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) gused[i] = used[i];
+    }
+    else if (n_hits <= 4 * lds_hits) {
+      int16_t* ph = reinterpret_cast<int16_t*>(dyn);
+      uint8_t* used = reinterpret_cast<uint8_t*>(ph + n_hits);
+      uint8_t* odd = used + n_hits;
+      for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) {
+        ph[i] = gph[i];
+        used[i] = 0;
+        odd[i] = module_parity(gids[i]);
+      }
+      cn.status |= status_lds_fallback;
+      __syncthreads();
+      const MixedHits hits {gxyz, ph, used, odd};
+      search_event<count>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
+      __syncthreads();
+      for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) gused[i] = used[i];
+    }
+    else {
+      for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) gused[i] = 0;
+      cn.status |= status_lds_fallback;
+      __threadfence_block();
+      __syncthreads();
+      const GlobalHits hits {gxyz, gph, gused, gids};
+      search_event<count>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
+      __syncthreads();
+    }
+    // The ring of tracks to follow as far as it was written (dev_tracks_to_follow).
+    uint32_t* gttf = ttf_all + (size_t) event_number * max_tracks_to_follow;
+    const unsigned ring_written = min(cn.ttf, max_tracks_to_follow);
+    for (unsigned i = threadIdx.x; i < ring_written; i += blockDim.x) gttf[i] = ring[i];
+    if constexpr (count) {
+      // Wave reduction, then one atomic per wave.
+      unsigned long long v[4] = {wc.seed_eval, wc.fwd_eval, wc.extrap, wc.steps};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+#pragma unroll
+        for (unsigned d = 32; d > 0; d >>= 1) v[k] += __shfl_xor(v[k], d, wave_size);
+        if (lane_id() == 0 && v[k]) atomicAdd(work_stats + k, v[k]);
+      }
+      wc.seed_eval = wc.fwd_eval = wc.extrap = wc.steps = 0;
     }
     __syncthreads();
-    const LdsHits hits {xs, ys, zs, ph, used, odd};
-    search_event<count>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
-    __syncthreads();
-    for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) gused[i] = used[i];
-  }
-  else if (n_hits <= 4 * lds_hits) {
-    int16_t* ph = reinterpret_cast<int16_t*>(dyn);
-    uint8_t* used = reinterpret_cast<uint8_t*>(ph + n_hits);
-    uint8_t* odd = used + n_hits;
-    for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) {
-      ph[i] = gph[i];
-      used[i] = 0;
-      odd[i] = module_parity(gids[i]);
-    }
-    cn.status |= status_lds_fallback;
-    __syncthreads();
-    const MixedHits hits {gxyz, ph, used, odd};
-    search_event<count>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
-    __syncthreads();
-    for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) gused[i] = used[i];
-  }
-  else {
-    for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) gused[i] = 0;
-    cn.status |= status_lds_fallback;
-    __threadfence_block();
-    __syncthreads();
-    const GlobalHits hits {gxyz, gph, gused, gids};
-    search_event<count>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
-    __syncthreads();
-  }
-  // The ring of tracks to follow as far as it was written (dev_tracks_to_follow).
-  uint32_t* gttf = ttf_all + (size_t) event_number * max_tracks_to_follow;
-  const unsigned ring_written = min(cn.ttf, max_tracks_to_follow);
-  for (unsigned i = threadIdx.x; i < ring_written; i += blockDim.x) gttf[i] = ring[i];
-  if constexpr (count) {
-    // Wave reduction, then one atomic per wave.
-    unsigned long long v[4] = {wc.seed_eval, wc.fwd_eval, wc.extrap, wc.steps};
+    if (threadIdx.x == 0) {
+      uint32_t* at = atomics_all + event_number * num_atomics;
+      at[atomic_three_hit] = cn.three;
+      at[atomic_ttf] = cn.ttf;
+      at[atomic_local_hits] = cn.local_hits;
+      number_of_velo_tracks[event_number] = min(cn.n_tracks, o.track_capacity);
+      if (cn.status && status) atomicOr(status, cn.status);
+      if (event_clock) {
+        unsigned long long* ec = event_clock + event_clock_words * event_number;
+        const unsigned long long t_end = wall_clock64();
+        ec[0] = t_start;
+        ec[1] = t_end;
+        if constexpr (count) {
+          phase_mark<count>(wc, phase_other);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-#pragma unroll
-      for (unsigned d = 32; d > 0; d >>= 1) v[k] += __shfl_xor(v[k], d, wave_size);
-      if (lane_id() == 0 && v[k]) atomicAdd(work_stats + k, v[k]);
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t* at = atomics_all + event_number * num_atomics;
-    at[atomic_three_hit] = cn.three;
-    at[atomic_ttf] = cn.ttf;
-    at[atomic_local_hits] = cn.local_hits;
-    number_of_velo_tracks[event_number] = min(cn.n_tracks, o.track_capacity);
-    if (cn.status && status) atomicOr(status, cn.status);
-    if (event_clock) {
-      unsigned long long* ec = event_clock + event_clock_words * event_number;
-      const unsigned long long t_end = wall_clock64();
-      ec[0] = t_start;
-      ec[1] = t_end;
-      if constexpr (count) {
-        phase_mark<count>(wc, phase_other);
-#pragma unroll
-        for (int k = 0; k < n_phases; ++k) ec[2 + k] = wc.phase[k];
+          for (int k = 0; k < n_phases; ++k) {
+            ec[2 + k] = wc.phase[k];
+            wc.phase[k] = 0;
+          }
+        }
       }
     }
+    __syncthreads(); // the next event reuses the LDS
   }
 }
 
@@ -1010,15 +1034,27 @@ hipError_t launch_search_by_triplet(const SearchLaunch& a, hipStream_t s)
   const unsigned max_hits = static_cast<unsigned>(((size_t) max_lds - fixed) / search_bytes_per_hit);
   const unsigned lds_hits = min(a.lds_hits ? a.lds_hits : search_default_lds_hits, max_hits);
   const size_t dyn = search_dynamic_lds(block, lds_hits);
+  // Persistent mode: as many workgroups as stay resident (they take events from the counter).
+  unsigned grid = a.n_list;
+  if (a.work_counter) {
+    int dev = 0, cus = 0, per_cu = 0;
+    const void* kernel = a.work_stats ? reinterpret_cast<const void*>(&k_search_by_triplet<true>)
+                                      : reinterpret_cast<const void*>(&k_search_by_triplet<false>);
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, static_cast<int>(block), dyn) == hipSuccess &&
+        cus > 0 && per_cu > 0)
+      grid = min(grid, static_cast<unsigned>(cus * per_cu));
+  }
 #define VELO_LAUNCH_SEARCH(C)                                                                                       \
   hipLaunchKernelGGL(                                                                                               \
-    k_search_by_triplet<C>, dim3(a.n_list), dim3(block), dyn, s, a.event_list, a.number_of_events,                  \
+    k_search_by_triplet<C>, dim3(grid), dim3(block), dyn, s, a.event_list, a.number_of_events,                      \
     a.sorted_container, a.offsets_estimated_input_size, a.module_cluster_num,                                       \
     reinterpret_cast<const Geometry*>(a.geometry), reinterpret_cast<TrackHits*>(a.tracks),                          \
     reinterpret_cast<TrackletHits*>(a.tracklets), a.tracks_to_follow,                                               \
     reinterpret_cast<TrackletHits*>(a.three_hit_tracks), a.hit_used, a.atomics_velo, a.rel_indices,                \
     a.number_of_velo_tracks, a.status, a.phi_tolerance, a.max_scatter, a.max_skipped_modules, lds_hits,             \
-    a.work_stats, a.event_clock)
+    a.work_stats, a.event_clock, a.n_list, a.work_counter)
   if (a.work_stats) VELO_LAUNCH_SEARCH(true);
   else VELO_LAUNCH_SEARCH(false);
 #undef VELO_LAUNCH_SEARCH

@@ -109,8 +109,9 @@ PRESETS = {
     "single_pv_200": dict(n_events=200, pileup=0, mean_tracks=120.0),
     # 1000 events, nu ~ Poisson(7.6) collisions x NegBinomial(16, k=1.5) tracks.
     "pileup_1000": dict(n_events=1000, pileup=1, nu=7.6),
-    # 20k events, as above plus 5 % of events at nu = 20.
-    "pileup_20k": dict(n_events=20000, pileup=1, nu=7.6, high_occupancy_fraction=0.05, high_nu=20.0),
+    # 20k events as single_pv_200 (one PV, Poisson(120) tracks, 3 % noise), of which 5 % are
+    # high-occupancy events of nu = 20 pile-up collisions.
+    "batched_20k": dict(n_events=20000, pileup=0, mean_tracks=120.0, high_occupancy_fraction=0.05, high_nu=20.0),
     # 10k events at nu ~ Poisson(15), 5 % noise, 1 % dead ASICs.
     "stress_10k": dict(n_events=10000, pileup=1, nu=15.0, noise_fraction=0.05, dead_asic_fraction=0.01),
 }

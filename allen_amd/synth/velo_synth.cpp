@@ -116,6 +116,12 @@ void generate_event(const velo_synth_config& c, const Geometry& g, unsigned ev, 
     const int n_coll = pn(rng);
     for (int i = 0; i < n_coll; ++i) draw_pv(static_cast<unsigned>(neg_binomial(rng, c.nb_mean, c.nb_k)));
   }
+  else if (c.high_occupancy_fraction > 0 && uni(rng) < c.high_occupancy_fraction) {
+    // A high-occupancy event in a single-collision workload: nu = high_nu pile-up collisions.
+    std::poisson_distribution<int> pn(c.high_nu);
+    const int n_coll = pn(rng);
+    for (int i = 0; i < n_coll; ++i) draw_pv(static_cast<unsigned>(neg_binomial(rng, c.nb_mean, c.nb_k)));
+  }
   else {
     std::poisson_distribution<int> pt(c.mean_tracks);
     draw_pv(static_cast<unsigned>(pt(rng)));

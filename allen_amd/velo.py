@@ -148,6 +148,8 @@ SIGNATURES = {
     "velo_sequence_status": (I, [P, P, ctypes.POINTER(U)]),
     "velo_sequence_set_tuning": (I, [P, ctypes.c_char_p, U]),
     "velo_sequence_stage_ms": (I, [P, ctypes.POINTER(F), ctypes.POINTER(U)]),
+    "velo_sequence_capture": (I, [P, ctypes.POINTER(SequenceInput), P]),
+    "velo_sequence_replay": (I, [P, P]),
 }
 
 STAGES = {"all": 1, "candidates": 2, "estimate": 3, "clustering": 4, "search": 5, "three_hit_filter": 6,
@@ -308,6 +310,26 @@ class VeloSequence:
         for k, v in props.items():
             setattr(inp, k, v)
         check(lib().velo_sequence_run(self.handle, ctypes.byref(inp), stream_handle(stream)), "velo_sequence_run")
+
+    def capture(self, ev: DeviceEvents, stream, **props):
+        """Captures one run into a HIP graph on `stream` (a torch.cuda.Stream, not the null stream)."""
+        inp = SequenceInput()
+        lib().velo_sequence_default_input(ctypes.byref(inp))
+        inp.dev_velo_raw_input = _ptr(ev.raw)
+        inp.dev_velo_raw_input_offsets = _ptr(ev.offsets)
+        inp.dev_velo_raw_input_sizes = _ptr(ev.sizes)
+        inp.dev_velo_raw_input_types = _ptr(ev.types)
+        inp.raw_input_bytes = ev.raw_bytes
+        inp.number_of_events = ev.n_events
+        inp.bank_version = ev.bank_version
+        for k, v in props.items():
+            setattr(inp, k, v)
+        check(lib().velo_sequence_capture(self.handle, ctypes.byref(inp), stream_handle(stream)),
+              "velo_sequence_capture")
+
+    def replay(self, stream):
+        """Launches the captured run on `stream`."""
+        check(lib().velo_sequence_replay(self.handle, stream_handle(stream)), "velo_sequence_replay")
 
     def time_stage(self, stage: str | None):
         """Brackets one stage of subsequent runs with HIP events (None: no timing)."""

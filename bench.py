@@ -24,8 +24,9 @@ WORKLOADS = {
     "pileup_1000": dict(n_events=1000, pileup=1, nu=7.6, seed=1),
     # 200 events of Poisson(120) tracks from one PV (the reference's CPU-runnable case).
     "single_pv_200": dict(n_events=200, pileup=0, mean_tracks=120.0, seed=1),
-    # 20k events, as pileup_1000 plus 5 % of events at nu = 20.
-    "pileup_20k": dict(n_events=20000, pileup=1, nu=7.6, high_occupancy_fraction=0.05, high_nu=20.0, seed=1),
+    # 20k events of one PV with Poisson(120) tracks, 5 % of them nu = 20 pile-up events.
+    "batched_20k": dict(n_events=20000, pileup=0, mean_tracks=120.0, high_occupancy_fraction=0.05, high_nu=20.0,
+                        seed=1),
     # 10k events at nu ~ Poisson(15), 5 % noise pixels, 1 % dead ASICs.
     "stress_10k": dict(n_events=10000, pileup=1, nu=15.0, noise_fraction=0.05, dead_asic_fraction=0.01, seed=1),
 }
@@ -62,6 +63,10 @@ def main():
     ap.add_argument("--search-block", type=int, default=0)
     ap.add_argument("--search-lds-hits", type=int, default=0)
     ap.add_argument("--search-event-order", choices=["largest_first", "natural"], default="largest_first")
+    ap.add_argument("--autotune", action="store_true",
+                    help="sweep the search_by_triplet launch knobs on this workload first and use the fastest")
+    ap.add_argument("--graph", action="store_true",
+                    help="time replays of the whole sequence captured as one HIP graph instead of eager runs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times (the reference kernel for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -100,6 +105,14 @@ def main():
     seq.set_tuning("search_largest_first", int(args.search_event_order == "largest_first"))
     stream = torch.cuda.current_stream()
 
+    tuned = None
+    if args.autotune:
+        from allen_amd import autotune
+        t0 = time.time()
+        res = autotune.tune(seq, dev)
+        tuned = res.knobs
+        log(f"[bench] autotune ({time.time() - t0:.1f}s, {len(res.trials)} trials): {res.knobs} "
+            f"search {res.search_ms:.4f} ms (default {res.default_search_ms:.4f} ms)")
     for _ in range(args.warmup):
         seq.run(dev)
     torch.cuda.synchronize()
@@ -167,14 +180,30 @@ def main():
     torch.cuda.synchronize()
     start = torch.cuda.Event(enable_timing=True)
     end = torch.cuda.Event(enable_timing=True)
-    start.record(stream)
-    for _ in range(args.steps):
-        seq.run(dev)
-    end.record(stream)
-    torch.cuda.synchronize()
-    total_ms = start.elapsed_time(end)
-    kern_ms, kern_runs = seq.stage_ms()
-    kern_avg_ms = kern_ms / kern_runs
+    if args.graph:
+        # The graph carries no stage events; the dominant kernel's launch time is the eager one.
+        seq.time_stage(None)
+        stream = torch.cuda.Stream()
+        seq.capture(dev, stream)
+        for _ in range(args.warmup):
+            seq.replay(stream)
+        stream.synchronize()
+        start.record(stream)
+        for _ in range(args.steps):
+            seq.replay(stream)
+        end.record(stream)
+        stream.synchronize()
+        total_ms = start.elapsed_time(end)
+        kern_avg_ms = stage_ms[dominant]
+    else:
+        start.record(stream)
+        for _ in range(args.steps):
+            seq.run(dev)
+        end.record(stream)
+        torch.cuda.synchronize()
+        total_ms = start.elapsed_time(end)
+        kern_ms, kern_runs = seq.stage_ms()
+        kern_avg_ms = kern_ms / kern_runs
     ms_per_step = total_ms / args.steps
     events_per_s = n * args.steps / (total_ms / 1e3)
     status = seq.status()
@@ -226,6 +255,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded generator, real VELO geometry, v4 super-pixel banks)",
         "config": {"workload": args.workload, "events_per_step": n, "path": "plain",
+                   "launch": "hip_graph" if args.graph else "eager",
+                   **({"search_knobs": tuned} if tuned else {}),
                    "raw_bytes_per_step": int(ev.raw.size), **{k: v for k, v in wl.items() if k != "n_events"}},
         "status": status,
         "roofline": roofline,

@@ -295,6 +295,9 @@ int velo_sequence_status(velo_sequence* s, hipStream_t stream, unsigned* status)
  *                       global memory
  *   "search_largest_first"  1 (default): search_by_triplet takes the events in descending order
  *                       of their hit count; 0: in event order. Outputs do not depend on it
+ *   "search_persistent" 1: search_by_triplet runs as a resident grid whose workgroups take the
+ *                       next event from a device counter (work stealing) instead of one
+ *                       workgroup per event; 0 (default). Outputs do not depend on it
  *   "fused_clustering"  1: clustering and phi sort fused per module pair (default); 0: the four
  *                       reference algorithms one after the other
  *   "timed_stage"       VELO_STAGE_* to bracket with HIP events on the run's stream (0 = none)
@@ -320,6 +323,12 @@ enum {
 /* Sum of the elapsed milliseconds of the timed stage over the runs since "timed_stage" was
  * last set, and the number of those runs (waits for them). */
 int velo_sequence_stage_ms(velo_sequence* s, float* total_ms, unsigned* runs);
+/* Captures one velo_sequence_run with these inputs on `stream` (not the null stream) into a
+ * HIP graph held by the sequence, replacing any earlier one; velo_sequence_replay launches it.
+ * The graph keeps the input pointers, event count and knobs of the capture; every size inside
+ * the run is read on the device, and buffers are cleared by kernels, not memset nodes. */
+int velo_sequence_capture(velo_sequence* s, const velo_sequence_input* in, hipStream_t stream);
+int velo_sequence_replay(velo_sequence* s, hipStream_t stream);
 
 #ifdef __cplusplus
 }

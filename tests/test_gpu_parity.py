@@ -62,6 +62,8 @@ TUNINGS = {
     "mixed_lds": {"search_block": 256, "search_lds_hits": 600},   # phi + used in LDS, xyz global
     "global_hits": {"search_block": 128, "search_lds_hits": 64},  # everything in global memory
     "event_order": {"search_largest_first": 0},                   # events in their own order
+    "persistent": {"search_persistent": 1},                       # resident grid, work stealing
+    "persistent_block64": {"search_persistent": 1, "search_block": 64, "search_largest_first": 0},
 }
 
 
@@ -82,6 +84,55 @@ def test_kalman_fp64_covariance_fallback(geometry, beamline, constants):
     assert status & velo.STATUS_OVERFLOW_MASK == 0
     errs = parity.compare(gpu, ref, True)
     assert not errs, "\n".join(errs[:20])
+
+
+def test_captured_sequence_replays(geometry, beamline, constants):
+    """The whole sequence captured as one HIP graph: three replays, with an eager run on other
+    events in between (which overwrites every buffer), each reproduce the eager result bit for
+    bit and the oracle."""
+    import torch
+    ev = synth.generate(**DATASETS["pileup"])
+    other = synth.generate(**dict(DATASETS["pileup"], n_events=40, seed=99))
+    print(f"[graph] events={ev.n_events} other={other.n_events}", flush=True)
+    ref = oracle.run(ev, geometry, beamline)
+    dev, dev_other = velo.DeviceEvents(ev), velo.DeviceEvents(other)
+    seq = velo.VeloSequence(constants, ev.n_events, max(ev.raw.size, other.raw.size))
+    seq.run(dev)
+    torch.cuda.synchronize()
+    eager = parity.gpu_buffers(seq, ref)
+    stream = torch.cuda.Stream()
+    seq.capture(dev, stream)
+    stream.synchronize()
+    for replay in range(3):
+        seq.run(dev_other)
+        torch.cuda.synchronize()
+        seq.replay(stream)
+        stream.synchronize()
+        got = parity.gpu_buffers(seq, ref)
+        for name, a in eager.items():
+            assert np.array_equal(np.asarray(got[name]).view(np.uint8), np.asarray(a).view(np.uint8)), \
+                f"replay {replay}: {name} differs from the eager run"
+        errs = parity.compare(got, ref, True)
+        assert not errs, "\n".join(errs[:20])
+    assert seq.status() & velo.STATUS_OVERFLOW_MASK == 0
+    seq.close()
+
+
+def test_autotune_keeps_outputs(geometry, beamline, constants):
+    """A reduced autotuner sweep: every candidate reproduces the default outputs (the tuner raises
+    otherwise) and the chosen knobs are applied to the sequence."""
+    from allen_amd import autotune
+    ev = synth.generate(**DATASETS["pileup"])
+    dev = velo.DeviceEvents(ev)
+    seq = velo.VeloSequence(constants, ev.n_events, ev.raw.size)
+    res = autotune.tune(seq, dev, reps=1, blocks=(64, 256), lds_hits=(512, 1760))
+    assert len(res.trials) == 8 and res.search_ms <= res.default_search_ms
+    assert res.knobs["search_block"] in (64, 256)
+    seq.run(dev)
+    ref = oracle.run(ev, geometry, beamline)
+    errs = parity.compare(parity.gpu_buffers(seq, ref), ref, True)
+    assert not errs, "\n".join(errs[:20])
+    seq.close()
 
 
 def test_dense_module_pairs_take_the_tiled_paths(geometry, beamline, constants):
