@@ -92,7 +92,7 @@ def main():
         return
     wl = WORKLOADS[args.workload]
     t0 = time.time()
-    ev = synth.generate(**wl, keep_truth=False, n_threads=min(16, os.cpu_count() or 1))
+    ev = synth.generate(**wl, keep_truth=True, n_threads=min(16, os.cpu_count() or 1))
     log(f"[bench] generated {ev.n_events} events ({ev.raw.size / 1e6:.2f} MB raw) in {time.time() - t0:.1f}s")
     geometry, beamline = synth.read_geometry(), synth.read_beamline()
     constants = velo.VeloConstants(geometry, beamline)
@@ -174,6 +174,12 @@ def main():
         "seeds": int(atomics[1::3].sum()),
     }
     log(f"[bench] {counts}")
+    # Physics of the timed workload's output against the generator's truth (01_velo category).
+    from allen_amd import checker
+    hits_n = seq.fetch("offsets_velo_track_hit_number", np.uint32, n_tracks + 1)
+    track_hits = seq.fetch("velo_track_hits", np.uint8, 4 * int(hits_n[-1]))
+    physics = checker.check_buffers(ev, off_all, hits_n, track_hits).as_dict()
+    log(f"[bench] physics: {physics}")
 
     # Timed region: K steps, the dominant stage bracketed by HIP events on the run's stream.
     seq.time_stage(dominant)
@@ -224,7 +230,8 @@ def main():
                              "peak_tops": round(VALU_PEAK_TOPS, 2), "frac": round(search_tops / VALU_PEAK_TOPS, 5),
                              "work": [int(x) for x in work]},
              "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
-             "search_events": search_events}
+             "search_events": search_events,
+             "physics": physics}
 
     cpu = None
     if not args.no_cpu_baseline:

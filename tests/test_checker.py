@@ -1,0 +1,57 @@
+"""The VELO track checker (allen_amd/checker.py) against a per-track loop restatement of the same
+rules, and on events whose tracks are known exactly."""
+import numpy as np
+
+import oracle
+from allen_amd import checker, synth
+
+
+def _loop_check(ev, r):
+    """Plain-Python restatement: majority particle per track, 70 % rule, 01_velo particles."""
+    off_all = r["offsets_all_velo_tracks"]
+    off_h = r["offsets_velo_track_hit_number"]
+    n = int(off_all[-1])
+    ids = r["velo_track_hits"][:4 * int(off_h[n])].view("<u4")
+    rec = {i for i, p in enumerate(ev.particles) if p["n_modules"] >= 3 and 2.0 < p["eta"] < 5.0}
+    matched, ghosts = {}, 0
+    for e in range(ev.n_events):
+        h = ev.hits[ev.hit_offsets[e]:ev.hit_offsets[e + 1]]
+        truth = {int(x["lhcb_id"]): int(x["particle"]) for x in h if x["particle"] != 0xFFFFFFFF}
+        for t in range(off_all[e], off_all[e + 1]):
+            ps = [truth.get(int(i), -1) for i in ids[off_h[t]:off_h[t + 1]]]
+            cand = [p for p in ps if p >= 0]
+            if not cand:
+                ghosts += 1
+                continue
+            counts = {p: cand.count(p) for p in set(cand)}
+            best = max(counts.values())
+            p = min(q for q, c in counts.items() if c == best)
+            if best / len(ps) >= 0.7:
+                matched.setdefault(p, []).append(t)
+            else:
+                ghosts += 1
+    reco = [p for p in matched if p in rec]
+    clones = sum(len(matched[p]) - 1 for p in reco)
+    return len(rec), len(reco), n, ghosts, clones
+
+
+def test_checker_against_loop(geometry, beamline):
+    ev = synth.generate(n_events=25, pileup=1, nu=7.6, seed=41)
+    r = oracle.run(ev, geometry, beamline)
+    res = checker.check_buffers(ev, r["offsets_all_velo_tracks"], r["offsets_velo_track_hit_number"],
+                                r["velo_track_hits"])
+    n_rec, n_reco, n_tr, ghosts, clones = _loop_check(ev, r)
+    assert (res.reconstructible, res.reconstructed, res.tracks, res.ghosts, res.clones) == \
+        (n_rec, n_reco, n_tr, ghosts, clones)
+    assert 0.9 < res.efficiency <= 1.0 and res.ghost_rate < 0.1
+
+
+def test_checker_on_clean_single_tracks(geometry, beamline):
+    """Noiseless single-pixel tracks: everything reconstructible is found, no ghosts, pure hits."""
+    ev = synth.generate(n_events=100, mean_tracks=2.0, noise_fraction=0.0, hit_efficiency=1.0,
+                        cluster_size_p=(1.0, 0.0, 0.0), seed=23)
+    r = oracle.run(ev, geometry, beamline)
+    res = checker.check_buffers(ev, r["offsets_all_velo_tracks"], r["offsets_velo_track_hit_number"],
+                                r["velo_track_hits"])
+    assert res.ghosts == 0 and res.hit_purity == 1.0
+    assert res.efficiency > 0.97
