@@ -88,8 +88,8 @@ def test_kalman_fp64_covariance_fallback(geometry, beamline, constants):
 
 def test_captured_sequence_replays(geometry, beamline, constants):
     """The whole sequence captured as one HIP graph: three replays, with an eager run on other
-    events in between (which overwrites every buffer), each reproduce the eager result bit for
-    bit and the oracle."""
+    events in between (which overwrites the buffers), each reproduce the eager result and the
+    oracle bit for bit."""
     import torch
     ev = synth.generate(**DATASETS["pileup"])
     other = synth.generate(**dict(DATASETS["pileup"], n_events=40, seed=99))
@@ -109,11 +109,12 @@ def test_captured_sequence_replays(geometry, beamline, constants):
         seq.replay(stream)
         stream.synchronize()
         got = parity.gpu_buffers(seq, ref)
-        for name, a in eager.items():
-            assert np.array_equal(np.asarray(got[name]).view(np.uint8), np.asarray(a).view(np.uint8)), \
-                f"replay {replay}: {name} differs from the eager run"
+        # Same rules as the oracle comparison: filled slots only (gap contents are unspecified and
+        # keep what the run on the other events left there).
+        errs = parity.compare(got, eager, True)
+        assert not errs, f"replay {replay} against the eager run:\n" + "\n".join(errs[:20])
         errs = parity.compare(got, ref, True)
-        assert not errs, "\n".join(errs[:20])
+        assert not errs, f"replay {replay} against the oracle:\n" + "\n".join(errs[:20])
     assert seq.status() & velo.STATUS_OVERFLOW_MASK == 0
     seq.close()
 
@@ -126,7 +127,7 @@ def test_autotune_keeps_outputs(geometry, beamline, constants):
     dev = velo.DeviceEvents(ev)
     seq = velo.VeloSequence(constants, ev.n_events, ev.raw.size)
     res = autotune.tune(seq, dev, reps=1, blocks=(64, 256), lds_hits=(512, 1760))
-    assert len(res.trials) == 8 and res.search_ms <= res.default_search_ms
+    assert len(res.trials) == 16 and res.search_ms <= res.default_search_ms
     assert res.knobs["search_block"] in (64, 256)
     seq.run(dev)
     ref = oracle.run(ev, geometry, beamline)
