@@ -451,6 +451,7 @@ struct velo_sequence {
   unsigned search_lds_hits = velo::search_default_lds_hits;
   bool search_largest_first = true;
   bool search_persistent = false;
+  bool clustering_resident_grid = false;
   unsigned kalman_fp64_fallback = 0;
   bool fused_clustering = true;
   bool count_work = false;
@@ -603,6 +604,10 @@ int velo_sequence_set_tuning(velo_sequence* s, const char* knob, unsigned value)
     s->kalman_fp64_fallback = value != 0;
     return VELO_OK;
   }
+  if (k == "clustering_resident_grid") {
+    s->clustering_resident_grid = value != 0;
+    return VELO_OK;
+  }
   if (k == "search_persistent") {
     s->search_persistent = value != 0;
     return VELO_OK;
@@ -719,6 +724,7 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream
     e.n_events = E;
     e.estimated_input_size = k2.estimated_input_size;
     e.sp_flags = sp_flags;
+    e.resident_grid = s->clustering_resident_grid;
     VELO_TRY(velo::launch_estimate_mp(e, stream));
   }
   else {
@@ -748,6 +754,7 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream
     f.module_cluster_num = mcn;
     f.sorted_container = sorted;
     f.unsorted_scratch = s->get<char>("velo_cluster_container");
+    f.resident_grid = s->clustering_resident_grid;
     VELO_TRY(velo::launch_cluster_mp(f, stream));
   }
   else {

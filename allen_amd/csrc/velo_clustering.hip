@@ -1281,11 +1281,14 @@ static unsigned resident_blocks(const void* kernel, unsigned block)
   return static_cast<unsigned>(cus) * static_cast<unsigned>(per_cu);
 }
 
-// One wave per item up to the resident capacity; beyond it the waves loop.
-static unsigned mp_grid(const void* kernel, unsigned n_items, unsigned* cache)
+// One wave per item, or (resident) at most the blocks that stay resident, whose waves loop
+// over the items. One wave per item balances uneven module pairs better (the dispatcher starts
+// a new wave wherever one ends), and is the default.
+static unsigned mp_grid(const void* kernel, unsigned n_items, bool resident, unsigned* cache)
 {
-  if (!*cache) *cache = resident_blocks(kernel, mp::waves_per_block * wave_size);
   const unsigned need = (n_items + mp::waves_per_block - 1) / mp::waves_per_block;
+  if (!resident) return need;
+  if (!*cache) *cache = resident_blocks(kernel, mp::waves_per_block * wave_size);
   return *cache ? min(need, *cache) : need;
 }
 
@@ -1297,7 +1300,7 @@ hipError_t launch_estimate_mp(const EstimateMpLaunch& a, hipStream_t s)
   const void* kernel = v == 2 ? reinterpret_cast<const void*>(&k_estimate_mp<2>)
                      : v == 3 ? reinterpret_cast<const void*>(&k_estimate_mp<3>)
                               : reinterpret_cast<const void*>(&k_estimate_mp<4>);
-  const unsigned grid = mp_grid(kernel, a.n_events * n_module_pairs, &cache[v]);
+  const unsigned grid = mp_grid(kernel, a.n_events * n_module_pairs, a.resident_grid, &cache[v]);
 #define VELO_LAUNCH_EMP(V)                                                                                          \
   hipLaunchKernelGGL(                                                                                               \
     k_estimate_mp<V>, dim3(grid), dim3(mp::waves_per_block * wave_size), 0, s, a.raw,                               \
@@ -1319,7 +1322,7 @@ hipError_t launch_cluster_mp(const ClusterMpLaunch& a, hipStream_t s)
   const void* kernel = v == 2 ? reinterpret_cast<const void*>(&k_cluster_mp<2>)
                      : v == 3 ? reinterpret_cast<const void*>(&k_cluster_mp<3>)
                               : reinterpret_cast<const void*>(&k_cluster_mp<4>);
-  const unsigned grid = mp_grid(kernel, a.n_events * n_module_pairs, &cache[v]);
+  const unsigned grid = mp_grid(kernel, a.n_events * n_module_pairs, a.resident_grid, &cache[v]);
 #define VELO_LAUNCH_CMP(V)                                                                                          \
   hipLaunchKernelGGL(                                                                                               \
     k_cluster_mp<V>, dim3(grid), dim3(mp::waves_per_block * wave_size), 0, s, a.raw,                                \

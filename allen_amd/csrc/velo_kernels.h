@@ -94,6 +94,7 @@ struct FusedClusterLaunch {
 
 struct EstimateMpLaunch {
   int version;
+  bool resident_grid = false; // one wave per item (default) or a resident grid looping over items
   const char* raw;
   const void* directory;
   const uint32_t* directory_count;
@@ -104,6 +105,7 @@ struct EstimateMpLaunch {
 
 struct ClusterMpLaunch {
   int version;
+  bool resident_grid = false;
   const char* raw;
   const void* directory;
   const uint32_t* directory_count;

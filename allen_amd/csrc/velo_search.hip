@@ -17,6 +17,12 @@
 
 namespace velo {
 
+// LDS bytes per hit of the full layout: x, y, z (f32), phi (i16), flags (u8).
+constexpr unsigned search_bytes_per_hit = 15;
+// Capacity of the list of unused hits of the previous module pair used by seeding step 1; a
+// larger module pair takes the pendulum walk.
+constexpr unsigned unused_list_cap = 1024;
+
 // Three 0/1 flags scanned at once over the block. Exclusive ranks in r[], totals in t[].
 // One barrier: consecutive scans must alternate between two scratch buffers (a scan's reads of
 // its buffer are ordered before the next scan's barrier, hence before the reuse two scans on).
@@ -75,19 +81,25 @@ struct ModulePairData {
 };
 
 // Hits of one event, structure-of-arrays in LDS.
+// Per-hit flags byte of the LDS layouts: bit 0 = used (dev_hit_used), bit 1 = module parity of
+// the hit's sensor, ((id >> 18) & 0xFF) / 4 % 2. Marking a hit used ORs bit 0 in; lanes that mark
+// the same hit concurrently all store the same byte, so the read-modify-write cannot lose a bit.
+constexpr uint8_t flag_used = 1, flag_odd = 2;
+
 struct LdsHits {
   const float* xs;
   const float* ys;
   const float* zs;
   const int16_t* ph;
-  uint8_t* used;
-  const uint8_t* odd; // module parity of the hit's sensor, ((id >> 18) & 0xFF) / 4 % 2
+  uint8_t* flags;
   __device__ __forceinline__ float x(unsigned i) const { return xs[i]; }
   __device__ __forceinline__ float y(unsigned i) const { return ys[i]; }
   __device__ __forceinline__ float z(unsigned i) const { return zs[i]; }
   __device__ __forceinline__ int16_t phi(unsigned i) const { return ph[i]; }
   __device__ __forceinline__ const int16_t* phi_begin() const { return ph; }
-  __device__ __forceinline__ bool module_odd(unsigned i) const { return odd[i] != 0; }
+  __device__ __forceinline__ bool used(unsigned i) const { return (flags[i] & flag_used) != 0; }
+  __device__ __forceinline__ void mark_used(unsigned i) const { flags[i] |= flag_used; }
+  __device__ __forceinline__ bool module_odd(unsigned i) const { return (flags[i] & flag_odd) != 0; }
 };
 
 // Module parity of a hit from its LHCbID (SearchByTriplet.cu:498-499).
@@ -97,13 +109,15 @@ __device__ __forceinline__ uint8_t module_parity(uint32_t id) { return static_ca
 struct GlobalHits {
   const float* xyz;
   const int16_t* ph;
-  uint8_t* used;
+  uint8_t* used_flags; // dev_hit_used
   const uint32_t* ids;
   __device__ __forceinline__ float x(unsigned i) const { return xyz[3 * i]; }
   __device__ __forceinline__ float y(unsigned i) const { return xyz[3 * i + 1]; }
   __device__ __forceinline__ float z(unsigned i) const { return xyz[3 * i + 2]; }
   __device__ __forceinline__ int16_t phi(unsigned i) const { return ph[i]; }
   __device__ __forceinline__ const int16_t* phi_begin() const { return ph; }
+  __device__ __forceinline__ bool used(unsigned i) const { return used_flags[i] != 0; }
+  __device__ __forceinline__ void mark_used(unsigned i) const { used_flags[i] = 1; }
   __device__ __forceinline__ bool module_odd(unsigned i) const { return module_parity(ids[i]) != 0; }
 };
 
@@ -149,8 +163,8 @@ __device__ __forceinline__ void phase_mark(WorkCount& wc, Phase p)
 // points at once (independent LDS reads) and keeps the eighth of the range that holds the
 // answer, so a module pair of n hits takes about log8(n) dependent round trips instead of
 // log2(n). `steps` counts the probes.
-template<bool count>
-__device__ __forceinline__ int counted_search(const int16_t* array, unsigned size, int16_t value, WorkCount& wc)
+template<bool count, typename T>
+__device__ __forceinline__ int counted_search(const T* array, unsigned size, T value, WorkCount& wc)
 {
   unsigned lo = 0, hi = size; // answer in [lo, hi]; array[< lo] < value <= array[>= hi]
   while (lo < hi) {
@@ -181,14 +195,15 @@ __device__ __forceinline__ int counted_search(const int16_t* array, unsigned siz
 struct MixedHits {
   const float* xyz;
   const int16_t* ph;
-  uint8_t* used;
-  const uint8_t* odd;
+  uint8_t* flags;
   __device__ __forceinline__ float x(unsigned i) const { return xyz[3 * i]; }
   __device__ __forceinline__ float y(unsigned i) const { return xyz[3 * i + 1]; }
   __device__ __forceinline__ float z(unsigned i) const { return xyz[3 * i + 2]; }
   __device__ __forceinline__ int16_t phi(unsigned i) const { return ph[i]; }
   __device__ __forceinline__ const int16_t* phi_begin() const { return ph; }
-  __device__ __forceinline__ bool module_odd(unsigned i) const { return odd[i] != 0; }
+  __device__ __forceinline__ bool used(unsigned i) const { return (flags[i] & flag_used) != 0; }
+  __device__ __forceinline__ void mark_used(unsigned i) const { flags[i] |= flag_used; }
+  __device__ __forceinline__ bool module_odd(unsigned i) const { return (flags[i] & flag_odd) != 0; }
 };
 
 struct SearchShared {
@@ -222,6 +237,7 @@ struct SeedScratch {
   uint16_t* pair_h2;
   uint8_t* n_h0;
   uint16_t* h1_chunk; // unused hits of the current module pair, one compaction chunk
+  uint16_t* unused;   // unused hits of the previous module pair (relative indices, ascending)
 };
 
 struct EventOutputs {
@@ -239,7 +255,7 @@ struct EventOutputs {
 struct Window4 {
   unsigned index[4];
   int16_t phi[4];
-  uint8_t used[4];
+  bool used[4];
   float x[4], y[4], z[4];
   template<typename H>
   __device__ __forceinline__ void
@@ -253,7 +269,7 @@ struct Window4 {
 #pragma unroll
     for (unsigned k = 0; k < 4; ++k) {
       phi[k] = hits.phi(index[k]);
-      used[k] = with_used ? hits.used[index[k]] : 0;
+      used[k] = with_used ? hits.used(index[k]) : false;
       x[k] = hits.x(index[k]);
       y[k] = hits.y(index[k]);
       z[k] = hits.z(index[k]);
@@ -287,6 +303,63 @@ __device__ __forceinline__ void find_forward_candidate(
   extrapolated_phi = phi;
 }
 
+// The first K unused hits of the previous module pair in the pendulum order of track_seeding
+// (SearchByTriplet.cu:348-370), from the ascending list `unused` (u entries) of the module
+// pair's unused hits (relative indices) instead of a walk over every position.
+// The pendulum visits position start + off, off = 0, +1, -1, +2, -2, ..., each wrapped once
+// into [0, n): position x gets rank 2d - 1 if d = (x - start) mod n lies in [1, n/2] (right of
+// start), else 2 ((start - x) mod n) (left, start itself included); ranks are a permutation of
+// 0..n-1, and the walk keeps the unused positions of the K smallest ranks, in rank order. Those
+// are among the K nearest unused entries on either side of start in the circular list, so the
+// 2K + 1 entries around its lower bound (all u if u <= 2K + 1) are ranked and the K smallest
+// taken. Writes hit_start + position to out[0..) and returns how many (<= K).
+template<unsigned K, bool count>
+__device__ __forceinline__ unsigned h0_candidates_from_list(
+  const uint16_t* unused,
+  unsigned u,
+  unsigned n,
+  unsigned hit_start,
+  int start,
+  uint16_t* out,
+  WorkCount& wc)
+{
+  if (u == 0) return 0;
+  const unsigned s0 = static_cast<unsigned>(start) >= n ? static_cast<unsigned>(start) - n : static_cast<unsigned>(start);
+  const unsigned p = static_cast<unsigned>(counted_search<count>(unused, u, static_cast<uint16_t>(s0), wc));
+  // key = rank << 16 | position (n <= 1024 on this path: rank < 2048, position < 1024).
+  constexpr unsigned M = 2 * K + 1;
+  constexpr unsigned none = 0xFFFFFFFFu;
+  const bool all = u <= M;
+  unsigned key[M];
+#pragma unroll
+  for (unsigned j = 0; j < M; ++j) {
+    unsigned ix;
+    if (all) {
+      ix = j < u ? j : 0u;
+    }
+    else {
+      const int q = static_cast<int>(p) + static_cast<int>(j) - static_cast<int>(K);
+      ix = static_cast<unsigned>(q < 0 ? q + static_cast<int>(u) : (q >= static_cast<int>(u) ? q - static_cast<int>(u) : q));
+    }
+    const unsigned x = unused[ix];
+    const unsigned d = x >= s0 ? x - s0 : x + n - s0;
+    const unsigned r = d == 0 ? 0u : (d <= n / 2 ? 2 * d - 1 : 2 * (n - d));
+    key[j] = (!all || j < u) ? (r << 16 | x) : none;
+  }
+  unsigned found = 0;
+  for (unsigned k = 0; k < K; ++k) {
+    unsigned best = none;
+#pragma unroll
+    for (unsigned j = 0; j < M; ++j) best = min(best, key[j]);
+    if (best == none) break;
+#pragma unroll
+    for (unsigned j = 0; j < M; ++j)
+      if (key[j] == best) key[j] = none;
+    out[found++] = static_cast<uint16_t>(hit_start + (best & 0xFFFFu));
+  }
+  return found;
+}
+
 // SearchByTriplet.cu:298-446, ordered, with the (h1, h0) pairs spread over lanes.
 //
 // The reference walks, per h1, its h0 candidates in pendulum order and, per h0, the h2 window
@@ -296,7 +369,7 @@ __device__ __forceinline__ void find_forward_candidate(
 // (first strict minimum), and finally one lane per h1 keeps the first strict minimum over its
 // h0 candidates in pendulum order: the same pair the sequential double loop selects.
 template<bool count, typename H>
-__device__ void track_seeding(
+__device__ __forceinline__ void track_seeding(
   WorkCount& wc,
   const H& hits,
   const SeedScratch& ss,
@@ -314,13 +387,27 @@ __device__ void track_seeding(
   const unsigned C = blockDim.x / K;
   const float dz_next = next.z[0] - prev.z[0];
 
+  // The list of unused hits of the previous module pair for step 1 (its writes are ordered
+  // before step 1 by the barrier of the compaction scan below).
+  const bool use_list = prev.hit_num <= unused_list_cap;
+  unsigned n_unused = 0;
+  if (use_list) {
+    for (unsigned c0 = 0; c0 < prev.hit_num; c0 += blockDim.x) {
+      const unsigned r = c0 + threadIdx.x;
+      const bool free_hit = r < prev.hit_num && !hits.used(prev.hit_start + r);
+      const Scan3 sc = scan3(sh, cn, free_hit, false, false);
+      if (free_hit) ss.unused[n_unused + sc.r[0]] = static_cast<uint16_t>(r);
+      n_unused += sc.t[0];
+    }
+  }
+
   // Unused hits of the current module pair, compacted in index order one chunk of blockDim
   // hits at a time (dev_rel_indices in global memory, the chunk's list in LDS); the seeds of a
   // chunk are made before the next chunk, which keeps the h1 order of the reference.
   uint32_t base = 0;
   for (unsigned c0 = 0; c0 < cur.hit_num; c0 += blockDim.x) {
     const unsigned r = c0 + threadIdx.x;
-    const bool act = r < cur.hit_num && !hits.used[cur.hit_start + r];
+    const bool act = r < cur.hit_num && !hits.used(cur.hit_start + r);
     const Scan3 sc = scan3(sh, cn, act, false, false);
     if (act) {
       o.h1_indices[base + sc.r[0]] = cur.hit_start + r;
@@ -341,26 +428,24 @@ __device__ void track_seeding(
           const int16_t h1_phi = hits.phi(h1_index);
           const int start = counted_search<count>(hits.phi_begin() + prev.hit_start, prev.hit_num, h1_phi, wc);
           phase_mark<count>(wc, phase_seed1_search);
-          // Pendulum order around `start`: start, +1, -1, +2, -2, ... (each wrapped once into
-          // the module pair). The used flags of 16 positions are read per round, all at once
-          // (positions past the module pair read its first hit and are ignored).
-          const int n = static_cast<int>(prev.hit_num);
-          constexpr unsigned R = 16;
-          for (unsigned i0 = 0; i0 < prev.hit_num && n_found < K; i0 += R) {
-            unsigned pos[R];
-            uint8_t flag[R];
-#pragma unroll
-            for (unsigned k = 0; k < R; ++k) {
-              const unsigned i = i0 + k;
-              const int off = (i & 1u) ? static_cast<int>((i + 1) / 2) : -static_cast<int>(i / 2);
-              const int p = start + off;
-              const int in_bounds = p < 0 ? p + n : (p >= n ? p - n : p);
-              pos[k] = prev.hit_start + (i < prev.hit_num ? static_cast<unsigned>(in_bounds) : 0u);
-              flag[k] = hits.used[pos[k]];
-            }
-#pragma unroll
-            for (unsigned k = 0; k < R; ++k) {
-              if (i0 + k < prev.hit_num && !flag[k] && n_found < K) ss.h0[threadIdx.x * K + n_found++] = pos[k];
+          uint16_t* h0_out = ss.h0 + threadIdx.x * K;
+          if (use_list) {
+            n_found = K == initial_seeding_h0_candidates
+                        ? h0_candidates_from_list<initial_seeding_h0_candidates, count>(
+                            ss.unused, n_unused, prev.hit_num, prev.hit_start, start, h0_out, wc)
+                        : h0_candidates_from_list<seeding_h0_candidates, count>(
+                            ss.unused, n_unused, prev.hit_num, prev.hit_start, start, h0_out, wc);
+          }
+          else {
+            // Module pairs beyond the list's capacity: the pendulum walk itself, start, +1, -1,
+            // +2, -2, ... (each wrapped once into the module pair).
+            const int n = static_cast<int>(prev.hit_num);
+            int phi_index = start;
+            for (unsigned i = 0; i < prev.hit_num && n_found < K; ++i) {
+              phi_index += (i & 1u) ? static_cast<int>(i) : -static_cast<int>(i);
+              const int in_bounds = phi_index < 0 ? phi_index + n : (phi_index >= n ? phi_index - n : phi_index);
+              const unsigned h0_index = prev.hit_start + static_cast<unsigned>(in_bounds);
+              if (!hits.used(h0_index)) h0_out[n_found++] = h0_index;
             }
           }
           ss.h1[threadIdx.x] = h1_index;
@@ -464,7 +549,7 @@ __device__ void track_seeding(
 
 // SearchByTriplet.cu:451-609, ordered.
 template<bool count, typename H>
-__device__ void track_forwarding(
+__device__ __forceinline__ void track_forwarding(
   WorkCount& wc,
   const H& hits,
   SearchShared& sh,
@@ -564,11 +649,11 @@ __device__ void track_forwarding(
     const Scan3 s = scan3(sh, cn, promote, reinsert, three);
     if (valid) {
       if (best_h2 != -1) {
-        hits.used[best_h2] = 1;
+        hits.mark_used(best_h2);
         if (promote) {
-          hits.used[th[0]] = 1;
-          hits.used[th[1]] = 1;
-          hits.used[th[2]] = 1;
+          hits.mark_used(th[0]);
+          hits.mark_used(th[1]);
+          hits.mark_used(th[2]);
           track_number = cn.n_tracks + s.r[0];
           if (track_number < o.track_capacity) {
             TrackHits& t = o.tracks[track_number];
@@ -613,7 +698,7 @@ __device__ void track_forwarding(
 }
 
 template<bool count, typename H>
-__device__ void search_event(
+__device__ __forceinline__ void search_event(
   WorkCount& wc,
   const H& hits,
   const SeedScratch& ss,
@@ -676,10 +761,11 @@ __device__ void search_event(
 }
 
 // One workgroup per event, or (work_counter set) a resident grid whose workgroups take the
-// next event slot from the counter until the slots run out. Dynamic LDS: the seeding scratch (13 B per lane), the ring of
-// tracks to follow (2048 x u32), then `lds_hits` hits as x[], y[], z[] (f32), phi[] (i16),
-// used[] (u8), module parity[] (u8). Events with more hits keep phi, used and parity in LDS
-// and read coordinates from global memory; beyond 4 x lds_hits everything stays global.
+// next event slot from the counter until the slots run out. Dynamic LDS: the seeding scratch
+// (13 B per lane), the ring of tracks to follow (2048 x u32), the list of unused hits of the
+// previous module pair (1024 x u16), then `lds_hits` hits as x[], y[], z[] (f32), phi[] (i16)
+// and a flags byte (used, module parity). Events with up to 5 x lds_hits hits keep phi and the
+// flags in LDS and read coordinates from global memory; larger events run on global memory.
 // The `count` instantiation also accumulates the work counters into work_stats[4] (one
 // atomic per wave). With event_clock, each workgroup stores at event_clock[10 * event_number]
 // its event's start and end time (wall clock ticks) and, in the counting instantiation, the
@@ -722,7 +808,8 @@ __global__ void k_search_by_triplet(
   ss.h1_chunk = ss.pair_h2 + blockDim.x;
   ss.n_h0 = reinterpret_cast<uint8_t*>(ss.h1_chunk + blockDim.x);
   uint32_t* ring = reinterpret_cast<uint32_t*>(dyn_all + ((13 * blockDim.x + 15) & ~15u));
-  char* dyn = reinterpret_cast<char*>(ring + max_tracks_to_follow);
+  ss.unused = reinterpret_cast<uint16_t*>(ring + max_tracks_to_follow);
+  char* dyn = reinterpret_cast<char*>(ss.unused + unused_list_cap);
   // One event per workgroup (work_counter == NULL), or a resident grid whose workgroups take
   // event slots from the counter until they run out (the counter starts at 0).
   for (unsigned round = 0;; ++round) {
@@ -780,37 +867,33 @@ __global__ void k_search_by_triplet(
       float* ys = xs + lds_hits;
       float* zs = ys + lds_hits;
       int16_t* ph = reinterpret_cast<int16_t*>(zs + lds_hits);
-      uint8_t* used = reinterpret_cast<uint8_t*>(ph + lds_hits);
-      uint8_t* odd = used + lds_hits;
+      uint8_t* flags = reinterpret_cast<uint8_t*>(ph + lds_hits);
       for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) {
         xs[i] = gxyz[3 * i];
         ys[i] = gxyz[3 * i + 1];
         zs[i] = gxyz[3 * i + 2];
         ph[i] = gph[i];
-        used[i] = 0;
-        odd[i] = module_parity(gids[i]);
+        flags[i] = module_parity(gids[i]) ? flag_odd : 0;
       }
       __syncthreads();
-      const LdsHits hits {xs, ys, zs, ph, used, odd};
+      const LdsHits hits {xs, ys, zs, ph, flags};
       search_event<count>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
       __syncthreads();
-      for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) gused[i] = used[i];
+      for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) gused[i] = flags[i] & flag_used;
     }
-    else if (n_hits <= 4 * lds_hits) {
+    else if (3 * n_hits <= search_bytes_per_hit * lds_hits) {
       int16_t* ph = reinterpret_cast<int16_t*>(dyn);
-      uint8_t* used = reinterpret_cast<uint8_t*>(ph + n_hits);
-      uint8_t* odd = used + n_hits;
+      uint8_t* flags = reinterpret_cast<uint8_t*>(ph + n_hits);
       for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) {
         ph[i] = gph[i];
-        used[i] = 0;
-        odd[i] = module_parity(gids[i]);
+        flags[i] = module_parity(gids[i]) ? flag_odd : 0;
       }
       cn.status |= status_lds_fallback;
       __syncthreads();
-      const MixedHits hits {gxyz, ph, used, odd};
+      const MixedHits hits {gxyz, ph, flags};
       search_event<count>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
       __syncthreads();
-      for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) gused[i] = used[i];
+      for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) gused[i] = flags[i] & flag_used;
     }
     else {
       for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) gused[i] = 0;
@@ -1008,12 +1091,10 @@ __global__ void k_three_hit_filter(
   if (threadIdx.x == 0) number_of_output[event_number] = base;
 }
 
-// Launch geometry: LDS bytes per hit of the full layout (x, y, z, phi, used, parity).
-constexpr unsigned search_bytes_per_hit = 16;
-
+// Launch geometry: the seeding scratch, the ring, the unused-hit list, then the hits.
 size_t search_dynamic_lds(unsigned block, unsigned lds_hits)
 {
-  return ((13 * (size_t) block + 15) & ~size_t(15)) + 4 * (size_t) max_tracks_to_follow +
+  return ((13 * (size_t) block + 15) & ~size_t(15)) + 4 * (size_t) max_tracks_to_follow + 2 * unused_list_cap +
          (size_t) lds_hits * search_bytes_per_hit;
 }
 

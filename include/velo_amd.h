@@ -295,6 +295,8 @@ int velo_sequence_status(velo_sequence* s, hipStream_t stream, unsigned* status)
  *                       global memory
  *   "search_largest_first"  1 (default): search_by_triplet takes the events in descending order
  *                       of their hit count; 0: in event order. Outputs do not depend on it
+ *   "clustering_resident_grid"  1: the fused clustering kernels run as a resident grid whose
+ *                       waves loop over (event, module pair) items; 0 (default): one wave per item
  *   "search_persistent" 1: search_by_triplet runs as a resident grid whose workgroups take the
  *                       next event from a device counter (work stealing) instead of one
  *                       workgroup per event; 0 (default). Outputs do not depend on it

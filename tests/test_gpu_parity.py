@@ -64,6 +64,7 @@ TUNINGS = {
     "event_order": {"search_largest_first": 0},                   # events in their own order
     "persistent": {"search_persistent": 1},                       # resident grid, work stealing
     "persistent_block64": {"search_persistent": 1, "search_block": 64, "search_largest_first": 0},
+    "clustering_resident": {"clustering_resident_grid": 1},       # clustering waves loop over items
 }
 
 
