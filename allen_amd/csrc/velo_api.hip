@@ -922,6 +922,45 @@ int velo_sequence_capture(velo_sequence* s, const velo_sequence_input* in, hipSt
   return rc;
 }
 
+size_t velo_mlp_scratch_floats(unsigned n)
+{
+  const size_t blocks = velo::mlp::mlp_blocks(n);
+  return blocks * velo::mlp::n_params + blocks;
+}
+
+int velo_mlp_train_step(
+  const float* dev_features,
+  const uint8_t* dev_labels,
+  unsigned n,
+  float* dev_params,
+  float* dev_adam_m,
+  float* dev_adam_v,
+  float* dev_scratch,
+  float lr,
+  unsigned step,
+  float* dev_grad,
+  float* dev_loss,
+  hipStream_t stream)
+{
+  if (!dev_features || !dev_labels || !n || !dev_params || !dev_scratch) return VELO_ERR_INVALID_ARGUMENT;
+  if (lr != 0.f && (!dev_adam_m || !dev_adam_v || step == 0)) return VELO_ERR_INVALID_ARGUMENT;
+  const unsigned blocks = velo::mlp::mlp_blocks(n);
+  float* partial = dev_scratch;
+  float* loss_partial = dev_scratch + (size_t) blocks * velo::mlp::n_params;
+  int rc = hip_status(velo::launch_mlp_train(
+    dev_features, dev_labels, n, 1.f / static_cast<float>(n), dev_params, partial, loss_partial, stream));
+  if (rc == VELO_OK)
+    rc = hip_status(velo::launch_mlp_reduce_adam(
+      partial, loss_partial, blocks, dev_params, dev_adam_m, dev_adam_v, dev_grad, dev_loss, lr, step, stream));
+  return rc;
+}
+
+int velo_mlp_score(const float* dev_features, unsigned n, const float* dev_params, float* dev_logits, hipStream_t stream)
+{
+  if (!dev_features || !dev_params || !dev_logits) return VELO_ERR_INVALID_ARGUMENT;
+  return hip_status(velo::launch_mlp_score(dev_features, n, dev_params, dev_logits, stream));
+}
+
 int velo_sequence_replay(velo_sequence* s, hipStream_t stream)
 {
   if (!s || !s->graph_exec) return VELO_ERR_INVALID_ARGUMENT;

@@ -226,6 +226,38 @@ hipError_t launch_copy_track_hit_number(const CopyHitNumberLaunch& a, hipStream_
 hipError_t launch_consolidate_tracks(const ConsolidateLaunch& a, hipStream_t s);
 hipError_t launch_kalman_filter(const KalmanLaunch& a, hipStream_t s);
 
+// Learned seed scorer (10 -> 32 -> 32 -> 1 MLP). Parameter layout (FP32): W1 [10][32], b1 [32],
+// W2 [32][32], b2 [32], w3 [32], b3.
+namespace mlp {
+constexpr unsigned n_inputs = 10, n_hidden = 32;
+constexpr unsigned param_w1 = 0, param_b1 = 320, param_w2 = 352, param_b2 = 1376, param_w3 = 1408, param_b3 = 1440;
+constexpr unsigned n_params = 1441;
+constexpr unsigned rows_per_block = 128;
+inline unsigned mlp_blocks(unsigned n) { return (n + rows_per_block - 1) / rows_per_block; }
+} // namespace mlp
+hipError_t launch_mlp_train(
+  const float* features,
+  const uint8_t* labels,
+  unsigned n,
+  float inv_norm,
+  const float* params,
+  float* partial,
+  float* loss_partial,
+  hipStream_t s);
+hipError_t launch_mlp_reduce_adam(
+  const float* partial,
+  const float* loss_partial,
+  unsigned n_blocks,
+  float* params,
+  float* m,
+  float* v,
+  float* grad_out,
+  float* loss_out,
+  float lr,
+  unsigned step,
+  hipStream_t s);
+hipError_t launch_mlp_score(const float* features, unsigned n, const float* params, float* logits, hipStream_t s);
+
 // Device exclusive scan with a leading zero: out[0] = 0, out[i+1] = sum(in[0..i]).
 size_t prefix_sum_scratch_words(unsigned n);
 hipError_t launch_prefix_sum(const uint32_t* in, unsigned n, uint32_t* out, uint32_t* scratch, hipStream_t s);

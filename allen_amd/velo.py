@@ -150,6 +150,9 @@ SIGNATURES = {
     "velo_sequence_stage_ms": (I, [P, ctypes.POINTER(F), ctypes.POINTER(U)]),
     "velo_sequence_capture": (I, [P, ctypes.POINTER(SequenceInput), P]),
     "velo_sequence_replay": (I, [P, P]),
+    "velo_mlp_scratch_floats": (ctypes.c_size_t, [U]),
+    "velo_mlp_train_step": (I, [P, P, U, P, P, P, P, F, U, P, P, P]),
+    "velo_mlp_score": (I, [P, U, P, P, P]),
 }
 
 STAGES = {"all": 1, "candidates": 2, "estimate": 3, "clustering": 4, "search": 5, "three_hit_filter": 6,

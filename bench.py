@@ -39,6 +39,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(workload: str, stage: str):
+    """HBM bytes per launch of `stage`'s kernel from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    committed for this workload (the last column of the stage's row), or None."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cols = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cols) >= 3 and cols[0] == stage and cols[-1].isdigit():
+            return int(cols[-1])
+    return None
+
+
 def algorithmic_bytes(ev, counts: dict) -> dict:
     """Algorithmic HBM bytes of each stage for one step (definitions in DESIGN.md)."""
     n = ev.n_events
@@ -220,7 +233,8 @@ def main():
         dom_bytes = alg["clustering"]
     achieved = dom_bytes / (kern_avg_ms / 1e3) / 1e9
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": measured_traffic(args.workload, dominant),
                 "algorithmic_bytes": dom_bytes, "avg_launch_ms": round(kern_avg_ms, 4)}
     # search_by_triplet is bounded by lane arithmetic, not bytes: 9 VALU ops per scored candidate,
     # ~12 per extrapolation (hit_phi_16), 3 per binary-search iteration.

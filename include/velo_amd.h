@@ -332,6 +332,34 @@ int velo_sequence_stage_ms(velo_sequence* s, float* total_ms, unsigned* runs);
 int velo_sequence_capture(velo_sequence* s, const velo_sequence_input* in, hipStream_t stream);
 int velo_sequence_replay(velo_sequence* s, hipStream_t stream);
 
+/* ---- Learned seed scorer (no reference counterpart): a 10 -> 32 -> 32 -> 1 MLP with ReLU on
+ * BF16 MFMA with FP32 accumulation, over the 10 features of a seed (velo_seed_features).
+ * Parameters: VELO_MLP_PARAMS FP32 values, W1 [10][32], b1 [32], W2 [32][32], b2 [32], w3 [32], b3;
+ * they are rounded to bf16 where they enter a matrix product. */
+#define VELO_MLP_PARAMS 1441
+#define VELO_SEED_FEATURES 10
+/* Scratch (floats) for velo_mlp_train_step over n rows. */
+size_t velo_mlp_scratch_floats(unsigned n);
+/* One step of full-batch gradient descent over n rows: mean binary cross-entropy of the logits
+ * against labels (0/1); Adam (beta 0.9, 0.999, eps 1e-8) with learning rate lr at 1-based step
+ * `step` updates dev_params, dev_adam_m, dev_adam_v; lr = 0 computes the gradient only.
+ * Optional outputs: dev_grad (VELO_MLP_PARAMS floats), dev_loss (sum of the rows' losses). */
+int velo_mlp_train_step(
+  const float* dev_features,
+  const uint8_t* dev_labels,
+  unsigned n,
+  float* dev_params,
+  float* dev_adam_m,
+  float* dev_adam_v,
+  float* dev_scratch,
+  float lr,
+  unsigned step,
+  float* dev_grad,
+  float* dev_loss,
+  hipStream_t stream);
+/* Logits of n rows of features. */
+int velo_mlp_score(const float* dev_features, unsigned n, const float* dev_params, float* dev_logits, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

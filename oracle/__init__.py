@@ -36,6 +36,7 @@ class VoInputs(ctypes.Structure):
         ("block_dim_x", ctypes.c_uint),
         ("compute_f64", ctypes.c_int),
         ("kalman_fp64_fallback", ctypes.c_int),
+        ("collect_seeds", ctypes.c_int),
     ]
 
 
@@ -97,6 +98,7 @@ class Result:
         "velo_kalman_beamline_states": np.float32, "velo_kalman_endvelo_states": np.float32,
         "velo_kalman_beamline_states_f64": np.float64, "velo_kalman_endvelo_states_f64": np.float64,
         "search_stats": np.uint64,
+        "seed_features": np.float32,
     }
 
     def __init__(self, handle, keep):

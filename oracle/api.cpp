@@ -171,6 +171,19 @@ struct Handle {
   vo::Sequence seq;
 };
 
+// Every seed written by seeding: (event, module pair of h1, h0, h1, h2) and its features.
+struct SeedLog : vo::SeedSink {
+  unsigned event = 0;
+  std::vector<uint32_t> records;
+  std::vector<float> features;
+  void begin_event(unsigned e) override { event = e; }
+  void seed(unsigned module_pair, unsigned h0, unsigned h1, unsigned h2, const float* f) override
+  {
+    records.insert(records.end(), {event, module_pair, h0, h1, h2});
+    features.insert(features.end(), f, f + vo::n_seed_features);
+  }
+};
+
 bool make_inputs(const vo_inputs* a, vo::RawInput& in, vo::Geometry& g, vo::SpTables& sp, vo::Params& p)
 {
   in.raw = a->raw;
@@ -214,7 +227,14 @@ void* vo_run(const vo_inputs* a)
   vo::Params p;
   if (!make_inputs(a, in, g, sp, p)) return nullptr;
   auto* h = new Handle;
-  vo::run_sequence(h->seq, in, a->n_events, a->event_start, g, sp, a->beamline, p, nullptr);
+  SeedLog log;
+  vo::run_sequence(h->seq, in, a->n_events, a->event_start, g, sp, a->beamline, p, a->collect_seeds ? &log : nullptr);
+  if (a->collect_seeds) {
+    auto* rec = h->seq.make<uint32_t>("seed_log", log.records.size());
+    std::copy(log.records.begin(), log.records.end(), rec);
+    auto* feat = h->seq.make<float>("seed_features", log.features.size());
+    std::copy(log.features.begin(), log.features.end(), feat);
+  }
   return h;
 }
 

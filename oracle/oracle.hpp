@@ -23,8 +23,17 @@ struct SearchStats {
 struct SeedSink {
   virtual ~SeedSink() = default;
   virtual void begin_event(unsigned) {}
-  virtual void candidate(unsigned h0, unsigned h1, unsigned h2, float scatter) = 0;
+  virtual void candidate(unsigned, unsigned, unsigned, float) {}
+  // A seed written by seeding (h1's module pair `module_pair`), with its scorer features.
+  virtual void seed(unsigned, unsigned, unsigned, unsigned, const float*) {}
 };
+
+// Scorer features of a seed (h0, h1, h2) whose h1 lies in module pair `module_pair`
+// (no reference counterpart): h2's residual to the (h0, h1) line and h1's residual to the
+// (h0, h2) line in x and y, the phi steps h0 -> h1 and h1 -> h2, the (h0, h1) slopes, the
+// scatter and the module pair, each scaled to order one.
+constexpr unsigned n_seed_features = 10;
+void seed_features(const Clusters& cc, unsigned h0, unsigned h1, unsigned h2, unsigned module_pair, float* f);
 
 void cache_sp_patterns(SpTables& t);
 

@@ -47,6 +47,37 @@ void sort_by_phi(
   }
 }
 
+void seed_features(const Clusters& cc, unsigned h0, unsigned h1, unsigned h2, unsigned module_pair, float* f)
+{
+  const float x0 = cc.x(h0), y0 = cc.y(h0), z0 = cc.z(h0);
+  const float x1 = cc.x(h1), y1 = cc.y(h1), z1 = cc.z(h1);
+  const float x2 = cc.x(h2), y2 = cc.y(h2), z2 = cc.z(h2);
+  // h2 against the (h0, h1) line: the seeding scatter's components.
+  const float td = 1.0f / (z1 - z0);
+  const float tx = (x1 - x0) * td;
+  const float ty = (y1 - y0) * td;
+  const float dz2 = z2 - z0;
+  const float dx2 = (x0 + tx * dz2) - x2;
+  const float dy2 = (y0 + ty * dz2) - y2;
+  // h1 against the (h0, h2) line.
+  const float td02 = 1.0f / (z2 - z0);
+  const float dz1 = z1 - z0;
+  const float dx1 = (x0 + ((x2 - x0) * td02) * dz1) - x1;
+  const float dy1 = (y0 + ((y2 - y0) * td02) * dz1) - y1;
+  const int16_t dphi01 = static_cast<int16_t>(cc.phi(h1) - cc.phi(h0));
+  const int16_t dphi12 = static_cast<int16_t>(cc.phi(h2) - cc.phi(h1));
+  f[0] = dx2 * 4.0f;
+  f[1] = dy2 * 4.0f;
+  f[2] = dx1 * 4.0f;
+  f[3] = dy1 * 4.0f;
+  f[4] = static_cast<float>(dphi01) * (1.0f / 512.0f);
+  f[5] = static_cast<float>(dphi12) * (1.0f / 512.0f);
+  f[6] = tx * 4.0f;
+  f[7] = ty * 4.0f;
+  f[8] = (dx2 * dx2 + dy2 * dy2) * 12.5f;
+  f[9] = static_cast<float>(module_pair) * (1.0f / 32.0f);
+}
+
 namespace {
 struct ModulePair {
   unsigned hit_start;
@@ -94,6 +125,7 @@ void track_seeding(
   float max_scatter,
   int16_t phi_tolerance,
   unsigned h0_candidates_to_consider,
+  unsigned module_pair,
   SearchStats& st,
   SeedSink* sink)
 {
@@ -158,6 +190,11 @@ void track_seeding(
           }
         }
       }
+    }
+    if (best_fit < max_scatter && sink) {
+      float f[n_seed_features];
+      seed_features(cc, best_h0, h1_index, best_h2, module_pair, f);
+      sink->seed(module_pair, best_h0, h1_index, best_h2, f);
     }
     if (best_fit < max_scatter) {
       const unsigned tn = atomics[atomic_ttf]++ % max_tracks_to_follow;
@@ -304,7 +341,7 @@ void search_by_triplet_event(
   const int16_t phi_tolerance_i16 = hit_phi_float_to_16(p.phi_tolerance);
   track_seeding(
     cc, mpd, hit_used, tracklets, tracks_to_follow, h1_rel_indices, atomics, p.max_scatter, phi_tolerance_i16,
-    initial_seeding_h0_candidates, st, sink);
+    initial_seeding_h0_candidates, first_module_pair - 1, st, sink);
   unsigned last_ttf = 0;
   --first_module_pair;
   while (first_module_pair > 1) {
@@ -319,7 +356,7 @@ void search_by_triplet_event(
       number_of_velo_tracks_event, phi_tolerance_i16, p.max_scatter, p.max_skipped_modules, st);
     track_seeding(
       cc, mpd, hit_used, tracklets, tracks_to_follow, h1_rel_indices, atomics, p.max_scatter, phi_tolerance_i16,
-      seeding_h0_candidates, st, sink);
+      seeding_h0_candidates, first_module_pair - 1, st, sink);
     --first_module_pair;
   }
   const unsigned prev_ttf = last_ttf;

@@ -28,6 +28,8 @@ typedef struct {
   unsigned block_dim_x;       /* velo_calculate_number_of_candidates block size (decides grid) */
   int compute_f64;            /* also run the FP64 Kalman restatement */
   int kalman_fp64_fallback;   /* FP32 Kalman with the FP64 covariance update on ill-conditioned steps */
+  int collect_seeds;          /* also return every seed: "seed_log" (u32 event, module pair, h0, h1, h2)
+                                 and "seed_features" (f32 x 10) */
 } vo_inputs;
 
 void vo_default_inputs(vo_inputs* a);

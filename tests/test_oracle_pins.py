@@ -485,3 +485,22 @@ def test_kalman_backward_flag(geometry, beamline):
     ev_st = r["velo_kalman_endvelo_states"]
     np.testing.assert_allclose(ev_st[2:6 * bwd.size:6], 770.0, atol=1e-3)
     del bl
+
+
+def test_mlp_restatement_gradient_by_finite_differences():
+    """oracle/mlp.py's gradient (without bf16 rounding) against central differences of its loss,
+    and its bf16 rounding on known values."""
+    from oracle import mlp
+    rng = np.random.default_rng(4)
+    x = rng.normal(0, 1, (50, 10))
+    y = (rng.random(50) < 0.4).astype(np.float64)
+    p = rng.normal(0, 0.4, mlp.N_PARAMS)
+    g = mlp.gradient(x, y, p, rounded=False)
+    eps = 1e-6
+    for q in list(range(0, 1441, 37)) + [1440, 1439, 1375, 351]:
+        d = np.zeros(mlp.N_PARAMS)
+        d[q] = eps
+        fd = (mlp.loss(x, y, p + d, False) - mlp.loss(x, y, p - d, False)) / (2 * eps) / x.shape[0]
+        assert abs(fd - g[q]) <= 1e-6 + 1e-5 * abs(fd), (q, fd, g[q])
+    vals = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -2.5, 1 + 2 ** -9], np.float32)
+    np.testing.assert_array_equal(mlp.bf16(vals), [1.0, 1.0, 1.0 + 2 ** -6, -2.5, 1.0])
