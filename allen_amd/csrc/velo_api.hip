@@ -452,6 +452,7 @@ struct velo_sequence {
   bool search_largest_first = true;
   bool search_persistent = false;
   bool clustering_resident_grid = false;
+  unsigned seed_log_capacity = 0; // seeds; buffers "seed_log_*" exist once the knob was set
   unsigned kalman_fp64_fallback = 0;
   bool fused_clustering = true;
   bool count_work = false;
@@ -602,6 +603,29 @@ int velo_sequence_set_tuning(velo_sequence* s, const char* knob, unsigned value)
   }
   if (k == "kalman_fp64_fallback") {
     s->kalman_fp64_fallback = value != 0;
+    return VELO_OK;
+  }
+  if (k == "seed_log") {
+    // (Re)allocates the log: counter, records (5 x u32) and features (10 x f32) per seed.
+    for (const char* name : {"seed_log_counter", "seed_log_records", "seed_log_features"}) {
+      auto it = s->bufs.find(name);
+      if (it != s->bufs.end()) {
+        (void) hipFree(it->second.first);
+        s->bufs.erase(it);
+      }
+    }
+    s->seed_log_capacity = 0;
+    if (!value) return VELO_OK;
+    const std::vector<std::pair<const char*, size_t>> layout = {
+      {"seed_log_counter", 4}, {"seed_log_records", 20 * (size_t) value},
+      {"seed_log_features", 4 * velo::n_seed_features * (size_t) value}};
+    for (const auto& kv : layout) {
+      void* p = nullptr;
+      const int rc = hip_status(hipMalloc(&p, kv.second));
+      if (rc != VELO_OK) return rc;
+      s->bufs[kv.first] = {p, kv.second};
+    }
+    s->seed_log_capacity = value;
     return VELO_OK;
   }
   if (k == "clustering_resident_grid") {
@@ -819,6 +843,13 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream
   k5.event_clock = s->count_work ? s->get<unsigned long long>("search_event_clock") : nullptr;
   if (k5.work_stats) VELO_TRY(velo::launch_fill_u32(reinterpret_cast<uint32_t*>(k5.work_stats), 8, 0, stream));
   VELO_MARK(mark_begin(VELO_STAGE_SEARCH));
+  if (s->seed_log_capacity) {
+    k5.seed_log.counter = s->get<uint32_t>("seed_log_counter");
+    k5.seed_log.records = s->get<uint32_t>("seed_log_records");
+    k5.seed_log.features = s->get<float>("seed_log_features");
+    k5.seed_log.capacity = s->seed_log_capacity;
+    VELO_TRY(velo::launch_fill_u32(k5.seed_log.counter, 1, 0, stream));
+  }
   if (s->search_persistent) {
     k5.work_counter = s->get<uint32_t>("search_counter");
     VELO_TRY(velo::launch_fill_u32(k5.work_counter, 1, 0, stream));

@@ -121,6 +121,16 @@ struct ClusterMpLaunch {
   char* unsorted_scratch;
 };
 
+// Every seed written by seeding, appended in no particular order: records (u32 event, module
+// pair of h1, h0, h1, h2; event-relative hit indices) and n_seed_features floats each.
+constexpr unsigned n_seed_features = 10;
+struct SeedLog {
+  uint32_t* counter = nullptr; // zero at launch; NULL = no log
+  uint32_t* records = nullptr;
+  float* features = nullptr;
+  unsigned capacity = 0;
+};
+
 struct SearchLaunch {
   const uint32_t* event_list;
   unsigned n_list;
@@ -148,6 +158,7 @@ struct SearchLaunch {
   // Optional: persistent mode, a resident grid whose workgroups take event slots from this
   // counter (zero at launch) instead of one workgroup per event.
   uint32_t* work_counter = nullptr;
+  SeedLog seed_log;
 };
 
 struct ThreeHitFilterLaunch {

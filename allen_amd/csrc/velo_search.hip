@@ -247,6 +247,8 @@ struct EventOutputs {
   uint32_t* ttf;        // the ring of tracks to follow, in LDS
   uint16_t* h1_indices;
   unsigned track_capacity;
+  unsigned event_number;
+  SeedLog seed_log;
 };
 
 // Four consecutive positions of an h2 window, read at once: position j of the window is hit
@@ -301,6 +303,41 @@ __device__ __forceinline__ void find_forward_candidate(
   candidate = static_cast<int16_t>(counted_search<count>(
     hits.phi_begin() + mp.hit_start, mp.hit_num, static_cast<int16_t>(phi - phi_tolerance), wc));
   extrapolated_phi = phi;
+}
+
+// Scorer features of a seed (h0, h1, h2) whose h1 lies in module pair `module_pair` (no
+// reference counterpart): h2's residual to the (h0, h1) line and h1's residual to the (h0, h2)
+// line in x and y, the phi steps h0 -> h1 and h1 -> h2, the (h0, h1) slopes, the scatter and the
+// module pair, each scaled to order one.
+template<typename H>
+__device__ __forceinline__ void
+seed_features(const H& hits, unsigned h0, unsigned h1, unsigned h2, unsigned module_pair, float* f)
+{
+  const float x0 = hits.x(h0), y0 = hits.y(h0), z0 = hits.z(h0);
+  const float x1 = hits.x(h1), y1 = hits.y(h1), z1 = hits.z(h1);
+  const float x2 = hits.x(h2), y2 = hits.y(h2), z2 = hits.z(h2);
+  const float td = 1.0f / (z1 - z0);
+  const float tx = (x1 - x0) * td;
+  const float ty = (y1 - y0) * td;
+  const float dz2 = z2 - z0;
+  const float dx2 = (x0 + tx * dz2) - x2;
+  const float dy2 = (y0 + ty * dz2) - y2;
+  const float td02 = 1.0f / (z2 - z0);
+  const float dz1 = z1 - z0;
+  const float dx1 = (x0 + ((x2 - x0) * td02) * dz1) - x1;
+  const float dy1 = (y0 + ((y2 - y0) * td02) * dz1) - y1;
+  const int16_t dphi01 = static_cast<int16_t>(hits.phi(h1) - hits.phi(h0));
+  const int16_t dphi12 = static_cast<int16_t>(hits.phi(h2) - hits.phi(h1));
+  f[0] = dx2 * 4.0f;
+  f[1] = dy2 * 4.0f;
+  f[2] = dx1 * 4.0f;
+  f[3] = dy1 * 4.0f;
+  f[4] = static_cast<float>(dphi01) * (1.0f / 512.0f);
+  f[5] = static_cast<float>(dphi12) * (1.0f / 512.0f);
+  f[6] = tx * 4.0f;
+  f[7] = ty * 4.0f;
+  f[8] = (dx2 * dx2 + dy2 * dy2) * 12.5f;
+  f[9] = static_cast<float>(module_pair) * (1.0f / 32.0f);
 }
 
 // The first K unused hits of the previous module pair in the pendulum order of track_seeding
@@ -368,7 +405,7 @@ __device__ __forceinline__ unsigned h0_candidates_from_list(
 // candidates of one h1 each, then every lane scores one (h1, h0) pair over its h2 window
 // (first strict minimum), and finally one lane per h1 keeps the first strict minimum over its
 // h0 candidates in pendulum order: the same pair the sequential double loop selects.
-template<bool count, typename H>
+template<bool count, bool log_seeds, typename H>
 __device__ __forceinline__ void track_seeding(
   WorkCount& wc,
   const H& hits,
@@ -379,6 +416,7 @@ __device__ __forceinline__ void track_seeding(
   const ModulePairData prev,
   const ModulePairData cur,
   const ModulePairData next,
+  unsigned module_pair,
   float max_scatter,
   int16_t phi_tolerance,
   unsigned h0_candidates_to_consider)
@@ -529,6 +567,24 @@ __device__ __forceinline__ void track_seeding(
         }
         h1_index = ss.h1[t];
         found = best_fit < max_scatter;
+      }
+      if constexpr (log_seeds) {
+        if (found) {
+          float f[n_seed_features];
+          seed_features(hits, best_h0, h1_index, best_h2, module_pair, f);
+          const unsigned slot = atomicAdd(o.seed_log.counter, 1u);
+          if (slot < o.seed_log.capacity) {
+            uint32_t* rec = o.seed_log.records + 5 * (size_t) slot;
+            rec[0] = o.event_number;
+            rec[1] = module_pair;
+            rec[2] = best_h0;
+            rec[3] = h1_index;
+            rec[4] = best_h2;
+            float* fo = o.seed_log.features + n_seed_features * (size_t) slot;
+#pragma unroll
+            for (unsigned k = 0; k < n_seed_features; ++k) fo[k] = f[k];
+          }
+        }
       }
       const Scan3 s = scan3(sh, cn, found, false, false);
       if (found) {
@@ -697,7 +753,7 @@ __device__ __forceinline__ void track_forwarding(
   }
 }
 
-template<bool count, typename H>
+template<bool count, bool log_seeds, typename H>
 __device__ __forceinline__ void search_event(
   WorkCount& wc,
   const H& hits,
@@ -714,9 +770,9 @@ __device__ __forceinline__ void search_event(
   phase_mark<count>(wc, phase_other);
   const int16_t phi_tolerance = f32_to_i16(phi_tolerance_f * convert_factor_i16);
   unsigned first_module_pair = n_module_pairs - 1;
-  track_seeding<count>(
+  track_seeding<count, log_seeds>(
     wc, hits, ss, sh, cn, o, sh.mp[first_module_pair], sh.mp[first_module_pair - 1], sh.mp[first_module_pair - 2],
-    max_scatter, phi_tolerance, initial_seeding_h0_candidates);
+    first_module_pair - 1, max_scatter, phi_tolerance, initial_seeding_h0_candidates);
   unsigned last_ttf = 0;
   --first_module_pair;
   while (first_module_pair > 1) {
@@ -732,9 +788,9 @@ __device__ __forceinline__ void search_event(
       max_skipped_modules);
     __syncthreads();
     phase_mark<count>(wc, phase_forwarding);
-    track_seeding<count>(
+    track_seeding<count, log_seeds>(
       wc, hits, ss, sh, cn, o, sh.mp[first_module_pair], sh.mp[first_module_pair - 1], sh.mp[first_module_pair - 2],
-      max_scatter, phi_tolerance, seeding_h0_candidates);
+      first_module_pair - 1, max_scatter, phi_tolerance, seeding_h0_candidates);
     --first_module_pair;
   }
   __syncthreads();
@@ -770,7 +826,7 @@ __device__ __forceinline__ void search_event(
 // atomic per wave). With event_clock, each workgroup stores at event_clock[10 * event_number]
 // its event's start and end time (wall clock ticks) and, in the counting instantiation, the
 // ticks spent in each Phase.
-template<bool count>
+template<bool count, bool log_seeds>
 __global__ void k_search_by_triplet(
   const uint32_t* event_list,
   const uint32_t* number_of_events,
@@ -794,7 +850,8 @@ __global__ void k_search_by_triplet(
   unsigned long long* work_stats,
   unsigned long long* event_clock,
   unsigned n_list,
-  uint32_t* work_counter)
+  uint32_t* work_counter,
+  SeedLog seed_log)
 {
   extern __shared__ __align__(16) char dyn_all[];
   __shared__ SearchShared sh;
@@ -845,6 +902,8 @@ __global__ void k_search_by_triplet(
     o.tracklets = tracklets_all + (size_t) event_number * max_tracks_to_follow;
     o.ttf = ring;
     o.h1_indices = rel_indices_all + hit_offset;
+    o.event_number = event_number;
+    o.seed_log = seed_log;
     // Slots of this event; the reference lets a busy event run into its neighbour's slots, here
     // nothing is written past the container and the overflow is reported.
     o.track_capacity = min(next_off > toff ? next_off - toff : 0u, container - min(toff, container));
@@ -877,7 +936,7 @@ __global__ void k_search_by_triplet(
       }
       __syncthreads();
       const LdsHits hits {xs, ys, zs, ph, flags};
-      search_event<count>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
+      search_event<count, log_seeds>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
       __syncthreads();
       for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) gused[i] = flags[i] & flag_used;
     }
@@ -891,7 +950,7 @@ __global__ void k_search_by_triplet(
       cn.status |= status_lds_fallback;
       __syncthreads();
       const MixedHits hits {gxyz, ph, flags};
-      search_event<count>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
+      search_event<count, log_seeds>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
       __syncthreads();
       for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) gused[i] = flags[i] & flag_used;
     }
@@ -901,7 +960,7 @@ __global__ void k_search_by_triplet(
       __threadfence_block();
       __syncthreads();
       const GlobalHits hits {gxyz, gph, gused, gids};
-      search_event<count>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
+      search_event<count, log_seeds>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
       __syncthreads();
     }
     // The ring of tracks to follow as far as it was written (dev_tracks_to_follow).
@@ -1119,25 +1178,27 @@ hipError_t launch_search_by_triplet(const SearchLaunch& a, hipStream_t s)
   unsigned grid = a.n_list;
   if (a.work_counter) {
     int dev = 0, cus = 0, per_cu = 0;
-    const void* kernel = a.work_stats ? reinterpret_cast<const void*>(&k_search_by_triplet<true>)
-                                      : reinterpret_cast<const void*>(&k_search_by_triplet<false>);
+    const void* kernel = a.work_stats        ? reinterpret_cast<const void*>(&k_search_by_triplet<true, false>)
+                         : a.seed_log.counter ? reinterpret_cast<const void*>(&k_search_by_triplet<false, true>)
+                                              : reinterpret_cast<const void*>(&k_search_by_triplet<false, false>);
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, static_cast<int>(block), dyn) == hipSuccess &&
         cus > 0 && per_cu > 0)
       grid = min(grid, static_cast<unsigned>(cus * per_cu));
   }
-#define VELO_LAUNCH_SEARCH(C)                                                                                       \
+#define VELO_LAUNCH_SEARCH(C, L)                                                                                    \
   hipLaunchKernelGGL(                                                                                               \
-    k_search_by_triplet<C>, dim3(grid), dim3(block), dyn, s, a.event_list, a.number_of_events,                      \
+    (k_search_by_triplet<C, L>), dim3(grid), dim3(block), dyn, s, a.event_list, a.number_of_events,                 \
     a.sorted_container, a.offsets_estimated_input_size, a.module_cluster_num,                                       \
     reinterpret_cast<const Geometry*>(a.geometry), reinterpret_cast<TrackHits*>(a.tracks),                          \
     reinterpret_cast<TrackletHits*>(a.tracklets), a.tracks_to_follow,                                               \
     reinterpret_cast<TrackletHits*>(a.three_hit_tracks), a.hit_used, a.atomics_velo, a.rel_indices,                \
     a.number_of_velo_tracks, a.status, a.phi_tolerance, a.max_scatter, a.max_skipped_modules, lds_hits,             \
-    a.work_stats, a.event_clock, a.n_list, a.work_counter)
-  if (a.work_stats) VELO_LAUNCH_SEARCH(true);
-  else VELO_LAUNCH_SEARCH(false);
+    a.work_stats, a.event_clock, a.n_list, a.work_counter, a.seed_log)
+  if (a.work_stats) VELO_LAUNCH_SEARCH(true, false);
+  else if (a.seed_log.counter) VELO_LAUNCH_SEARCH(false, true);
+  else VELO_LAUNCH_SEARCH(false, false);
 #undef VELO_LAUNCH_SEARCH
   return hipGetLastError();
 }

@@ -330,6 +330,15 @@ class VeloSequence:
         check(lib().velo_sequence_capture(self.handle, ctypes.byref(inp), stream_handle(stream)),
               "velo_sequence_capture")
 
+    def seed_log(self):
+        """(records u32 [n, 5], features f32 [n, 10], seeds seen) of the last run's seed log."""
+        seen = int(self.fetch("seed_log_counter", np.uint32, 1)[0])
+        cap = self.buffer_ptr("seed_log_records")[1] // 20
+        n = min(seen, cap)
+        rec = self.fetch("seed_log_records", np.uint32, 5 * n).reshape(n, 5)
+        feat = self.fetch("seed_log_features", np.float32, 10 * n).reshape(n, 10)
+        return rec, feat, seen
+
     def replay(self, stream):
         """Launches the captured run on `stream`."""
         check(lib().velo_sequence_replay(self.handle, stream_handle(stream)), "velo_sequence_replay")

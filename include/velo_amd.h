@@ -295,6 +295,11 @@ int velo_sequence_status(velo_sequence* s, hipStream_t stream, unsigned* status)
  *                       global memory
  *   "search_largest_first"  1 (default): search_by_triplet takes the events in descending order
  *                       of their hit count; 0: in event order. Outputs do not depend on it
+ *   "seed_log"          N > 0: search_by_triplet also records up to N seeds of each run, in no
+ *                       particular order, into "seed_log_records" (u32 event, module pair of h1,
+ *                       h0, h1, h2 as event-relative hit indices), "seed_log_features" (10 x f32,
+ *                       the scorer's inputs) and "seed_log_counter" (seeds seen, may exceed N);
+ *                       0 (default) frees the log
  *   "clustering_resident_grid"  1: the fused clustering kernels run as a resident grid whose
  *                       waves loop over (event, module pair) items; 0 (default): one wave per item
  *   "search_persistent" 1: search_by_triplet runs as a resident grid whose workgroups take the

@@ -57,3 +57,30 @@ def test_mlp_training_reduces_loss():
     assert last < 0.35 * first, (first, last)
     acc = ((net.score(xd).cpu().numpy() > 0) == (y == 1)).mean()
     assert acc > 0.97
+
+
+def test_seed_log_matches_oracle(geometry, beamline):
+    """The GPU seed log holds exactly the oracle's seeds (as a set: the GPU appends them in no
+    particular order) with bit-identical features, and the plain outputs are unchanged."""
+    import oracle
+    from allen_amd import synth, velo
+    from tests import parity
+    ev = synth.generate(n_events=60, pileup=1, nu=7.6, seed=61)
+    ref = oracle.run(ev, geometry, beamline, collect_seeds=1)
+    constants = velo.VeloConstants(geometry, beamline)
+    seq = velo.VeloSequence(constants, ev.n_events, ev.raw.size)
+    n_ref = ref["seed_log"].size // 5
+    seq.set_tuning("seed_log", n_ref + 100)
+    seq.run(velo.DeviceEvents(ev))
+    rec, feat, seen = seq.seed_log()
+    assert seen == n_ref and rec.shape[0] == n_ref
+    want_rec = ref["seed_log"].reshape(-1, 5)
+    want_feat = ref["seed_features"].reshape(-1, 10)
+    key = lambda r: np.lexsort(r.T[::-1])
+    gi, wi = key(rec), key(want_rec)
+    np.testing.assert_array_equal(rec[gi], want_rec[wi])
+    np.testing.assert_array_equal(feat[gi].view(np.uint32), want_feat[wi].view(np.uint32))
+    errs = parity.compare(parity.gpu_buffers(seq, ref), ref, True)
+    assert not errs, "\n".join(errs[:20])
+    seq.close()
+    constants.close()
