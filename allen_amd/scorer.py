@@ -72,3 +72,134 @@ class MLP:
 
     def host_params(self) -> np.ndarray:
         return self.params.cpu().numpy().copy()
+
+
+# ---------------------------------------------------------------------------------------------
+# Training data: the seeds the plain search writes, labelled by whether they end in a track
+# that the checker matches to a generated particle (>= 70 % of its hits).
+
+def seed_labels(events, seq, records: np.ndarray) -> np.ndarray:
+    """1 for each seed (rows of `records`: event, module pair, h0, h1, h2) whose three hits open a
+    consolidated track of the last run that the checker matches to a particle, else 0."""
+    from . import checker
+    n = events.n_events
+    est = seq.fetch("offsets_estimated_input_size", np.uint32, 26 * n + 1).astype(np.int64)
+    ids = seq.fetch("sorted_velo_cluster_container", np.uint8, 4 * int(est[-1])).view("<u4")
+    seed_ids = ids[est[0:26 * n:26][records[:, 0]][:, None] + records[:, 2:5].astype(np.int64)]
+    off_all = seq.fetch("offsets_all_velo_tracks", np.uint32, n + 1)
+    n_tracks = int(off_all[-1])
+    off_h = seq.fetch("offsets_velo_track_hit_number", np.uint32, n_tracks + 1).astype(np.int64)
+    th = seq.fetch("velo_track_hits", np.uint8, 4 * int(off_h[-1])).view("<u4")
+    res = checker.check(events, off_all, off_h, th)
+    track_event = np.repeat(np.arange(n), np.diff(off_all.astype(np.int64)))
+    first3 = th[off_h[:-1, None] + np.arange(3)[None, :]]
+    good = {(int(e), int(a), int(b), int(c))
+            for e, (a, b, c), m in zip(track_event, first3, res.track_particle >= 0) if m}
+    return np.fromiter(((int(e), int(a), int(b), int(c)) in good
+                        for e, (a, b, c) in zip(records[:, 0], seed_ids)), dtype=np.uint8, count=records.shape[0])
+
+
+def collect(constants, batches, log=None):
+    """Runs the plain GPU search over each batch of events (an iterable of synth.Events with
+    truth) with the seed log on; returns features (float32 [n, 10]) and labels (uint8 [n])."""
+    feats, labs = [], []
+    for ev in batches:
+        seq = velo.VeloSequence(constants, ev.n_events, ev.raw.size)
+        seq.set_tuning("seed_log", 400 * ev.n_events)
+        seq.run(velo.DeviceEvents(ev))
+        rec, f, seen = seq.seed_log()
+        if seen > rec.shape[0]:
+            raise velo.VeloError(f"seed log overflow: {seen} seeds, capacity {rec.shape[0]}")
+        lab = seed_labels(ev, seq, rec)
+        seq.close()
+        feats.append(f)
+        labs.append(lab)
+        if log:
+            log(f"[scorer] {ev.n_events} events: {rec.shape[0]} seeds, {lab.mean():.3f} positive")
+    return np.concatenate(feats), np.concatenate(labs)
+
+
+def train(features: np.ndarray, labels: np.ndarray, epochs: int = 20, batch: int = 65536, lr: float = 3e-3,
+          seed: int = 0, log=None) -> MLP:
+    """Adam on shuffled minibatches on the GPU."""
+    torch = velo._torch()
+    xd = torch.from_numpy(np.ascontiguousarray(features, np.float32)).cuda()
+    yd = torch.from_numpy(np.ascontiguousarray(labels, np.uint8)).cuda()
+    net = MLP(seed=seed)
+    rng = np.random.default_rng(seed)
+    n = labels.size
+    for ep in range(epochs):
+        perm = torch.from_numpy(rng.permutation(n)).cuda()
+        xs, ys = xd[perm].contiguous(), yd[perm].contiguous()
+        tot = 0.0
+        for b0 in range(0, n, batch):
+            loss, _ = net.train_step(xs[b0:b0 + batch], ys[b0:b0 + batch], lr)
+            tot += loss * min(batch, n - b0)
+        if log:
+            log(f"[scorer] epoch {ep}: mean loss {tot / n:.4f}")
+    return net
+
+
+def roc(scores: np.ndarray, labels: np.ndarray) -> dict:
+    """ROC area, and for the pruning thresholds the fraction of seeds removed and of positive seeds
+    lost (a seed is kept when its logit >= threshold)."""
+    order = np.argsort(-scores, kind="stable")
+    y = labels[order].astype(np.float64)
+    pos, neg = y.sum(), (1 - y).sum()
+    tpr = np.concatenate([[0], np.cumsum(y) / max(pos, 1)])
+    fpr = np.concatenate([[0], np.cumsum(1 - y) / max(neg, 1)])
+    auc = float(np.trapezoid(tpr, fpr)) if neg and pos else float("nan")
+    table = []
+    for lost in (0.0005, 0.001, 0.002, 0.005, 0.01):
+        k = int(np.ceil((1 - lost) * pos))
+        kept_pos = np.cumsum(y)
+        idx = int(np.searchsorted(kept_pos, k)) if k > 0 else 0
+        thr = float(scores[order][min(idx, y.size - 1)])
+        keep = scores >= thr
+        table.append({"positive_lost": round(float(1 - (labels[keep] == 1).sum() / max(pos, 1)), 5),
+                      "seeds_removed": round(float(1 - keep.mean()), 5), "threshold": thr})
+    return {"auc": auc, "positives": int(pos), "negatives": int(neg), "thresholds": table}
+
+
+def save(path: str, net: MLP, meta: dict) -> None:
+    np.savez(path, params=net.host_params(), **{k: np.asarray(v) for k, v in meta.items()})
+
+
+def load_params(path: str) -> np.ndarray:
+    with np.load(path, allow_pickle=False) as z:
+        return z["params"].astype(np.float32)
+
+
+def main(argv=None):
+    """Trains the scorer on seeds of the plain GPU search and prints its evaluation as JSON."""
+    import argparse
+    import json
+    import sys
+    from . import synth
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--train-events", type=int, default=16000)
+    ap.add_argument("--test-events", type=int, default=2000)
+    ap.add_argument("--chunk", type=int, default=2000)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--out", default="")
+    args = ap.parse_args(argv)
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    constants = velo.VeloConstants(synth.read_geometry(), synth.read_beamline())
+    # Config-1-style events (one PV, Poisson(120) tracks, 3 % noise), disjoint seeds for testing.
+    batches = lambda n, s0: (synth.generate(n_events=min(args.chunk, n - c), pileup=0, mean_tracks=120.0,
+                                            seed=s0 + i) for i, c in enumerate(range(0, n, args.chunk)))
+    x, y = collect(constants, batches(args.train_events, 1000 * args.seed), log)
+    xt, yt = collect(constants, batches(args.test_events, 1000 * args.seed + 500), log)
+    log(f"[scorer] training on {y.size} seeds ({y.mean():.4f} positive), testing on {yt.size}")
+    net = train(x, y, epochs=args.epochs, seed=args.seed, log=log)
+    torch = velo._torch()
+    st = net.score(torch.from_numpy(xt).cuda()).cpu().numpy()
+    out = {"train_seeds": int(y.size), "test_seeds": int(yt.size), "test": roc(st, yt)}
+    if args.out:
+        save(args.out, net, {"train_seeds": y.size})
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

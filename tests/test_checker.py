@@ -55,3 +55,42 @@ def test_checker_on_clean_single_tracks(geometry, beamline):
                                 r["velo_track_hits"])
     assert res.ghosts == 0 and res.hit_purity == 1.0
     assert res.efficiency > 0.97
+
+
+class _OracleBuffers:
+    """The fetch() of a GPU sequence, served from an oracle run (same buffer names)."""
+
+    def __init__(self, r):
+        self.r = r
+
+    def fetch(self, name, dtype, count):
+        a = np.ascontiguousarray(self.r[name]).view(np.uint8).view(dtype)
+        return a[:count]
+
+
+def test_seed_labels_against_loop(geometry, beamline):
+    """scorer.seed_labels on the oracle's own seed log: a seed is positive iff a consolidated
+    track starts with its three hits and is matched (>= 70 % of hits from one particle)."""
+    from allen_amd import scorer
+    ev = synth.generate(n_events=12, pileup=1, nu=7.6, seed=43)
+    r = oracle.run(ev, geometry, beamline, collect_seeds=1)
+    rec = r["seed_log"].reshape(-1, 5).astype(np.int64)
+    lab = scorer.seed_labels(ev, _OracleBuffers(r), rec)
+    # Loop restatement.
+    est = r["offsets_estimated_input_size"].astype(np.int64)
+    total = int(est[-1])
+    ids = r["sorted_velo_cluster_container"][:4 * total].view("<u4")
+    off_all = r["offsets_all_velo_tracks"]
+    off_h = r["offsets_velo_track_hit_number"]
+    tids = r["velo_track_hits"][:4 * int(off_h[-1])].view("<u4")
+    res = checker.check(ev, off_all, off_h, tids)
+    want = []
+    for e, _, h0, h1, h2 in rec:
+        trip = [int(ids[est[26 * e] + h]) for h in (h0, h1, h2)]
+        hit = 0
+        for t in range(off_all[e], off_all[e + 1]):
+            if tids[off_h[t]:off_h[t] + 3].tolist() == trip and res.track_particle[t] >= 0:
+                hit = 1
+        want.append(hit)
+    np.testing.assert_array_equal(lab, want)
+    assert 0.3 < lab.mean() <= 1.0 and lab.sum() > 100
