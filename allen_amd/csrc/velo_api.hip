@@ -453,6 +453,8 @@ struct velo_sequence {
   bool search_persistent = false;
   bool clustering_resident_grid = false;
   unsigned seed_log_capacity = 0; // seeds; buffers "seed_log_*" exist once the knob was set
+  bool scorer_on = false;         // buffer "scorer_params" holds the scorer's parameters
+  float scorer_threshold = 0.f;
   unsigned kalman_fp64_fallback = 0;
   bool fused_clustering = true;
   bool count_work = false;
@@ -607,7 +609,7 @@ int velo_sequence_set_tuning(velo_sequence* s, const char* knob, unsigned value)
   }
   if (k == "seed_log") {
     // (Re)allocates the log: counter, records (5 x u32) and features (10 x f32) per seed.
-    for (const char* name : {"seed_log_counter", "seed_log_records", "seed_log_features"}) {
+    for (const char* name : {"seed_log_counter", "seed_log_records", "seed_log_features", "seed_log_logits"}) {
       auto it = s->bufs.find(name);
       if (it != s->bufs.end()) {
         (void) hipFree(it->second.first);
@@ -618,7 +620,7 @@ int velo_sequence_set_tuning(velo_sequence* s, const char* knob, unsigned value)
     if (!value) return VELO_OK;
     const std::vector<std::pair<const char*, size_t>> layout = {
       {"seed_log_counter", 4}, {"seed_log_records", 20 * (size_t) value},
-      {"seed_log_features", 4 * velo::n_seed_features * (size_t) value}};
+      {"seed_log_features", 4 * velo::n_seed_features * (size_t) value}, {"seed_log_logits", 4 * (size_t) value}};
     for (const auto& kv : layout) {
       void* p = nullptr;
       const int rc = hip_status(hipMalloc(&p, kv.second));
@@ -848,7 +850,12 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream
     k5.seed_log.records = s->get<uint32_t>("seed_log_records");
     k5.seed_log.features = s->get<float>("seed_log_features");
     k5.seed_log.capacity = s->seed_log_capacity;
+    k5.seed_log.logits = s->get<float>("seed_log_logits");
     VELO_TRY(velo::launch_fill_u32(k5.seed_log.counter, 1, 0, stream));
+  }
+  if (s->scorer_on) {
+    k5.scorer_params = s->get<float>("scorer_params");
+    k5.scorer_threshold = s->scorer_threshold;
   }
   if (s->search_persistent) {
     k5.work_counter = s->get<uint32_t>("search_counter");
@@ -990,6 +997,27 @@ int velo_mlp_score(const float* dev_features, unsigned n, const float* dev_param
 {
   if (!dev_features || !dev_params || !dev_logits) return VELO_ERR_INVALID_ARGUMENT;
   return hip_status(velo::launch_mlp_score(dev_features, n, dev_params, dev_logits, stream));
+}
+
+int velo_sequence_set_scorer(velo_sequence* s, const float* params, float threshold)
+{
+  if (!s) return VELO_ERR_INVALID_ARGUMENT;
+  if (!params) {
+    s->scorer_on = false;
+    return VELO_OK;
+  }
+  if (!s->bufs.count("scorer_params")) {
+    void* p = nullptr;
+    const int rc = hip_status(hipMalloc(&p, 4 * velo::mlp::n_params));
+    if (rc != VELO_OK) return rc;
+    s->bufs["scorer_params"] = {p, 4 * velo::mlp::n_params};
+  }
+  const int rc = hip_status(
+    hipMemcpy(s->get<float>("scorer_params"), params, 4 * velo::mlp::n_params, hipMemcpyHostToDevice));
+  if (rc != VELO_OK) return rc;
+  s->scorer_on = true;
+  s->scorer_threshold = threshold;
+  return VELO_OK;
 }
 
 int velo_sequence_replay(velo_sequence* s, hipStream_t stream)

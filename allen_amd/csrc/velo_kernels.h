@@ -121,6 +121,10 @@ struct ClusterMpLaunch {
   char* unsorted_scratch;
 };
 
+// What search_by_triplet does with the seeds it writes: nothing more, record them (SeedLog), or
+// score them with the learned scorer and keep those at or above the threshold.
+enum { seed_mode_plain = 0, seed_mode_log = 1, seed_mode_scored = 2 };
+
 // Every seed written by seeding, appended in no particular order: records (u32 event, module
 // pair of h1, h0, h1, h2; event-relative hit indices) and n_seed_features floats each.
 constexpr unsigned n_seed_features = 10;
@@ -129,6 +133,7 @@ struct SeedLog {
   uint32_t* records = nullptr;
   float* features = nullptr;
   unsigned capacity = 0;
+  float* logits = nullptr; // scored mode: each seed's logit
 };
 
 struct SearchLaunch {
@@ -159,6 +164,9 @@ struct SearchLaunch {
   // counter (zero at launch) instead of one workgroup per event.
   uint32_t* work_counter = nullptr;
   SeedLog seed_log;
+  // Scored mode: the scorer's FP32 parameters (mlp::n_params) and the logit threshold.
+  const float* scorer_params = nullptr;
+  float scorer_threshold = 0.f;
 };
 
 struct ThreeHitFilterLaunch {

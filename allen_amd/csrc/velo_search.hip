@@ -14,6 +14,7 @@
 // a larger event runs the same code on the global containers.
 #include "velo_common.cuh"
 #include "velo_kernels.h"
+#include "velo_mfma.cuh"
 
 namespace velo {
 
@@ -240,6 +241,8 @@ struct SeedScratch {
   uint16_t* unused;   // unused hits of the previous module pair (relative indices, ascending)
 };
 
+struct ScorerLds;
+
 struct EventOutputs {
   TrackHits* tracks;
   TrackletHits* three_hit_tracks;
@@ -249,6 +252,8 @@ struct EventOutputs {
   unsigned track_capacity;
   unsigned event_number;
   SeedLog seed_log;
+  const ScorerLds* scorer;
+  float scorer_threshold;
 };
 
 // Four consecutive positions of an h2 window, read at once: position j of the window is hit
@@ -303,6 +308,82 @@ __device__ __forceinline__ void find_forward_candidate(
   candidate = static_cast<int16_t>(counted_search<count>(
     hits.phi_begin() + mp.hit_start, mp.hit_num, static_cast<int16_t>(phi - phi_tolerance), wc));
   extrapolated_phi = phi;
+}
+
+// The learned scorer's weights in LDS (scored mode): W1 (K padded to 16) and W2 as bf16, the
+// biases and w3 (bf16-rounded) as FP32.
+struct ScorerLds {
+  uint16_t w1[16 * 32];
+  uint16_t w2[32 * 32];
+  float b1[32], b2[32], w3[32];
+  float b3;
+};
+constexpr unsigned scorer_lds_bytes = (sizeof(ScorerLds) + 15) & ~15u;
+
+__device__ void load_scorer(const float* p, ScorerLds& w)
+{
+  using namespace mlp;
+  for (unsigned t = threadIdx.x; t < 32 * 32; t += blockDim.x) {
+    if (t < 16 * 32) w.w1[t] = t / 32 < n_inputs ? to_bf16(p[param_w1 + t]) : 0;
+    w.w2[t] = to_bf16(p[param_w2 + t]);
+  }
+  for (unsigned t = threadIdx.x; t < 32; t += blockDim.x) {
+    w.b1[t] = p[param_b1 + t];
+    w.b2[t] = p[param_b2 + t];
+    w.w3[t] = from_bf16(to_bf16(p[param_w3 + t]));
+  }
+  if (threadIdx.x == 0) w.b3 = p[param_b3];
+}
+
+// Logit of the seed of lane l (features fp[0..9], fp[10..15] zero) for all 64 lanes of a wave,
+// on MFMA: the same arithmetic as k_mlp_score. Seeds 32 rb + r form row block rb; each lane
+// gathers its operand features with shuffles, the hidden layer goes through `tile` (32 x 32
+// bf16), the logits through `tmp` (64 floats). The whole wave must be active.
+__device__ __forceinline__ float scorer_logit(const float* fp, const ScorerLds& w, uint16_t* tile, float* tmp)
+{
+  using namespace mlp;
+  const unsigned l = lane_id(), h = l / 32, c = l % 32;
+#pragma unroll
+  for (unsigned rb = 0; rb < 2; ++rb) {
+    const unsigned src = 32 * rb + c;
+    f32x16 acc = zero16();
+#pragma unroll
+    for (unsigned k0 = 0; k0 < 16; k0 += 8) {
+      bf16x4 a, b;
+#pragma unroll
+      for (unsigned i = 0; i < 4; ++i) {
+        const float lo = __shfl(fp[k0 + i], src, wave_size);
+        const float hi = __shfl(fp[k0 + 4 + i], src, wave_size);
+        a[i] = static_cast<short>(to_bf16(h ? hi : lo));
+        b[i] = static_cast<short>(w.w1[(k0 + 4 * h + i) * 32 + c]);
+      }
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (unsigned v = 0; v < 16; ++v) {
+      const float z = acc[v] + w.b1[c];
+      tile[d_row(v) * 32 + c] = to_bf16(z > 0.f ? z : 0.f);
+    }
+    wave_sync();
+    acc = mma32(tile, 32, 1, w.w2, 32, 1, 32, zero16());
+    float q[16];
+#pragma unroll
+    for (unsigned v = 0; v < 16; ++v) {
+      const float z = acc[v] + w.b2[c];
+      q[v] = (z > 0.f ? z : 0.f) * w.w3[c];
+    }
+#pragma unroll
+    for (unsigned d = 1; d < 32; d <<= 1) {
+#pragma unroll
+      for (unsigned v = 0; v < 16; ++v) q[v] += __shfl_xor(q[v], d, wave_size);
+    }
+    if (c == 0) {
+#pragma unroll
+      for (unsigned v = 0; v < 16; ++v) tmp[32 * rb + d_row(v)] = q[v] + w.b3;
+    }
+    wave_sync();
+  }
+  return tmp[l];
 }
 
 // Scorer features of a seed (h0, h1, h2) whose h1 lies in module pair `module_pair` (no
@@ -405,7 +486,7 @@ __device__ __forceinline__ unsigned h0_candidates_from_list(
 // candidates of one h1 each, then every lane scores one (h1, h0) pair over its h2 window
 // (first strict minimum), and finally one lane per h1 keeps the first strict minimum over its
 // h0 candidates in pendulum order: the same pair the sequential double loop selects.
-template<bool count, bool log_seeds, typename H>
+template<bool count, int seed_mode, typename H>
 __device__ __forceinline__ void track_seeding(
   WorkCount& wc,
   const H& hits,
@@ -568,7 +649,7 @@ __device__ __forceinline__ void track_seeding(
         h1_index = ss.h1[t];
         found = best_fit < max_scatter;
       }
-      if constexpr (log_seeds) {
+      if constexpr (seed_mode == seed_mode_log) {
         if (found) {
           float f[n_seed_features];
           seed_features(hits, best_h0, h1_index, best_h2, module_pair, f);
@@ -584,6 +665,36 @@ __device__ __forceinline__ void track_seeding(
 #pragma unroll
             for (unsigned k = 0; k < n_seed_features; ++k) fo[k] = f[k];
           }
+        }
+      }
+      if constexpr (seed_mode == seed_mode_scored) {
+        // The seeds of a sub-chunk are all in wave 0 (C <= 64). A seed is kept when its logit
+        // reaches the threshold; a sub-chunk whose seeds would all go keeps them (plain search
+        // as fallback). The scratch of steps 1-2 is dead here and holds the hidden layer.
+        if (threadIdx.x < wave_size) {
+          float fp[16];
+#pragma unroll
+          for (unsigned k = 0; k < 16; ++k) fp[k] = 0.f;
+          if (found) seed_features(hits, best_h0, h1_index, best_h2, module_pair, fp);
+          const float logit =
+            scorer_logit(fp, *o.scorer, reinterpret_cast<uint16_t*>(ss.pair_scatter), reinterpret_cast<float*>(ss.n_h0));
+          const bool keep = found && logit >= o.scorer_threshold;
+          if (found && o.seed_log.counter) {
+            const unsigned slot = atomicAdd(o.seed_log.counter, 1u);
+            if (slot < o.seed_log.capacity) {
+              uint32_t* rec = o.seed_log.records + 5 * (size_t) slot;
+              rec[0] = o.event_number;
+              rec[1] = module_pair;
+              rec[2] = best_h0;
+              rec[3] = h1_index;
+              rec[4] = best_h2;
+              float* fo = o.seed_log.features + n_seed_features * (size_t) slot;
+#pragma unroll
+              for (unsigned k = 0; k < n_seed_features; ++k) fo[k] = fp[k];
+              if (o.seed_log.logits) o.seed_log.logits[slot] = logit;
+            }
+          }
+          if (__ballot(keep)) found = keep;
         }
       }
       const Scan3 s = scan3(sh, cn, found, false, false);
@@ -753,7 +864,7 @@ __device__ __forceinline__ void track_forwarding(
   }
 }
 
-template<bool count, bool log_seeds, typename H>
+template<bool count, int seed_mode, typename H>
 __device__ __forceinline__ void search_event(
   WorkCount& wc,
   const H& hits,
@@ -770,7 +881,7 @@ __device__ __forceinline__ void search_event(
   phase_mark<count>(wc, phase_other);
   const int16_t phi_tolerance = f32_to_i16(phi_tolerance_f * convert_factor_i16);
   unsigned first_module_pair = n_module_pairs - 1;
-  track_seeding<count, log_seeds>(
+  track_seeding<count, seed_mode>(
     wc, hits, ss, sh, cn, o, sh.mp[first_module_pair], sh.mp[first_module_pair - 1], sh.mp[first_module_pair - 2],
     first_module_pair - 1, max_scatter, phi_tolerance, initial_seeding_h0_candidates);
   unsigned last_ttf = 0;
@@ -788,7 +899,7 @@ __device__ __forceinline__ void search_event(
       max_skipped_modules);
     __syncthreads();
     phase_mark<count>(wc, phase_forwarding);
-    track_seeding<count, log_seeds>(
+    track_seeding<count, seed_mode>(
       wc, hits, ss, sh, cn, o, sh.mp[first_module_pair], sh.mp[first_module_pair - 1], sh.mp[first_module_pair - 2],
       first_module_pair - 1, max_scatter, phi_tolerance, seeding_h0_candidates);
     --first_module_pair;
@@ -826,7 +937,7 @@ __device__ __forceinline__ void search_event(
 // atomic per wave). With event_clock, each workgroup stores at event_clock[10 * event_number]
 // its event's start and end time (wall clock ticks) and, in the counting instantiation, the
 // ticks spent in each Phase.
-template<bool count, bool log_seeds>
+template<bool count, int seed_mode>
 __global__ void k_search_by_triplet(
   const uint32_t* event_list,
   const uint32_t* number_of_events,
@@ -851,7 +962,9 @@ __global__ void k_search_by_triplet(
   unsigned long long* event_clock,
   unsigned n_list,
   uint32_t* work_counter,
-  SeedLog seed_log)
+  SeedLog seed_log,
+  const float* scorer_params,
+  float scorer_threshold)
 {
   extern __shared__ __align__(16) char dyn_all[];
   __shared__ SearchShared sh;
@@ -867,6 +980,12 @@ __global__ void k_search_by_triplet(
   uint32_t* ring = reinterpret_cast<uint32_t*>(dyn_all + ((13 * blockDim.x + 15) & ~15u));
   ss.unused = reinterpret_cast<uint16_t*>(ring + max_tracks_to_follow);
   char* dyn = reinterpret_cast<char*>(ss.unused + unused_list_cap);
+  ScorerLds* scorer = nullptr;
+  if constexpr (seed_mode == seed_mode_scored) {
+    scorer = reinterpret_cast<ScorerLds*>(dyn + (((size_t) lds_hits * search_bytes_per_hit + 15) & ~size_t(15)));
+    load_scorer(scorer_params, *scorer);
+    __syncthreads();
+  }
   // One event per workgroup (work_counter == NULL), or a resident grid whose workgroups take
   // event slots from the counter until they run out (the counter starts at 0).
   for (unsigned round = 0;; ++round) {
@@ -904,6 +1023,8 @@ __global__ void k_search_by_triplet(
     o.h1_indices = rel_indices_all + hit_offset;
     o.event_number = event_number;
     o.seed_log = seed_log;
+    o.scorer = scorer;
+    o.scorer_threshold = scorer_threshold;
     // Slots of this event; the reference lets a busy event run into its neighbour's slots, here
     // nothing is written past the container and the overflow is reported.
     o.track_capacity = min(next_off > toff ? next_off - toff : 0u, container - min(toff, container));
@@ -936,7 +1057,7 @@ __global__ void k_search_by_triplet(
       }
       __syncthreads();
       const LdsHits hits {xs, ys, zs, ph, flags};
-      search_event<count, log_seeds>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
+      search_event<count, seed_mode>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
       __syncthreads();
       for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) gused[i] = flags[i] & flag_used;
     }
@@ -950,7 +1071,7 @@ __global__ void k_search_by_triplet(
       cn.status |= status_lds_fallback;
       __syncthreads();
       const MixedHits hits {gxyz, ph, flags};
-      search_event<count, log_seeds>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
+      search_event<count, seed_mode>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
       __syncthreads();
       for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) gused[i] = flags[i] & flag_used;
     }
@@ -960,7 +1081,7 @@ __global__ void k_search_by_triplet(
       __threadfence_block();
       __syncthreads();
       const GlobalHits hits {gxyz, gph, gused, gids};
-      search_event<count, log_seeds>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
+      search_event<count, seed_mode>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
       __syncthreads();
     }
     // The ring of tracks to follow as far as it was written (dev_tracks_to_follow).
@@ -1169,18 +1290,28 @@ hipError_t launch_search_by_triplet(const SearchLaunch& a, hipStream_t s)
         hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
       return hipErrorInvalidDevice;
   }
-  const size_t fixed = search_dynamic_lds(block, 0) + 1024;
+  // Scored mode: the seeds of a sub-chunk must fit wave 0 and the hidden layer the seeding
+  // scratch, which holds for 256 lanes; its weights take LDS from the hits (same residency).
+  const bool scored = a.scorer_params != nullptr;
+  if (scored && block != 256) return hipErrorInvalidConfiguration;
+  const size_t extra = scored ? scorer_lds_bytes + 16 : 0;
+  const size_t fixed = search_dynamic_lds(block, 0) + extra + 1024;
   if ((size_t) max_lds <= fixed) return hipErrorInvalidConfiguration;
   const unsigned max_hits = static_cast<unsigned>(((size_t) max_lds - fixed) / search_bytes_per_hit);
-  const unsigned lds_hits = min(a.lds_hits ? a.lds_hits : search_default_lds_hits, max_hits);
-  const size_t dyn = search_dynamic_lds(block, lds_hits);
+  const unsigned default_hits =
+    scored ? search_default_lds_hits - static_cast<unsigned>((extra + search_bytes_per_hit - 1) / search_bytes_per_hit)
+           : search_default_lds_hits;
+  const unsigned lds_hits = min(a.lds_hits ? a.lds_hits : default_hits, max_hits);
+  const size_t dyn = search_dynamic_lds(block, lds_hits) + extra;
   // Persistent mode: as many workgroups as stay resident (they take events from the counter).
   unsigned grid = a.n_list;
   if (a.work_counter) {
     int dev = 0, cus = 0, per_cu = 0;
-    const void* kernel = a.work_stats        ? reinterpret_cast<const void*>(&k_search_by_triplet<true, false>)
-                         : a.seed_log.counter ? reinterpret_cast<const void*>(&k_search_by_triplet<false, true>)
-                                              : reinterpret_cast<const void*>(&k_search_by_triplet<false, false>);
+    const void* kernel = a.work_stats      ? reinterpret_cast<const void*>(&k_search_by_triplet<true, seed_mode_plain>)
+                         : a.scorer_params ? reinterpret_cast<const void*>(&k_search_by_triplet<false, seed_mode_scored>)
+                         : a.seed_log.counter
+                           ? reinterpret_cast<const void*>(&k_search_by_triplet<false, seed_mode_log>)
+                           : reinterpret_cast<const void*>(&k_search_by_triplet<false, seed_mode_plain>);
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, static_cast<int>(block), dyn) == hipSuccess &&
@@ -1195,10 +1326,11 @@ hipError_t launch_search_by_triplet(const SearchLaunch& a, hipStream_t s)
     reinterpret_cast<TrackletHits*>(a.tracklets), a.tracks_to_follow,                                               \
     reinterpret_cast<TrackletHits*>(a.three_hit_tracks), a.hit_used, a.atomics_velo, a.rel_indices,                \
     a.number_of_velo_tracks, a.status, a.phi_tolerance, a.max_scatter, a.max_skipped_modules, lds_hits,             \
-    a.work_stats, a.event_clock, a.n_list, a.work_counter, a.seed_log)
-  if (a.work_stats) VELO_LAUNCH_SEARCH(true, false);
-  else if (a.seed_log.counter) VELO_LAUNCH_SEARCH(false, true);
-  else VELO_LAUNCH_SEARCH(false, false);
+    a.work_stats, a.event_clock, a.n_list, a.work_counter, a.seed_log, a.scorer_params, a.scorer_threshold)
+  if (a.work_stats) VELO_LAUNCH_SEARCH(true, seed_mode_plain);
+  else if (a.scorer_params) VELO_LAUNCH_SEARCH(false, seed_mode_scored);
+  else if (a.seed_log.counter) VELO_LAUNCH_SEARCH(false, seed_mode_log);
+  else VELO_LAUNCH_SEARCH(false, seed_mode_plain);
 #undef VELO_LAUNCH_SEARCH
   return hipGetLastError();
 }

@@ -150,6 +150,7 @@ SIGNATURES = {
     "velo_sequence_stage_ms": (I, [P, ctypes.POINTER(F), ctypes.POINTER(U)]),
     "velo_sequence_capture": (I, [P, ctypes.POINTER(SequenceInput), P]),
     "velo_sequence_replay": (I, [P, P]),
+    "velo_sequence_set_scorer": (I, [P, P, F]),
     "velo_mlp_scratch_floats": (ctypes.c_size_t, [U]),
     "velo_mlp_train_step": (I, [P, P, U, P, P, P, P, F, U, P, P, P]),
     "velo_mlp_score": (I, [P, U, P, P, P]),
@@ -338,6 +339,17 @@ class VeloSequence:
         rec = self.fetch("seed_log_records", np.uint32, 5 * n).reshape(n, 5)
         feat = self.fetch("seed_log_features", np.float32, 10 * n).reshape(n, 10)
         return rec, feat, seen
+
+    def set_scorer(self, params, threshold: float = 0.0):
+        """NN-pruned seeding with these scorer parameters (None: plain search)."""
+        if params is None:
+            check(lib().velo_sequence_set_scorer(self.handle, None, 0.0), "set_scorer")
+            return
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        check(lib().velo_sequence_set_scorer(self.handle, p.ctypes.data, float(threshold)), "set_scorer")
+
+    def seed_logits(self, n: int) -> np.ndarray:
+        return self.fetch("seed_log_logits", np.float32, n)
 
     def replay(self, stream):
         """Launches the captured run on `stream`."""

@@ -336,6 +336,12 @@ int velo_sequence_stage_ms(velo_sequence* s, float* total_ms, unsigned* runs);
  * the run is read on the device, and buffers are cleared by kernels, not memset nodes. */
 int velo_sequence_capture(velo_sequence* s, const velo_sequence_input* in, hipStream_t stream);
 int velo_sequence_replay(velo_sequence* s, hipStream_t stream);
+/* NN-pruned seeding: with params (VELO_MLP_PARAMS host floats, copied) search_by_triplet scores
+ * every seed with the learned scorer and drops those whose logit is below `threshold`, except
+ * that a group of seeds (one seeding pass over up to 64 h1 hits) that would lose all its seeds
+ * keeps them; NULL restores the plain search. Needs "search_block" 256. With "seed_log" on, the
+ * log also holds each seed's logit ("seed_log_logits"). */
+int velo_sequence_set_scorer(velo_sequence* s, const float* params, float threshold);
 
 /* ---- Learned seed scorer (no reference counterpart): a 10 -> 32 -> 32 -> 1 MLP with ReLU on
  * BF16 MFMA with FP32 accumulation, over the 10 features of a seed (velo_seed_features).

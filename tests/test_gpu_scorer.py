@@ -84,3 +84,66 @@ def test_seed_log_matches_oracle(geometry, beamline):
     assert not errs, "\n".join(errs[:20])
     seq.close()
     constants.close()
+
+
+def _scored_run(geometry, beamline, ev, params, threshold, log_cap=0):
+    from allen_amd import velo
+    constants = velo.VeloConstants(geometry, beamline)
+    seq = velo.VeloSequence(constants, ev.n_events, ev.raw.size)
+    if log_cap:
+        seq.set_tuning("seed_log", log_cap)
+    seq.set_scorer(params, threshold)
+    seq.run(velo.DeviceEvents(ev))
+    return seq, constants
+
+
+def test_scored_search_extreme_thresholds_equal_plain(geometry, beamline):
+    """Threshold -inf keeps every seed; +inf would drop every group of seeds, which the plain
+    fallback keeps: both reproduce the plain search (and the oracle) exactly."""
+    import oracle
+    from allen_amd import scorer, synth
+    from tests import parity
+    ev = synth.generate(n_events=40, pileup=1, nu=7.6, seed=71)
+    ref = oracle.run(ev, geometry, beamline)
+    p = scorer.init_params(9)
+    for thr in (-1e30, 1e30):
+        seq, constants = _scored_run(geometry, beamline, ev, p, thr)
+        errs = parity.compare(parity.gpu_buffers(seq, ref), ref, True)
+        assert not errs, f"threshold {thr}:\n" + "\n".join(errs[:20])
+        seq.close()
+        constants.close()
+
+
+def test_scored_search_logits_match_score_kernel(geometry, beamline):
+    """The logits computed inside search_by_triplet equal k_mlp_score's on the same features."""
+    import torch
+    from allen_amd import scorer, synth
+    ev = synth.generate(n_events=40, pileup=1, nu=7.6, seed=72)
+    p = scorer.init_params(11)
+    seq, constants = _scored_run(geometry, beamline, ev, p, -1e30, log_cap=40000)
+    rec, feat, seen = seq.seed_log()
+    assert 0 < seen == rec.shape[0]
+    logits = seq.seed_logits(seen)
+    want = scorer.MLP(p).score(torch.from_numpy(feat).cuda()).cpu().numpy()
+    np.testing.assert_allclose(logits, want, rtol=1e-5, atol=1e-5)
+    seq.close()
+    constants.close()
+
+
+def test_scored_search_prunes(geometry, beamline):
+    """A threshold at the median logit drops seeds: fewer tracks to follow, no overflow."""
+    from allen_amd import scorer, synth, velo
+    ev = synth.generate(n_events=40, pileup=1, nu=7.6, seed=73)
+    p = scorer.init_params(12)
+    seq, constants = _scored_run(geometry, beamline, ev, p, -1e30, log_cap=40000)
+    _, _, seen = seq.seed_log()
+    med = float(np.median(seq.seed_logits(seen)))
+    plain_ttf = int(seq.fetch("atomics_velo", np.uint32, 3 * ev.n_events)[1::3].sum())
+    seq.close()
+    constants.close()
+    seq, constants = _scored_run(geometry, beamline, ev, p, med)
+    pruned_ttf = int(seq.fetch("atomics_velo", np.uint32, 3 * ev.n_events)[1::3].sum())
+    assert seq.status() & velo.STATUS_OVERFLOW_MASK == 0
+    assert pruned_ttf < plain_ttf
+    seq.close()
+    constants.close()
