@@ -1,7 +1,7 @@
 // The learned seed scorer: a 10 -> 32 -> 32 -> 1 MLP with ReLU on BF16 MFMA (FP32 accumulation),
 // trained on the seeds the plain search produces (no reference counterpart).
 //
-//   k_mlp_train        forward, binary cross-entropy and backward of 128 seeds per workgroup
+//   k_mlp_train        forward, binary cross-entropy and backward of 64 seeds per workgroup
 //                      (32 per wave), gradients summed per workgroup into partial[block][1441]
 //   k_mlp_reduce_adam  sums the partial gradients in block order (deterministic) and applies
 //                      one Adam step to the FP32 master parameters
@@ -33,10 +33,10 @@ struct Weights {
   float b3;
 };
 
-// Per wave: the activations of its 32 rows.
+// Per wave: the activations of its 32 rows ([Z1 > 0] is [a1b != 0]: rounding a positive FP32 to
+// bf16 never gives zero).
 struct WaveTiles {
   uint16_t xb[32 * 32];  // bf16(x), columns >= 10 zero
-  float z1[32 * 32];
   uint16_t a1b[32 * 32];
   float z2[32 * 32];
   uint16_t dz2b[32 * 32];
@@ -59,7 +59,7 @@ __device__ void load_weights(const float* p, Weights& w)
   if (threadIdx.x == 0) w.b3 = p[param_b3];
 }
 
-// Forward of the wave's rows (row0 = first row, n rows valid): fills xb, z1, a1b, z2 and
+// Forward of the wave's rows (row0 = first row, n rows valid): fills xb, a1b, z2 and
 // returns this lane's logit for row l % 32 (both half-waves hold it).
 __device__ float forward(const float* x, unsigned row0, unsigned n, const Weights& w, WaveTiles& t)
 {
@@ -74,7 +74,6 @@ __device__ float forward(const float* x, unsigned row0, unsigned n, const Weight
   for (unsigned v = 0; v < 16; ++v) {
     const unsigned r = d_row(v), c = d_col();
     const float z = acc[v] + w.b1[c];
-    t.z1[r * 32 + c] = z;
     t.a1b[r * 32 + c] = to_bf16(z > 0.f ? z : 0.f);
   }
   wave_sync();
@@ -165,7 +164,7 @@ __global__ void __launch_bounds__(rows_per_block) k_mlp_train(
 #pragma unroll
   for (unsigned v = 0; v < 16; ++v) {
     const unsigned rr = d_row(v), c = d_col();
-    const float d = t.z1[rr * 32 + c] > 0.f ? acc[v] : 0.f;
+    const float d = t.a1b[rr * 32 + c] != 0 ? acc[v] : 0.f;
     dz1[rr * 32 + c] = d;
     t.dz1b[rr * 32 + c] = to_bf16(d);
   }
