@@ -251,7 +251,7 @@ namespace mlp {
 constexpr unsigned n_inputs = 10, n_hidden = 32;
 constexpr unsigned param_w1 = 0, param_b1 = 320, param_w2 = 352, param_b2 = 1376, param_w3 = 1408, param_b3 = 1440;
 constexpr unsigned n_params = 1441;
-constexpr unsigned rows_per_block = 64; // two waves: the kernels' LDS stays within 64 KiB
+constexpr unsigned rows_per_block = 64; // seeds per workgroup of the training and scoring kernels (two waves)
 inline unsigned mlp_blocks(unsigned n) { return (n + rows_per_block - 1) / rows_per_block; }
 } // namespace mlp
 hipError_t launch_mlp_train(

@@ -23,7 +23,9 @@
 namespace velo {
 namespace mlp {
 
-constexpr unsigned rows_per_wave = 32, waves = rows_per_block / rows_per_wave;
+// A wave owns 32 rows (the M of one 32 x 32 product), so a workgroup of rows_per_block rows has
+// `waves` waves of 64 lanes.
+constexpr unsigned rows_per_wave = 32, waves = rows_per_block / rows_per_wave, block_threads = waves * wave_size;
 
 // Shared per workgroup: the bf16 weights (W1 padded to 32 input rows), biases and w3.
 struct Weights {
@@ -100,7 +102,7 @@ __device__ float forward(const float* x, unsigned row0, unsigned n, const Weight
 
 using namespace mlp;
 
-__global__ void __launch_bounds__(rows_per_block) k_mlp_train(
+__global__ void __launch_bounds__(block_threads) k_mlp_train(
   const float* features,
   const uint8_t* labels,
   unsigned n,
@@ -240,7 +242,7 @@ __global__ void k_mlp_reduce_adam(
   params[q] -= lr * mh / (sqrtf(vh) + eps);
 }
 
-__global__ void __launch_bounds__(rows_per_block) k_mlp_score(
+__global__ void __launch_bounds__(block_threads) k_mlp_score(
   const float* features,
   unsigned n,
   const float* params,
@@ -269,7 +271,7 @@ hipError_t launch_mlp_train(
 {
   if (!n) return hipSuccess;
   hipLaunchKernelGGL(
-    k_mlp_train, dim3(mlp_blocks(n)), dim3(rows_per_block), 0, s, features, labels, n, inv_norm, params, partial,
+    k_mlp_train, dim3(mlp_blocks(n)), dim3(block_threads), 0, s, features, labels, n, inv_norm, params, partial,
     loss_partial);
   return hipGetLastError();
 }
@@ -296,7 +298,7 @@ hipError_t launch_mlp_reduce_adam(
 hipError_t launch_mlp_score(const float* features, unsigned n, const float* params, float* logits, hipStream_t s)
 {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_mlp_score, dim3(mlp_blocks(n)), dim3(rows_per_block), 0, s, features, n, params, logits);
+  hipLaunchKernelGGL(k_mlp_score, dim3(mlp_blocks(n)), dim3(block_threads), 0, s, features, n, params, logits);
   return hipGetLastError();
 }
 

@@ -335,30 +335,49 @@ __device__ void load_scorer(const float* p, ScorerLds& w)
   if (threadIdx.x == 0) w.b3 = p[param_b3];
 }
 
-// Logit of the seed of lane l (features fp[0..9], fp[10..15] zero) for all 64 lanes of a wave,
-// on MFMA: the same arithmetic as k_mlp_score. Seeds 32 rb + r form row block rb; each lane
-// gathers its operand features with shuffles, the hidden layer goes through `tile` (32 x 32
-// bf16), the logits through `tmp` (64 floats). The whole wave must be active.
-__device__ __forceinline__ float scorer_logit(const float* fp, const ScorerLds& w, uint16_t* tile, float* tmp)
+// Logit of the seed of lane l (features fp[0..9]) for the lanes set in `live`, on MFMA: the
+// products of k_mlp_score with the output layer summed in another order. Seeds 32 rb + r form
+// row block rb, computed only when one of its seeds is live. The row block's features go through
+// `tile` as bf16 rows of 16 (the first 1 KiB), then the hidden layer overwrites them (32 x 32
+// bf16); the logits go through `tmp` (64 floats). The whole wave must be active; `live` is
+// wave-uniform (a ballot).
+__device__ __forceinline__ float
+scorer_logit(const float* fp, uint64_t live, const ScorerLds& w, uint16_t* tile, float* tmp)
 {
   using namespace mlp;
   const unsigned l = lane_id(), h = l / 32, c = l % 32;
 #pragma unroll
   for (unsigned rb = 0; rb < 2; ++rb) {
-    const unsigned src = 32 * rb + c;
+    if (!static_cast<uint32_t>(live >> (32 * rb))) continue;
+    if (h == rb) {
+      // This lane's seed is row c of the block: features 0..9, then zeros.
+      uint32_t row[8];
+#pragma unroll
+      for (unsigned k = 0; k < 8; ++k) {
+        const uint32_t lo = 2 * k < n_inputs ? to_bf16(fp[2 * k]) : 0u;
+        const uint32_t hi = 2 * k + 1 < n_inputs ? to_bf16(fp[2 * k + 1]) : 0u;
+        row[k] = lo | hi << 16;
+      }
+      uint4* dst = reinterpret_cast<uint4*>(tile + 16 * c);
+      dst[0] = make_uint4(row[0], row[1], row[2], row[3]);
+      dst[1] = make_uint4(row[4], row[5], row[6], row[7]);
+    }
+    wave_sync();
     f32x16 acc = zero16();
 #pragma unroll
     for (unsigned k0 = 0; k0 < 16; k0 += 8) {
+      const uint2 av = *reinterpret_cast<const uint2*>(tile + 16 * c + k0 + 4 * h);
       bf16x4 a, b;
+      a[0] = static_cast<short>(av.x & 0xFFFFu);
+      a[1] = static_cast<short>(av.x >> 16);
+      a[2] = static_cast<short>(av.y & 0xFFFFu);
+      a[3] = static_cast<short>(av.y >> 16);
 #pragma unroll
-      for (unsigned i = 0; i < 4; ++i) {
-        const float lo = __shfl(fp[k0 + i], src, wave_size);
-        const float hi = __shfl(fp[k0 + 4 + i], src, wave_size);
-        a[i] = static_cast<short>(to_bf16(h ? hi : lo));
-        b[i] = static_cast<short>(w.w1[(k0 + 4 * h + i) * 32 + c]);
-      }
+      for (unsigned i = 0; i < 4; ++i) b[i] = static_cast<short>(w.w1[(k0 + 4 * h + i) * 32 + c]);
       acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
     }
+    // The feature rows were read into registers before the products: the hidden layer may
+    // overwrite them (a wave's LDS accesses complete in order).
 #pragma unroll
     for (unsigned v = 0; v < 16; ++v) {
       const float z = acc[v] + w.b1[c];
@@ -372,15 +391,22 @@ __device__ __forceinline__ float scorer_logit(const float* fp, const ScorerLds& 
       const float z = acc[v] + w.b2[c];
       q[v] = (z > 0.f ? z : 0.f) * w.w3[c];
     }
+    // Sum over the 32 columns (lanes c of each half-wave) by recursive halving: at the step of
+    // lane bit m the lane keeps the half of its values selected by that bit and adds the
+    // partner's copy of it. After the steps 16, 8, 4, 2 a lane holds the value of v = c >> 1
+    // summed over 16 columns; the step 1 adds the last pair.
 #pragma unroll
-    for (unsigned d = 1; d < 32; d <<= 1) {
+    for (unsigned n = 8, m = 16; n >= 1; n >>= 1, m >>= 1) {
+      const bool upper = c & m;
 #pragma unroll
-      for (unsigned v = 0; v < 16; ++v) q[v] += __shfl_xor(q[v], d, wave_size);
+      for (unsigned j = 0; j < n; ++j) {
+        const float send = upper ? q[j] : q[j + n];
+        const float keep = upper ? q[j + n] : q[j];
+        q[j] = keep + __shfl_xor(send, m, wave_size);
+      }
     }
-    if (c == 0) {
-#pragma unroll
-      for (unsigned v = 0; v < 16; ++v) tmp[32 * rb + d_row(v)] = q[v] + w.b3;
-    }
+    const float total = q[0] + __shfl_xor(q[0], 1, wave_size);
+    if (!(c & 1)) tmp[32 * rb + d_row(c >> 1)] = total + w.b3;
     wave_sync();
   }
   return tmp[l];
@@ -672,12 +698,15 @@ __device__ __forceinline__ void track_seeding(
         // reaches the threshold; a sub-chunk whose seeds would all go keeps them (plain search
         // as fallback). The scratch of steps 1-2 is dead here and holds the hidden layer.
         if (threadIdx.x < wave_size) {
-          float fp[16];
+          float fp[n_seed_features];
 #pragma unroll
-          for (unsigned k = 0; k < 16; ++k) fp[k] = 0.f;
+          for (unsigned k = 0; k < n_seed_features; ++k) fp[k] = 0.f;
           if (found) seed_features(hits, best_h0, h1_index, best_h2, module_pair, fp);
-          const float logit =
-            scorer_logit(fp, *o.scorer, reinterpret_cast<uint16_t*>(ss.pair_scatter), reinterpret_cast<float*>(ss.n_h0));
+          const uint64_t live = __ballot(found);
+          const float logit = live ? scorer_logit(
+                                       fp, live, *o.scorer, reinterpret_cast<uint16_t*>(ss.pair_scatter),
+                                       reinterpret_cast<float*>(ss.n_h0))
+                                   : 0.f;
           const bool keep = found && logit >= o.scorer_threshold;
           if (found && o.seed_log.counter) {
             const unsigned slot = atomicAdd(o.seed_log.counter, 1u);

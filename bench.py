@@ -52,6 +52,31 @@ def measured_traffic(workload: str, stage: str):
     return None
 
 
+def physics_of(seq, ev) -> dict:
+    """Efficiency, ghost and clone rates of the last run's tracks against the generator's truth (01_velo)."""
+    from allen_amd import checker
+    n = ev.n_events
+    off_all = seq.fetch("offsets_all_velo_tracks", np.uint32, n + 1)
+    n_tracks = int(off_all[-1])
+    hits_n = seq.fetch("offsets_velo_track_hit_number", np.uint32, n_tracks + 1)
+    track_hits = seq.fetch("velo_track_hits", np.uint8, 4 * int(hits_n[-1]))
+    return checker.check_buffers(ev, off_all, hits_n, track_hits).as_dict()
+
+
+def timed_steps(seq, dev, steps: int, stream) -> float:
+    """Milliseconds per step of `steps` eager runs, HIP events on the run's stream."""
+    import torch
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for _ in range(steps):
+        seq.run(dev)
+    end.record(stream)
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / steps
+
+
 def algorithmic_bytes(ev, counts: dict) -> dict:
     """Algorithmic HBM bytes of each stage for one step (definitions in DESIGN.md)."""
     n = ev.n_events
@@ -73,13 +98,21 @@ def main():
     ap.add_argument("--workload", default="pileup_1000", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores (capped at 16)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="CPU work of the all-core baseline sample (the single-thread sample takes half)")
     ap.add_argument("--search-block", type=int, default=0)
     ap.add_argument("--search-lds-hits", type=int, default=0)
     ap.add_argument("--search-event-order", choices=["largest_first", "natural"], default="largest_first")
+    ap.add_argument("--persistent", action="store_true",
+                    help="search_by_triplet as a persistent grid whose workgroups take events from a work counter")
     ap.add_argument("--autotune", action="store_true",
                     help="sweep the search_by_triplet launch knobs on this workload first and use the fastest")
     ap.add_argument("--graph", action="store_true",
                     help="time replays of the whole sequence captured as one HIP graph instead of eager runs")
+    ap.add_argument("--scorer", default="",
+                    help="NN-pruned seeding with the scorer parameters in this .npz (allen_amd.scorer); the plain "
+                         "search is timed on the same events first and reported beside it")
+    ap.add_argument("--threshold", type=float, default=0.0, help="logit below which a seed is pruned")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times (the reference kernel for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -116,7 +149,19 @@ def main():
     if args.search_lds_hits:
         seq.set_tuning("search_lds_hits", args.search_lds_hits)
     seq.set_tuning("search_largest_first", int(args.search_event_order == "largest_first"))
+    seq.set_tuning("search_persistent", int(args.persistent))
     stream = torch.cuda.current_stream()
+
+    plain = None
+    if args.scorer:
+        from allen_amd import scorer
+        for _ in range(args.warmup):
+            seq.run(dev)
+        plain_ms = timed_steps(seq, dev, args.steps, stream)
+        plain = {"ms_per_step": round(plain_ms, 4), "events_per_s": round(ev.n_events / (plain_ms / 1e3), 1),
+                 "physics": physics_of(seq, ev)}
+        log(f"[bench] plain search: {plain}")
+        seq.set_scorer(scorer.load_params(args.scorer), args.threshold)
 
     tuned = None
     if args.autotune:
@@ -187,11 +232,9 @@ def main():
         "seeds": int(atomics[1::3].sum()),
     }
     log(f"[bench] {counts}")
-    # Physics of the timed workload's output against the generator's truth (01_velo category).
-    from allen_amd import checker
-    hits_n = seq.fetch("offsets_velo_track_hit_number", np.uint32, n_tracks + 1)
-    track_hits = seq.fetch("velo_track_hits", np.uint8, 4 * int(hits_n[-1]))
-    physics = checker.check_buffers(ev, off_all, hits_n, track_hits).as_dict()
+    # The counting run above uses the plain search; the physics is that of the timed path.
+    seq.run(dev)
+    physics = physics_of(seq, ev)
     log(f"[bench] physics: {physics}")
 
     # Timed region: K steps, the dominant stage bracketed by HIP events on the run's stream.
@@ -252,14 +295,21 @@ def main():
         import oracle
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         sample = synth.generate(**dict(wl, n_events=min(n, 1000)), keep_truth=False)
-        reps = 4
-        eps, secs = oracle.throughput(sample, geometry, beamline, threads, reps)
-        eps1, secs1 = oracle.throughput(synth.generate(**dict(wl, n_events=200), keep_truth=False), geometry,
-                                        beamline, 1, 1)
+
+        def timed(n_threads, seconds):
+            # One calibration pass, then enough repetitions of the sample for about `seconds` of CPU work.
+            _, s1 = oracle.throughput(sample, geometry, beamline, n_threads, 1)
+            reps = max(1, int(round(seconds / max(s1, 1e-3))))
+            eps, secs = oracle.throughput(sample, geometry, beamline, n_threads, reps)
+            return eps, secs, reps
+
+        eps, secs, reps = timed(threads, args.cpu_seconds)
+        eps1, secs1, reps1 = timed(1, args.cpu_seconds / 2)
+        log(f"[bench] cpu baseline: {eps:.1f} events/s on {threads} threads, {eps1:.1f} on one")
         cpu = {"value": round(eps, 1), "unit": "events/s", "cores": threads, "kind": "port",
-               "sample": f"{sample.n_events} events x {reps} repetitions of the same workload, one event per task "
-                         f"on {threads} std::threads ({secs:.1f} s); single thread: {eps1:.1f} events/s on 200 "
-                         f"events ({secs1:.1f} s)",
+               "sample": f"{sample.n_events} events of the same workload x {reps} repetitions, one event per task "
+                         f"on {threads} std::threads ({secs:.1f} s); single thread: {sample.n_events} events x "
+                         f"{reps1} repetitions ({secs1:.1f} s) -> {eps1:.1f} events/s",
                "single_thread_events_per_s": round(eps1, 1)}
 
     out = {
@@ -275,13 +325,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded generator, real VELO geometry, v4 super-pixel banks)",
-        "config": {"workload": args.workload, "events_per_step": n, "path": "plain",
+        "config": {"workload": args.workload, "events_per_step": n, "path": "nn_pruned" if args.scorer else "plain",
+                   **({"scorer": os.path.basename(args.scorer), "threshold": args.threshold} if args.scorer else {}),
                    "launch": "hip_graph" if args.graph else "eager",
+                   "search_grid": "persistent" if args.persistent else "one_workgroup_per_event",
                    **({"search_knobs": tuned} if tuned else {}),
                    "raw_bytes_per_step": int(ev.raw.size), **{k: v for k, v in wl.items() if k != "n_events"}},
         "status": status,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        **({"plain": plain, "speedup_vs_plain": round(plain["ms_per_step"] / ms_per_step, 4)} if plain else {}),
         **extra,
     }
     print(json.dumps(out), flush=True)
