@@ -448,7 +448,7 @@ struct velo_sequence {
   size_t sp_cap = 0, cluster_cap = 0, track_cap = 0, hits_cap = 0;
   std::map<std::string, std::pair<void*, size_t>> bufs;
   unsigned search_block = 256;
-  unsigned search_lds_hits = velo::search_default_lds_hits;
+  unsigned search_lds_hits = 0; // 0: the launcher's default for the kernel variant
   bool search_largest_first = true;
   bool search_persistent = false;
   bool clustering_resident_grid = false;
@@ -600,7 +600,7 @@ int velo_sequence_set_tuning(velo_sequence* s, const char* knob, unsigned value)
   }
   if (k == "search_lds_hits") {
     if (value > 8192) return VELO_ERR_INVALID_ARGUMENT;
-    s->search_lds_hits = value ? value : velo::search_default_lds_hits;
+    s->search_lds_hits = value;
     return VELO_OK;
   }
   if (k == "kalman_fp64_fallback") {

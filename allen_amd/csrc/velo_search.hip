@@ -312,9 +312,11 @@ __device__ __forceinline__ void find_forward_candidate(
 
 // The learned scorer's weights in LDS (scored mode): W1 (K padded to 16) and W2 as bf16, the
 // biases and w3 (bf16-rounded) as FP32.
+// The scorer's weights in LDS, transposed for the products below: w1t[j][k] = bf16(W1[k][j])
+// (k < 16, zero from 10), w2t[j2][j1] = bf16(W2[j1][j2]); biases and bf16(w3) in FP32.
 struct ScorerLds {
-  uint16_t w1[16 * 32];
-  uint16_t w2[32 * 32];
+  uint16_t w1t[32 * 16];
+  uint16_t w2t[32 * 32];
   float b1[32], b2[32], w3[32];
   float b3;
 };
@@ -324,8 +326,12 @@ __device__ void load_scorer(const float* p, ScorerLds& w)
 {
   using namespace mlp;
   for (unsigned t = threadIdx.x; t < 32 * 32; t += blockDim.x) {
-    if (t < 16 * 32) w.w1[t] = t / 32 < n_inputs ? to_bf16(p[param_w1 + t]) : 0;
-    w.w2[t] = to_bf16(p[param_w2 + t]);
+    const unsigned j = t / 32, k = t % 32;
+    w.w2t[t] = to_bf16(p[param_w2 + k * 32 + j]);
+    if (t < 32 * 16) {
+      const unsigned jj = t / 16, kk = t % 16;
+      w.w1t[t] = kk < n_inputs ? to_bf16(p[param_w1 + kk * 32 + jj]) : 0;
+    }
   }
   for (unsigned t = threadIdx.x; t < 32; t += blockDim.x) {
     w.b1[t] = p[param_b1 + t];
@@ -335,81 +341,83 @@ __device__ void load_scorer(const float* p, ScorerLds& w)
   if (threadIdx.x == 0) w.b3 = p[param_b3];
 }
 
-// Logit of the seed of lane l (features fp[0..9]) for the lanes set in `live`, on MFMA: the
-// products of k_mlp_score with the output layer summed in another order. Seeds 32 rb + r form
-// row block rb, computed only when one of its seeds is live. The row block's features go through
-// `tile` as bf16 rows of 16 (the first 1 KiB), then the hidden layer overwrites them (32 x 32
-// bf16); the logits go through `tmp` (64 floats). The whole wave must be active; `live` is
-// wave-uniform (a ballot).
-__device__ __forceinline__ float
-scorer_logit(const float* fp, uint64_t live, const ScorerLds& w, uint16_t* tile, float* tmp)
+__device__ __forceinline__ mlp::bf16x4 bf16x4_from(uint2 v)
+{
+  mlp::bf16x4 r;
+  r[0] = static_cast<short>(v.x & 0xFFFFu);
+  r[1] = static_cast<short>(v.x >> 16);
+  r[2] = static_cast<short>(v.y & 0xFFFFu);
+  r[3] = static_cast<short>(v.y >> 16);
+  return r;
+}
+
+// Logit of the seed of lane l (features fp[0..9]) for the lanes set in `live` (a ballot, so
+// wave-uniform), with the products of k_mlp_score computed transposed: Z1^T = W1b^T xb^T and
+// Z2^T = W2b^T A1b^T on v_mfma_f32_32x32x8_bf16. Seeds 32 rb + c form row block rb (computed
+// only when one of its seeds is live), with seed c in column c. In that layout the hidden layer
+// leaves the first product exactly where the second takes its operand (element 4 s + i of
+// lane l is hidden unit 8 s + 4 (l / 32) + i of seed l % 32), and the output layer is a sum
+// over the lane's 16 hidden units plus one exchange with lane l ^ 32 (the other 16), so the
+// wave needs no LDS beyond the weights. The output sum runs in another order than
+// k_mlp_score's. The whole wave must be active.
+__device__ __forceinline__ float scorer_logit(const float* fp, uint64_t live, const ScorerLds& w)
 {
   using namespace mlp;
   const unsigned l = lane_id(), h = l / 32, c = l % 32;
+  // Features of the seed in lane l ^ 32: lanes l < 32 send features 4..7 (the operand of the
+  // upper half in row block 0), lanes l >= 32 send 0..3, 8 and 9 (that of the lower half in row
+  // block 1).
+  float other[6];
+#pragma unroll
+  for (unsigned i = 0; i < 4; ++i) other[i] = __shfl_xor(h ? fp[i] : fp[4 + i], 32, wave_size);
+  other[4] = __shfl_xor(fp[8], 32, wave_size);
+  other[5] = __shfl_xor(fp[9], 32, wave_size);
+  float logit = 0.f;
 #pragma unroll
   for (unsigned rb = 0; rb < 2; ++rb) {
     if (!static_cast<uint32_t>(live >> (32 * rb))) continue;
-    if (h == rb) {
-      // This lane's seed is row c of the block: features 0..9, then zeros.
-      uint32_t row[8];
+    const bool own = h == rb; // this lane's seed is column c of the row block
+    // B operand of the first product: features 4 h + i (k-step 0) and 8 + 4 h + i (k-step 1).
+    bf16x4 x0, x1;
 #pragma unroll
-      for (unsigned k = 0; k < 8; ++k) {
-        const uint32_t lo = 2 * k < n_inputs ? to_bf16(fp[2 * k]) : 0u;
-        const uint32_t hi = 2 * k + 1 < n_inputs ? to_bf16(fp[2 * k + 1]) : 0u;
-        row[k] = lo | hi << 16;
-      }
-      uint4* dst = reinterpret_cast<uint4*>(tile + 16 * c);
-      dst[0] = make_uint4(row[0], row[1], row[2], row[3]);
-      dst[1] = make_uint4(row[4], row[5], row[6], row[7]);
+    for (unsigned i = 0; i < 4; ++i) {
+      const float f0 = own ? fp[4 * h + i] : other[i];
+      x0[i] = static_cast<short>(to_bf16(f0));
+      const float f1 = h ? 0.f : (i < 2 ? (own ? fp[8 + i] : other[4 + i]) : 0.f);
+      x1[i] = static_cast<short>(to_bf16(f1));
     }
-    wave_sync();
     f32x16 acc = zero16();
+    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
+      bf16x4_from(*reinterpret_cast<const uint2*>(w.w1t + 16 * c + 4 * h)), x0, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
+      bf16x4_from(*reinterpret_cast<const uint2*>(w.w1t + 16 * c + 8 + 4 * h)), x1, acc, 0, 0, 0);
+    f32x16 acc2 = zero16();
 #pragma unroll
-    for (unsigned k0 = 0; k0 < 16; k0 += 8) {
-      const uint2 av = *reinterpret_cast<const uint2*>(tile + 16 * c + k0 + 4 * h);
-      bf16x4 a, b;
-      a[0] = static_cast<short>(av.x & 0xFFFFu);
-      a[1] = static_cast<short>(av.x >> 16);
-      a[2] = static_cast<short>(av.y & 0xFFFFu);
-      a[3] = static_cast<short>(av.y >> 16);
+    for (unsigned st = 0; st < 4; ++st) {
+      const float4 b1 = *reinterpret_cast<const float4*>(w.b1 + 8 * st + 4 * h);
+      const float z[4] = {acc[4 * st] + b1.x, acc[4 * st + 1] + b1.y, acc[4 * st + 2] + b1.z,
+                          acc[4 * st + 3] + b1.w};
+      bf16x4 a1;
 #pragma unroll
-      for (unsigned i = 0; i < 4; ++i) b[i] = static_cast<short>(w.w1[(k0 + 4 * h + i) * 32 + c]);
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+      for (unsigned i = 0; i < 4; ++i) a1[i] = static_cast<short>(to_bf16(z[i] > 0.f ? z[i] : 0.f));
+      acc2 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
+        bf16x4_from(*reinterpret_cast<const uint2*>(w.w2t + 32 * c + 8 * st + 4 * h)), a1, acc2, 0, 0, 0);
     }
-    // The feature rows were read into registers before the products: the hidden layer may
-    // overwrite them (a wave's LDS accesses complete in order).
+    float part = 0.f;
 #pragma unroll
-    for (unsigned v = 0; v < 16; ++v) {
-      const float z = acc[v] + w.b1[c];
-      tile[d_row(v) * 32 + c] = to_bf16(z > 0.f ? z : 0.f);
+    for (unsigned st = 0; st < 4; ++st) {
+      const float4 b2 = *reinterpret_cast<const float4*>(w.b2 + 8 * st + 4 * h);
+      const float4 w3 = *reinterpret_cast<const float4*>(w.w3 + 8 * st + 4 * h);
+      const float z[4] = {acc2[4 * st] + b2.x, acc2[4 * st + 1] + b2.y, acc2[4 * st + 2] + b2.z,
+                          acc2[4 * st + 3] + b2.w};
+      const float ww[4] = {w3.x, w3.y, w3.z, w3.w};
+#pragma unroll
+      for (unsigned i = 0; i < 4; ++i) part += (z[i] > 0.f ? z[i] : 0.f) * ww[i];
     }
-    wave_sync();
-    acc = mma32(tile, 32, 1, w.w2, 32, 1, 32, zero16());
-    float q[16];
-#pragma unroll
-    for (unsigned v = 0; v < 16; ++v) {
-      const float z = acc[v] + w.b2[c];
-      q[v] = (z > 0.f ? z : 0.f) * w.w3[c];
-    }
-    // Sum over the 32 columns (lanes c of each half-wave) by recursive halving: at the step of
-    // lane bit m the lane keeps the half of its values selected by that bit and adds the
-    // partner's copy of it. After the steps 16, 8, 4, 2 a lane holds the value of v = c >> 1
-    // summed over 16 columns; the step 1 adds the last pair.
-#pragma unroll
-    for (unsigned n = 8, m = 16; n >= 1; n >>= 1, m >>= 1) {
-      const bool upper = c & m;
-#pragma unroll
-      for (unsigned j = 0; j < n; ++j) {
-        const float send = upper ? q[j] : q[j + n];
-        const float keep = upper ? q[j + n] : q[j];
-        q[j] = keep + __shfl_xor(send, m, wave_size);
-      }
-    }
-    const float total = q[0] + __shfl_xor(q[0], 1, wave_size);
-    if (!(c & 1)) tmp[32 * rb + d_row(c >> 1)] = total + w.b3;
-    wave_sync();
+    const float sum = part + __shfl_xor(part, 32, wave_size);
+    if (own) logit = sum + w.b3;
   }
-  return tmp[l];
+  return logit;
 }
 
 // Scorer features of a seed (h0, h1, h2) whose h1 lies in module pair `module_pair` (no
@@ -696,17 +704,14 @@ __device__ __forceinline__ void track_seeding(
       if constexpr (seed_mode == seed_mode_scored) {
         // The seeds of a sub-chunk are all in wave 0 (C <= 64). A seed is kept when its logit
         // reaches the threshold; a sub-chunk whose seeds would all go keeps them (plain search
-        // as fallback). The scratch of steps 1-2 is dead here and holds the hidden layer.
+        // as fallback).
         if (threadIdx.x < wave_size) {
           float fp[n_seed_features];
 #pragma unroll
           for (unsigned k = 0; k < n_seed_features; ++k) fp[k] = 0.f;
           if (found) seed_features(hits, best_h0, h1_index, best_h2, module_pair, fp);
           const uint64_t live = __ballot(found);
-          const float logit = live ? scorer_logit(
-                                       fp, live, *o.scorer, reinterpret_cast<uint16_t*>(ss.pair_scatter),
-                                       reinterpret_cast<float*>(ss.n_h0))
-                                   : 0.f;
+          const float logit = live ? scorer_logit(fp, live, *o.scorer) : 0.f;
           const bool keep = found && logit >= o.scorer_threshold;
           if (found && o.seed_log.counter) {
             const unsigned slot = atomicAdd(o.seed_log.counter, 1u);
@@ -910,12 +915,14 @@ __device__ __forceinline__ void search_event(
   phase_mark<count>(wc, phase_other);
   const int16_t phi_tolerance = f32_to_i16(phi_tolerance_f * convert_factor_i16);
   unsigned first_module_pair = n_module_pairs - 1;
-  track_seeding<count, seed_mode>(
-    wc, hits, ss, sh, cn, o, sh.mp[first_module_pair], sh.mp[first_module_pair - 1], sh.mp[first_module_pair - 2],
-    first_module_pair - 1, max_scatter, phi_tolerance, initial_seeding_h0_candidates);
   unsigned last_ttf = 0;
-  --first_module_pair;
-  while (first_module_pair > 1) {
+  const auto seed = [&](unsigned h0_candidates) {
+    track_seeding<count, seed_mode>(
+      wc, hits, ss, sh, cn, o, sh.mp[first_module_pair], sh.mp[first_module_pair - 1], sh.mp[first_module_pair - 2],
+      first_module_pair - 1, max_scatter, phi_tolerance, h0_candidates);
+    --first_module_pair;
+  };
+  const auto forward = [&]() {
     const unsigned prev_ttf = last_ttf;
     last_ttf = cn.ttf;
     const unsigned diff_ttf = last_ttf - prev_ttf;
@@ -928,10 +935,23 @@ __device__ __forceinline__ void search_event(
       max_skipped_modules);
     __syncthreads();
     phase_mark<count>(wc, phase_forwarding);
-    track_seeding<count, seed_mode>(
-      wc, hits, ss, sh, cn, o, sh.mp[first_module_pair], sh.mp[first_module_pair - 1], sh.mp[first_module_pair - 2],
-      first_module_pair - 1, max_scatter, phi_tolerance, seeding_h0_candidates);
-    --first_module_pair;
+  };
+  if constexpr (seed_mode == seed_mode_scored) {
+    // One inlined copy of track_seeding (the largest body, here with the scorer) keeps the
+    // kernel's code small.
+    for (;;) {
+      seed(first_module_pair == n_module_pairs - 1 ? initial_seeding_h0_candidates : seeding_h0_candidates);
+      if (first_module_pair <= 1) break;
+      forward();
+    }
+  }
+  else {
+    // The first call peeled: both copies see a constant h0 window.
+    seed(initial_seeding_h0_candidates);
+    while (first_module_pair > 1) {
+      forward();
+      seed(seeding_h0_candidates);
+    }
   }
   __syncthreads();
   const unsigned prev_ttf = last_ttf;
