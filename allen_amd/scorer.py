@@ -5,10 +5,15 @@ The network is 10 -> 32 -> 32 -> 1 with ReLU. Its matrix products run on BF16 MF
 from __future__ import annotations
 
 import ctypes
+import json
+import os
 
 import numpy as np
 
 from . import velo
+
+# The parameters trained by main() with its defaults, and their evaluation (threshold included).
+TRAINED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "seed_scorer.npz")
 
 N_PARAMS = 1441
 N_FEATURES = 10
@@ -163,6 +168,13 @@ def roc(scores: np.ndarray, labels: np.ndarray) -> dict:
 
 def save(path: str, net: MLP, meta: dict) -> None:
     np.savez(path, params=net.host_params(), **{k: np.asarray(v) for k, v in meta.items()})
+
+
+def trained() -> tuple[np.ndarray, float]:
+    """The shipped parameters and their operating threshold (a seed is kept at logit >= it)."""
+    with open(os.path.splitext(TRAINED)[0] + ".json") as f:
+        meta = json.load(f)
+    return load_params(TRAINED), float(meta["threshold"])
 
 
 def load_params(path: str) -> np.ndarray:

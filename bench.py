@@ -110,9 +110,11 @@ def main():
     ap.add_argument("--graph", action="store_true",
                     help="time replays of the whole sequence captured as one HIP graph instead of eager runs")
     ap.add_argument("--scorer", default="",
-                    help="NN-pruned seeding with the scorer parameters in this .npz (allen_amd.scorer); the plain "
-                         "search is timed on the same events first and reported beside it")
-    ap.add_argument("--threshold", type=float, default=0.0, help="logit below which a seed is pruned")
+                    help="NN-pruned seeding with the scorer parameters in this .npz (allen_amd.scorer), or 'shipped' "
+                         "for the package's trained scorer; the plain search is timed on the same events first and "
+                         "reported beside it")
+    ap.add_argument("--threshold", type=float, default=None,
+                    help="logit below which a seed is pruned (default: the shipped scorer's operating threshold)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times (the reference kernel for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -161,7 +163,13 @@ def main():
         plain = {"ms_per_step": round(plain_ms, 4), "events_per_s": round(ev.n_events / (plain_ms / 1e3), 1),
                  "physics": physics_of(seq, ev)}
         log(f"[bench] plain search: {plain}")
-        seq.set_scorer(scorer.load_params(args.scorer), args.threshold)
+        if args.scorer == "shipped":
+            params, shipped_threshold = scorer.trained()
+        else:
+            params, shipped_threshold = scorer.load_params(args.scorer), 0.0
+        if args.threshold is None:
+            args.threshold = shipped_threshold
+        seq.set_scorer(params, args.threshold)
 
     tuned = None
     if args.autotune:
