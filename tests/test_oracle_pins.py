@@ -504,3 +504,43 @@ def test_mlp_restatement_gradient_by_finite_differences():
         assert abs(fd - g[q]) <= 1e-6 + 1e-5 * abs(fd), (q, fd, g[q])
     vals = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -2.5, 1 + 2 ** -9], np.float32)
     np.testing.assert_array_equal(mlp.bf16(vals), [1.0, 1.0, 1.0 + 2 ** -6, -2.5, 1.0])
+
+
+def test_seed_features_against_geometry(geometry, beamline):
+    """The scorer's ten seed features, restated from their meaning in float64 on the hits the
+    seed log names: h2's residual to the line through h0 and h1 and h1's residual to the line
+    through h0 and h2 (written with interpolation weights, predicted minus measured), the 16-bit
+    phi steps with wrap-around, the (h0, h1) slopes, the squared h2 residual and h1's module
+    pair (which must contain h1)."""
+    ev = synth.generate(n_events=30, pileup=1, nu=7.6, seed=62)
+    r = oracle.run(ev, geometry, beamline, collect_seeds=1)
+    rec = r["seed_log"].reshape(-1, 5).astype(np.int64)
+    got = r["seed_features"].reshape(-1, 10).astype(np.float64)
+    assert rec.shape[0] > 3000
+    n = ev.n_events
+    est = r["offsets_estimated_input_size"].astype(np.int64)
+    t = int(est[-1])
+    cont = np.asarray(r["sorted_velo_cluster_container"]).view(np.uint8)
+    xyz = cont[4 * t:16 * t].view("<f4").reshape(-1, 3).astype(np.float64)
+    phi = cont[16 * t:18 * t].view("<i2").astype(np.int64)
+    base = est[0:26 * n:26][rec[:, 0]]
+    h0, h1, h2 = (base + rec[:, k] for k in (2, 3, 4))
+    # h1 lies in the module pair the record names.
+    mp_lo = est[26 * rec[:, 0] + rec[:, 1]]
+    mp_hi = est[26 * rec[:, 0] + rec[:, 1] + 1]
+    assert np.all((mp_lo <= h1) & (h1 < mp_hi))
+    p0, p1, p2 = xyz[h0], xyz[h1], xyz[h2]
+    z0, z1, z2 = p0[:, 2], p1[:, 2], p2[:, 2]
+    w2 = ((z2 - z0) / (z1 - z0))[:, None]
+    r2 = (1 - w2) * p0[:, :2] + w2 * p1[:, :2] - p2[:, :2]
+    w1 = ((z1 - z0) / (z2 - z0))[:, None]
+    r1 = (1 - w1) * p0[:, :2] + w1 * p2[:, :2] - p1[:, :2]
+    wrap = lambda d: (d + 32768) % 65536 - 32768
+    slopes = (p1[:, :2] - p0[:, :2]) / (z1 - z0)[:, None]
+    want = np.column_stack([4 * r2, 4 * r1, wrap(phi[h1] - phi[h0]) / 512, wrap(phi[h2] - phi[h1]) / 512,
+                            4 * slopes, 12.5 * (r2 ** 2).sum(axis=1), rec[:, 1] / 32])
+    np.testing.assert_allclose(got[:, :4], want[:, :4], rtol=0, atol=2e-4)
+    np.testing.assert_array_equal(got[:, 4:6], want[:, 4:6])
+    np.testing.assert_allclose(got[:, 6:8], want[:, 6:8], rtol=1e-5, atol=1e-7)
+    np.testing.assert_allclose(got[:, 8], want[:, 8], rtol=1e-3, atol=2e-4)
+    np.testing.assert_array_equal(got[:, 9], want[:, 9])
