@@ -2,7 +2,10 @@
 
 One step = the whole VELO path (super-pixel decoding + masked clustering, phi sort,
 search_by_triplet, three-hit filter, consolidation, simplified Kalman fit) over one slice
-of 1000 synthetic events with the raw banks already resident in HBM.
+of 1000 synthetic events with the raw banks already resident in HBM. Slices are dealt to
+three HIP streams in turn, one sequence per stream, which is how Allen measures GPU
+throughput (one stream per host thread, scanned over the stream count); the one-stream
+figure is reported beside it.
 
 Prints ONE JSON line (rank 0). Progress goes to stderr.
 """
@@ -63,20 +66,6 @@ def physics_of(seq, ev) -> dict:
     return checker.check_buffers(ev, off_all, hits_n, track_hits).as_dict()
 
 
-def timed_steps(seq, dev, steps: int, stream) -> float:
-    """Milliseconds per step of `steps` eager runs, HIP events on the run's stream."""
-    import torch
-    start = torch.cuda.Event(enable_timing=True)
-    end = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    start.record(stream)
-    for _ in range(steps):
-        seq.run(dev)
-    end.record(stream)
-    torch.cuda.synchronize()
-    return start.elapsed_time(end) / steps
-
-
 def algorithmic_bytes(ev, counts: dict) -> dict:
     """Algorithmic HBM bytes of each stage for one step (definitions in DESIGN.md)."""
     n = ev.n_events
@@ -103,6 +92,10 @@ def main():
     ap.add_argument("--search-block", type=int, default=0)
     ap.add_argument("--search-lds-hits", type=int, default=0)
     ap.add_argument("--search-event-order", choices=["largest_first", "natural"], default="largest_first")
+    ap.add_argument("--streams", type=int, default=3,
+                    help="slices in flight: one sequence and one HIP stream per slice, as Allen runs one stream per "
+                         "host thread and measures GPU throughput over the stream count; steps are dealt to the "
+                         "streams in turn (3 is the best count on one MI355X, whose process has 4 hardware queues)")
     ap.add_argument("--persistent", action="store_true",
                     help="search_by_triplet as a persistent grid whose workgroups take events from a work counter")
     ap.add_argument("--autotune", action="store_true",
@@ -145,21 +138,64 @@ def main():
     geometry, beamline = synth.read_geometry(), synth.read_beamline()
     constants = velo.VeloConstants(geometry, beamline)
     dev = velo.DeviceEvents(ev)
-    seq = velo.VeloSequence(constants, ev.n_events, ev.raw.size)
+    knobs = {"search_largest_first": int(args.search_event_order == "largest_first"),
+             "search_persistent": int(args.persistent)}
     if args.search_block:
-        seq.set_tuning("search_block", args.search_block)
+        knobs["search_block"] = args.search_block
     if args.search_lds_hits:
-        seq.set_tuning("search_lds_hits", args.search_lds_hits)
-    seq.set_tuning("search_largest_first", int(args.search_event_order == "largest_first"))
-    seq.set_tuning("search_persistent", int(args.persistent))
+        knobs["search_lds_hits"] = args.search_lds_hits
+    # One sequence per stream (slices in flight), configured alike; the first is also the one the
+    # diagnostics below run on.
+    n_streams = max(1, args.streams)
+    seqs = [velo.VeloSequence(constants, ev.n_events, ev.raw.size) for _ in range(n_streams)]
+    seq = seqs[0]
+    for sq in seqs:
+        for k, v in knobs.items():
+            sq.set_tuning(k, v)
     stream = torch.cuda.current_stream()
+    streams = [stream] if n_streams == 1 and not args.graph else [torch.cuda.Stream() for _ in range(n_streams)]
+
+    def timed(steps: int, warmup: int, stage: str | None, used: int = n_streams) -> float:
+        """Milliseconds of `steps` steps dealt to the first `used` streams in turn (eager runs or
+        graph replays), HIP events around them on the current stream; `stage` is bracketed in the
+        first sequence."""
+        if args.graph:
+            for sq, st in zip(seqs[:used], streams[:used]):
+                sq.time_stage(None)
+                sq.capture(dev, st)
+
+        def launch(i):
+            sq, st = seqs[i % used], streams[i % used]
+            if args.graph:
+                sq.replay(st)
+            else:
+                sq.run(dev, stream=st)
+
+        for i in range(warmup):
+            launch(i)
+        torch.cuda.synchronize()
+        if stage and not args.graph:
+            seq.time_stage(stage)
+        main_stream = torch.cuda.current_stream()
+        start = torch.cuda.Event(enable_timing=True)
+        end = torch.cuda.Event(enable_timing=True)
+        start.record(main_stream)
+        for st in streams[:used]:
+            st.wait_event(start)
+        for i in range(steps):
+            launch(i)
+        for st in streams[:used]:
+            main_stream.wait_stream(st)
+        end.record(main_stream)
+        torch.cuda.synchronize()
+        return start.elapsed_time(end)
 
     plain = None
+    params = None
     if args.scorer:
         from allen_amd import scorer
-        for _ in range(args.warmup):
-            seq.run(dev)
-        plain_ms = timed_steps(seq, dev, args.steps, stream)
+        plain_ms = timed(args.steps, args.warmup, None) / args.steps
+        seq.run(dev)
         plain = {"ms_per_step": round(plain_ms, 4), "events_per_s": round(ev.n_events / (plain_ms / 1e3), 1),
                  "physics": physics_of(seq, ev)}
         log(f"[bench] plain search: {plain}")
@@ -169,7 +205,8 @@ def main():
             params, shipped_threshold = scorer.load_params(args.scorer), 0.0
         if args.threshold is None:
             args.threshold = shipped_threshold
-        seq.set_scorer(params, args.threshold)
+        for sq in seqs:
+            sq.set_scorer(params, args.threshold)
 
     tuned = None
     if args.autotune:
@@ -177,6 +214,9 @@ def main():
         t0 = time.time()
         res = autotune.tune(seq, dev)
         tuned = res.knobs
+        knobs.update(tuned)
+        for sq in seqs[1:]:
+            autotune.apply(sq, tuned)
         log(f"[bench] autotune ({time.time() - t0:.1f}s, {len(res.trials)} trials): {res.knobs} "
             f"search {res.search_ms:.4f} ms (default {res.default_search_ms:.4f} ms)")
     for _ in range(args.warmup):
@@ -245,35 +285,21 @@ def main():
     physics = physics_of(seq, ev)
     log(f"[bench] physics: {physics}")
 
-    # Timed region: K steps, the dominant stage bracketed by HIP events on the run's stream.
-    seq.time_stage(dominant)
-    torch.cuda.synchronize()
-    start = torch.cuda.Event(enable_timing=True)
-    end = torch.cuda.Event(enable_timing=True)
+    # The same steps on one stream, for reference (outside the timed region).
+    single_ms = timed(args.steps, args.warmup, None, 1) / args.steps if n_streams > 1 else None
+    # Timed region: K steps; without a graph the dominant stage is bracketed by HIP events on its
+    # stream in the first sequence.
+    total_ms = timed(args.steps, args.warmup, dominant)
     if args.graph:
         # The graph carries no stage events; the dominant kernel's launch time is the eager one.
-        seq.time_stage(None)
-        stream = torch.cuda.Stream()
-        seq.capture(dev, stream)
-        for _ in range(args.warmup):
-            seq.replay(stream)
-        stream.synchronize()
-        start.record(stream)
-        for _ in range(args.steps):
-            seq.replay(stream)
-        end.record(stream)
-        stream.synchronize()
-        total_ms = start.elapsed_time(end)
         kern_avg_ms = stage_ms[dominant]
     else:
-        start.record(stream)
-        for _ in range(args.steps):
-            seq.run(dev)
-        end.record(stream)
-        torch.cuda.synchronize()
-        total_ms = start.elapsed_time(end)
         kern_ms, kern_runs = seq.stage_ms()
         kern_avg_ms = kern_ms / kern_runs
+        seq.time_stage(None)
+    for sq in seqs[1:]:
+        if physics_of(sq, ev) != physics:
+            raise RuntimeError("a second stream's slice gave other tracks than the first")
     ms_per_step = total_ms / args.steps
     events_per_s = n * args.steps / (total_ms / 1e3)
     status = seq.status()
@@ -335,7 +361,7 @@ def main():
         "data": "synthetic (seeded generator, real VELO geometry, v4 super-pixel banks)",
         "config": {"workload": args.workload, "events_per_step": n, "path": "nn_pruned" if args.scorer else "plain",
                    **({"scorer": os.path.basename(args.scorer), "threshold": args.threshold} if args.scorer else {}),
-                   "launch": "hip_graph" if args.graph else "eager",
+                   "launch": "hip_graph" if args.graph else "eager", "streams": n_streams,
                    "search_grid": "persistent" if args.persistent else "one_workgroup_per_event",
                    **({"search_knobs": tuned} if tuned else {}),
                    "raw_bytes_per_step": int(ev.raw.size), **{k: v for k, v in wl.items() if k != "n_events"}},
@@ -343,9 +369,13 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         **({"plain": plain, "speedup_vs_plain": round(plain["ms_per_step"] / ms_per_step, 4)} if plain else {}),
+        **({"single_stream": {"ms_per_step": round(single_ms, 4), "events_per_s": round(n / (single_ms / 1e3), 1)}}
+           if single_ms else {}),
         **extra,
     }
     print(json.dumps(out), flush=True)
+    for sq in seqs[1:]:
+        sq.close()
     seq.close()
     constants.close()
 
