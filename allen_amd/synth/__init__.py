@@ -38,6 +38,7 @@ class SynthConfig(ctypes.Structure):
         ("cluster_size_p", ctypes.c_double * 3),
         ("bank_version", ctypes.c_int),
         ("shuffle_sp", ctypes.c_int),
+        ("retina", ctypes.c_int),
     ]
 
 
@@ -93,6 +94,7 @@ class Events:
     particle_offsets: np.ndarray | None = None
     hits: np.ndarray | None = None
     hit_offsets: np.ndarray | None = None
+    retina: bool = False     # banks are retina clusters (type 73) rather than super pixels
 
     @property
     def n_sp_words(self) -> int:
@@ -145,7 +147,7 @@ def generate(n_threads: int = 8, keep_truth: bool = True, **overrides) -> Events
         ev = Events(
             n_events=cfg.n_events, bank_version=cfg.bank_version,
             raw=get("raw", np.uint8), offsets=get("offsets", np.uint32),
-            sizes=get("sizes", np.uint32), types=get("types", np.uint32))
+            sizes=get("sizes", np.uint32), types=get("types", np.uint32), retina=bool(cfg.retina))
         if keep_truth:
             ev.particles = get("particles", PARTICLE_DTYPE)
             ev.particle_offsets = get("particle_offsets", np.uint32)

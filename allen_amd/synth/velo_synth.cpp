@@ -100,6 +100,17 @@ void generate_event(const velo_synth_config& c, const Geometry& g, unsigned ev, 
   }
 
   std::vector<std::vector<Pixel>> sensor_pixels(n_sensors);
+  // Retina encoding: one word per generated cluster (centroid in quarter pixels).
+  std::vector<std::vector<uint32_t>> sensor_retina(n_sensors);
+  auto retina_word = [&](unsigned sensor, unsigned sc, unsigned sr, unsigned n, uint32_t* lhcb_id) {
+    const unsigned cx4 = 4 * sc / n, cy4 = 4 * sr / n;
+    const unsigned cx = cx4 >> 2, cy = cy4 >> 2, fx = cx4 & 3u, fy = cy4 & 3u;
+    const uint32_t cid = ((fx != 0) << 27) | ((fy != 0) << 26) | (sensor << 18) | ((cx >> 8) << 16) |
+                         ((cx & 0xFFu) << 8) | cy;
+    *lhcb_id = (0x2u << 28) | (cid & 0x0FFFFFFFu);
+    const uint32_t w = (cx << 12) | (fx << 10) | (cy << 2) | fy;
+    return c.bank_version > 3 ? (w | ((sensor & 1u) << 22)) : w;
+  };
 
   // Collisions and tracks.
   struct Vtx {
@@ -204,6 +215,7 @@ void generate_event(const velo_synth_config& c, const Geometry& g, unsigned ev, 
         h.event = ev;
         h.particle = pidx;
         h.lhcb_id = channel_lhcb_id(sensor, sc / kept, sr / kept);
+        if (c.retina) sensor_retina[sensor].push_back(retina_word(sensor, sc, sr, kept, &h.lhcb_id));
         h.sensor = sensor;
         h.n_pixels = kept;
         h.x = gx;
@@ -235,6 +247,7 @@ void generate_event(const velo_synth_config& c, const Geometry& g, unsigned ev, 
     h.event = ev;
     h.particle = 0xFFFFFFFFu;
     h.lhcb_id = channel_lhcb_id(sensor, col, row);
+    if (c.retina) sensor_retina[sensor].push_back(retina_word(sensor, col, row, 1, &h.lhcb_id));
     h.sensor = sensor;
     h.n_pixels = 1;
     out.hits.push_back(h);
@@ -271,7 +284,36 @@ void generate_event(const velo_synth_config& c, const Geometry& g, unsigned ev, 
   // Banks.
   std::vector<std::vector<uint32_t>> banks;
   std::vector<uint32_t> source_ids;
-  if (c.bank_version > 3) {
+  uint8_t bank_type = 63;
+  if (c.retina) {
+    bank_type = 73;
+    // Identical words of a sensor merged, in generation order.
+    for (auto& w : sensor_retina) {
+      std::vector<uint32_t> u;
+      for (const uint32_t x : w)
+        if (std::find(u.begin(), u.end(), x) == u.end()) u.push_back(x);
+      if (c.shuffle_sp) std::shuffle(u.begin(), u.end(), rng);
+      w = std::move(u);
+    }
+    if (c.bank_version > 3) {
+      for (unsigned pair = 0; pair < n_sensors / 2; ++pair) {
+        std::vector<uint32_t> w = sensor_retina[2 * pair];
+        w.insert(w.end(), sensor_retina[2 * pair + 1].begin(), sensor_retina[2 * pair + 1].end());
+        if (w.size() & 1u) w.push_back(0u);
+        banks.push_back(std::move(w));
+        source_ids.push_back(0x2000u | pair);
+      }
+    }
+    else {
+      for (unsigned sensor = 0; sensor < n_sensors; ++sensor) {
+        std::vector<uint32_t> w = sensor_retina[sensor];
+        w.insert(w.begin(), static_cast<uint32_t>(w.size()));
+        banks.push_back(std::move(w));
+        source_ids.push_back(sensor);
+      }
+    }
+  }
+  else if (c.bank_version > 3) {
     for (unsigned pair = 0; pair < n_sensors / 2; ++pair) {
       std::vector<uint32_t> w = sensor_words[2 * pair];
       w.insert(w.end(), sensor_words[2 * pair + 1].begin(), sensor_words[2 * pair + 1].end());
@@ -301,7 +343,7 @@ void generate_event(const velo_synth_config& c, const Geometry& g, unsigned ev, 
     out.block.insert(out.block.end(), banks[b].begin(), banks[b].end());
     offset += 4 * (1 + banks[b].size());
     out.bank_sizes.push_back(static_cast<uint16_t>(4 * banks[b].size()));
-    out.bank_types.push_back(63);
+    out.bank_types.push_back(bank_type);
   }
   out.block[off_pos + nb] = offset;
 }
@@ -347,7 +389,7 @@ void* velo_synth_generate(const velo_synth_config* c, const char* geometry, size
     delete gp;
     return nullptr;
   }
-  if (c->bank_version != 2 && c->bank_version != 4) {
+  if (c->retina ? (c->bank_version != 3 && c->bank_version != 4) : (c->bank_version != 2 && c->bank_version != 4)) {
     delete gp;
     return nullptr;
   }

@@ -7,6 +7,14 @@
  *              SP word bit 31 = no 8-neighbour SP on the same sensor, bit 23 = sensor in pair,
  *              bits 22..14 = SP column, 13..8 = SP row, 7..0 = pixels (bit = 4*col + row);
  *   version 2: 208 banks per event, one per sensor, first word = number of SP words.
+ * With `retina` set, every generated cluster is encoded instead as one retina cluster word
+ * (bank type 73, the FPGA-clustered format decode_retinaclusters reads):
+ *   version 4: 104 banks per event, one per sensor pair, bit 22 = sensor in pair;
+ *   version 3: 208 banks per event, one per sensor, first word = number of cluster words;
+ *   word bits 21..12 = column of the centroid, 11..10 = its quarter-pixel fraction,
+ *   9..2 = row, 1..0 = its quarter-pixel fraction (centroids truncated to quarter pixels);
+ *   identical words of a sensor are merged, and a bank with an odd number of words is padded
+ *   with a zero word.
  * The event layout is the one main/src/Transpose.cpp:430-498 writes (see velo_amd.h). */
 #ifndef VELO_SYNTH_H
 #define VELO_SYNTH_H
@@ -32,8 +40,9 @@ typedef struct {
   double noise_fraction;         /* noise pixels per signal pixel */
   double dead_asic_fraction;
   double cluster_size_p[3];      /* probabilities of 1, 2 and 3 pixel clusters */
-  int bank_version;              /* 4 or 2 */
+  int bank_version;              /* 4 or 2 (super pixels); 4 or 3 (retina) */
   int shuffle_sp;                /* shuffle SP words inside each bank */
+  int retina;                    /* retina cluster banks instead of super-pixel banks */
 } velo_synth_config;
 
 typedef struct {
@@ -48,7 +57,8 @@ typedef struct {
 typedef struct {
   uint32_t event;
   uint32_t particle;             /* index into the particle table of the batch, or 0xFFFFFFFF for noise */
-  uint32_t lhcb_id;              /* LHCbID of the centroid pixel of the generated cluster */
+  uint32_t lhcb_id;              /* LHCbID of the centroid pixel of the generated cluster (retina: with
+                                    the fraction flags of the encoded centroid in bits 26-27) */
   uint16_t sensor;
   uint16_t n_pixels;
   float x, y, z;                 /* true intersection point */

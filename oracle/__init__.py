@@ -37,6 +37,7 @@ class VoInputs(ctypes.Structure):
         ("compute_f64", ctypes.c_int),
         ("kalman_fp64_fallback", ctypes.c_int),
         ("collect_seeds", ctypes.c_int),
+        ("retina", ctypes.c_int),
     ]
 
 
@@ -99,6 +100,7 @@ class Result:
         "velo_kalman_beamline_states_f64": np.float64, "velo_kalman_endvelo_states_f64": np.float64,
         "search_stats": np.uint64,
         "seed_features": np.float32,
+        "hit_sorting_key": np.int64,
     }
 
     def __init__(self, handle, keep):
@@ -135,6 +137,7 @@ def make_inputs(events, geometry: bytes, beamline: np.ndarray, **props):
     a.geometry = geometry
     a.geometry_bytes = len(geometry)
     a.beamline = _ptr(beamline)
+    a.retina = int(getattr(events, "retina", False))
     for k, v in props.items():
         setattr(a, k, v)
     return a, keep

@@ -67,6 +67,7 @@ struct Params {
   unsigned block_dim_x = 256;
   bool f64 = true;
   bool kalman_fp64_fallback = false;
+  bool retina = false;
 };
 
 void run_sequence(
@@ -85,31 +86,54 @@ void run_sequence(
   const uint32_t* el = event_list.data();
   const unsigned nl = n_events;
 
-  // velo_calculate_number_of_candidates + prefix_sum_offsets_velo_candidates
-  auto* ncand = s.make<uint32_t>("number_of_candidates", n_events);
-  auto* bank_index = s.make<uint32_t>("velo_bank_index", n_sensors);
-  calculate_number_of_candidates(in, el, nl, event_start, prm.block_dim_x, ncand, bank_index);
-  auto* cand_off = s.make<uint32_t>("candidates_offsets", n_events + 1);
-  const uint32_t total_candidates = prefix_sum(ncand, n_events, cand_off);
+  uint32_t total_clusters = 0;
+  uint32_t* est_off = nullptr;
+  uint32_t* mcnum = nullptr;
+  char* sorted = nullptr;
+  if (prm.retina) {
+    // velo_reconstruction.py:18-71: calculate_number_of_retinaclusters_each_sensor_pair,
+    // prefix_sum_offsets_estimated_input_size, decode_retinaclusters
+    const unsigned slots = retina_sensor_slots(in.version);
+    auto* size = s.make<uint32_t>("each_sensor_pair_size", (size_t) n_events * slots);
+    auto* bank_index = s.make<uint32_t>("retina_bank_index", slots);
+    calculate_number_of_retinaclusters_each_sensor_pair(in, el, nl, event_start, size, bank_index);
+    auto* off = s.make<uint32_t>("offsets_each_sensor_pair_size", (size_t) n_events * slots + 1);
+    total_clusters = prefix_sum(size, (size_t) n_events * slots, off);
+    mcnum = s.make<uint32_t>("module_cluster_num", n_events * n_module_pairs);
+    est_off = s.make<uint32_t>("offsets_estimated_input_size", n_events * n_module_pairs + 1);
+    sorted = s.make<char>("sorted_velo_cluster_container", (size_t) total_clusters * cluster_element_size);
+    auto* perm = s.make<uint32_t>("hit_permutations", total_clusters);
+    auto* keys = s.make<int64_t>("hit_sorting_key", total_clusters);
+    decode_retinaclusters(
+      in, el, nl, n_events, event_start, g, off, bank_index, mcnum, est_off, sorted, perm, keys);
+  }
+  else {
+    // velo_calculate_number_of_candidates + prefix_sum_offsets_velo_candidates
+    auto* ncand = s.make<uint32_t>("number_of_candidates", n_events);
+    auto* bank_index = s.make<uint32_t>("velo_bank_index", n_sensors);
+    calculate_number_of_candidates(in, el, nl, event_start, prm.block_dim_x, ncand, bank_index);
+    auto* cand_off = s.make<uint32_t>("candidates_offsets", n_events + 1);
+    const uint32_t total_candidates = prefix_sum(ncand, n_events, cand_off);
 
-  // velo_estimate_input_size + prefix_sum_offsets_estimated_input_size
-  auto* est = s.make<uint32_t>("estimated_input_size", n_events * n_module_pairs);
-  auto* mcn = s.make<uint32_t>("module_candidate_num", n_events);
-  auto* cands = s.make<uint32_t>("cluster_candidates", total_candidates);
-  estimate_input_size(in, el, nl, event_start, cand_off, est, mcn, cands);
-  auto* est_off = s.make<uint32_t>("offsets_estimated_input_size", n_events * n_module_pairs + 1);
-  const uint32_t total_clusters = prefix_sum(est, n_events * n_module_pairs, est_off);
+    // velo_estimate_input_size + prefix_sum_offsets_estimated_input_size
+    auto* est = s.make<uint32_t>("estimated_input_size", n_events * n_module_pairs);
+    auto* mcn = s.make<uint32_t>("module_candidate_num", n_events);
+    auto* cands = s.make<uint32_t>("cluster_candidates", total_candidates);
+    estimate_input_size(in, el, nl, event_start, cand_off, est, mcn, cands);
+    est_off = s.make<uint32_t>("offsets_estimated_input_size", n_events * n_module_pairs + 1);
+    total_clusters = prefix_sum(est, n_events * n_module_pairs, est_off);
 
-  // velo_masked_clustering
-  auto* mcnum = s.make<uint32_t>("module_cluster_num", n_events * n_module_pairs);
-  auto* container = s.make<char>("velo_cluster_container", (size_t) total_clusters * cluster_element_size);
-  masked_clustering(
-    in, el, nl, n_events, event_start, g, sp, est_off, mcn, cands, cand_off, bank_index, mcnum, container);
+    // velo_masked_clustering
+    mcnum = s.make<uint32_t>("module_cluster_num", n_events * n_module_pairs);
+    auto* container = s.make<char>("velo_cluster_container", (size_t) total_clusters * cluster_element_size);
+    masked_clustering(
+      in, el, nl, n_events, event_start, g, sp, est_off, mcn, cands, cand_off, bank_index, mcnum, container);
 
-  // velo_sort_by_phi
-  auto* perm = s.make<uint32_t>("hit_permutation", total_clusters);
-  auto* sorted = s.make<char>("sorted_velo_cluster_container", (size_t) total_clusters * cluster_element_size);
-  sort_by_phi(el, nl, n_events, est_off, mcnum, container, sorted, perm);
+    // velo_sort_by_phi
+    auto* perm = s.make<uint32_t>("hit_permutation", total_clusters);
+    sorted = s.make<char>("sorted_velo_cluster_container", (size_t) total_clusters * cluster_element_size);
+    sort_by_phi(el, nl, n_events, est_off, mcnum, container, sorted, perm);
+  }
 
   // velo_search_by_triplet
   const unsigned tcs = track_container_size(total_clusters, n_events);
@@ -200,6 +224,7 @@ bool make_inputs(const vo_inputs* a, vo::RawInput& in, vo::Geometry& g, vo::SpTa
   p.block_dim_x = a->block_dim_x ? a->block_dim_x : 256;
   p.f64 = a->compute_f64 != 0;
   p.kalman_fp64_fallback = a->kalman_fp64_fallback != 0;
+  p.retina = a->retina != 0;
   return true;
 }
 } // namespace

@@ -58,6 +58,7 @@ constexpr unsigned skipped_pos = 28;
 // LHCb raw bank types accepted by the VELO decoders (mdf/lhcb/Event/RawBank.h:69,124)
 constexpr unsigned bank_type_velo = 8;
 constexpr unsigned bank_type_vp = 63;
+constexpr unsigned bank_type_retina = 73; // VPRetinaCluster (mdf/lhcb/Event/RawBank.h:134)
 
 // backend/include/BackendCommonInterface.h:49
 constexpr float pi_f = 3.141592654f;

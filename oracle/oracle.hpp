@@ -79,6 +79,30 @@ void masked_clustering(
   uint32_t* module_pair_cluster_num,
   char* cluster_container);
 
+// Retina-cluster decoding (retina.cpp). Sensor-pair slots per event: 208 (v2/v3) or 104 (v4).
+unsigned retina_sensor_slots(int version);
+void calculate_number_of_retinaclusters_each_sensor_pair(
+  const RawInput& in,
+  const uint32_t* event_list,
+  unsigned n_list,
+  unsigned event_start,
+  uint32_t* each_sensor_pair_size,
+  uint32_t* retina_bank_index);
+void decode_retinaclusters(
+  const RawInput& in,
+  const uint32_t* event_list,
+  unsigned n_list,
+  unsigned n_events,
+  unsigned event_start,
+  const Geometry& g,
+  const uint32_t* offsets_each_sensor_pair_size,
+  const uint32_t* retina_bank_index,
+  uint32_t* module_pair_cluster_num,
+  uint32_t* offsets_module_pair_cluster,
+  char* cluster_container,
+  uint32_t* hit_permutations,
+  int64_t* hit_sorting_key);
+
 void sort_by_phi(
   const uint32_t* event_list,
   unsigned n_list,

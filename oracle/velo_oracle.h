@@ -30,6 +30,8 @@ typedef struct {
   int kalman_fp64_fallback;   /* FP32 Kalman with the FP64 covariance update on ill-conditioned steps */
   int collect_seeds;          /* also return every seed: "seed_log" (u32 event, module pair, h0, h1, h2)
                                  and "seed_features" (f32 x 10) */
+  int retina;                 /* the banks are retina clusters (type 73): decode_retinaclusters replaces
+                                 the candidates, estimate, masked clustering and phi sort */
 } vo_inputs;
 
 void vo_default_inputs(vo_inputs* a);

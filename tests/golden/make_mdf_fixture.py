@@ -1,5 +1,6 @@
-"""Writes tests/golden/mdf_minbias_vp.npz: the VELO super-pixel raw banks (type 63, VP) of the
-10 events of the reference's bundled MDF file, in Allen's event layout.
+"""Writes tests/golden/mdf_minbias_vp.npz and mdf_minbias_retina.npz: the VELO super-pixel raw
+banks (type 63, VP) and the retina cluster banks (type 73, VPRetinaCluster) of the 10 events of
+the reference's bundled MDF file, each in Allen's event layout.
 
 Run in a checkout that has the reference at /root/reference:
     python tests/golden/make_mdf_fixture.py
@@ -16,8 +17,8 @@ import sys
 import numpy as np
 
 SRC = "/root/reference/input/minbias/mdf/MiniBrunel_2018_MinBias_FTv4_DIGI_retinacluster_v1.mdf"
-OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "mdf_minbias_vp.npz")
-VP = 63
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUTPUTS = {63: os.path.join(HERE, "mdf_minbias_vp.npz"), 73: os.path.join(HERE, "mdf_minbias_retina.npz")}
 SUBHEADER = {0: 20 + 52 - 2, 1: 20 + 28, 2: 20, 3: 20 + 28}
 
 
@@ -90,7 +91,8 @@ def allen_layout(events, bank_type):
 
 if __name__ == "__main__":
     evs = read_events(sys.argv[1] if len(sys.argv) > 1 else SRC)
-    raw, offsets, sizes, types, version = allen_layout(evs, VP)
-    np.savez_compressed(OUT, raw=raw, offsets=offsets, sizes=sizes, types=types, version=np.int32(version),
-                        n_events=np.int32(len(evs)))
-    print(f"{len(evs)} events, VP bank version {version}, raw {raw.size} bytes -> {OUT}")
+    for bank_type, out in OUTPUTS.items():
+        raw, offsets, sizes, types, version = allen_layout(evs, bank_type)
+        np.savez_compressed(out, raw=raw, offsets=offsets, sizes=sizes, types=types, version=np.int32(version),
+                            n_events=np.int32(len(evs)))
+        print(f"{len(evs)} events, bank type {bank_type} version {version}, raw {raw.size} bytes -> {out}")

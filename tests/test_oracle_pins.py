@@ -544,3 +544,98 @@ def test_seed_features_against_geometry(geometry, beamline):
     np.testing.assert_allclose(got[:, 6:8], want[:, 6:8], rtol=1e-5, atol=1e-7)
     np.testing.assert_allclose(got[:, 8], want[:, 8], rtol=1e-3, atol=2e-4)
     np.testing.assert_array_equal(got[:, 9], want[:, 9])
+
+
+# ---------------------------------------------------------------------------- retina decoding
+def _check_retina(ev, geometry, geo, beamline):
+    """Every module pair of the oracle's retina decoding against the textbook decoding of the
+    banks: the same LHCbIDs (one cluster per non-zero word), float64 positions to 1e-5 mm, the
+    (phi, id) order, and module-pair ranges that hold every word of the pair's banks (zero
+    padding words included)."""
+    r = oracle.run(ev, geometry, beamline)
+    est, cnt = r["offsets_estimated_input_size"], r["module_cluster_num"]
+    ids, xyz, phi = parity.cluster_views(r["sorted_velo_cluster_container"], int(est[-1]))
+    n_checked = 0
+    words_per_mp = np.zeros(ev.n_events * 26, np.int64)
+    for e in range(ev.n_events):
+        for source, words in textbook.iter_retina_banks(ev.raw, ev.offsets, ev.sizes, ev.types, ev.bank_version, e):
+            sensor0 = source if ev.bank_version in (2, 3) else (source & 0x1FF) << 1
+            words_per_mp[e * 26 + sensor0 // 8] += words.size
+        by_mp = {}
+        for sensor, lhcb, g in textbook.retina_clusters(ev, geo, e):
+            by_mp.setdefault(sensor // 8, []).append((lhcb, g))
+        for mp in range(26):
+            k = e * 26 + mp
+            a, c = int(est[k]), int(cnt[k])
+            want = by_mp.get(mp, [])
+            assert c == len(want), (e, mp)
+            key = phi[a:a + c].astype(np.int64) * (1 << 33) + ids[a:a + c].astype(np.int64)
+            assert np.all(np.diff(key) > 0)
+            got = {int(i): p.astype(np.float64) for i, p in zip(ids[a:a + c], xyz[a:a + c])}
+            for lhcb, g in want:
+                np.testing.assert_allclose(got[lhcb], g, rtol=0, atol=1e-5)
+            n_checked += c
+    np.testing.assert_array_equal(est, np.concatenate([[0], np.cumsum(words_per_mp)]))
+    return r, n_checked
+
+
+@pytest.mark.parametrize("version", [4, 3])
+def test_retina_decoding_of_synthetic_banks_against_textbook(version, geometry, geo, beamline):
+    ev = synth.generate(n_events=12, pileup=1, nu=7.6, seed=41, retina=1, bank_version=version)
+    r, n = _check_retina(ev, geometry, geo, beamline)
+    assert n > 10000
+    if version == 4:
+        assert int(r["offsets_estimated_input_size"][-1]) > n  # zero padding words present
+
+
+def test_retina_decoding_of_real_banks_against_textbook(geometry, geo, beamline):
+    """The retina cluster banks (type 73, version 3) of the reference's bundled MDF file."""
+    ev = textbook.load_mdf_fixture(os.path.join(GOLDEN, "mdf_minbias_retina.npz"))
+    ev.retina = True
+    assert ev.bank_version == 3
+    _, n = _check_retina(ev, geometry, geo, beamline)
+    assert n > 20000
+
+
+def test_retina_clusters_of_real_data_match_super_pixel_clusters(geometry, beamline):
+    """The same 10 real events carry both encodings of their pixels: clusters the FPGA made
+    (retina banks) and raw super pixels clustered here. Decoded retina clusters sit on super-pixel
+    clusters of the same sensor: the median distance is zero, 94 % are within 0.03 mm (about half
+    a pixel); the rest are clusters the two clusterings split differently."""
+    ret = textbook.load_mdf_fixture(os.path.join(GOLDEN, "mdf_minbias_retina.npz"))
+    ret.retina = True
+    sp = textbook.load_mdf_fixture(os.path.join(GOLDEN, "mdf_minbias_vp.npz"))
+    rr, rs = oracle.run(ret, geometry, beamline), oracle.run(sp, geometry, beamline)
+
+    def filled(r, e):
+        est, cnt = r["offsets_estimated_input_size"], r["module_cluster_num"]
+        ids, xyz, _ = parity.cluster_views(r["sorted_velo_cluster_container"], int(est[-1]))
+        idx = np.concatenate([np.arange(est[e * 26 + m], est[e * 26 + m] + cnt[e * 26 + m]) for m in range(26)])
+        idx = idx.astype(np.int64)
+        return (ids[idx] >> 18) & 0xFF, xyz[idx].astype(np.float64)
+
+    dist = []
+    for e in range(ret.n_events):
+        sr, xr = filled(rr, e)
+        ss, xs = filled(rs, e)
+        for k in range(sr.size):
+            on = ss == sr[k]
+            if on.any():
+                dist.append(np.min(np.linalg.norm(xs[on, :2] - xr[k, :2], axis=1)))
+    dist = np.array(dist)
+    assert dist.size > 20000
+    assert np.median(dist) == 0.0
+    assert (dist < 0.03).mean() > 0.93
+    # Both give the same number of tracks to within a few per cent.
+    n_ret, n_sp = int(rr["offsets_all_velo_tracks"][-1]), int(rs["offsets_all_velo_tracks"][-1])
+    assert abs(n_ret - n_sp) < 0.03 * n_sp
+
+
+def test_retina_synthetic_tracking_efficiency(geometry, beamline):
+    """The whole path on synthetic retina banks finds the generated particles (01_velo)."""
+    from allen_amd import checker
+    ev = synth.generate(n_events=20, pileup=0, mean_tracks=120.0, seed=42, retina=1)
+    r = oracle.run(ev, geometry, beamline)
+    res = checker.check_buffers(ev, r["offsets_all_velo_tracks"], r["offsets_velo_track_hit_number"],
+                                r["velo_track_hits"])
+    assert res.efficiency > 0.99 and res.ghost_rate < 0.005

@@ -94,3 +94,45 @@ def load_mdf_fixture(path):
     z = np.load(path)
     return Events(n_events=int(z["n_events"]), bank_version=int(z["version"]), raw=z["raw"],
                   offsets=z["offsets"], sizes=z["sizes"], types=z["types"])
+
+
+def iter_retina_banks(raw, offsets, sizes, types, version, event):
+    """(sourceID, words) of each retina cluster bank (type 73) of an event in Allen's layout."""
+    base = int(offsets[event])
+    n_banks = int(raw[base:base + 4].view("<u4")[0])
+    bank_off = raw[base + 4:base + 4 + 4 * (n_banks + 1)].view("<u4")
+    payload = base + 4 + 4 * (n_banks + 1)
+    s16 = sizes.view("<u2")[int(sizes[event]):int(sizes[event]) + n_banks]
+    t8 = types.view(np.uint8)[int(types[event]):int(types[event]) + n_banks]
+    for b in range(n_banks):
+        if t8[b] != 73:
+            continue
+        p = payload + int(bank_off[b])
+        source = int(raw[p:p + 4].view("<u4")[0])
+        if version in (2, 3):
+            count = int(raw[p + 4:p + 8].view("<u4")[0])
+            words = raw[p + 8:p + 8 + 4 * count].view("<u4")
+        else:
+            words = raw[p + 4:p + 4 + int(s16[b])].view("<u4")
+        yield source, words
+
+
+def retina_clusters(ev, geometry, event):
+    """Every non-zero retina word of an event as (sensor, LHCbID, float64 global position): the
+    centroid column and row with their quarter-pixel fractions (v3/v4 word layout: column
+    21..12, its fraction 11..10, row 9..2, its fraction 1..0; v4: bit 22 = sensor in pair);
+    the LHCbID carries the fraction flags in bits 27 (column) and 26 (row)."""
+    out = []
+    for source, words in iter_retina_banks(ev.raw, ev.offsets, ev.sizes, ev.types, ev.bank_version, event):
+        for w in words.tolist():
+            if w == 0:
+                continue
+            sensor = source if ev.bank_version in (2, 3) else (((source & 0x1FF) << 1) | ((w >> 22) & 1))
+            cx, qx, cy, qy = (w >> 12) & 0x3FF, (w >> 10) & 3, (w >> 2) & 0xFF, w & 3
+            lx = float(geometry["local_x"][cx]) + qx / 4 * float(geometry["x_pitch"][cx])
+            ly = (cy + qy / 4 + 0.5) * 0.055
+            m = geometry["ltg"][sensor].astype(np.float64)
+            g = np.array([m[0] * lx + m[1] * ly + m[9], m[3] * lx + m[4] * ly + m[10], m[6] * lx + m[7] * ly + m[11]])
+            lhcb = lhcb_id(sensor, cx, cy) | (int(qx != 0) << 27) | (int(qy != 0) << 26)
+            out.append((sensor, lhcb, g))
+    return out
