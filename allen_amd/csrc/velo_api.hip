@@ -240,6 +240,67 @@ int velo_calculate_number_of_candidates(const velo_calculate_number_of_candidate
   return rc;
 }
 
+unsigned velo_retina_sensor_slots(int bank_version) { return velo::retina_sensor_slots(bank_version); }
+
+int velo_calculate_number_of_retinaclusters_each_sensor_pair(
+  const velo_calculate_number_of_retinaclusters_params* p,
+  hipStream_t stream)
+{
+  if (!p || !p->dev_each_sensor_pair_size || !p->dev_retina_bank_index) return VELO_ERR_INVALID_ARGUMENT;
+  if (p->mep_layout) return VELO_ERR_UNSUPPORTED_LAYOUT;
+  const unsigned slots = velo::retina_sensor_slots(p->host_raw_bank_version);
+  int rc = hip_status(
+    velo::launch_fill_u32(p->dev_each_sensor_pair_size, (size_t) p->host_number_of_events * slots, 0, stream));
+  if (rc != VELO_OK) return rc;
+  if (p->host_raw_bank_version < 0) return hip_status(velo::launch_fill_u32(p->dev_retina_bank_index, slots, 0, stream));
+  if (!version_ok(p->host_raw_bank_version)) return VELO_ERR_BANK_VERSION;
+  velo::RetinaSizesLaunch a;
+  a.version = p->host_raw_bank_version;
+  a.raw = p->dev_velo_retina_raw_input;
+  a.offsets = p->dev_velo_retina_raw_input_offsets;
+  a.sizes = p->dev_velo_retina_raw_input_sizes;
+  a.types = p->dev_velo_retina_raw_input_types;
+  a.event_list = p->dev_event_list;
+  a.n_list = p->event_list_size;
+  a.event_start = p->event_start;
+  a.each_sensor_pair_size = p->dev_each_sensor_pair_size;
+  a.retina_bank_index = p->dev_retina_bank_index;
+  return hip_status(velo::launch_retina_sensor_sizes(a, stream));
+}
+
+int velo_decode_retinaclusters(const velo_decode_retinaclusters_params* p, hipStream_t stream)
+{
+  if (!p || !p->constants || !p->dev_module_cluster_num || !p->dev_offsets_module_pair_cluster ||
+      !p->dev_velo_cluster_container || !p->dev_hit_permutations || !p->dev_hit_sorting_key ||
+      !p->dev_number_of_events)
+    return VELO_ERR_INVALID_ARGUMENT;
+  if (p->mep_layout) return VELO_ERR_UNSUPPORTED_LAYOUT;
+  const size_t items = (size_t) p->host_number_of_events * velo::n_module_pairs;
+  int rc = hip_status(velo::launch_fill_u32(p->dev_module_cluster_num, items, 0, stream));
+  if (rc == VELO_OK) rc = hip_status(velo::launch_fill_u32(p->dev_offsets_module_pair_cluster, items + 1, 0, stream));
+  if (rc != VELO_OK || p->host_raw_bank_version < 0) return rc;
+  if (!version_ok(p->host_raw_bank_version)) return VELO_ERR_BANK_VERSION;
+  velo::RetinaDecodeLaunch a;
+  a.version = p->host_raw_bank_version;
+  a.raw = p->dev_velo_retina_raw_input;
+  a.offsets = p->dev_velo_retina_raw_input_offsets;
+  a.sizes = p->dev_velo_retina_raw_input_sizes;
+  a.types = p->dev_velo_retina_raw_input_types;
+  a.event_list = p->dev_event_list;
+  a.n_list = p->event_list_size;
+  a.event_start = p->event_start;
+  a.geometry = p->constants->dev_geometry;
+  a.offsets_each_sensor_pair_size = p->dev_offsets_each_sensor_pair_size;
+  a.retina_bank_index = p->dev_retina_bank_index;
+  a.number_of_events = p->dev_number_of_events;
+  a.module_cluster_num = p->dev_module_cluster_num;
+  a.offsets_module_pair_cluster = p->dev_offsets_module_pair_cluster;
+  a.cluster_container = p->dev_velo_cluster_container;
+  a.hit_permutations = p->dev_hit_permutations;
+  a.hit_sorting_key = p->dev_hit_sorting_key;
+  return hip_status(velo::launch_retina_decode(a, stream));
+}
+
 int velo_estimate_input_size(const velo_estimate_input_size_params* p, hipStream_t stream)
 {
   if (!p || !p->dev_estimated_input_size || !p->dev_module_candidate_num) return VELO_ERR_INVALID_ARGUMENT;
@@ -498,7 +559,9 @@ int velo_sequence_create(const velo_constants* c, unsigned max_events, size_t ma
   s->track_cap = velo::track_container_size(T, max_events) + 1;
   s->hits_cap = s->track_cap * (velo::max_track_size + 3);
   const size_t scan_words =
-    std::max<size_t>(velo::prefix_sum_scratch_words(26 * max_events), velo::prefix_sum_scratch_words(2 * s->track_cap)) + 1;
+    std::max({velo::prefix_sum_scratch_words(26 * max_events), velo::prefix_sum_scratch_words(velo::n_sensors * max_events),
+              velo::prefix_sum_scratch_words(2 * s->track_cap)}) +
+    1;
   const std::vector<std::pair<const char*, size_t>> layout = {
     {"number_of_events", 4},
     {"status", 4},
@@ -541,6 +604,11 @@ int velo_sequence_create(const velo_constants* c, unsigned max_events, size_t ma
     {"sp_flags", s->sp_cap},
     {"bank_directory", 16 * 8 * 26 * E},
     {"bank_directory_count", 4 * 26 * E},
+    {"each_sensor_pair_size", 4 * velo::n_sensors * E},
+    {"offsets_each_sensor_pair_size", 4 * (velo::n_sensors * E + 1)},
+    {"retina_bank_index", 4 * velo::n_sensors},
+    {"hit_permutations", 4 * s->sp_cap},
+    {"hit_sorting_key", 8 * s->sp_cap},
   };
   for (const auto& kv : layout) {
     void* p = nullptr;
@@ -705,119 +773,166 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream
   VELO_TRY(velo::launch_fill_u32(n_events_dev, 1, E, stream));
   VELO_TRY(velo::launch_fill_u32(status, 1, 0, stream));
 
-  velo::CandidatesLaunch k1;
-  k1.version = in->bank_version;
-  k1.raw = in->dev_velo_raw_input;
-  k1.offsets = in->dev_velo_raw_input_offsets;
-  k1.sizes = in->dev_velo_raw_input_sizes;
-  k1.types = in->dev_velo_raw_input_types;
-  k1.event_list = nullptr;
-  k1.n_list = E;
-  k1.event_start = 0;
-  k1.block_dim_x = 256;
-  k1.number_of_candidates = s->get<uint32_t>("number_of_candidates");
-  k1.bank_index = s->get<uint32_t>("velo_bank_index");
-  if (s->fused_clustering) {
-    k1.directory = s->get<void>("bank_directory");
-    k1.directory_count = s->get<uint32_t>("bank_directory_count");
-  }
-  VELO_MARK(mark_begin(VELO_STAGE_CANDIDATES));
-  VELO_TRY(velo::launch_number_of_candidates(k1, stream));
-  VELO_MARK(mark_end(VELO_STAGE_CANDIDATES));
-  VELO_TRY(velo::launch_prefix_sum(k1.number_of_candidates, E, s->get<uint32_t>("candidates_offsets"), scratch, stream));
-
-  velo::EstimateLaunch k2;
-  k2.version = in->bank_version;
-  k2.raw = k1.raw;
-  k2.offsets = k1.offsets;
-  k2.sizes = k1.sizes;
-  k2.types = k1.types;
-  k2.event_list = nullptr;
-  k2.n_list = E;
-  k2.event_start = 0;
-  k2.candidates_offsets = s->get<uint32_t>("candidates_offsets");
-  k2.estimated_input_size = s->get<uint32_t>("estimated_input_size");
-  k2.module_candidate_num = s->get<uint32_t>("module_candidate_num");
-  k2.cluster_candidates = s->get<uint32_t>("cluster_candidates");
-  auto* sp_flags = s->get<uint8_t>("sp_flags");
-  VELO_MARK(mark_begin(VELO_STAGE_ESTIMATE));
-  if (s->fused_clustering) {
-    velo::EstimateMpLaunch e;
-    e.version = in->bank_version;
-    e.raw = k1.raw;
-    e.directory = k1.directory;
-    e.directory_count = k1.directory_count;
-    e.n_events = E;
-    e.estimated_input_size = k2.estimated_input_size;
-    e.sp_flags = sp_flags;
-    e.resident_grid = s->clustering_resident_grid;
-    VELO_TRY(velo::launch_estimate_mp(e, stream));
-  }
-  else {
-    VELO_TRY(velo::launch_fill_u32(k2.module_candidate_num, E, 0, stream));
-    VELO_TRY(velo::launch_estimate_input_size(k2, stream));
-  }
-  VELO_MARK(mark_end(VELO_STAGE_ESTIMATE));
   auto* est_off = s->get<uint32_t>("offsets_estimated_input_size");
-  VELO_TRY(velo::launch_prefix_sum(k2.estimated_input_size, 26 * E, est_off, scratch, stream));
-
   auto* mcn = s->get<uint32_t>("module_cluster_num");
   auto* sorted = s->get<char>("sorted_velo_cluster_container");
-  VELO_MARK(mark_begin(VELO_STAGE_CLUSTERING));
-  if (s->fused_clustering) {
-    velo::ClusterMpLaunch f;
-    f.version = in->bank_version;
-    f.raw = k1.raw;
-    f.directory = k1.directory;
-    f.directory_count = k1.directory_count;
-    f.n_events = E;
-    f.offsets_estimated_input_size = est_off;
-    f.sp_flags = sp_flags;
-    f.geometry = s->constants->dev_geometry;
-    f.sp_patterns = s->constants->dev_sp_patterns;
-    f.sp_fx = s->constants->dev_sp_fx;
-    f.sp_fy = s->constants->dev_sp_fy;
-    f.module_cluster_num = mcn;
-    f.sorted_container = sorted;
-    f.unsorted_scratch = s->get<char>("velo_cluster_container");
-    f.resident_grid = s->clustering_resident_grid;
-    VELO_TRY(velo::launch_cluster_mp(f, stream));
+  if (in->retina) {
+    // velo_reconstruction.py:18-71: retina cluster counts per slot, their prefix sum, and the
+    // decoding, which writes the module-pair offsets, counts and the sorted container.
+    const unsigned slots = velo::retina_sensor_slots(in->bank_version);
+    velo::RetinaSizesLaunch r1;
+    r1.version = in->bank_version;
+    r1.raw = in->dev_velo_raw_input;
+    r1.offsets = in->dev_velo_raw_input_offsets;
+    r1.sizes = in->dev_velo_raw_input_sizes;
+    r1.types = in->dev_velo_raw_input_types;
+    r1.event_list = nullptr;
+    r1.n_list = E;
+    r1.event_start = 0;
+    r1.each_sensor_pair_size = s->get<uint32_t>("each_sensor_pair_size");
+    r1.retina_bank_index = s->get<uint32_t>("retina_bank_index");
+    VELO_MARK(mark_begin(VELO_STAGE_CANDIDATES));
+    VELO_TRY(velo::launch_fill_u32(r1.retina_bank_index, slots, 0xFFFFFFFFu, stream));
+    VELO_TRY(velo::launch_retina_sensor_sizes(r1, stream));
+    VELO_MARK(mark_end(VELO_STAGE_CANDIDATES));
+    auto* sp_off = s->get<uint32_t>("offsets_each_sensor_pair_size");
+    VELO_MARK(mark_begin(VELO_STAGE_ESTIMATE));
+    VELO_TRY(velo::launch_prefix_sum(r1.each_sensor_pair_size, slots * E, sp_off, scratch, stream));
+    VELO_MARK(mark_end(VELO_STAGE_ESTIMATE));
+    velo::RetinaDecodeLaunch r2;
+    r2.version = in->bank_version;
+    r2.raw = r1.raw;
+    r2.offsets = r1.offsets;
+    r2.sizes = r1.sizes;
+    r2.types = r1.types;
+    r2.event_list = nullptr;
+    r2.n_list = E;
+    r2.event_start = 0;
+    r2.geometry = s->constants->dev_geometry;
+    r2.offsets_each_sensor_pair_size = sp_off;
+    r2.retina_bank_index = r1.retina_bank_index;
+    r2.number_of_events = n_events_dev;
+    r2.module_cluster_num = mcn;
+    r2.offsets_module_pair_cluster = est_off;
+    r2.cluster_container = sorted;
+    r2.hit_permutations = s->get<uint32_t>("hit_permutations");
+    r2.hit_sorting_key = s->get<int64_t>("hit_sorting_key");
+    VELO_MARK(mark_begin(VELO_STAGE_CLUSTERING));
+    VELO_TRY(velo::launch_retina_decode(r2, stream));
+    VELO_MARK(mark_end(VELO_STAGE_CLUSTERING));
   }
   else {
-    velo::MaskedClusteringLaunch k3;
-    k3.version = in->bank_version;
-    k3.raw = k1.raw;
-    k3.offsets = k1.offsets;
-    k3.sizes = k1.sizes;
-    k3.types = k1.types;
-    k3.event_list = nullptr;
-    k3.n_list = E;
-    k3.event_start = 0;
-    k3.number_of_events = n_events_dev;
-    k3.offsets_estimated_input_size = est_off;
-    k3.module_candidate_num = k2.module_candidate_num;
-    k3.cluster_candidates = k2.cluster_candidates;
-    k3.candidates_offsets = k2.candidates_offsets;
-    k3.bank_index = k1.bank_index;
-    k3.geometry = s->constants->dev_geometry;
-    k3.sp_patterns = s->constants->dev_sp_patterns;
-    k3.sp_fx = s->constants->dev_sp_fx;
-    k3.sp_fy = s->constants->dev_sp_fy;
-    k3.module_cluster_num = mcn;
-    k3.cluster_container = s->get<char>("velo_cluster_container");
-    VELO_TRY(velo::launch_masked_clustering(k3, stream));
-    velo::SortLaunch k4;
-    k4.event_list = nullptr;
-    k4.n_list = E;
-    k4.number_of_events = n_events_dev;
-    k4.offsets_estimated_input_size = est_off;
-    k4.module_cluster_num = mcn;
-    k4.cluster_container = k3.cluster_container;
-    k4.sorted_container = sorted;
-    k4.hit_permutation = s->get<uint32_t>("hit_permutation");
-    VELO_TRY(velo::launch_sort_by_phi(k4, stream));
+    velo::CandidatesLaunch k1;
+    k1.version = in->bank_version;
+    k1.raw = in->dev_velo_raw_input;
+    k1.offsets = in->dev_velo_raw_input_offsets;
+    k1.sizes = in->dev_velo_raw_input_sizes;
+    k1.types = in->dev_velo_raw_input_types;
+    k1.event_list = nullptr;
+    k1.n_list = E;
+    k1.event_start = 0;
+    k1.block_dim_x = 256;
+    k1.number_of_candidates = s->get<uint32_t>("number_of_candidates");
+    k1.bank_index = s->get<uint32_t>("velo_bank_index");
+    if (s->fused_clustering) {
+      k1.directory = s->get<void>("bank_directory");
+      k1.directory_count = s->get<uint32_t>("bank_directory_count");
+    }
+    VELO_MARK(mark_begin(VELO_STAGE_CANDIDATES));
+    VELO_TRY(velo::launch_number_of_candidates(k1, stream));
+    VELO_MARK(mark_end(VELO_STAGE_CANDIDATES));
+    VELO_TRY(velo::launch_prefix_sum(k1.number_of_candidates, E, s->get<uint32_t>("candidates_offsets"), scratch, stream));
+
+    velo::EstimateLaunch k2;
+    k2.version = in->bank_version;
+    k2.raw = k1.raw;
+    k2.offsets = k1.offsets;
+    k2.sizes = k1.sizes;
+    k2.types = k1.types;
+    k2.event_list = nullptr;
+    k2.n_list = E;
+    k2.event_start = 0;
+    k2.candidates_offsets = s->get<uint32_t>("candidates_offsets");
+    k2.estimated_input_size = s->get<uint32_t>("estimated_input_size");
+    k2.module_candidate_num = s->get<uint32_t>("module_candidate_num");
+    k2.cluster_candidates = s->get<uint32_t>("cluster_candidates");
+    auto* sp_flags = s->get<uint8_t>("sp_flags");
+    VELO_MARK(mark_begin(VELO_STAGE_ESTIMATE));
+    if (s->fused_clustering) {
+      velo::EstimateMpLaunch e;
+      e.version = in->bank_version;
+      e.raw = k1.raw;
+      e.directory = k1.directory;
+      e.directory_count = k1.directory_count;
+      e.n_events = E;
+      e.estimated_input_size = k2.estimated_input_size;
+      e.sp_flags = sp_flags;
+      e.resident_grid = s->clustering_resident_grid;
+      VELO_TRY(velo::launch_estimate_mp(e, stream));
+    }
+    else {
+      VELO_TRY(velo::launch_fill_u32(k2.module_candidate_num, E, 0, stream));
+      VELO_TRY(velo::launch_estimate_input_size(k2, stream));
+    }
+    VELO_MARK(mark_end(VELO_STAGE_ESTIMATE));
+    VELO_TRY(velo::launch_prefix_sum(k2.estimated_input_size, 26 * E, est_off, scratch, stream));
+
+    VELO_MARK(mark_begin(VELO_STAGE_CLUSTERING));
+    if (s->fused_clustering) {
+      velo::ClusterMpLaunch f;
+      f.version = in->bank_version;
+      f.raw = k1.raw;
+      f.directory = k1.directory;
+      f.directory_count = k1.directory_count;
+      f.n_events = E;
+      f.offsets_estimated_input_size = est_off;
+      f.sp_flags = sp_flags;
+      f.geometry = s->constants->dev_geometry;
+      f.sp_patterns = s->constants->dev_sp_patterns;
+      f.sp_fx = s->constants->dev_sp_fx;
+      f.sp_fy = s->constants->dev_sp_fy;
+      f.module_cluster_num = mcn;
+      f.sorted_container = sorted;
+      f.unsorted_scratch = s->get<char>("velo_cluster_container");
+      f.resident_grid = s->clustering_resident_grid;
+      VELO_TRY(velo::launch_cluster_mp(f, stream));
+    }
+    else {
+      velo::MaskedClusteringLaunch k3;
+      k3.version = in->bank_version;
+      k3.raw = k1.raw;
+      k3.offsets = k1.offsets;
+      k3.sizes = k1.sizes;
+      k3.types = k1.types;
+      k3.event_list = nullptr;
+      k3.n_list = E;
+      k3.event_start = 0;
+      k3.number_of_events = n_events_dev;
+      k3.offsets_estimated_input_size = est_off;
+      k3.module_candidate_num = k2.module_candidate_num;
+      k3.cluster_candidates = k2.cluster_candidates;
+      k3.candidates_offsets = k2.candidates_offsets;
+      k3.bank_index = k1.bank_index;
+      k3.geometry = s->constants->dev_geometry;
+      k3.sp_patterns = s->constants->dev_sp_patterns;
+      k3.sp_fx = s->constants->dev_sp_fx;
+      k3.sp_fy = s->constants->dev_sp_fy;
+      k3.module_cluster_num = mcn;
+      k3.cluster_container = s->get<char>("velo_cluster_container");
+      VELO_TRY(velo::launch_masked_clustering(k3, stream));
+      velo::SortLaunch k4;
+      k4.event_list = nullptr;
+      k4.n_list = E;
+      k4.number_of_events = n_events_dev;
+      k4.offsets_estimated_input_size = est_off;
+      k4.module_cluster_num = mcn;
+      k4.cluster_container = k3.cluster_container;
+      k4.sorted_container = sorted;
+      k4.hit_permutation = s->get<uint32_t>("hit_permutation");
+      VELO_TRY(velo::launch_sort_by_phi(k4, stream));
+    }
+    VELO_MARK(mark_end(VELO_STAGE_CLUSTERING));
   }
-  VELO_MARK(mark_end(VELO_STAGE_CLUSTERING));
 
   velo::SearchLaunch k5;
   k5.event_list = nullptr;

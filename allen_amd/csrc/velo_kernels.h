@@ -25,6 +25,42 @@ struct CandidatesLaunch {
   uint32_t* directory_count = nullptr;
 };
 
+// Retina-cluster decoding (velo_retina.hip).
+struct RetinaSizesLaunch {
+  int version;
+  const char* raw;
+  const uint32_t* offsets;
+  const uint32_t* sizes;
+  const uint32_t* types;
+  const uint32_t* event_list; // NULL = identity
+  unsigned n_list;
+  unsigned event_start;
+  uint32_t* each_sensor_pair_size; // slots per event: 208 (v2/v3) or 104 (v4)
+  uint32_t* retina_bank_index;     // slots
+};
+struct RetinaDecodeLaunch {
+  int version;
+  const char* raw;
+  const uint32_t* offsets;
+  const uint32_t* sizes;
+  const uint32_t* types;
+  const uint32_t* event_list; // NULL = identity
+  unsigned n_list;
+  unsigned event_start;
+  const void* geometry;
+  const uint32_t* offsets_each_sensor_pair_size;
+  const uint32_t* retina_bank_index;
+  const uint32_t* number_of_events;
+  uint32_t* module_cluster_num;         // 26 per event
+  uint32_t* offsets_module_pair_cluster; // 26 per event + 1
+  char* cluster_container;
+  uint32_t* hit_permutations;
+  int64_t* hit_sorting_key;
+};
+unsigned retina_sensor_slots(int version);
+hipError_t launch_retina_sensor_sizes(const RetinaSizesLaunch& a, hipStream_t s);
+hipError_t launch_retina_decode(const RetinaDecodeLaunch& a, hipStream_t s);
+
 struct EstimateLaunch {
   int version;
   const char* raw;

@@ -62,6 +62,21 @@ MaskedClusteringParams = _params("MaskedClusteringParams", [
     ("dev_number_of_events", P), ("dev_velo_bank_index", P), ("dev_module_cluster_num", P),
     ("dev_velo_cluster_container", P), ("constants", P), ("event_start", U), ("mep_layout", I)])
 
+RetinaSizesParams = _params("RetinaSizesParams", [
+    ("host_number_of_events", U), ("host_raw_bank_version", I), ("dev_event_list", P), ("event_list_size", U),
+    ("dev_velo_retina_raw_input", P), ("dev_velo_retina_raw_input_offsets", P),
+    ("dev_velo_retina_raw_input_sizes", P), ("dev_velo_retina_raw_input_types", P), ("dev_retina_bank_index", P),
+    ("dev_each_sensor_pair_size", P), ("event_start", U), ("mep_layout", I)])
+
+RetinaDecodeParams = _params("RetinaDecodeParams", [
+    ("host_total_number_of_velo_clusters", U), ("host_raw_bank_version", I), ("host_number_of_events", U),
+    ("dev_velo_retina_raw_input", P), ("dev_velo_retina_raw_input_offsets", P),
+    ("dev_velo_retina_raw_input_sizes", P), ("dev_velo_retina_raw_input_types", P),
+    ("dev_offsets_each_sensor_pair_size", P), ("dev_retina_bank_index", P), ("dev_event_list", P),
+    ("event_list_size", U), ("dev_number_of_events", P), ("dev_module_cluster_num", P),
+    ("dev_offsets_module_pair_cluster", P), ("dev_velo_cluster_container", P), ("dev_hit_permutations", P),
+    ("dev_hit_sorting_key", P), ("constants", P), ("event_start", U), ("mep_layout", I)])
+
 SortParams = _params("SortParams", [
     ("host_number_of_events", U), ("host_total_number_of_velo_clusters", U), ("dev_event_list", P),
     ("event_list_size", U), ("dev_offsets_estimated_input_size", P), ("dev_module_cluster_num", P),
@@ -112,7 +127,8 @@ PrefixSumParams = _params("PrefixSumParams", [
 SequenceInput = _params("SequenceInput", [
     ("dev_velo_raw_input", P), ("dev_velo_raw_input_offsets", P), ("dev_velo_raw_input_sizes", P),
     ("dev_velo_raw_input_types", P), ("raw_input_bytes", ctypes.c_size_t), ("number_of_events", U),
-    ("bank_version", I), ("phi_tolerance", F), ("max_scatter", F), ("max_chi2", F), ("max_skipped_modules", U)])
+    ("bank_version", I), ("phi_tolerance", F), ("max_scatter", F), ("max_chi2", F), ("max_skipped_modules", U),
+    ("retina", I)])
 
 # Every function the header declares, with its ctypes signature.
 SIGNATURES = {
@@ -133,6 +149,9 @@ SIGNATURES = {
     "velo_calculate_number_of_candidates": (I, [ctypes.POINTER(CandidatesParams), P]),
     "velo_estimate_input_size": (I, [ctypes.POINTER(EstimateParams), P]),
     "velo_masked_clustering": (I, [ctypes.POINTER(MaskedClusteringParams), P]),
+    "velo_retina_sensor_slots": (U, [I]),
+    "velo_calculate_number_of_retinaclusters_each_sensor_pair": (I, [ctypes.POINTER(RetinaSizesParams), P]),
+    "velo_decode_retinaclusters": (I, [ctypes.POINTER(RetinaDecodeParams), P]),
     "velo_sort_by_phi": (I, [ctypes.POINTER(SortParams), P]),
     "velo_search_by_triplet": (I, [ctypes.POINTER(SearchParams), P]),
     "velo_track_container_size": (U, [U, U]),
@@ -257,6 +276,7 @@ class DeviceEvents:
         torch = _torch()
         self.n_events = events.n_events
         self.bank_version = events.bank_version
+        self.retina = bool(getattr(events, "retina", False))
         self.raw_bytes = int(events.raw.size)
         self.raw = torch.from_numpy(events.raw).to(device)
         self.offsets = torch.from_numpy(events.offsets.view(np.int32)).to(device)
@@ -311,6 +331,7 @@ class VeloSequence:
         inp.raw_input_bytes = ev.raw_bytes
         inp.number_of_events = ev.n_events
         inp.bank_version = ev.bank_version
+        inp.retina = int(ev.retina)
         for k, v in props.items():
             setattr(inp, k, v)
         check(lib().velo_sequence_run(self.handle, ctypes.byref(inp), stream_handle(stream)), "velo_sequence_run")
@@ -326,6 +347,7 @@ class VeloSequence:
         inp.raw_input_bytes = ev.raw_bytes
         inp.number_of_events = ev.n_events
         inp.bank_version = ev.bank_version
+        inp.retina = int(ev.retina)
         for k, v in props.items():
             setattr(inp, k, v)
         check(lib().velo_sequence_capture(self.handle, ctypes.byref(inp), stream_handle(stream)),

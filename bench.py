@@ -85,6 +85,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="pileup_1000", choices=sorted(WORKLOADS))
+    ap.add_argument("--decoding", choices=["super_pixel", "retina"], default="super_pixel",
+                    help="raw banks as super pixels (masked clustering) or as retina clusters made by the "
+                         "front-end (decode_retinaclusters), Allen's two VELO decodings")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores (capped at 16)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
@@ -131,7 +134,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": n_bytes, "launches": 5}), flush=True)
         return
-    wl = WORKLOADS[args.workload]
+    wl = dict(WORKLOADS[args.workload])
+    if args.decoding == "retina":
+        wl["retina"] = 1
     t0 = time.time()
     ev = synth.generate(**wl, keep_truth=True, n_threads=min(16, os.cpu_count() or 1))
     log(f"[bench] generated {ev.n_events} events ({ev.raw.size / 1e6:.2f} MB raw) in {time.time() - t0:.1f}s")
@@ -359,7 +364,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded generator, real VELO geometry, v4 super-pixel banks)",
-        "config": {"workload": args.workload, "events_per_step": n, "path": "nn_pruned" if args.scorer else "plain",
+        "config": {"workload": args.workload, "events_per_step": n, "decoding": args.decoding,
+                   "path": "nn_pruned" if args.scorer else "plain",
                    **({"scorer": os.path.basename(args.scorer), "threshold": args.threshold} if args.scorer else {}),
                    "launch": "hip_graph" if args.graph else "eager", "streams": n_streams,
                    "search_grid": "persistent" if args.persistent else "one_workgroup_per_event",

@@ -147,6 +147,61 @@ typedef struct {
 } velo_masked_clustering_params;
 int velo_masked_clustering(const velo_masked_clustering_params* p, hipStream_t stream);
 
+/* ---- calculate_number_of_retinaclusters_each_sensor_pair_t
+ * (device/velo/retinacluster_decoding/include/CalculateNumberOfRetinaClustersPerSensor.cuh:9-21):
+ * the number of cluster words of every retina bank (type 73) per slot: one slot per sensor for
+ * versions 2/3 (bank count word), per sensor pair for version 4 (bank size / 4); slots without
+ * a retina bank get 0. The first event of the list also records each slot's bank number
+ * (every bank type, sourceID & 0x1FF); other entries of dev_retina_bank_index are left as they
+ * are, as in the reference. */
+typedef struct {
+  unsigned host_number_of_events;
+  int host_raw_bank_version;
+  const unsigned* dev_event_list; unsigned event_list_size; /* MASK_INPUT */
+  const char* dev_velo_retina_raw_input;
+  const unsigned* dev_velo_retina_raw_input_offsets;
+  const unsigned* dev_velo_retina_raw_input_sizes;
+  const unsigned* dev_velo_retina_raw_input_types;
+  unsigned* dev_retina_bank_index;      /* 208 (v2/v3) or 104 (v4) */
+  unsigned* dev_each_sensor_pair_size;  /* host_number_of_events x 208 (v2/v3) or 104 (v4) */
+  unsigned event_start;
+  int mep_layout;
+} velo_calculate_number_of_retinaclusters_params;
+int velo_calculate_number_of_retinaclusters_each_sensor_pair(
+  const velo_calculate_number_of_retinaclusters_params* p,
+  hipStream_t stream);
+/* Slots per event of the retina path: 208 for bank versions 2/3, 104 for version 4. */
+unsigned velo_retina_sensor_slots(int bank_version);
+
+/* ---- decode_retinaclusters_t (device/velo/retinacluster_decoding/include/DecodeRetinaClusters.cuh:13-48):
+ * decodes every word, keys it by (phi as u16) << 48 | LHCbID (the largest key for a zero word)
+ * and writes the clusters of each module pair sorted by key from dev_offsets_module_pair_cluster
+ * on; dev_module_cluster_num counts the non-zero words. Where words share a key the slot holds
+ * the last of them in word order. The reference's dev_velo_clusters views are built on the
+ * caller's side. No scratch is needed. */
+typedef struct {
+  unsigned host_total_number_of_velo_clusters;
+  int host_raw_bank_version;
+  unsigned host_number_of_events;
+  const char* dev_velo_retina_raw_input;
+  const unsigned* dev_velo_retina_raw_input_offsets;
+  const unsigned* dev_velo_retina_raw_input_sizes;
+  const unsigned* dev_velo_retina_raw_input_types;
+  const unsigned* dev_offsets_each_sensor_pair_size; /* slots x events + 1 */
+  const unsigned* dev_retina_bank_index;
+  const unsigned* dev_event_list; unsigned event_list_size;
+  const unsigned* dev_number_of_events;
+  unsigned* dev_module_cluster_num;            /* 26 x events */
+  unsigned* dev_offsets_module_pair_cluster;   /* 26 x events + 1 */
+  char* dev_velo_cluster_container;            /* 18 x host_total_number_of_velo_clusters bytes */
+  unsigned* dev_hit_permutations;              /* host_total_number_of_velo_clusters */
+  int64_t* dev_hit_sorting_key;                /* host_total_number_of_velo_clusters */
+  const velo_constants* constants;
+  unsigned event_start;
+  int mep_layout;
+} velo_decode_retinaclusters_params;
+int velo_decode_retinaclusters(const velo_decode_retinaclusters_params* p, hipStream_t stream);
+
 /* ---- velo_sort_by_phi_t (device/velo/search_by_triplet/include/SortByPhi.cuh:14-28) */
 typedef struct {
   unsigned host_number_of_events;
@@ -277,6 +332,8 @@ typedef struct {
   int bank_version;
   float phi_tolerance, max_scatter, max_chi2;
   unsigned max_skipped_modules;
+  int retina;                      /* the banks are retina clusters (type 73): the retina decoding
+                                      replaces candidates, estimate, masked clustering and sort */
 } velo_sequence_input;
 int velo_sequence_create(const velo_constants* c, unsigned max_events, size_t max_raw_bytes, velo_sequence** out);
 int velo_sequence_destroy(velo_sequence* s);

@@ -34,29 +34,48 @@ def filled_slots(offsets_est: np.ndarray, counts: np.ndarray) -> np.ndarray:
     return np.concatenate(idx) if idx else np.zeros(0, np.int64)
 
 
+def is_retina(ref) -> bool:
+    """Whether the oracle's run decoded retina cluster banks."""
+    try:
+        ref["each_sensor_pair_size"]
+        return True
+    except KeyError:
+        return False
+
+
 def gpu_buffers(seq: velo.VeloSequence, ref) -> dict:
     """Fetches the GPU sequence's buffers with the sizes of the oracle's run `ref`."""
-    n_events = ref["number_of_candidates"].size
+    n_events = ref["number_of_velo_tracks"].size
+    retina = is_retina(ref)
     out = {}
 
     def f(name, dtype, count):
         out[name] = seq.fetch(name, dtype, int(count))
 
-    for name in ["number_of_candidates", "module_candidate_num", "number_of_velo_tracks",
-                 "number_of_three_hit_tracks_output"]:
+    for name in ["number_of_velo_tracks", "number_of_three_hit_tracks_output"]:
         f(name, np.uint32, n_events)
-    for name in ["candidates_offsets", "offsets_velo_tracks", "offsets_three_hit_tracks_filtered",
-                 "offsets_all_velo_tracks"]:
+    for name in ["offsets_velo_tracks", "offsets_three_hit_tracks_filtered", "offsets_all_velo_tracks"]:
         f(name, np.uint32, n_events + 1)
-    f("velo_bank_index", np.uint32, 208)
-    f("estimated_input_size", np.uint32, N_MP * n_events)
     f("offsets_estimated_input_size", np.uint32, N_MP * n_events + 1)
     f("module_cluster_num", np.uint32, N_MP * n_events)
     total = int(ref["offsets_estimated_input_size"][-1])
-    for name in ["velo_cluster_container", "sorted_velo_cluster_container"]:
-        f(name, np.uint8, 18 * total)
-    f("cluster_candidates", np.uint32, ref["cluster_candidates"].size)
-    f("hit_permutation", np.uint32, total)
+    f("sorted_velo_cluster_container", np.uint8, 18 * total)
+    if retina:
+        slots = ref["retina_bank_index"].size
+        f("each_sensor_pair_size", np.uint32, slots * n_events)
+        f("offsets_each_sensor_pair_size", np.uint32, slots * n_events + 1)
+        f("retina_bank_index", np.uint32, slots)
+        f("hit_permutations", np.uint32, total)
+        f("hit_sorting_key", np.int64, total)
+    else:
+        for name in ["number_of_candidates", "module_candidate_num"]:
+            f(name, np.uint32, n_events)
+        f("candidates_offsets", np.uint32, n_events + 1)
+        f("velo_bank_index", np.uint32, 208)
+        f("estimated_input_size", np.uint32, N_MP * n_events)
+        f("velo_cluster_container", np.uint8, 18 * total)
+        f("cluster_candidates", np.uint32, ref["cluster_candidates"].size)
+        f("hit_permutation", np.uint32, total)
     f("tracks", TRACK, ref["tracks"].size)
     f("tracklets", TRACKLET, ref["tracklets"].size)
     f("tracks_to_follow", np.uint32, ref["tracks_to_follow"].size)
@@ -93,20 +112,32 @@ def _mismatch(name, a, b, limit=5):
 def compare(gpu: dict, ref, fused: bool) -> list[str]:
     """Returns the list of mismatch descriptions (empty when everything agrees)."""
     errs = []
-    n_events = ref["number_of_candidates"].size
+    n_events = ref["number_of_velo_tracks"].size
+    retina = is_retina(ref)
 
     def eq(name, a=None, b=None):
         m = _mismatch(name, gpu[name] if a is None else a, ref[name] if b is None else b)
         if m:
             errs.append(m)
 
-    for name in ["number_of_candidates", "candidates_offsets", "velo_bank_index", "estimated_input_size",
-                 "offsets_estimated_input_size", "module_cluster_num"]:
+    names = ["each_sensor_pair_size", "offsets_each_sensor_pair_size", "retina_bank_index", "hit_sorting_key",
+             "hit_permutations"] if retina else ["number_of_candidates", "candidates_offsets", "velo_bank_index",
+                                                 "estimated_input_size"]
+    for name in names + ["offsets_estimated_input_size", "module_cluster_num"]:
         eq(name)
     est = ref["offsets_estimated_input_size"]
     total = int(est[-1])
     counts = ref["module_cluster_num"]
     slots = filled_slots(est, counts)
+    if retina:
+        # Module pairs with zero words also fill the slot after their clusters (the last zero
+        # word decoded), as the reference does.
+        zero = (np.diff(est.astype(np.int64)) > counts)
+        extra = (est[:-1].astype(np.int64) + counts)[zero]
+        g_ids, g_xyz, g_phi = cluster_views(gpu["sorted_velo_cluster_container"], total)
+        r_ids, r_xyz, r_phi = cluster_views(ref["sorted_velo_cluster_container"], total)
+        eq("sorted.zero_word_slot", np.stack([g_ids[extra], g_xyz[extra].view("<u4")[:, 0]]),
+           np.stack([r_ids[extra], r_xyz[extra].view("<u4")[:, 0]]))
 
     # Sorted container: filled slots equal, bit for bit.
     g_ids, g_xyz, g_phi = cluster_views(gpu["sorted_velo_cluster_container"], total)
@@ -115,7 +146,7 @@ def compare(gpu: dict, ref, fused: bool) -> list[str]:
     eq("sorted.phi", g_phi[slots], r_phi[slots])
     eq("sorted.xyz", g_xyz[slots].view("<u4"), r_xyz[slots].view("<u4"))
 
-    if not fused:
+    if not fused and not retina:
         eq("module_candidate_num")
         # Candidate lists: same multiset per event.
         co = ref["candidates_offsets"]

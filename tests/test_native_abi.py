@@ -22,7 +22,8 @@ def test_header_declares_the_entry_points():
     for required in ["velo_calculate_number_of_candidates", "velo_estimate_input_size", "velo_masked_clustering",
                      "velo_sort_by_phi", "velo_search_by_triplet", "velo_three_hit_tracks_filter",
                      "velo_copy_track_hit_number", "velo_consolidate_tracks", "velo_kalman_filter",
-                     "velo_prefix_sum", "velo_cache_sp_patterns", "velo_sequence_run"]:
+                     "velo_prefix_sum", "velo_cache_sp_patterns", "velo_sequence_run",
+                     "velo_calculate_number_of_retinaclusters_each_sensor_pair", "velo_decode_retinaclusters"]:
         assert required in names
 
 
@@ -52,6 +53,8 @@ def test_params_structs_match_header_layout():
                "velo_consolidate_tracks_params": velo.ConsolidateParams,
                "velo_kalman_filter_params": velo.KalmanParams,
                "velo_prefix_sum_params": velo.PrefixSumParams,
+               "velo_calculate_number_of_retinaclusters_params": velo.RetinaSizesParams,
+               "velo_decode_retinaclusters_params": velo.RetinaDecodeParams,
                "velo_sequence_input": velo.SequenceInput}
     for cname, py in mirrors.items():
         body = structs[cname]
