@@ -302,7 +302,7 @@ def main():
         kern_ms, kern_runs = seq.stage_ms()
         kern_avg_ms = kern_ms / kern_runs
         seq.time_stage(None)
-    for sq in seqs[1:]:
+    for sq in seqs[1:min(n_streams, args.steps)]:  # the sequences the timed steps ran
         if physics_of(sq, ev) != physics:
             raise RuntimeError("a second stream's slice gave other tracks than the first")
     ms_per_step = total_ms / args.steps
@@ -316,7 +316,7 @@ def main():
     achieved = dom_bytes / (kern_avg_ms / 1e3) / 1e9
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": measured_traffic(args.workload, dominant),
+                "traffic": measured_traffic(args.workload + ("_retina" if args.decoding == "retina" else ""), dominant),
                 "algorithmic_bytes": dom_bytes, "avg_launch_ms": round(kern_avg_ms, 4)}
     # search_by_triplet is bounded by lane arithmetic, not bytes: 9 VALU ops per scored candidate,
     # ~12 per extrapolation (hit_phi_16), 3 per binary-search iteration.
