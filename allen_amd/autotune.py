@@ -10,6 +10,8 @@ Each candidate is timed with the sequence's own stage events (HIP events on the 
 
 Results are cached in a JSON file keyed by the device name and a signature of the workload (event count and hit-count distribution), so a later process with a similar workload reuses them.
 
+`tune_throughput` tunes for a different regime: several sequences in flight on their own streams (or one batch much larger than the GPU holds at once), where the whole job's events per second, not one launch's latency, is the objective. There the best search workgroup is smaller than for a single slice (more events resident per CU), so it times whole steps with a caller-supplied measurement.
+
 Usage::
 
     from allen_amd import autotune
@@ -44,6 +46,12 @@ class Result:
     def to_json(self) -> dict:
         return {"knobs": self.knobs, "search_ms": self.search_ms, "default_search_ms": self.default_search_ms,
                 "trials": self.trials}
+
+
+# Candidates of the throughput tuning: workgroup size x LDS hit budget (the budget sets how many
+# workgroups fit a CU's LDS, the size how many lanes an event gets).
+THROUGHPUT_CANDIDATES = tuple({"search_block": b, "search_lds_hits": h}
+                              for b in (128, 192, 256) for h in (512, 800, 1200, 1760))
 
 
 def default_knobs() -> dict:
@@ -91,6 +99,34 @@ def workload_signature(dev: velo.DeviceEvents, seq: velo.VeloSequence) -> str:
 def _device_name() -> str:
     import torch
     return torch.cuda.get_device_name(0)
+
+
+def tune_throughput(seqs, measure, n_events: int, candidates=THROUGHPUT_CANDIDATES, log=None) -> Result:
+    """Applies each candidate to every sequence of `seqs`, calls `measure()` (milliseconds per
+    step with the knobs in place, running every sequence), and keeps the fastest whose outputs
+    (first sequence, last run) equal the default configuration's byte for byte. Applies the
+    winner to every sequence and returns it; `search_ms` holds milliseconds per step here."""
+    def apply_all(knobs):
+        for sq in seqs:
+            apply(sq, knobs)
+
+    base = {"search_block": 256, "search_lds_hits": 0}
+    apply_all(base)
+    default_ms = measure()
+    reference = _outputs_digest(seqs[0], n_events)
+    best_knobs, best_ms, trials = dict(base), default_ms, []
+    for knobs in candidates:
+        apply_all(knobs)
+        ms = measure()
+        if _outputs_digest(seqs[0], n_events) != reference:
+            raise velo.VeloError(f"knobs {knobs} changed the outputs of the sequence")
+        trials.append({**knobs, "ms_per_step": round(ms, 4)})
+        if log:
+            log(f"[autotune] {knobs} -> {ms:.4f} ms per step")
+        if ms < best_ms:
+            best_knobs, best_ms = dict(knobs), ms
+    apply_all(best_knobs)
+    return Result(best_knobs, best_ms, default_ms, trials)
 
 
 def tune(seq: velo.VeloSequence, dev: velo.DeviceEvents, reps: int = 3, cache: str | None = None,

@@ -189,6 +189,44 @@ def test_autotune_keeps_outputs(geometry, beamline, constants):
     seq.close()
 
 
+def test_throughput_autotune_on_two_streams(geometry, beamline, constants):
+    """The throughput tuner over two sequences on two streams: every candidate keeps the outputs,
+    the winner is applied to both, and both then reproduce the oracle."""
+    import torch
+    from allen_amd import autotune
+    ev = synth.generate(**DATASETS["pileup"])
+    print(f"[autotune-throughput] events={ev.n_events}", flush=True)
+    dev = velo.DeviceEvents(ev)
+    seqs = [velo.VeloSequence(constants, ev.n_events, ev.raw.size) for _ in range(2)]
+    streams = [torch.cuda.Stream() for _ in range(2)]
+
+    def measure():
+        torch.cuda.synchronize()
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        main = torch.cuda.current_stream()
+        start.record(main)
+        for st in streams:
+            st.wait_event(start)
+        for i in range(4):
+            seqs[i % 2].run(dev, stream=streams[i % 2])
+        for st in streams:
+            main.wait_stream(st)
+        end.record(main)
+        torch.cuda.synchronize()
+        return start.elapsed_time(end) / 4
+
+    cands = ({"search_block": 128, "search_lds_hits": 512}, {"search_block": 192, "search_lds_hits": 800})
+    res = autotune.tune_throughput(seqs, measure, ev.n_events, candidates=cands)
+    assert len(res.trials) == 2 and res.search_ms <= res.default_search_ms
+    ref = oracle.run(ev, geometry, beamline)
+    for sq, st in zip(seqs, streams):
+        sq.run(dev, stream=st)
+        torch.cuda.synchronize()
+        errs = parity.compare(parity.gpu_buffers(sq, ref), ref, True)
+        assert not errs, "\n".join(errs[:20])
+        sq.close()
+
+
 def test_dense_module_pairs_take_the_tiled_paths(geometry, beamline, constants):
     """Events dense enough that module pairs exceed the per-wave LDS slices of clustering."""
     DATASETS["dense"] = dict(n_events=4, pileup=1, nu=60.0, noise_fraction=0.3, seed=17)
