@@ -150,6 +150,10 @@ SIGNATURES = {
     "velo_estimate_input_size": (I, [ctypes.POINTER(EstimateParams), P]),
     "velo_masked_clustering": (I, [ctypes.POINTER(MaskedClusteringParams), P]),
     "velo_retina_sensor_slots": (U, [I]),
+    "velo_mdf_read": (I, [ctypes.c_char_p, U, U, ctypes.POINTER(P)]),
+    "velo_mdf_events_info": (I, [P, ctypes.POINTER(U), ctypes.POINTER(I), ctypes.POINTER(ctypes.c_size_t)]),
+    "velo_mdf_events_buffer": (I, [P, ctypes.c_char_p, ctypes.POINTER(P), ctypes.POINTER(ctypes.c_size_t)]),
+    "velo_mdf_events_free": (None, [P]),
     "velo_calculate_number_of_retinaclusters_each_sensor_pair": (I, [ctypes.POINTER(RetinaSizesParams), P]),
     "velo_decode_retinaclusters": (I, [ctypes.POINTER(RetinaDecodeParams), P]),
     "velo_sort_by_phi": (I, [ctypes.POINTER(SortParams), P]),

@@ -37,7 +37,8 @@ enum {
   VELO_ERR_BANK_VERSION = 2,    /* like StrException in EstimateInputSize.cu:265-268 */
   VELO_ERR_CAPACITY = 3,        /* a run exceeds what a velo_sequence was created for */
   VELO_ERR_HIP = 4,
-  VELO_ERR_UNSUPPORTED_LAYOUT = 5 /* mep_layout != 0 is not supported */
+  VELO_ERR_UNSUPPORTED_LAYOUT = 5, /* mep_layout != 0, or a compressed MDF record */
+  VELO_ERR_IO = 6                  /* a file cannot be read or is not a well-formed MDF file */
 };
 
 /* Device status bits written by the kernels when a fixed-capacity structure of the
@@ -71,6 +72,19 @@ const float* velo_constants_dev_beamline(const velo_constants* c);
 void velo_cache_sp_patterns(uint8_t* patterns, uint8_t* sizes, float* fx, float* fy);
 /* Host copy of the super-pixel tables (ClusteringCommon.cpp:6-104): 256 + 256 B, 2 x 512 floats. */
 int velo_constants_sp_tables(const velo_constants* c, uint8_t* patterns, uint8_t* sizes, float* fx, float* fy);
+
+/* ---- MDF ingestion (host only; no device needed). Reads the raw banks of one type (63: VELO
+ * super pixels, 73: retina clusters; as Allen's provider, main/src/Provider.cpp:240-248, takes one
+ * of the two encodings) from an uncompressed MDF file (mdf/include/mdf_header.hpp:45-220,
+ * mdf/lhcb/Event/RawBank.h:196-250) into Allen's event layout (main/src/Transpose.cpp:220-498,
+ * banks sorted by sourceID): the "raw", "offsets", "sizes" and "types" buffers that
+ * velo_sequence_input and the single algorithms take, in host memory owned by the handle.
+ * max_events = 0 reads every record. All banks of the type must share one version. */
+typedef struct velo_mdf_events velo_mdf_events;
+int velo_mdf_read(const char* path, unsigned bank_type, unsigned max_events, velo_mdf_events** out);
+int velo_mdf_events_info(const velo_mdf_events* e, unsigned* n_events, int* bank_version, size_t* raw_bytes);
+int velo_mdf_events_buffer(const velo_mdf_events* e, const char* name, const void** host_ptr, size_t* bytes);
+void velo_mdf_events_free(velo_mdf_events* e);
 
 /* ---- host_prefix_sum_t (host/prefix_sum/src/HostPrefixSum.cpp:76-107), as a device scan:
  * dev_output_buffer[0] = 0, dev_output_buffer[i+1] = sum(dev_input_buffer[0..i]) (u32 wrap).
