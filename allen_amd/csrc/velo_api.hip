@@ -302,6 +302,33 @@ int velo_decode_retinaclusters(const velo_decode_retinaclusters_params* p, hipSt
   return hip_status(velo::launch_retina_decode(a, stream));
 }
 
+int velo_pv_beamline(const velo_pv_beamline_params* p, hipStream_t stream)
+{
+  if (!p || !p->constants || !p->dev_offsets_all_velo_tracks || !p->dev_velo_kalman_beamline_states ||
+      !p->dev_number_of_events || !p->dev_pvtracks || !p->dev_zpeaks || !p->dev_number_of_zpeaks ||
+      !p->dev_pvtracks_denom || !p->dev_multi_fit_vertices || !p->dev_number_of_multi_fit_vertices ||
+      !p->dev_multi_final_vertices || !p->dev_number_of_multi_final_vertices)
+    return VELO_ERR_INVALID_ARGUMENT;
+  velo::PVLaunch a;
+  a.event_list = p->dev_event_list;
+  a.n_list = p->dev_event_list ? p->event_list_size : p->host_number_of_events;
+  a.number_of_events = p->dev_number_of_events;
+  a.offsets_all_velo_tracks = p->dev_offsets_all_velo_tracks;
+  a.beamline_states = p->dev_velo_kalman_beamline_states;
+  a.beamline = p->constants->dev_beamline;
+  a.pvtracks = reinterpret_cast<velo::PVTrackOut*>(p->dev_pvtracks);
+  a.zhisto = p->dev_zhisto;
+  a.zpeaks = p->dev_zpeaks;
+  a.number_of_zpeaks = p->dev_number_of_zpeaks;
+  a.pvtracks_denom = p->dev_pvtracks_denom;
+  a.multi_fit_vertices = static_cast<velo::PVVertexOut*>(p->dev_multi_fit_vertices);
+  a.number_of_multi_fit_vertices = p->dev_number_of_multi_fit_vertices;
+  a.multi_final_vertices = static_cast<velo::PVVertexOut*>(p->dev_multi_final_vertices);
+  a.number_of_multi_final_vertices = p->dev_number_of_multi_final_vertices;
+  a.status = p->dev_status;
+  return hip_status(velo::launch_pv_beamline(a, stream));
+}
+
 int velo_estimate_input_size(const velo_estimate_input_size_params* p, hipStream_t stream)
 {
   if (!p || !p->dev_estimated_input_size || !p->dev_module_candidate_num) return VELO_ERR_INVALID_ARGUMENT;
@@ -518,6 +545,7 @@ struct velo_sequence {
   bool scorer_on = false;         // buffer "scorer_params" holds the scorer's parameters
   float scorer_threshold = 0.f;
   unsigned kalman_fp64_fallback = 0;
+  unsigned pv_finder = 0;         // 1: PV finder after the fit; 2: also the z histograms ("zhisto")
   bool fused_clustering = true;
   bool count_work = false;
   // HIP events bracketing one stage of the run (0 = none), see velo_sequence_stage_ms.
@@ -670,6 +698,25 @@ int velo_sequence_set_tuning(velo_sequence* s, const char* knob, unsigned value)
   if (k == "search_lds_hits") {
     if (value > 8192) return VELO_ERR_INVALID_ARGUMENT;
     s->search_lds_hits = value;
+    return VELO_OK;
+  }
+  if (k == "pv_finder") {
+    // Buffers of the PV finder, made on first use (the z histograms only for value 2).
+    if (value > 2) return VELO_ERR_INVALID_ARGUMENT;
+    const size_t E = s->max_events, T = 2 * s->track_cap;
+    const std::vector<std::pair<const char*, size_t>> layout = {
+      {"pvtracks", 28 * T}, {"zpeaks", 4 * VELO_PV_MAX_VERTICES * E}, {"number_of_zpeaks", 4 * E},
+      {"pvtracks_denom", 4 * T}, {"multi_fit_vertices", 48 * VELO_PV_MAX_VERTICES * E},
+      {"number_of_multi_fit_vertices", 4 * E}, {"multi_final_vertices", 48 * VELO_PV_MAX_VERTICES * E},
+      {"number_of_multi_final_vertices", 4 * E}, {"zhisto", value == 2 ? 4 * VELO_PV_HISTO_BINS * E : 0}};
+    for (const auto& kv : layout) {
+      if (!kv.second || s->bufs.count(kv.first)) continue;
+      void* p = nullptr;
+      const int rc = hip_status(hipMalloc(&p, kv.second));
+      if (rc != VELO_OK) return rc;
+      s->bufs[kv.first] = {p, kv.second};
+    }
+    s->pv_finder = value;
     return VELO_OK;
   }
   if (k == "kalman_fp64_fallback") {
@@ -1049,6 +1096,28 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream
   VELO_MARK(mark_begin(VELO_STAGE_KALMAN));
   VELO_TRY(velo::launch_kalman_filter(k9, stream));
   VELO_MARK(mark_end(VELO_STAGE_KALMAN));
+  if (s->pv_finder) {
+    velo::PVLaunch pv;
+    pv.event_list = nullptr;
+    pv.n_list = E;
+    pv.number_of_events = n_events_dev;
+    pv.offsets_all_velo_tracks = k7.offsets_all_velo_tracks;
+    pv.beamline_states = s->get<float>("velo_kalman_beamline_states");
+    pv.beamline = s->constants->dev_beamline;
+    pv.pvtracks = s->get<velo::PVTrackOut>("pvtracks");
+    pv.zhisto = s->pv_finder == 2 ? s->get<float>("zhisto") : nullptr;
+    pv.zpeaks = s->get<float>("zpeaks");
+    pv.number_of_zpeaks = s->get<uint32_t>("number_of_zpeaks");
+    pv.pvtracks_denom = s->get<float>("pvtracks_denom");
+    pv.multi_fit_vertices = s->get<velo::PVVertexOut>("multi_fit_vertices");
+    pv.number_of_multi_fit_vertices = s->get<uint32_t>("number_of_multi_fit_vertices");
+    pv.multi_final_vertices = s->get<velo::PVVertexOut>("multi_final_vertices");
+    pv.number_of_multi_final_vertices = s->get<uint32_t>("number_of_multi_final_vertices");
+    pv.status = status;
+    VELO_MARK(mark_begin(VELO_STAGE_PV));
+    VELO_TRY(velo::launch_pv_beamline(pv, stream));
+    VELO_MARK(mark_end(VELO_STAGE_PV));
+  }
   VELO_MARK(mark_end(VELO_STAGE_ALL));
 #undef VELO_MARK
 #undef VELO_TRY

@@ -61,6 +61,37 @@ unsigned retina_sensor_slots(int version);
 hipError_t launch_retina_sensor_sizes(const RetinaSizesLaunch& a, hipStream_t s);
 hipError_t launch_retina_decode(const RetinaDecodeLaunch& a, hipStream_t s);
 
+// Beamline PV finder (velo_pv.hip). PVTrack (TrackBeamLineVertexFinder.cuh:14-31) with plain
+// floats, PV::Vertex (PV_Definitions.cuh:64-97), 48 bytes.
+struct PVTrackOut {
+  float z, x, y, tx, ty, W_00, W_11;
+};
+struct PVVertexOut {
+  float x, y, z, chi2;
+  int32_t ndof;
+  float cov00, cov10, cov11, cov20, cov21, cov22, nTracks;
+};
+static_assert(sizeof(PVVertexOut) == 48, "layout");
+struct PVLaunch {
+  const uint32_t* event_list; // NULL = identity
+  unsigned n_list;
+  const uint32_t* number_of_events;
+  const uint32_t* offsets_all_velo_tracks;
+  const float* beamline_states;
+  const float* beamline;
+  PVTrackOut* pvtracks;
+  float* zhisto; // optional
+  float* zpeaks;
+  uint32_t* number_of_zpeaks;
+  float* pvtracks_denom;
+  PVVertexOut* multi_fit_vertices;
+  uint32_t* number_of_multi_fit_vertices;
+  PVVertexOut* multi_final_vertices;
+  uint32_t* number_of_multi_final_vertices;
+  uint32_t* status; // optional
+};
+hipError_t launch_pv_beamline(const PVLaunch& a, hipStream_t s);
+
 struct EstimateLaunch {
   int version;
   const char* raw;

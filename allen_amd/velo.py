@@ -62,6 +62,18 @@ MaskedClusteringParams = _params("MaskedClusteringParams", [
     ("dev_number_of_events", P), ("dev_velo_bank_index", P), ("dev_module_cluster_num", P),
     ("dev_velo_cluster_container", P), ("constants", P), ("event_start", U), ("mep_layout", I)])
 
+PVParams = _params("PVParams", [
+    ("host_number_of_events", U), ("dev_event_list", P), ("event_list_size", U), ("dev_number_of_events", P),
+    ("dev_offsets_all_velo_tracks", P), ("dev_velo_kalman_beamline_states", P), ("dev_pvtracks", P),
+    ("dev_zhisto", P), ("dev_zpeaks", P), ("dev_number_of_zpeaks", P), ("dev_pvtracks_denom", P),
+    ("dev_multi_fit_vertices", P), ("dev_number_of_multi_fit_vertices", P), ("dev_multi_final_vertices", P),
+    ("dev_number_of_multi_final_vertices", P), ("dev_status", P), ("constants", P)])
+
+# PV::Vertex (PV_Definitions.cuh:64-97).
+PV_VERTEX_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("chi2", "<f4"), ("ndof", "<i4"),
+                            ("cov00", "<f4"), ("cov10", "<f4"), ("cov11", "<f4"), ("cov20", "<f4"), ("cov21", "<f4"),
+                            ("cov22", "<f4"), ("nTracks", "<f4")])
+
 RetinaSizesParams = _params("RetinaSizesParams", [
     ("host_number_of_events", U), ("host_raw_bank_version", I), ("dev_event_list", P), ("event_list_size", U),
     ("dev_velo_retina_raw_input", P), ("dev_velo_retina_raw_input_offsets", P),
@@ -150,6 +162,7 @@ SIGNATURES = {
     "velo_estimate_input_size": (I, [ctypes.POINTER(EstimateParams), P]),
     "velo_masked_clustering": (I, [ctypes.POINTER(MaskedClusteringParams), P]),
     "velo_retina_sensor_slots": (U, [I]),
+    "velo_pv_beamline": (I, [ctypes.POINTER(PVParams), P]),
     "velo_mdf_read": (I, [ctypes.c_char_p, U, U, ctypes.POINTER(P)]),
     "velo_mdf_events_info": (I, [P, ctypes.POINTER(U), ctypes.POINTER(I), ctypes.POINTER(ctypes.c_size_t)]),
     "velo_mdf_events_buffer": (I, [P, ctypes.c_char_p, ctypes.POINTER(P), ctypes.POINTER(ctypes.c_size_t)]),
@@ -180,7 +193,7 @@ SIGNATURES = {
 }
 
 STAGES = {"all": 1, "candidates": 2, "estimate": 3, "clustering": 4, "search": 5, "three_hit_filter": 6,
-          "consolidate": 7, "kalman": 8}
+          "consolidate": 7, "kalman": 8, "pv": 9}
 
 _lib = None
 

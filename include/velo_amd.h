@@ -331,6 +331,39 @@ typedef struct {
 } velo_kalman_filter_params;
 int velo_kalman_filter(const velo_kalman_filter_params* p, hipStream_t stream);
 
+/* ---- Beamline primary-vertex finder, the consumer of the beamline states: the six algorithms
+ * pv_beamline_extrapolate_t, pv_beamline_histo_t, pv_beamline_peak_t,
+ * pv_beamline_calculate_denom_t, pv_beamline_multi_fitter_t and pv_beamline_cleanup_t
+ * (device/PV/beamlinePV/include/pv_beamline_*.cuh, wired in
+ * configuration/python/AllenConf/primary_vertex_reconstruction.py) in one call, one workgroup per
+ * event with the z histogram in LDS. Outputs carry the reference's names and layouts: PVTrack is
+ * 7 floats (z, x, y, tx, ty, W_00, W_11), PV::Vertex 48 bytes (x, y, z, chi2, int ndof, cov00,
+ * cov10, cov11, cov20, cov21, cov22, nTracks), 32 per event; dev_zhisto (3392 floats per event)
+ * is written only when not NULL. Capacities the reference leaves unchecked (extrema, clusters,
+ * vertices) raise VELO_STATUS_PV_OVERFLOW in dev_status (optional). */
+#define VELO_PV_MAX_VERTICES 32
+#define VELO_PV_HISTO_BINS 3392
+#define VELO_STATUS_PV_OVERFLOW (1u << 16)
+typedef struct {
+  unsigned host_number_of_events;
+  const unsigned* dev_event_list; unsigned event_list_size;
+  const unsigned* dev_number_of_events;
+  const unsigned* dev_offsets_all_velo_tracks;
+  const float* dev_velo_kalman_beamline_states;
+  float* dev_pvtracks;                        /* 7 floats per track */
+  float* dev_zhisto;                          /* optional: 3392 per event */
+  float* dev_zpeaks;                          /* 32 per event */
+  unsigned* dev_number_of_zpeaks;
+  float* dev_pvtracks_denom;                  /* 1 per track */
+  void* dev_multi_fit_vertices;               /* 32 x 48 bytes per event */
+  unsigned* dev_number_of_multi_fit_vertices;
+  void* dev_multi_final_vertices;             /* 32 x 48 bytes per event */
+  unsigned* dev_number_of_multi_final_vertices;
+  unsigned* dev_status;                       /* optional */
+  const velo_constants* constants;            /* beamline position */
+} velo_pv_beamline_params;
+int velo_pv_beamline(const velo_pv_beamline_params* p, hipStream_t stream);
+
 /* ---- Whole sequence: the nine algorithms and five scans above, enqueued back to back on one
  * stream with every size read on the device (no host synchronisation inside a run), so the run
  * can be captured in a hipGraph.  Clustering and the phi sort are fused per module pair.
@@ -396,7 +429,8 @@ enum {
   VELO_STAGE_THREE_HIT_FILTER = 6, /* velo_three_hit_tracks_filter */
   VELO_STAGE_CONSOLIDATE = 7,      /* velo_consolidate_tracks */
   VELO_STAGE_KALMAN = 8,           /* velo_kalman_filter */
-  VELO_STAGE_COUNT = 8
+  VELO_STAGE_PV = 9,               /* the beamline PV finder (knob "pv_finder") */
+  VELO_STAGE_COUNT = 9
 };
 /* Sum of the elapsed milliseconds of the timed stage over the runs since "timed_stage" was
  * last set, and the number of those runs (waits for them). */

@@ -38,6 +38,7 @@ class VoInputs(ctypes.Structure):
         ("kalman_fp64_fallback", ctypes.c_int),
         ("collect_seeds", ctypes.c_int),
         ("retina", ctypes.c_int),
+        ("pv_finder", ctypes.c_int),
     ]
 
 
@@ -101,7 +102,12 @@ class Result:
         "search_stats": np.uint64,
         "seed_features": np.float32,
         "hit_sorting_key": np.int64,
+        "pvtracks": np.float32, "zhisto": np.float32, "zpeaks": np.float32, "pvtracks_denom": np.float32,
+        "multi_fit_vertices": np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("chi2", "<f4"), ("ndof", "<i4"),
+                                        ("cov00", "<f4"), ("cov10", "<f4"), ("cov11", "<f4"), ("cov20", "<f4"),
+                                        ("cov21", "<f4"), ("cov22", "<f4"), ("nTracks", "<f4")]),
     }
+    DTYPES["multi_final_vertices"] = DTYPES["multi_fit_vertices"]
 
     def __init__(self, handle, keep):
         self._h = handle

@@ -68,6 +68,7 @@ struct Params {
   bool f64 = true;
   bool kalman_fp64_fallback = false;
   bool retina = false;
+  bool pv_finder = false;
 };
 
 void run_sequence(
@@ -180,6 +181,20 @@ void run_sequence(
   double* ev64 = prm.f64 ? s.make<double>("velo_kalman_endvelo_states_f64", (size_t) nt * 14) : nullptr;
   kalman_filter(el, nl, n_events, off_all, off_hits, track_hits, beamline, bwd, bl, ev, bl64, ev64, prm.kalman_fp64_fallback);
 
+  if (prm.pv_finder) {
+    // primary_vertex_reconstruction.py: pv_beamline_extrapolate ... pv_beamline_cleanup
+    auto* pvtracks = s.make<PVTrack>("pvtracks", nt);
+    auto* zhisto = s.make<float>("zhisto", (size_t) pv_nbins * n_events);
+    auto* zpeaks = s.make<float>("zpeaks", (size_t) pv_max_vertices * n_events);
+    auto* nzp = s.make<uint32_t>("number_of_zpeaks", n_events);
+    auto* denom = s.make<float>("pvtracks_denom", nt);
+    auto* fit = s.make<PVVertex>("multi_fit_vertices", (size_t) pv_max_vertices * n_events);
+    auto* nfit = s.make<uint32_t>("number_of_multi_fit_vertices", n_events);
+    auto* fin = s.make<PVVertex>("multi_final_vertices", (size_t) pv_max_vertices * n_events);
+    auto* nfin = s.make<uint32_t>("number_of_multi_final_vertices", n_events);
+    pv_beamline(n_events, off_all, bl, beamline, pvtracks, zhisto, zpeaks, nzp, denom, fit, nfit, fin, nfin, s.status);
+  }
+
   auto* st = s.make<uint64_t>("search_stats", 4);
   st[0] = s.stats.seed_evaluations;
   st[1] = s.stats.forward_evaluations;
@@ -225,6 +240,7 @@ bool make_inputs(const vo_inputs* a, vo::RawInput& in, vo::Geometry& g, vo::SpTa
   p.f64 = a->compute_f64 != 0;
   p.kalman_fp64_fallback = a->kalman_fp64_fallback != 0;
   p.retina = a->retina != 0;
+  p.pv_finder = a->pv_finder != 0;
   return true;
 }
 } // namespace

@@ -151,6 +151,35 @@ void three_hit_tracks_filter(
 
 uint32_t prefix_sum(const uint32_t* input, size_t n, uint32_t* output);
 
+// Beamline PV finder (pv.cpp). PVTrack (TrackBeamLineVertexFinder.cuh:14-31) and PV::Vertex
+// (PV_Definitions.cuh:64-97), both plain floats (48-byte vertex).
+struct PVTrack {
+  float z, x, y, tx, ty, W_00, W_11;
+};
+struct PVVertex {
+  float x, y, z, chi2;
+  int32_t ndof;
+  float cov00, cov10, cov11, cov20, cov21, cov22, nTracks;
+};
+static_assert(sizeof(PVVertex) == 48, "layout");
+constexpr uint32_t pv_status_overflow = 1u << 16;
+constexpr unsigned pv_nbins = 3392, pv_max_vertices = 32;
+void pv_beamline(
+  unsigned n_events,
+  const uint32_t* offsets_all_velo_tracks,
+  const float* beamline_states,
+  const float* beamline,
+  PVTrack* pvtracks,
+  float* zhisto,
+  float* zpeaks,
+  uint32_t* number_of_zpeaks,
+  float* pvtracks_denom,
+  PVVertex* multi_fit_vertices,
+  uint32_t* number_of_multi_fit_vertices,
+  PVVertex* multi_final_vertices,
+  uint32_t* number_of_multi_final_vertices,
+  uint32_t& status);
+
 void copy_track_hit_number(
   unsigned n_events,
   const uint32_t* offsets_estimated_input_size,

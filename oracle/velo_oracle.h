@@ -32,6 +32,7 @@ typedef struct {
                                  and "seed_features" (f32 x 10) */
   int retina;                 /* the banks are retina clusters (type 73): decode_retinaclusters replaces
                                  the candidates, estimate, masked clustering and phi sort */
+  int pv_finder;              /* also run the beamline PV finder on the beamline states */
 } vo_inputs;
 
 void vo_default_inputs(vo_inputs* a);

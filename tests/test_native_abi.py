@@ -55,6 +55,7 @@ def test_params_structs_match_header_layout():
                "velo_prefix_sum_params": velo.PrefixSumParams,
                "velo_calculate_number_of_retinaclusters_params": velo.RetinaSizesParams,
                "velo_decode_retinaclusters_params": velo.RetinaDecodeParams,
+               "velo_pv_beamline_params": velo.PVParams,
                "velo_sequence_input": velo.SequenceInput}
     for cname, py in mirrors.items():
         body = structs[cname]

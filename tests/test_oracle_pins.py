@@ -639,3 +639,69 @@ def test_retina_synthetic_tracking_efficiency(geometry, beamline):
     res = checker.check_buffers(ev, r["offsets_all_velo_tracks"], r["offsets_velo_track_hit_number"],
                                 r["velo_track_hits"])
     assert res.efficiency > 0.99 and res.ghost_rate < 0.005
+
+
+# ---------------------------------------------------------------------------- beamline PV finder
+def _windowed_tracks(pvt, beamline):
+    """Tracks the histogram takes (pv_beamline_histo.cu:40-62), restated in numpy."""
+    z, x, y, tx, ty, w0, w1 = pvt.T.astype(np.float64)
+    ok = (-541.0 < z) & (z < 307.0)
+    ok &= (x - beamline[0]) ** 2 * w0 + (y - beamline[1]) ** 2 * w1 < 10.0
+    ok &= 1.0 / np.sqrt(tx * tx * w0 + ty * ty * w1) < np.where(z < -341.0, 10.0, 1.5)
+    return ok
+
+
+def test_pv_histogram_is_a_sum_of_unit_gaussians(geometry, beamline):
+    """Each accepted track adds increments that telescope to exactly one (the integral of its
+    approximate Gaussian over its window), all non-negative, centred on its z: the histogram's
+    total is the number of accepted tracks, and its first moment their mean z."""
+    ev = synth.generate(n_events=30, pileup=1, nu=7.6, seed=82)
+    r = oracle.run(ev, geometry, beamline, pv_finder=1)
+    h = r["zhisto"].reshape(ev.n_events, 3392).astype(np.float64)
+    pvt = r["pvtracks"].reshape(-1, 7)
+    off = r["offsets_all_velo_tracks"].astype(np.int64)
+    centres = -541.0 + (np.arange(3392) + 0.5) * 0.25
+    assert h.min() >= 0.0
+    for e in range(ev.n_events):
+        t = pvt[off[e]:off[e + 1]]
+        ok = _windowed_tracks(t, beamline)
+        assert abs(h[e].sum() - ok.sum()) < 1e-3
+        if ok.sum():
+            mean_z = (h[e] * centres).sum() / h[e].sum()
+            assert abs(mean_z - t[ok, 0].astype(np.float64).mean()) < 0.2
+
+
+def test_pv_finder_finds_the_generated_vertices(geometry, beamline):
+    """Pile-up events: generated vertices with at least five reconstructible tracks are found
+    within 1 mm in z (94 %), with a median |dz| below 0.1 mm, on the beamline, and almost every
+    reconstructed vertex sits on a generated one."""
+    ev = synth.generate(n_events=60, pileup=1, nu=7.6, seed=83)
+    r = oracle.run(ev, geometry, beamline, pv_finder=1)
+    n = ev.n_events
+    nfin = r["number_of_multi_final_vertices"]
+    fin = r["multi_final_vertices"].reshape(n, 32)
+    p, po = ev.particles, ev.particle_offsets
+    found, total, dz, fakes, n_rec = 0, 0, [], 0, 0
+    for e in range(n):
+        pe = p[po[e]:po[e + 1]]
+        sel = (pe["n_modules"] >= 3) & (pe["eta"] > 2) & (pe["eta"] < 5)
+        pvs, counts = np.unique(pe["pv"][sel], return_counts=True)
+        zt = np.array([pe["z0"][pe["pv"] == v][0] for v in pvs[counts >= 5]], np.float64)
+        zall = np.unique(pe["z0"]).astype(np.float64)
+        rec = fin[e][:nfin[e]]
+        n_rec += rec.size
+        total += zt.size
+        for z in zt:
+            d = np.abs(rec["z"] - z)
+            if d.size and d.min() < 1.0:
+                found += 1
+                dz.append(d.min())
+        for v in rec:
+            if zall.size == 0 or np.abs(zall - v["z"]).min() > 1.0:
+                fakes += 1
+            assert v["x"] ** 2 + v["y"] ** 2 < 0.3
+    assert total > 300
+    assert found / total > 0.9
+    assert np.median(dz) < 0.1
+    assert fakes < 0.03 * n_rec
+    assert r["status"][0] == 0
