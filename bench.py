@@ -88,6 +88,9 @@ def main():
     ap.add_argument("--decoding", choices=["super_pixel", "retina"], default="super_pixel",
                     help="raw banks as super pixels (masked clustering) or as retina clusters made by the "
                          "front-end (decode_retinaclusters), Allen's two VELO decodings")
+    ap.add_argument("--pv-finder", action="store_true",
+                    help="also run the beamline primary-vertex finder on the beamline states (the path's consumer; "
+                         "the cpu baseline then runs it too)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores (capped at 16)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
@@ -159,6 +162,8 @@ def main():
     for sq in seqs:
         for k, v in knobs.items():
             sq.set_tuning(k, v)
+        if args.pv_finder:
+            sq.set_tuning("pv_finder", 1)
     stream = torch.cuda.current_stream()
     streams = [stream] if n_streams == 1 and not args.graph else [torch.cuda.Stream() for _ in range(n_streams)]
 
@@ -240,7 +245,8 @@ def main():
 
     # Per-stage times (diagnostic, outside the timed region) to find the dominant kernel.
     stage_ms = {}
-    for stage in ["candidates", "estimate", "clustering", "search", "three_hit_filter", "consolidate", "kalman"]:
+    for stage in ["candidates", "estimate", "clustering", "search", "three_hit_filter", "consolidate", "kalman"] + \
+            (["pv"] if args.pv_finder else []):
         seq.time_stage(stage)
         for _ in range(2):
             seq.run(dev)
@@ -344,9 +350,9 @@ def main():
 
         def timed(n_threads, seconds):
             # One calibration pass, then enough repetitions of the sample for about `seconds` of CPU work.
-            _, s1 = oracle.throughput(sample, geometry, beamline, n_threads, 1)
+            _, s1 = oracle.throughput(sample, geometry, beamline, n_threads, 1, pv_finder=int(args.pv_finder))
             reps = max(1, int(round(seconds / max(s1, 1e-3))))
-            eps, secs = oracle.throughput(sample, geometry, beamline, n_threads, reps)
+            eps, secs = oracle.throughput(sample, geometry, beamline, n_threads, reps, pv_finder=int(args.pv_finder))
             return eps, secs, reps
 
         eps, secs, reps = timed(threads, args.cpu_seconds)
@@ -372,6 +378,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded generator, real VELO geometry, v4 super-pixel banks)",
         "config": {"workload": args.workload, "events_per_step": n, "decoding": args.decoding,
+                   "pv_finder": bool(args.pv_finder),
                    "path": "nn_pruned" if args.scorer else "plain",
                    **({"scorer": os.path.basename(args.scorer), "threshold": args.threshold} if args.scorer else {}),
                    "launch": "hip_graph" if args.graph else "eager", "streams": n_streams,

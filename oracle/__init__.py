@@ -157,8 +157,8 @@ def run(events, geometry: bytes, beamline: np.ndarray, **props) -> Result:
     return Result(h, keep)
 
 
-def throughput(events, geometry: bytes, beamline: np.ndarray, n_threads: int, repetitions: int = 1):
-    a, keep = make_inputs(events, geometry, beamline)
+def throughput(events, geometry: bytes, beamline: np.ndarray, n_threads: int, repetitions: int = 1, **props):
+    a, keep = make_inputs(events, geometry, beamline, **props)
     secs = ctypes.c_double()
     eps = lib().vo_throughput(ctypes.byref(a), n_threads, repetitions, ctypes.byref(secs))
     del keep
