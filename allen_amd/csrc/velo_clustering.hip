@@ -893,8 +893,8 @@ hipError_t launch_cluster_sorted(const FusedClusterLaunch& a, hipStream_t s)
 // larger than the slice runs the same code on global memory.
 namespace mp {
 constexpr unsigned waves_per_block = 4;
-constexpr unsigned word_cap = 448;
-constexpr unsigned cluster_cap = 320;
+constexpr unsigned word_cap = 384;
+constexpr unsigned cluster_cap = 224;
 
 // Banks of one module pair (at most 8), kept in the wave's LDS slice.
 struct Banks {
@@ -1069,10 +1069,17 @@ __global__ void __launch_bounds__(mp::waves_per_block* wave_size) k_estimate_mp(
 // The wave-private LDS slices of k_cluster_mp.
 struct ClusterSlices {
   uint32_t* lw;
-  int16_t* lphi;
-  uint32_t* lid;
+  uint64_t* lkey;
   float* lxyz;
 };
+
+// (phi, id) as one unsigned 64-bit key: phi (signed) in the high word, offset to unsigned. Key
+// order is the sort's (phi, id) order (SortByPhi.cu:90-125), and distinct clusters of a module
+// pair have distinct ids, so the rank of a cluster is the number of keys below its own.
+__device__ __forceinline__ uint64_t phi_id_key(int16_t phi, uint32_t id)
+{
+  return (static_cast<uint64_t>(static_cast<uint16_t>(phi) ^ 0x8000u) << 32) | id;
+}
 
 // K3 + K4 of one module pair (idx = event * 26 + module pair) by one wave.
 template<int version>
@@ -1103,8 +1110,7 @@ __device__ __forceinline__ void cluster_item(
   uint32_t* lw = sl.lw;
   if (resident) mp::load_words(B, 0, n_words, lw);
   const Clusters scratch {unsorted_scratch, total};
-  int16_t* lphi = sl.lphi;
-  uint32_t* lid = sl.lid;
+  uint64_t* lkey = sl.lkey;
   float* lxyz = sl.lxyz;
   unsigned n = 0; // clusters so far (wave-uniform)
   // Appends up to two clusters per lane in lane order (wave scan of the per-lane counts).
@@ -1116,8 +1122,7 @@ __device__ __forceinline__ void cluster_item(
       const ClusterOut& c = j == 0 ? c0 : c1;
       if (slot < estimate) {
         if (clusters_in_lds) {
-          lphi[slot] = c.phi;
-          lid[slot] = c.id;
+          lkey[slot] = phi_id_key(c.phi, c.id);
           lxyz[3 * slot] = c.x;
           lxyz[3 * slot + 1] = c.y;
           lxyz[3 * slot + 2] = c.z;
@@ -1185,13 +1190,16 @@ __device__ __forceinline__ void cluster_item(
   const Clusters out {sorted_container, total};
   if (clusters_in_lds) {
     for (unsigned i = lane_id(); i < n; i += wave_size) {
+      const uint64_t key = lkey[i];
       ClusterOut c;
-      c.id = lid[i];
-      c.phi = lphi[i];
+      c.id = static_cast<uint32_t>(key);
+      c.phi = static_cast<int16_t>(static_cast<uint16_t>(key >> 32) ^ 0x8000u);
       c.x = lxyz[3 * i];
       c.y = lxyz[3 * i + 1];
       c.z = lxyz[3 * i + 2];
-      store_cluster(out, start + phi_rank(lphi, lid, n, i), c);
+      unsigned rank = 0;
+      for (unsigned j = 0; j < n; ++j) rank += lkey[j] < key;
+      store_cluster(out, start + rank, c);
     }
     return;
   }
@@ -1213,16 +1221,12 @@ __device__ __forceinline__ void cluster_item(
     for (unsigned t0 = 0; t0 < n; t0 += mp::cluster_cap) {
       const unsigned tn = min(mp::cluster_cap, n - t0);
       mp::wave_sync_lds();
-      for (unsigned j = lane_id(); j < tn; j += wave_size) {
-        lphi[j] = scratch.phis()[start + t0 + j];
-        lid[j] = scratch.ids()[start + t0 + j];
-      }
+      for (unsigned j = lane_id(); j < tn; j += wave_size)
+        lkey[j] = phi_id_key(scratch.phis()[start + t0 + j], scratch.ids()[start + t0 + j]);
       mp::wave_sync_lds();
       if (mine) {
-        for (unsigned j = 0; j < tn; ++j) {
-          const int16_t pj = lphi[j];
-          rank += c.phi > pj || (i != t0 + j && c.phi == pj && c.id > lid[j]);
-        }
+        const uint64_t key = phi_id_key(c.phi, c.id);
+        for (unsigned j = 0; j < tn; ++j) rank += lkey[j] < key;
       }
     }
     if (mine) store_cluster(out, start + rank, c);
@@ -1232,7 +1236,7 @@ __device__ __forceinline__ void cluster_item(
 // Resident grid: each wave takes items idx, idx + stride, ... and fetches the next item's
 // bank directory record before working on the current one.
 template<int version>
-__global__ void __launch_bounds__(mp::waves_per_block* wave_size) k_cluster_mp(
+__global__ void __launch_bounds__(mp::waves_per_block* wave_size, 6) k_cluster_mp(
   const char* raw,
   const BankEntry* directory,
   const uint32_t* directory_count,
@@ -1248,8 +1252,7 @@ __global__ void __launch_bounds__(mp::waves_per_block* wave_size) k_cluster_mp(
   char* unsorted_scratch)
 {
   __shared__ uint32_t s_words[mp::waves_per_block][mp::word_cap];
-  __shared__ int16_t s_phi[mp::waves_per_block][mp::cluster_cap];
-  __shared__ uint32_t s_id[mp::waves_per_block][mp::cluster_cap];
+  __shared__ uint64_t s_key[mp::waves_per_block][mp::cluster_cap];
   __shared__ float s_xyz[mp::waves_per_block][3 * mp::cluster_cap];
   __shared__ mp::Banks s_banks[mp::waves_per_block];
   const unsigned wave = threadIdx.x / wave_size;
@@ -1258,7 +1261,7 @@ __global__ void __launch_bounds__(mp::waves_per_block* wave_size) k_cluster_mp(
   unsigned idx = blockIdx.x * mp::waves_per_block + wave;
   if (idx >= n_items) return;
   const unsigned total = offsets_estimated_input_size[n_items];
-  const ClusterSlices sl {s_words[wave], s_phi[wave], s_id[wave], s_xyz[wave]};
+  const ClusterSlices sl {s_words[wave], s_key[wave], s_xyz[wave]};
   mp::DirFetch cur = mp::fetch_dir(directory, directory_count, idx);
   for (; idx < n_items; idx += stride) {
     mp::DirFetch next = cur;
