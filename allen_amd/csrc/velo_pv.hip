@@ -1,17 +1,23 @@
 // Beamline primary-vertex finder for gfx950: the consumer of the VELO beamline states
 // (device/PV/beamlinePV/src/pv_beamline_{extrapolate,histo,peak,calculate_denom,multi_fitter,
 // cleanup}.cu). The reference runs six kernels with the z histogram in HBM and float atomics;
-// here one workgroup per event runs them all with the histogram in LDS:
+// here one workgroup per event runs them all with the histogram in LDS, and every floating-point
+// sum keeps the CPU target's order, so the histogram and the seeds are bit-identical to it:
 //
-//   1. tracks: PVTrack (z, x, y, tx, ty, 1/c00, 1/c11) of each track, and its histogram window
-//   2. histogram: each lane owns 14 consecutive bins and adds, track by track in track order,
-//      the Gaussian-integral increments the reference adds with atomics, so every bin sums its
-//      terms in the CPU target's order (no atomics, identical sums)
-//   3. peaks: the CPU target's protocluster scan and extrema split (pv_beamline_peak.cu, the
-//      branch without warp intrinsics) on one lane, reading the histogram from LDS
+//   1. tracks: PVTrack (z, x, y, tx, ty, 1/c00, 1/c11) of each track
+//   2. histogram: lane t owns the bins t, t+256, ...; a track's window (about 30 consecutive bins)
+//      spreads over as many lanes, and each lane adds, track by track in track order, the
+//      Gaussian-integral increments the reference adds with atomics (no atomics, identical sums)
+//   3. peaks: the CPU target's protocluster scan (pv_beamline_peak.cu, the branch without warp
+//      intrinsics) in three steps: the empty/non-empty transitions found by all lanes and ranked
+//      by a block scan; the integral of each candidate protocluster summed in bin order by one
+//      lane each; the extrema split of each kept protocluster by one lane each, its seeds ranked
+//      by a block scan
 //   4. denominators: one lane per track, seeds in order
-//   5. multi-fitter: one lane per seed, tracks in order (the CPU target's sums), vertices kept in
-//      seed order by a ballot
+//   5. multi-fitter: the seeds iterate in lockstep; per iteration all lanes compute the (seed,
+//      track) weights and their weighted terms into LDS, one lane per (seed, sum) adds the terms
+//      in track order (the CPU target's sums), and one lane per seed solves and updates it;
+//      vertices kept in seed order by a ballot
 //   6. cleanup: one lane per vertex, unique vertices kept in order by a ballot
 //
 // Capacities the reference does not check (extrema per protocluster, clusters, vertices) are
@@ -34,17 +40,24 @@ constexpr float chi2CutExp = 0.135335283f, maxDeltaZConverged = 0.0005f;
 constexpr unsigned pp_minNumTracksPerVertex = 4, SMOG2_minNumTracksPerVertex = 3;
 constexpr float minChi2Dist = 25.f;
 // PV_Definitions.cuh
-constexpr unsigned max_vertices = 32, max_clusters = 200, max_clusteredges = 200, max_subclusters = 50;
-constexpr unsigned block = 256, bins_per_lane = (nbins + block - 1) / block; // 14
-constexpr unsigned track_chunk = 1024; // histogram windows staged per pass
+constexpr unsigned max_vertices = 32, max_clusteredges = 200, max_subclusters = 50;
+constexpr unsigned max_protoclusters = max_clusteredges / 2;
+
+constexpr unsigned block = 256, block_waves = block / wave_size;
+constexpr unsigned bins_per_lane = (nbins + block - 1) / block; // 14 consecutive bins per lane in the transition scan
+constexpr unsigned hist_words = block * bins_per_lane;          // 3584
+constexpr unsigned pool_words = 8192;  // LDS shared by the phases (32 KiB)
+constexpr unsigned track_chunk = 1024; // histogram windows staged per pass, three words each
+constexpr unsigned n_terms = 11;       // per (seed, track): ten weighted terms and the selection flag
 constexpr uint32_t status_overflow = 1u << 16;
+static_assert(hist_words + 3 * track_chunk <= pool_words, "histogram staging");
+static_assert(hist_words + (nbins + 1) / 2 + max_clusteredges / 2 <= pool_words, "transition list");
+static_assert(block >= max_protoclusters && wave_size >= max_vertices, "one lane per protocluster and per seed");
+static_assert((block & (block - 1)) == 0, "bin ownership uses a mask");
 
 struct Extremum {
   unsigned short index;
   float value, integral;
-};
-struct Cluster {
-  unsigned short izfirst, izlast, izmax;
 };
 
 __device__ __forceinline__ float gauss_integral(float x)
@@ -53,6 +66,27 @@ __device__ __forceinline__ float gauss_integral(float x)
   const float xi = x / a;
   const float eta = 1.f - xi * xi;
   return 0.5f + xi * (0.5f + eta * (0.25f + eta * 0.1875f));
+}
+
+// pv_beamline_peak.cu: a bin is empty below dz / (10 maxTrackZ0Err); the scan starts "empty".
+__device__ __forceinline__ bool bin_empty(const float* h, unsigned i)
+{
+  if (i == 0) return true;
+  const float zBin = zmin + i * dz;
+  const float Z0Err = zBin < SMOG2_pp_separation ? SMOG2_maxTrackZ0Err : pp_maxTrackZ0Err;
+  const float inv_maxTrackZ0Err = 1.f / (10.f * Z0Err);
+  const float threshold = dz * inv_maxTrackZ0Err;
+  return h[i] < threshold;
+}
+
+// The seed position of a cluster from its highest bin and the two next to it.
+__device__ __forceinline__ float seed_position(const float* h, unsigned short izmax)
+{
+  const float* b = h + izmax;
+  const float d1 = *b - *(b - 1);
+  const float d2 = *b - *(b + 1);
+  const float idz = d1 + d2 > 0 ? 0.5f * (d1 - d2) / (d1 + d2) : 0.0f;
+  return zmin + dz * (izmax + idz + 0.5f);
 }
 } // namespace pv
 
@@ -75,14 +109,16 @@ __global__ void __launch_bounds__(pv::block) k_pv_beamline(
   uint32_t* number_of_multi_final_vertices,
   uint32_t* status)
 {
-  __shared__ float h[block * bins_per_lane];
-  __shared__ uint32_t s_window[track_chunk]; // minbin | maxbin << 16, or 0 for no window
-  __shared__ float s_zerr[track_chunk];
-  __shared__ float s_z[track_chunk];
+  // pool: the histogram (phases 2-3) followed by the window staging (2) or the transitions and
+  // protocluster edges (3); then the track chunks and their (seed, track) terms (5).
+  __shared__ float pool[pool_words];
   __shared__ float s_zpeaks[max_vertices];
-  __shared__ unsigned s_npeaks;
+  __shared__ float4 s_seed[max_vertices]; // seeds not converged: vx, vy, vz, seed z
+  __shared__ float s_sum[n_terms * max_vertices];
   __shared__ PVVertexOut s_fit[max_vertices];
-  __shared__ unsigned s_nfit;
+  __shared__ uint32_t s_scan[block_waves + 1];
+  __shared__ unsigned s_nact, s_nfit;
+  float* const h = pool;
   const unsigned t = threadIdx.x, l = lane_id();
   const unsigned event_number = event_list ? event_list[blockIdx.x] : blockIdx.x;
   const unsigned total = offsets_all_velo_tracks[*number_of_events];
@@ -105,116 +141,136 @@ __global__ void __launch_bounds__(pv::block) k_pv_beamline(
     p.W_11 = 1.f / c[3];
     pvtracks[t0 + i] = p;
   }
-  for (unsigned b = t; b < block * bins_per_lane; b += block) h[b] = 0.f;
+  for (unsigned b = t; b < hist_words; b += block) h[b] = 0.f;
   __syncthreads(); // the tracks are read back by other lanes below (global memory, same workgroup)
 
   // 2. Histogram (pv_beamline_histo.cu:28-86), chunks of tracks in order.
-  const unsigned lo = t * bins_per_lane, hi = min(lo + bins_per_lane, static_cast<unsigned>(nbins));
-  for (unsigned c0 = 0; c0 < nt; c0 += track_chunk) {
-    const unsigned cn = min(track_chunk, nt - c0);
-    for (unsigned i = t; i < cn; i += block) {
-      const PVTrackOut trk = pvtracks[t0 + c0 + i];
-      uint32_t window = 0;
-      float zerr = 0.f;
-      if (zmin < trk.z && trk.z < zmax) {
-        const float diffx2 = (trk.x - blx) * (trk.x - blx);
-        const float diffy2 = (trk.y - bly) * (trk.y - bly);
-        const float blchi2 = diffx2 * trk.W_00 + diffy2 * trk.W_11;
-        if (blchi2 < maxTrackBLChi2) {
-          const float zbin = (trk.z - zmin) / dz;
-          const float zweight = trk.tx * trk.tx * trk.W_00 + trk.ty * trk.ty * trk.W_11;
-          zerr = 1.f / sqrtf(zweight);
-          const float maxTrackZ0Err = trk.z < SMOG2_pp_separation ? SMOG2_maxTrackZ0Err : pp_maxTrackZ0Err;
-          if (zerr < maxTrackZ0Err) {
-            const float a = sqrtf(float(2 * order_polynomial + 3));
-            const float halfwindow = a * zerr / dz;
-            const int minbin = max(int(zbin - halfwindow), 1);
-            const int maxbin = min(int(zbin + halfwindow), nbins - 2);
-            if (maxbin >= minbin) window = static_cast<uint32_t>(minbin) | (static_cast<uint32_t>(maxbin) << 16);
+  {
+    uint32_t* const s_window = reinterpret_cast<uint32_t*>(pool + hist_words); // minbin | maxbin << 16, 0: none
+    float* const s_zerr = pool + hist_words + track_chunk;
+    float* const s_z = pool + hist_words + 2 * track_chunk;
+    for (unsigned c0 = 0; c0 < nt; c0 += track_chunk) {
+      const unsigned cn = min(track_chunk, nt - c0);
+      for (unsigned i = t; i < cn; i += block) {
+        const PVTrackOut trk = pvtracks[t0 + c0 + i];
+        uint32_t window = 0;
+        float zerr = 0.f;
+        if (zmin < trk.z && trk.z < zmax) {
+          const float diffx2 = (trk.x - blx) * (trk.x - blx);
+          const float diffy2 = (trk.y - bly) * (trk.y - bly);
+          const float blchi2 = diffx2 * trk.W_00 + diffy2 * trk.W_11;
+          if (blchi2 < maxTrackBLChi2) {
+            const float zbin = (trk.z - zmin) / dz;
+            const float zweight = trk.tx * trk.tx * trk.W_00 + trk.ty * trk.ty * trk.W_11;
+            zerr = 1.f / sqrtf(zweight);
+            const float maxTrackZ0Err = trk.z < SMOG2_pp_separation ? SMOG2_maxTrackZ0Err : pp_maxTrackZ0Err;
+            if (zerr < maxTrackZ0Err) {
+              const float a = sqrtf(float(2 * order_polynomial + 3));
+              const float halfwindow = a * zerr / dz;
+              const int minbin = max(int(zbin - halfwindow), 1);
+              const int maxbin = min(int(zbin + halfwindow), nbins - 2);
+              if (maxbin >= minbin) window = static_cast<uint32_t>(minbin) | (static_cast<uint32_t>(maxbin) << 16);
+            }
           }
         }
+        s_window[i] = window;
+        s_zerr[i] = zerr;
+        s_z[i] = trk.z;
       }
-      s_window[i] = window;
-      s_zerr[i] = zerr;
-      s_z[i] = trk.z;
-    }
-    __syncthreads();
-    for (unsigned i = 0; i < cn; ++i) {
-      const uint32_t window = s_window[i];
-      if (!window) continue;
-      const unsigned minbin = window & 0xFFFFu, maxbin = window >> 16;
-      if (maxbin < lo || minbin >= hi) continue;
-      const float zerr = s_zerr[i], z = s_z[i];
-      const unsigned b0 = max(minbin, lo), b1 = min(maxbin, hi - 1);
-      for (unsigned b = b0; b <= b1; ++b) {
-        // The increment of bin b: the integral up to its upper edge (1 for the last bin)
-        // minus the integral up to its lower edge (0 for the first bin).
-        const float upper = b < maxbin ? gauss_integral((zmin + (int) (b + 1) * dz - z) / zerr) : 1.f;
-        const float lower = b > minbin ? gauss_integral((zmin + (int) b * dz - z) / zerr) : 0.f;
-        h[b] += upper - lower;
+      __syncthreads();
+      for (unsigned i = 0; i < cn; ++i) {
+        const uint32_t window = s_window[i];
+        if (!window) continue;
+        const unsigned minbin = window & 0xFFFFu, maxbin = window >> 16;
+        unsigned b = minbin + ((t - minbin) & (block - 1)); // this lane's first bin in the window
+        if (b > maxbin) continue;
+        const float zerr = s_zerr[i], z = s_z[i];
+        for (; b <= maxbin; b += block) {
+          // The increment of bin b: the integral up to its upper edge (1 for the last bin)
+          // minus the integral up to its lower edge (0 for the first bin).
+          const float upper = b < maxbin ? gauss_integral((zmin + (int) (b + 1) * dz - z) / zerr) : 1.f;
+          const float lower = b > minbin ? gauss_integral((zmin + (int) b * dz - z) / zerr) : 0.f;
+          h[b] += upper - lower;
+        }
       }
+      __syncthreads();
     }
-    __syncthreads();
   }
   if (zhisto_out)
     for (unsigned b = t; b < nbins; b += block) zhisto_out[(size_t) nbins * event_number + b] = h[b];
 
-  // 3. Peaks (pv_beamline_peak.cu:23-236, the CPU target's branch), on one lane.
-  if (t == 0) {
-    unsigned short clusteredges[max_clusteredges];
-    int number_of_clusteredges = 0;
-    bool prevempty = true;
-    float integral = h[0];
-    for (unsigned short i = 1; i < nbins; ++i) {
-      const float zBin = zmin + i * dz;
-      const float Z0Err = zBin < SMOG2_pp_separation ? SMOG2_maxTrackZ0Err : pp_maxTrackZ0Err;
-      const float inv_maxTrackZ0Err = 1.f / (10.f * Z0Err);
-      const float threshold = dz * inv_maxTrackZ0Err;
-      const float v = h[i];
-      integral += v;
-      const bool empty = v < threshold;
-      if (empty != prevempty) {
-        const float minInSeed = zBin < SMOG2_pp_separation ? SMOG2_minTracksInSeed : pp_minTracksInSeed;
-        if (prevempty || integral > minInSeed) {
-          if (number_of_clusteredges < (int) max_clusteredges) clusteredges[number_of_clusteredges] = i;
-          else st |= status_overflow;
-          number_of_clusteredges++;
-        }
-        else {
-          number_of_clusteredges--;
-        }
-        prevempty = empty;
-        integral = v;
+  // 3. Peaks (pv_beamline_peak.cu:23-236, the CPU target's branch).
+  uint16_t* const trans = reinterpret_cast<uint16_t*>(pool + hist_words);
+  uint16_t* const edges = reinterpret_cast<uint16_t*>(pool + hist_words + (nbins + 1) / 2);
+  // 3a. The bins where the scan flips between empty and non-empty, in bin order. They alternate,
+  // the first one opening a protocluster (the scan starts empty).
+  uint32_t n_trans;
+  {
+    const unsigned lo = t * bins_per_lane, hi = min(lo + bins_per_lane, static_cast<unsigned>(nbins));
+    uint32_t bits = 0;
+    if (lo < hi) {
+      bool prev = bin_empty(h, lo == 0 ? 0 : lo - 1);
+      for (unsigned i = max(lo, 1u); i < hi; ++i) {
+        const bool e = bin_empty(h, i);
+        if (e != prev) bits |= 1u << (i - lo);
+        prev = e;
       }
     }
-    number_of_clusteredges = min(number_of_clusteredges, (int) max_clusteredges);
-    unsigned number_of_clusters = 0;
-    float* zp = zpeaks + (size_t) max_vertices * event_number;
-    // Only the first max_vertices clusters become seeds; the cluster mean needs izmax alone.
-    auto add_cluster = [&](unsigned short izmax) {
-      if (number_of_clusters < max_vertices) {
-        const float* b = h + izmax;
-        const float d1 = *b - *(b - 1);
-        const float d2 = *b - *(b + 1);
-        const float idz = d1 + d2 > 0 ? 0.5f * (d1 - d2) / (d1 + d2) : 0.0f;
-        const float z = zmin + dz * (izmax + idz + 0.5f);
-        zp[number_of_clusters] = z;
-        s_zpeaks[number_of_clusters] = z;
+    unsigned k = block_exclusive_scan(__popc(bits), s_scan, &n_trans);
+    for (; bits; bits &= bits - 1u, ++k) trans[k] = static_cast<uint16_t>(lo + __ffs(bits) - 1);
+  }
+  __syncthreads();
+  // 3b. Candidate m runs from transition 2m to transition 2m+1. The scan keeps it when the
+  // histogram summed over it in bin order (the integral the scan carries) exceeds the seed
+  // minimum at its end, and stores the edges of the first 100 kept; a protocluster opened after
+  // those raises the overflow bit.
+  unsigned n_kept = 0;
+  for (unsigned m0 = 0; m0 < n_trans / 2; m0 += block) {
+    const unsigned m = m0 + t;
+    bool keep = false;
+    unsigned ibegin = 0, iend = 0;
+    if (m < n_trans / 2) {
+      ibegin = trans[2 * m];
+      iend = trans[2 * m + 1];
+      float integral = h[ibegin];
+      for (unsigned i = ibegin + 1; i <= iend; ++i) integral += h[i];
+      const float zBin = zmin + iend * dz;
+      const float minInSeed = zBin < SMOG2_pp_separation ? SMOG2_minTracksInSeed : pp_minTracksInSeed;
+      keep = integral > minInSeed;
+    }
+    uint32_t kept;
+    const unsigned rank = n_kept + block_flag_scan(keep, s_scan, &kept);
+    if (keep) {
+      if (rank < max_protoclusters) {
+        edges[2 * rank] = static_cast<uint16_t>(ibegin);
+        edges[2 * rank + 1] = static_cast<uint16_t>(iend);
       }
-      else {
-        st |= status_overflow;
-      }
-      ++number_of_clusters;
+      if (rank == max_protoclusters - 1 && 2 * (m + 1) < n_trans) st |= status_overflow;
+    }
+    n_kept += kept;
+  }
+  __syncthreads();
+  const unsigned n_proto = min(n_kept, max_protoclusters);
+  // 3c. Lane p splits protocluster p at its extrema; its seeds follow those of the protoclusters
+  // before it.
+  uint32_t n_clusters;
+  {
+    const float mindip = minDipDensity * dz;
+    const float minpeak = minDensity * dz;
+    Extremum extrema[max_vertices + 1];
+    int N = 0;
+    unsigned n_sub = 0, n_seeds = 0;
+    auto subcluster_kept = [&](int i) {
+      const float z_extrema = zmin + extrema[2 * i].index * dz;
+      const float minInSeed = z_extrema < SMOG2_pp_separation ? SMOG2_minTracksInSeed : pp_minTracksInSeed;
+      return extrema[2 * i].integral - extrema[2 * i - 2].integral > minInSeed;
     };
-    const unsigned Nproto = number_of_clusteredges / 2;
-    for (unsigned short p = 0; p < Nproto; ++p) {
-      const unsigned short ibegin = clusteredges[p * 2];
-      const unsigned short iend = clusteredges[p * 2 + 1];
-      const float mindip = minDipDensity * dz;
-      const float minpeak = minDensity * dz;
-      // One entry more than the capacity, zeroed: the split below reads extrema[N] when N is even
-      // (one past the last extremum, unset in the reference); here it reads zeros.
-      Extremum extrema[max_vertices + 1] = {};
+    if (t < n_proto) {
+      // One entry more than the capacity, zeroed: the split below reads extrema[N] when N is
+      // even (one past the last extremum, unset in the reference); here it reads zeros.
+      for (unsigned i = 0; i <= max_vertices; ++i) extrema[i] = Extremum {0, 0.f, 0.f};
+      const unsigned short ibegin = edges[2 * t];
+      const unsigned short iend = edges[2 * t + 1];
       int number_of_extrema = 0;
       auto push = [&](unsigned short index, float value, float integ) {
         if (number_of_extrema < (int) max_vertices) extrema[number_of_extrema++] = Extremum {index, value, integ};
@@ -253,35 +309,43 @@ __global__ void __launch_bounds__(pv::block) k_pv_beamline(
         integ += value;
       }
       push(iend, h[iend], integ);
-      const int N = number_of_extrema;
-      unsigned number_of_subclusters = 0;
-      if (N > 3) {
-        for (int i = 1; i < (N / 2) + 1; ++i) {
-          const float z_extrema = zmin + extrema[2 * i].index * dz;
-          const float minInSeed = z_extrema < SMOG2_pp_separation ? SMOG2_minTracksInSeed : pp_minTracksInSeed;
-          if (extrema[2 * i].integral - extrema[2 * i - 2].integral > minInSeed) {
-            if (number_of_subclusters < max_subclusters) {
-              ++number_of_subclusters;
-              if (number_of_clusters < max_clusters) add_cluster(extrema[2 * i - 1].index);
-              else st |= status_overflow;
-            }
-            else {
-              st |= status_overflow;
-            }
+      N = number_of_extrema;
+      if (N > 3)
+        for (int i = 1; i < (N / 2) + 1; ++i)
+          if (subcluster_kept(i)) {
+            if (n_sub < max_subclusters) ++n_sub;
+            else st |= status_overflow;
           }
-        }
-      }
-      if (number_of_subclusters == 0 && extrema[1].value >= minpeak) {
-        if (number_of_clusters < max_clusters) add_cluster(extrema[1].index);
-        else st |= status_overflow;
-      }
+      n_seeds = n_sub > 0 ? n_sub : (extrema[1].value >= minpeak ? 1u : 0u);
     }
-    const unsigned n_peaks = min(number_of_clusters, max_vertices);
-    number_of_zpeaks[event_number] = n_peaks;
-    s_npeaks = n_peaks;
+    unsigned k = block_exclusive_scan(n_seeds, s_scan, &n_clusters);
+    float* zp = zpeaks + (size_t) max_vertices * event_number;
+    auto add_cluster = [&](unsigned short izmax) {
+      if (k < max_vertices) {
+        const float z = seed_position(h, izmax);
+        zp[k] = z;
+        s_zpeaks[k] = z;
+      }
+      else {
+        st |= status_overflow;
+      }
+      ++k;
+    };
+    if (n_sub > 0) {
+      unsigned taken = 0;
+      for (int i = 1; i < (N / 2) + 1 && taken < n_sub; ++i)
+        if (subcluster_kept(i)) {
+          ++taken;
+          add_cluster(extrema[2 * i - 1].index);
+        }
+    }
+    else if (n_seeds) {
+      add_cluster(extrema[1].index);
+    }
   }
+  const unsigned n_peaks = min(n_clusters, max_vertices);
+  if (t == 0) number_of_zpeaks[event_number] = n_peaks;
   __syncthreads();
-  const unsigned n_peaks = s_npeaks;
 
   // 4. Denominators (pv_beamline_calculate_denom.cu:24-46).
   for (unsigned i = t; i < nt; i += block) {
@@ -297,103 +361,170 @@ __global__ void __launch_bounds__(pv::block) k_pv_beamline(
   }
   __syncthreads();
 
-  // 5. Multi-fitter (pv_beamline_multi_fitter.cu:28-185): lane s of wave 0 fits seed s.
-  if (t < wave_size) {
-    bool ok = false;
-    PVVertexOut vertex {};
-    if (l < n_peaks) {
-      bool converged = false, accept = true;
-      float vtxcov[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-      float vx = blx, vy = bly;
-      const float seed_z = s_zpeaks[l];
-      float vz = seed_z;
-      float chi2tot = 0.f, sum_weights = 0.f;
-      unsigned nselected = 0;
-      const unsigned minTracks = seed_z <= SMOG2_pp_separation ? SMOG2_minNumTracksPerVertex : pp_minNumTracksPerVertex;
-      for (unsigned iter = 0; iter < maxFitIter && !converged; ++iter) {
-        float a00 = 0.f, a11 = 0.f, a20 = 0.f, a21 = 0.f, a22 = 0.f;
-        float g0 = 0.f, g1 = 0.f, g2 = 0.f;
-        float local_chi2tot = 0.f, local_sum_weights = 0.f;
-        nselected = 0;
-        for (unsigned i = 0; i < nt; ++i) {
-          const PVTrackOut trk = pvtracks[t0 + i];
-          if (trk.z < zmin || trk.z >= zmax) continue;
-          const float H_20 = -trk.tx, H_21 = -trk.ty;
-          const float HWH_20 = H_20 * trk.W_00, HWH_21 = H_21 * trk.W_11;
-          const float HWH_22 = H_20 * H_20 * trk.W_00 + H_21 * H_21 * trk.W_11;
-          const float dzv = vz - trk.z;
-          const float rx = vx - (trk.x + trk.tx * dzv), ry = vy - (trk.y + trk.ty * dzv);
-          const float chi2 = rx * rx * trk.W_00 + ry * ry * trk.W_11;
+  // 5. Multi-fitter (pv_beamline_multi_fitter.cu:28-185). Lane s of wave 0 holds seed s; the
+  // seeds not yet converged iterate together.
+  bool converged = true, accept = false;
+  float vtxcov[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  float vx = blx, vy = bly, vz = 0.f, seedz = 0.f, chi2tot = 0.f, sum_weights = 0.f;
+  unsigned nselected = 0, minTracks = 0;
+  if (t < n_peaks) {
+    converged = false;
+    accept = true;
+    seedz = s_zpeaks[t];
+    vz = seedz;
+    minTracks = seedz <= SMOG2_pp_separation ? SMOG2_minNumTracksPerVertex : pp_minNumTracksPerVertex;
+  }
+  for (unsigned iter = 0; iter < maxFitIter; ++iter) {
+    unsigned my_j = 0;
+    if (t < wave_size) {
+      const uint64_t m = __ballot(!converged);
+      my_j = __popcll(m & lanes_below_mask());
+      if (!converged) s_seed[my_j] = make_float4(vx, vy, vz, seedz);
+      if (l == 0) s_nact = __popcll(m);
+    }
+    __syncthreads();
+    const unsigned S = s_nact, rows = n_terms * S;
+    if (S == 0) break;
+    // Tracks per chunk: eight track words and `rows` terms per track fit the pool. Odd, so that
+    // the row stride (the chunk size made odd) stays within it and the rows start in distinct
+    // banks.
+    unsigned C = pool_words / (8 + rows);
+    C -= (C & 1u) ^ 1u;
+    float acc0 = 0.f, acc1 = 0.f;
+    for (unsigned c0 = 0; c0 < nt; c0 += C) {
+      const unsigned cn = min(C, nt - c0), cs = cn | 1u;
+      float* const tw = pool;            // z, x, y, tx, ty, W_00, W_11, denominator: rows of cs
+      float* const term = pool + 8 * cs; // row k * S + j: term k of seed j, one column per track
+      for (unsigned i = t; i < cn; i += block) {
+        const PVTrackOut p = pvtracks[t0 + c0 + i];
+        tw[i] = p.z;
+        tw[cs + i] = p.x;
+        tw[2 * cs + i] = p.y;
+        tw[3 * cs + i] = p.tx;
+        tw[4 * cs + i] = p.ty;
+        tw[5 * cs + i] = p.W_00;
+        tw[6 * cs + i] = p.W_11;
+        tw[7 * cs + i] = pvtracks_denom[t0 + c0 + i];
+      }
+      __syncthreads();
+      // Pair (j, i) = (seed, track) for the thread's index j * cn + i, stepping by the block.
+      const unsigned step_j = block / cn, step_i = block - step_j * cn;
+      unsigned j = t / cn, i = t - j * cn;
+      while (j < S) {
+        const float4 sv = s_seed[j];
+        const float z = tw[i], x = tw[cs + i], y = tw[2 * cs + i], tx = tw[3 * cs + i], ty = tw[4 * cs + i];
+        const float W_00 = tw[5 * cs + i], W_11 = tw[6 * cs + i], denom = tw[7 * cs + i];
+        float q[n_terms] = {};
+        if (!(z < zmin || z >= zmax)) {
+          const float H_20 = -tx, H_21 = -ty;
+          const float HWH_20 = H_20 * W_00, HWH_21 = H_21 * W_11;
+          const float HWH_22 = H_20 * H_20 * W_00 + H_21 * H_21 * W_11;
+          const float dzv = sv.z - z;
+          const float rx = sv.x - (x + tx * dzv), ry = sv.y - (y + ty * dzv);
+          const float chi2 = rx * rx * W_00 + ry * ry * W_11;
           if (chi2 < maxChi2) {
-            ++nselected;
-            const float dz_seed = seed_z - trk.z;
-            const float sx = blx - (trk.x + trk.tx * dz_seed), sy = bly - (trk.y + trk.ty * dz_seed);
-            const float chi2_seed = sx * sx * trk.W_00 + sy * sy * trk.W_11;
+            q[10] = 1.f;
+            const float dz_seed = sv.w - z;
+            const float sx = blx - (x + tx * dz_seed), sy = bly - (y + ty * dz_seed);
+            const float chi2_seed = sx * sx * W_00 + sy * sy * W_11;
             const float exp_chi2_0 = expf(chi2_seed * (-0.5f));
             const float nom = expf(chi2 * (-0.5f));
             const float den = chi2CutExp + nom;
-            const float w = nom / (den + pvtracks_denom[t0 + i] - exp_chi2_0);
+            const float w = nom / (den + denom - exp_chi2_0);
             if (w > minWeight) {
-              const float hx = rx * trk.W_00, hy = ry * trk.W_11, hz = -trk.tx * rx * trk.W_00 - trk.ty * ry * trk.W_11;
-              g0 = g0 + hx * w;
-              g1 = g1 + hy * w;
-              g2 = g2 + hz * w;
-              a00 += w * trk.W_00;
-              a11 += w * trk.W_11;
-              a20 += w * HWH_20;
-              a21 += w * HWH_21;
-              a22 += w * HWH_22;
-              local_chi2tot += w * chi2;
-              local_sum_weights += w;
+              const float hx = rx * W_00, hy = ry * W_11, hz = -tx * rx * W_00 - ty * ry * W_11;
+              q[0] = hx * w;
+              q[1] = hy * w;
+              q[2] = hz * w;
+              q[3] = w * W_00;
+              q[4] = w * W_11;
+              q[5] = w * HWH_20;
+              q[6] = w * HWH_21;
+              q[7] = w * HWH_22;
+              q[8] = w * chi2;
+              q[9] = w;
             }
           }
         }
-        chi2tot = 0.f + local_chi2tot;
-        sum_weights = 0.f + local_sum_weights;
-        if (nselected >= minTracks) {
-          const float det = a00 * (a22 * a11 - a21 * a21) + a20 * (-a11 * a20);
-          const float inv_det = 1.f / det;
-          vtxcov[0] = (a22 * a11 - a21 * a21) * inv_det;
-          vtxcov[1] = -(-a20 * a21) * inv_det;
-          vtxcov[2] = (a22 * a00 - a20 * a20) * inv_det;
-          vtxcov[3] = (-a20 * a11) * inv_det;
-          vtxcov[4] = -(a21 * a00) * inv_det;
-          vtxcov[5] = (a11 * a00) * inv_det;
-          const float dx = -1.f * (vtxcov[0] * g0 + vtxcov[1] * g1 + vtxcov[3] * g2);
-          const float dy = -1.f * (vtxcov[1] * g0 + vtxcov[2] * g1 + vtxcov[4] * g2);
-          const float dzd = -1.f * (vtxcov[3] * g0 + vtxcov[4] * g1 + vtxcov[5] * g2);
-          chi2tot += dx * g0 + dy * g1 + dzd * g2;
-          vx = vx + dx;
-          vy = vy + dy;
-          vz = vz + dzd;
-          converged = fabsf(dzd) < maxDeltaZConverged;
-        }
-        else {
-          converged = true;
-          accept = false;
+#pragma unroll
+        for (unsigned k = 0; k < n_terms; ++k) term[(k * S + j) * cs + i] = q[k];
+        i += step_i;
+        j += step_j;
+        if (i >= cn) {
+          i -= cn;
+          ++j;
         }
       }
-      if (accept) {
-        vertex.x = vx;
-        vertex.y = vy;
-        vertex.z = vz;
-        vertex.chi2 = chi2tot;
-        vertex.ndof = 0;
-        vertex.cov00 = vtxcov[0];
-        vertex.cov10 = vtxcov[1];
-        vertex.cov11 = vtxcov[2];
-        vertex.cov20 = vtxcov[3];
-        vertex.cov21 = vtxcov[4];
-        vertex.cov22 = vtxcov[5];
-        vertex.nTracks = sum_weights;
-        const float bdx = vx - blx, bdy = vy - bly;
-        const float rho2 = bdx * bdx + bdy * bdy;
-        const unsigned minT = vz <= SMOG2_pp_separation ? SMOG2_minNumTracksPerVertex : pp_minNumTracksPerVertex;
-        ok = nselected >= minT && rho2 < maxVertexRho2;
+      __syncthreads();
+      // Sums in track order; a zero term leaves a sum as it is, as a skipped track does.
+      if (t < rows) {
+        const float* r = term + t * cs;
+        for (unsigned i = 0; i < cn; ++i) acc0 = acc0 + r[i];
+      }
+      if (t + block < rows) {
+        const float* r = term + (t + block) * cs;
+        for (unsigned i = 0; i < cn; ++i) acc1 = acc1 + r[i];
+      }
+      __syncthreads();
+    }
+    if (t < rows) s_sum[t] = acc0;
+    if (t + block < rows) s_sum[t + block] = acc1;
+    __syncthreads();
+    if (t < wave_size && !converged) {
+      const unsigned j = my_j;
+      const float g0 = s_sum[j], g1 = s_sum[S + j], g2 = s_sum[2 * S + j];
+      const float a00 = s_sum[3 * S + j], a11 = s_sum[4 * S + j], a20 = s_sum[5 * S + j];
+      const float a21 = s_sum[6 * S + j], a22 = s_sum[7 * S + j];
+      chi2tot = 0.f + s_sum[8 * S + j];
+      sum_weights = 0.f + s_sum[9 * S + j];
+      nselected = static_cast<unsigned>(s_sum[10 * S + j]);
+      if (nselected >= minTracks) {
+        const float det = a00 * (a22 * a11 - a21 * a21) + a20 * (-a11 * a20);
+        const float inv_det = 1.f / det;
+        vtxcov[0] = (a22 * a11 - a21 * a21) * inv_det;
+        vtxcov[1] = -(-a20 * a21) * inv_det;
+        vtxcov[2] = (a22 * a00 - a20 * a20) * inv_det;
+        vtxcov[3] = (-a20 * a11) * inv_det;
+        vtxcov[4] = -(a21 * a00) * inv_det;
+        vtxcov[5] = (a11 * a00) * inv_det;
+        const float dx = -1.f * (vtxcov[0] * g0 + vtxcov[1] * g1 + vtxcov[3] * g2);
+        const float dy = -1.f * (vtxcov[1] * g0 + vtxcov[2] * g1 + vtxcov[4] * g2);
+        const float dzd = -1.f * (vtxcov[3] * g0 + vtxcov[4] * g1 + vtxcov[5] * g2);
+        chi2tot += dx * g0 + dy * g1 + dzd * g2;
+        vx = vx + dx;
+        vy = vy + dy;
+        vz = vz + dzd;
+        converged = fabsf(dzd) < maxDeltaZConverged;
+      }
+      else {
+        converged = true;
+        accept = false;
       }
     }
+  }
+  if (t < wave_size) {
+    bool ok = false;
+    PVVertexOut vertex {};
+    if (accept) {
+      vertex.x = vx;
+      vertex.y = vy;
+      vertex.z = vz;
+      vertex.chi2 = chi2tot;
+      vertex.ndof = 0;
+      vertex.cov00 = vtxcov[0];
+      vertex.cov10 = vtxcov[1];
+      vertex.cov11 = vtxcov[2];
+      vertex.cov20 = vtxcov[3];
+      vertex.cov21 = vtxcov[4];
+      vertex.cov22 = vtxcov[5];
+      vertex.nTracks = sum_weights;
+      const float bdx = vx - blx, bdy = vy - bly;
+      const float rho2 = bdx * bdx + bdy * bdy;
+      const unsigned minT = vz <= SMOG2_pp_separation ? SMOG2_minNumTracksPerVertex : pp_minNumTracksPerVertex;
+      ok = nselected >= minT && rho2 < maxVertexRho2;
+    }
     const uint64_t mask = __ballot(ok);
-    const unsigned slot = __popcll(mask & ((1ull << l) - 1ull));
+    const unsigned slot = __popcll(mask & lanes_below_mask());
     if (ok) {
       multi_fit_vertices[(size_t) max_vertices * event_number + slot] = vertex;
       s_fit[slot] = vertex;
@@ -422,7 +553,7 @@ __global__ void __launch_bounds__(pv::block) k_pv_beamline(
       }
     }
     const uint64_t mask = __ballot(unique);
-    if (unique) multi_final_vertices[(size_t) max_vertices * event_number + __popcll(mask & ((1ull << l) - 1ull))] = v1;
+    if (unique) multi_final_vertices[(size_t) max_vertices * event_number + __popcll(mask & lanes_below_mask())] = v1;
     if (l == 0) number_of_multi_final_vertices[event_number] = __popcll(mask);
   }
   if (st && status) atomicOr(status, st);

@@ -77,6 +77,39 @@ def test_pv_finder_matches_oracle(name, geometry, beamline, constants):
         assert ref["number_of_multi_final_vertices"].sum() > 0
 
 
+CROWDED = {
+    "nu60": dict(n_events=8, pileup=1, nu=60.0, pv_sigma_z=150.0, seed=96),
+    "nu120": dict(n_events=6, pileup=1, nu=120.0, pv_sigma_z=200.0, seed=97),
+}
+
+
+@pytest.mark.parametrize("name", list(CROWDED))
+def test_pv_finder_seed_overflow_matches_oracle(name, geometry, beamline, constants):
+    """Events with more seeds than the 32 kept (and, at nu 120, more tracks than one histogram
+    chunk and many multi-fitter chunks): the first 32 seeds and their vertices as the oracle's,
+    and the same overflow bit."""
+    ev = synth.generate(**CROWDED[name])
+    print(f"[pv] crowded dataset={name} events={ev.n_events}", flush=True)
+    ref = oracle.run(ev, geometry, beamline, pv_finder=1)
+    assert int(ref["status"][0]) == 1 << 16, "the oracle should flag the seed overflow and nothing else"
+    seq = velo.VeloSequence(constants, ev.n_events, max(ev.raw.size, 4096))
+    seq.set_tuning("pv_finder", 2)
+    seq.run(velo.DeviceEvents(ev))
+    assert seq.status() & ~velo.STATUS_LDS_FALLBACK == 1 << 16  # crowded events may take the global-memory path
+    n = ev.n_events
+    nt = int(ref["offsets_all_velo_tracks"][-1])
+    gpu = {"pvtracks": seq.fetch("pvtracks", np.float32, 7 * nt), "zhisto": seq.fetch("zhisto", np.float32, 3392 * n),
+           "zpeaks": seq.fetch("zpeaks", np.float32, 32 * n),
+           "number_of_zpeaks": seq.fetch("number_of_zpeaks", np.uint32, n),
+           "pvtracks_denom": seq.fetch("pvtracks_denom", np.float32, nt)}
+    for kind in ("multi_fit", "multi_final"):
+        gpu[f"{kind}_vertices"] = seq.fetch(f"{kind}_vertices", V, 32 * n)
+        gpu[f"number_of_{kind}_vertices"] = seq.fetch(f"number_of_{kind}_vertices", np.uint32, n)
+    seq.close()
+    _compare(gpu, ref, n, nt)
+    assert (ref["number_of_zpeaks"] == 32).sum() >= n // 2
+
+
 def test_pv_drop_in_entry_point(geometry, beamline, constants):
     """velo_pv_beamline called through the C ABI on the sequence's beamline states, with an
     event list in reverse order, gives the oracle's vertices."""
