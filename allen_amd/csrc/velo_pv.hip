@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(pv::block) k_pv_beamline(
 {
   // pool: the histogram (phases 2-3) followed by the window staging (2) or the transitions and
   // protocluster edges (3); then the track chunks and their (seed, track) terms (5).
-  __shared__ float pool[pool_words];
+  __shared__ __align__(16) float pool[pool_words];
   __shared__ float s_zpeaks[max_vertices];
   __shared__ float4 s_seed[max_vertices]; // seeds not converged: vx, vy, vz, seed z
   __shared__ float s_sum[n_terms * max_vertices];
@@ -178,19 +178,53 @@ __global__ void __launch_bounds__(pv::block) k_pv_beamline(
         s_z[i] = trk.z;
       }
       __syncthreads();
-      for (unsigned i = 0; i < cn; ++i) {
-        const uint32_t window = s_window[i];
-        if (!window) continue;
-        const unsigned minbin = window & 0xFFFFu, maxbin = window >> 16;
-        unsigned b = minbin + ((t - minbin) & (block - 1)); // this lane's first bin in the window
-        if (b > maxbin) continue;
-        const float zerr = s_zerr[i], z = s_z[i];
-        for (; b <= maxbin; b += block) {
-          // The increment of bin b: the integral up to its upper edge (1 for the last bin)
-          // minus the integral up to its lower edge (0 for the first bin).
-          const float upper = b < maxbin ? gauss_integral((zmin + (int) (b + 1) * dz - z) / zerr) : 1.f;
-          const float lower = b > minbin ? gauss_integral((zmin + (int) b * dz - z) / zerr) : 0.f;
-          h[b] += upper - lower;
+      // The lanes of wave w own the bins whose residue mod 256 lies in [64w, 64w + 64). Each
+      // wave looks at 64 tracks at a time, keeps those whose window reaches one of its bins (a
+      // ballot), and walks them in track order with the window broadcast from the lane that
+      // read it.
+      const unsigned wave_first = t & ~(wave_size - 1) & (block - 1);
+      for (unsigned g = 0; g < cn; g += wave_size) {
+        uint32_t window = 0;
+        float zerr = 0.f, z = 0.f;
+        bool reaches = false;
+        if (g + l < cn) {
+          window = s_window[g + l];
+          zerr = s_zerr[g + l];
+          z = s_z[g + l];
+          if (window) {
+            const unsigned minbin = window & 0xFFFFu, len = (window >> 16) - minbin + 1;
+            const unsigned s = (minbin - wave_first) & (block - 1);
+            reaches = len >= block || s < wave_size || s + len > block;
+          }
+        }
+        for (uint64_t mask = __ballot(reaches); mask; mask &= mask - 1) {
+          const int k = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+          const uint32_t win = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(window), k));
+          const float ze = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(zerr), k));
+          const float zz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(z), k));
+          const unsigned minbin = win & 0xFFFFu, maxbin = win >> 16;
+          // The increment of bin b: the integral up to its upper edge (1 for the last bin) minus
+          // the integral up to its lower edge (0 for the first bin).
+          auto edge = [&](unsigned e) { return gauss_integral((zmin + (int) e * dz - zz) / ze); };
+          unsigned b = minbin + ((t - minbin) & (block - 1)); // this lane's first bin in the window
+          if (maxbin - minbin < block) {
+            // At most one bin per lane; the lower edge of bin b is the upper edge of bin b - 1,
+            // which the previous lane holds (same expression, same value).
+            const bool in = b <= maxbin;
+            const float upper = in ? (b < maxbin ? edge(b + 1) : 1.f) : 0.f;
+            const float below = __shfl_up(upper, 1, wave_size);
+            if (in) {
+              const float lower = b > minbin ? (l > 0 ? below : edge(b)) : 0.f;
+              h[b] += upper - lower;
+            }
+          }
+          else {
+            for (; b <= maxbin; b += block) {
+              const float upper = b < maxbin ? edge(b + 1) : 1.f;
+              const float lower = b > minbin ? edge(b) : 0.f;
+              h[b] += upper - lower;
+            }
+          }
         }
       }
       __syncthreads();
@@ -233,7 +267,15 @@ __global__ void __launch_bounds__(pv::block) k_pv_beamline(
       ibegin = trans[2 * m];
       iend = trans[2 * m + 1];
       float integral = h[ibegin];
-      for (unsigned i = ibegin + 1; i <= iend; ++i) integral += h[i];
+      unsigned i = ibegin + 1;
+      for (; i + 8 <= iend + 1; i += 8) { // eight reads in flight, added in order
+        float v[8];
+#pragma unroll
+        for (unsigned u = 0; u < 8; ++u) v[u] = h[i + u];
+#pragma unroll
+        for (unsigned u = 0; u < 8; ++u) integral += v[u];
+      }
+      for (; i <= iend; ++i) integral += h[i];
       const float zBin = zmin + iend * dz;
       const float minInSeed = zBin < SMOG2_pp_separation ? SMOG2_minTracksInSeed : pp_minTracksInSeed;
       keep = integral > minInSeed;
@@ -279,9 +321,8 @@ __global__ void __launch_bounds__(pv::block) k_pv_beamline(
       bool rising = true;
       float integ = h[ibegin];
       push(ibegin, h[ibegin], integ);
-      for (unsigned short i = ibegin; i < iend; ++i) {
-        const float value = h[i];
-        const bool stillrising = h[i + 1] > value;
+      auto step = [&](unsigned short i, float value, float next) {
+        const bool stillrising = next > value;
         if (rising && !stillrising && value >= minpeak) {
           const int n = number_of_extrema;
           if (n >= 2) {
@@ -307,6 +348,19 @@ __global__ void __launch_bounds__(pv::block) k_pv_beamline(
         }
         rising = stillrising;
         integ += value;
+      };
+      // Bins ibegin .. iend - 1 in order, the next eight values read ahead.
+      float value = h[ibegin];
+      for (unsigned i0 = ibegin; i0 < iend; i0 += 8) {
+        float next[8];
+#pragma unroll
+        for (unsigned u = 0; u < 8; ++u) next[u] = h[min(i0 + 1 + u, static_cast<unsigned>(iend))];
+#pragma unroll
+        for (unsigned u = 0; u < 8; ++u)
+          if (i0 + u < iend) {
+            step(static_cast<unsigned short>(i0 + u), value, next[u]);
+            value = next[u];
+          }
       }
       push(iend, h[iend], integ);
       N = number_of_extrema;
@@ -385,35 +439,30 @@ __global__ void __launch_bounds__(pv::block) k_pv_beamline(
     __syncthreads();
     const unsigned S = s_nact, rows = n_terms * S;
     if (S == 0) break;
-    // Tracks per chunk: eight track words and `rows` terms per track fit the pool. Odd, so that
-    // the row stride (the chunk size made odd) stays within it and the rows start in distinct
-    // banks.
-    unsigned C = pool_words / (8 + rows);
-    C -= (C & 1u) ^ 1u;
+    // Tracks per chunk: eight track words and `rows` terms per track fit the pool.
+    const unsigned C = pool_words / (8 + rows);
+    // Accumulator thread t < rows adds term k of seed j (t = 11 j + k); 256 + t for more seeds.
+    const unsigned aj0 = t / n_terms, ak0 = t - aj0 * n_terms;
+    const unsigned aj1 = (t + block) / n_terms, ak1 = t + block - aj1 * n_terms;
     float acc0 = 0.f, acc1 = 0.f;
     for (unsigned c0 = 0; c0 < nt; c0 += C) {
-      const unsigned cn = min(C, nt - c0), cs = cn | 1u;
-      float* const tw = pool;            // z, x, y, tx, ty, W_00, W_11, denominator: rows of cs
-      float* const term = pool + 8 * cs; // row k * S + j: term k of seed j, one column per track
+      const unsigned cn = min(C, nt - c0);
+      float* const tw = pool;            // track i: z, x, y, tx, ty, W_00, W_11, denominator at 8 i
+      float* const term = pool + 8 * cn; // pair (j, i): its terms at 11 (j cn + i)
       for (unsigned i = t; i < cn; i += block) {
         const PVTrackOut p = pvtracks[t0 + c0 + i];
-        tw[i] = p.z;
-        tw[cs + i] = p.x;
-        tw[2 * cs + i] = p.y;
-        tw[3 * cs + i] = p.tx;
-        tw[4 * cs + i] = p.ty;
-        tw[5 * cs + i] = p.W_00;
-        tw[6 * cs + i] = p.W_11;
-        tw[7 * cs + i] = pvtracks_denom[t0 + c0 + i];
+        *reinterpret_cast<float4*>(tw + 8 * i) = make_float4(p.z, p.x, p.y, p.tx);
+        *reinterpret_cast<float4*>(tw + 8 * i + 4) = make_float4(p.ty, p.W_00, p.W_11, pvtracks_denom[t0 + c0 + i]);
       }
       __syncthreads();
-      // Pair (j, i) = (seed, track) for the thread's index j * cn + i, stepping by the block.
+      // Pair p = j cn + i = (seed, track), stepping by the block.
       const unsigned step_j = block / cn, step_i = block - step_j * cn;
       unsigned j = t / cn, i = t - j * cn;
-      while (j < S) {
+      for (unsigned pair = t; pair < S * cn; pair += block) {
         const float4 sv = s_seed[j];
-        const float z = tw[i], x = tw[cs + i], y = tw[2 * cs + i], tx = tw[3 * cs + i], ty = tw[4 * cs + i];
-        const float W_00 = tw[5 * cs + i], W_11 = tw[6 * cs + i], denom = tw[7 * cs + i];
+        const float4 ta = *reinterpret_cast<const float4*>(tw + 8 * i);
+        const float4 tb = *reinterpret_cast<const float4*>(tw + 8 * i + 4);
+        const float z = ta.x, x = ta.y, y = ta.z, tx = ta.w, ty = tb.x, W_00 = tb.y, W_11 = tb.z, denom = tb.w;
         float q[n_terms] = {};
         if (!(z < zmin || z >= zmax)) {
           const float H_20 = -tx, H_21 = -ty;
@@ -446,8 +495,9 @@ __global__ void __launch_bounds__(pv::block) k_pv_beamline(
             }
           }
         }
+        float* const out = term + n_terms * pair;
 #pragma unroll
-        for (unsigned k = 0; k < n_terms; ++k) term[(k * S + j) * cs + i] = q[k];
+        for (unsigned k = 0; k < n_terms; ++k) out[k] = q[k];
         i += step_i;
         j += step_j;
         if (i >= cn) {
@@ -456,28 +506,34 @@ __global__ void __launch_bounds__(pv::block) k_pv_beamline(
         }
       }
       __syncthreads();
-      // Sums in track order; a zero term leaves a sum as it is, as a skipped track does.
-      if (t < rows) {
-        const float* r = term + t * cs;
-        for (unsigned i = 0; i < cn; ++i) acc0 = acc0 + r[i];
-      }
-      if (t + block < rows) {
-        const float* r = term + (t + block) * cs;
-        for (unsigned i = 0; i < cn; ++i) acc1 = acc1 + r[i];
-      }
+      // Sums in track order, eight reads in flight; a zero term leaves a sum as it is, as a
+      // skipped track does.
+      auto accumulate = [&](float& acc, unsigned aj, unsigned ak) {
+        const float* r = term + n_terms * aj * cn + ak;
+        unsigned i = 0;
+        for (; i + 8 <= cn; i += 8) {
+          float v[8];
+#pragma unroll
+          for (unsigned u = 0; u < 8; ++u) v[u] = r[n_terms * (i + u)];
+#pragma unroll
+          for (unsigned u = 0; u < 8; ++u) acc = acc + v[u];
+        }
+        for (; i < cn; ++i) acc = acc + r[n_terms * i];
+      };
+      if (t < rows) accumulate(acc0, aj0, ak0);
+      if (t + block < rows) accumulate(acc1, aj1, ak1);
       __syncthreads();
     }
     if (t < rows) s_sum[t] = acc0;
     if (t + block < rows) s_sum[t + block] = acc1;
     __syncthreads();
     if (t < wave_size && !converged) {
-      const unsigned j = my_j;
-      const float g0 = s_sum[j], g1 = s_sum[S + j], g2 = s_sum[2 * S + j];
-      const float a00 = s_sum[3 * S + j], a11 = s_sum[4 * S + j], a20 = s_sum[5 * S + j];
-      const float a21 = s_sum[6 * S + j], a22 = s_sum[7 * S + j];
-      chi2tot = 0.f + s_sum[8 * S + j];
-      sum_weights = 0.f + s_sum[9 * S + j];
-      nselected = static_cast<unsigned>(s_sum[10 * S + j]);
+      const float* sum = s_sum + n_terms * my_j;
+      const float g0 = sum[0], g1 = sum[1], g2 = sum[2];
+      const float a00 = sum[3], a11 = sum[4], a20 = sum[5], a21 = sum[6], a22 = sum[7];
+      chi2tot = 0.f + sum[8];
+      sum_weights = 0.f + sum[9];
+      nselected = static_cast<unsigned>(sum[10]);
       if (nselected >= minTracks) {
         const float det = a00 * (a22 * a11 - a21 * a21) + a20 * (-a11 * a20);
         const float inv_det = 1.f / det;
