@@ -14,10 +14,11 @@
 //      lane each; the extrema split of each kept protocluster by one lane each, its seeds ranked
 //      by a block scan
 //   4. denominators: one lane per track, seeds in order
-//   5. multi-fitter: the seeds iterate in lockstep; per iteration all lanes compute the (seed,
-//      track) weights and their weighted terms into LDS, one lane per (seed, sum) adds the terms
-//      in track order (the CPU target's sums), and one lane per seed solves and updates it;
-//      vertices kept in seed order by a ballot
+//   5. multi-fitter: the seeds iterate in lockstep; per iteration all lanes select the (seed,
+//      track) pairs within the chi2 cut, compute the weights and weighted terms of the selected
+//      pairs into LDS ranked seed by seed in track order, one lane per (seed, sum) adds them in
+//      that order (the CPU target's sums), and one lane per seed solves and updates it; vertices
+//      kept in seed order by a ballot
 //   6. cleanup: one lane per vertex, unique vertices kept in order by a ballot
 //
 // Capacities the reference does not check (extrema per protocluster, clusters, vertices) are
@@ -48,9 +49,13 @@ constexpr unsigned bins_per_lane = (nbins + block - 1) / block; // 14 consecutiv
 constexpr unsigned hist_words = block * bins_per_lane;          // 3584
 constexpr unsigned pool_words = 8192;  // LDS shared by the phases (32 KiB)
 constexpr unsigned track_chunk = 1024; // histogram windows staged per pass, three words each
-constexpr unsigned n_terms = 11;       // per (seed, track): ten weighted terms and the selection flag
+constexpr unsigned fit_tracks = 512;   // multi-fitter: tracks resident in LDS per chunk, eight words each
+constexpr unsigned fit_terms = 10;     // per selected (seed, track) pair: the ten weighted terms
+constexpr unsigned n_sums = fit_terms + 1; // per seed: the ten sums and the number of selected tracks
 constexpr uint32_t status_overflow = 1u << 16;
+constexpr int dpp_wave_shr1 = 0x138; // DPP control: every lane reads the previous lane (GFX9 wave shift)
 static_assert(hist_words + 3 * track_chunk <= pool_words, "histogram staging");
+static_assert(8 * fit_tracks + max_vertices * (fit_tracks / 32) + 4 + fit_terms * block <= pool_words, "fit pool");
 static_assert(hist_words + (nbins + 1) / 2 + max_clusteredges / 2 <= pool_words, "transition list");
 static_assert(block >= max_protoclusters && wave_size >= max_vertices, "one lane per protocluster and per seed");
 static_assert((block & (block - 1)) == 0, "bin ownership uses a mask");
@@ -114,7 +119,8 @@ __global__ void __launch_bounds__(pv::block) k_pv_beamline(
   __shared__ __align__(16) float pool[pool_words];
   __shared__ float s_zpeaks[max_vertices];
   __shared__ float4 s_seed[max_vertices]; // seeds not converged: vx, vy, vz, seed z
-  __shared__ float s_sum[n_terms * max_vertices];
+  __shared__ float s_sum[n_sums * max_vertices];
+  __shared__ unsigned s_base[max_vertices], s_count[max_vertices], s_nact_sel;
   __shared__ PVVertexOut s_fit[max_vertices];
   __shared__ uint32_t s_scan[block_waves + 1];
   __shared__ unsigned s_nact, s_nfit;
@@ -211,11 +217,14 @@ __global__ void __launch_bounds__(pv::block) k_pv_beamline(
             // At most one bin per lane; the lower edge of bin b is the upper edge of bin b - 1,
             // which the previous lane holds (same expression, same value).
             const bool in = b <= maxbin;
+            const float hb = in ? h[b] : 0.f; // read ahead of the integrals
             const float upper = in ? (b < maxbin ? edge(b + 1) : 1.f) : 0.f;
-            const float below = __shfl_up(upper, 1, wave_size);
+            // The previous lane's value: a DPP shift of the whole wave by one lane.
+            const float below =
+              __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(upper), dpp_wave_shr1, 0xF, 0xF, false));
             if (in) {
               const float lower = b > minbin ? (l > 0 ? below : edge(b)) : 0.f;
-              h[b] += upper - lower;
+              h[b] = hb + (upper - lower);
             }
           }
           else {
@@ -416,7 +425,15 @@ __global__ void __launch_bounds__(pv::block) k_pv_beamline(
   __syncthreads();
 
   // 5. Multi-fitter (pv_beamline_multi_fitter.cu:28-185). Lane s of wave 0 holds seed s; the
-  // seeds not yet converged iterate together.
+  // seeds not yet converged iterate together. Per iteration and chunk of tracks (all of them when
+  // there are at most fit_tracks):
+  //   a. all lanes test the (seed, track) pairs: a pair is selected when the track is in the z
+  //      range and its chi2 to the vertex is below maxChi2 (a bit per pair in LDS);
+  //   b. the selected pairs are ranked seed by seed in track order (popcounts and a wave scan);
+  //   c. all lanes compute the weights and weighted terms of the selected pairs (two exponentials
+  //      and a division each) into LDS at their rank, as many as the pool holds per round;
+  //   d. lane 10 j + k adds term k of seed j over its pairs in rank order: the CPU target's sums,
+  //      which skip the pairs not selected.
   bool converged = true, accept = false;
   float vtxcov[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   float vx = blx, vy = bly, vz = 0.f, seedz = 0.f, chi2tot = 0.f, sum_weights = 0.f;
@@ -428,6 +445,63 @@ __global__ void __launch_bounds__(pv::block) k_pv_beamline(
     vz = seedz;
     minTracks = seedz <= SMOG2_pp_separation ? SMOG2_minNumTracksPerVertex : pp_minNumTracksPerVertex;
   }
+  float* const tw = pool; // track i of the chunk: z, x, y, tx, ty, W_00, W_11, denominator at 8 i
+  auto stage = [&](unsigned c0, unsigned cn) {
+    for (unsigned i = t; i < cn; i += block) {
+      const PVTrackOut p = pvtracks[t0 + c0 + i];
+      *reinterpret_cast<float4*>(tw + 8 * i) = make_float4(p.z, p.x, p.y, p.tx);
+      *reinterpret_cast<float4*>(tw + 8 * i + 4) = make_float4(p.ty, p.W_00, p.W_11, pvtracks_denom[t0 + c0 + i]);
+    }
+  };
+  // The chi2 of track i to vertex j (maxChi2 outside the z range: not selected).
+  auto pair_chi2 = [&](unsigned j, unsigned i, float& rx, float& ry) {
+    const float4 sv = s_seed[j];
+    const float4 ta = *reinterpret_cast<const float4*>(tw + 8 * i);
+    const float4 tb = *reinterpret_cast<const float4*>(tw + 8 * i + 4);
+    const float z = ta.x, x = ta.y, y = ta.z, tx = ta.w, W_00 = tb.y, W_11 = tb.z;
+    const float dzv = sv.z - z;
+    rx = sv.x - (x + tx * dzv);
+    ry = sv.y - (y + tb.x * dzv);
+    const float chi2 = rx * rx * W_00 + ry * ry * W_11;
+    return (z < zmin || z >= zmax) ? maxChi2 : chi2;
+  };
+  // The weighted terms of a selected pair (zeros when its weight is not above minWeight).
+  auto pair_terms = [&](unsigned j, unsigned i, float* out) {
+    float rx, ry;
+    const float chi2 = pair_chi2(j, i, rx, ry);
+    const float4 sv = s_seed[j];
+    const float4 ta = *reinterpret_cast<const float4*>(tw + 8 * i);
+    const float4 tb = *reinterpret_cast<const float4*>(tw + 8 * i + 4);
+    const float z = ta.x, x = ta.y, y = ta.z, tx = ta.w, ty = tb.x, W_00 = tb.y, W_11 = tb.z, denom = tb.w;
+    const float H_20 = -tx, H_21 = -ty;
+    const float HWH_20 = H_20 * W_00, HWH_21 = H_21 * W_11;
+    const float HWH_22 = H_20 * H_20 * W_00 + H_21 * H_21 * W_11;
+    float q[fit_terms] = {};
+    const float dz_seed = sv.w - z;
+    const float sx = blx - (x + tx * dz_seed), sy = bly - (y + ty * dz_seed);
+    const float chi2_seed = sx * sx * W_00 + sy * sy * W_11;
+    const float exp_chi2_0 = expf(chi2_seed * (-0.5f));
+    const float nom = expf(chi2 * (-0.5f));
+    const float den = chi2CutExp + nom;
+    const float w = nom / (den + denom - exp_chi2_0);
+    if (w > minWeight) {
+      const float hx = rx * W_00, hy = ry * W_11, hz = -tx * rx * W_00 - ty * ry * W_11;
+      q[0] = hx * w;
+      q[1] = hy * w;
+      q[2] = hz * w;
+      q[3] = w * W_00;
+      q[4] = w * W_11;
+      q[5] = w * HWH_20;
+      q[6] = w * HWH_21;
+      q[7] = w * HWH_22;
+      q[8] = w * chi2;
+      q[9] = w;
+    }
+#pragma unroll
+    for (unsigned k = 0; k < fit_terms; ++k) out[k] = q[k];
+  };
+  const bool resident = nt <= fit_tracks;
+  if (resident) stage(0, nt); // read after the first barrier below
   for (unsigned iter = 0; iter < maxFitIter; ++iter) {
     unsigned my_j = 0;
     if (t < wave_size) {
@@ -437,98 +511,98 @@ __global__ void __launch_bounds__(pv::block) k_pv_beamline(
       if (l == 0) s_nact = __popcll(m);
     }
     __syncthreads();
-    const unsigned S = s_nact, rows = n_terms * S;
+    const unsigned S = s_nact, rows = fit_terms * S;
     if (S == 0) break;
-    // Tracks per chunk: eight track words and `rows` terms per track fit the pool.
-    const unsigned C = pool_words / (8 + rows);
-    // Accumulator thread t < rows adds term k of seed j (t = 11 j + k); 256 + t for more seeds.
-    const unsigned aj0 = t / n_terms, ak0 = t - aj0 * n_terms;
-    const unsigned aj1 = (t + block) / n_terms, ak1 = t + block - aj1 * n_terms;
+    // Accumulator thread t < rows adds term k of seed j (t = 10 j + k); t + 256 as well for more
+    // than 25 seeds.
+    const unsigned aj0 = t / fit_terms, ak0 = t - aj0 * fit_terms;
+    const unsigned aj1 = (t + block) / fit_terms, ak1 = t + block - aj1 * fit_terms;
     float acc0 = 0.f, acc1 = 0.f;
-    for (unsigned c0 = 0; c0 < nt; c0 += C) {
-      const unsigned cn = min(C, nt - c0);
-      float* const tw = pool;            // track i: z, x, y, tx, ty, W_00, W_11, denominator at 8 i
-      float* const term = pool + 8 * cn; // pair (j, i): its terms at 11 (j cn + i)
-      for (unsigned i = t; i < cn; i += block) {
-        const PVTrackOut p = pvtracks[t0 + c0 + i];
-        *reinterpret_cast<float4*>(tw + 8 * i) = make_float4(p.z, p.x, p.y, p.tx);
-        *reinterpret_cast<float4*>(tw + 8 * i + 4) = make_float4(p.ty, p.W_00, p.W_11, pvtracks_denom[t0 + c0 + i]);
-      }
+    unsigned n_selected = 0; // lane j < S of wave 0: pairs selected for seed j
+    for (unsigned c0 = 0; c0 < nt; c0 += fit_tracks) {
+      const unsigned cn = min(fit_tracks, nt - c0), mw = (cn + 31) / 32;
+      uint32_t* const sel = reinterpret_cast<uint32_t*>(pool + 8 * cn); // seed j: words j mw .. j mw + mw
+      const unsigned term_at = (8 * cn + S * mw + 3) & ~3u;
+      float* const term = pool + term_at;
+      const unsigned cap = (pool_words - term_at) / fit_terms; // selected pairs per round
+      if (!resident) stage(c0, cn);
+      for (unsigned u = t; u < S * mw; u += block) sel[u] = 0u;
       __syncthreads();
-      // Pair p = j cn + i = (seed, track), stepping by the block.
-      const unsigned step_j = block / cn, step_i = block - step_j * cn;
-      unsigned j = t / cn, i = t - j * cn;
-      for (unsigned pair = t; pair < S * cn; pair += block) {
-        const float4 sv = s_seed[j];
-        const float4 ta = *reinterpret_cast<const float4*>(tw + 8 * i);
-        const float4 tb = *reinterpret_cast<const float4*>(tw + 8 * i + 4);
-        const float z = ta.x, x = ta.y, y = ta.z, tx = ta.w, ty = tb.x, W_00 = tb.y, W_11 = tb.z, denom = tb.w;
-        float q[n_terms] = {};
-        if (!(z < zmin || z >= zmax)) {
-          const float H_20 = -tx, H_21 = -ty;
-          const float HWH_20 = H_20 * W_00, HWH_21 = H_21 * W_11;
-          const float HWH_22 = H_20 * H_20 * W_00 + H_21 * H_21 * W_11;
-          const float dzv = sv.z - z;
-          const float rx = sv.x - (x + tx * dzv), ry = sv.y - (y + ty * dzv);
-          const float chi2 = rx * rx * W_00 + ry * ry * W_11;
-          if (chi2 < maxChi2) {
-            q[10] = 1.f;
-            const float dz_seed = sv.w - z;
-            const float sx = blx - (x + tx * dz_seed), sy = bly - (y + ty * dz_seed);
-            const float chi2_seed = sx * sx * W_00 + sy * sy * W_11;
-            const float exp_chi2_0 = expf(chi2_seed * (-0.5f));
-            const float nom = expf(chi2 * (-0.5f));
-            const float den = chi2CutExp + nom;
-            const float w = nom / (den + denom - exp_chi2_0);
-            if (w > minWeight) {
-              const float hx = rx * W_00, hy = ry * W_11, hz = -tx * rx * W_00 - ty * ry * W_11;
-              q[0] = hx * w;
-              q[1] = hy * w;
-              q[2] = hz * w;
-              q[3] = w * W_00;
-              q[4] = w * W_11;
-              q[5] = w * HWH_20;
-              q[6] = w * HWH_21;
-              q[7] = w * HWH_22;
-              q[8] = w * chi2;
-              q[9] = w;
-            }
+      // a. Selection bits.
+      {
+        const unsigned step_j = block / cn, step_i = block - step_j * cn;
+        unsigned j = t / cn, i = t - j * cn;
+        for (unsigned pair = t; pair < S * cn; pair += block) {
+          float rx, ry;
+          if (pair_chi2(j, i, rx, ry) < maxChi2) atomicOr(&sel[j * mw + (i >> 5)], 1u << (i & 31));
+          i += step_i;
+          j += step_j;
+          if (i >= cn) {
+            i -= cn;
+            ++j;
           }
         }
-        float* const out = term + n_terms * pair;
-#pragma unroll
-        for (unsigned k = 0; k < n_terms; ++k) out[k] = q[k];
-        i += step_i;
-        j += step_j;
-        if (i >= cn) {
-          i -= cn;
-          ++j;
-        }
       }
       __syncthreads();
-      // Sums in track order, eight reads in flight; a zero term leaves a sum as it is, as a
-      // skipped track does.
-      auto accumulate = [&](float& acc, unsigned aj, unsigned ak) {
-        const float* r = term + n_terms * aj * cn + ak;
-        unsigned i = 0;
-        for (; i + 8 <= cn; i += 8) {
-          float v[8];
-#pragma unroll
-          for (unsigned u = 0; u < 8; ++u) v[u] = r[n_terms * (i + u)];
-#pragma unroll
-          for (unsigned u = 0; u < 8; ++u) acc = acc + v[u];
+      // b. Ranks: seed j's pairs take ranks s_base[j] .. s_base[j] + s_count[j], in track order.
+      if (t < wave_size) {
+        unsigned n = 0;
+        if (l < S)
+          for (unsigned w = 0; w < mw; ++w) n += __popc(sel[l * mw + w]);
+        const unsigned inc = wave_inclusive_scan(n);
+        if (l < S) {
+          s_base[l] = inc - n;
+          s_count[l] = n;
         }
-        for (; i < cn; ++i) acc = acc + r[n_terms * i];
-      };
-      if (t < rows) accumulate(acc0, aj0, ak0);
-      if (t + block < rows) accumulate(acc1, aj1, ak1);
+        if (l == wave_size - 1) s_nact_sel = inc;
+        n_selected += n;
+      }
       __syncthreads();
+      const unsigned total_selected = s_nact_sel;
+      for (unsigned r0 = 0; r0 < total_selected; r0 += cap) {
+        const unsigned rn = min(cap, total_selected - r0);
+        // c. Terms of the pairs of ranks r0 .. r0 + rn.
+        for (unsigned x = t; x < rn; x += block) {
+          const unsigned rank = r0 + x;
+          unsigned j = 0;
+          while (j + 1 < S && s_base[j + 1] <= rank) ++j;
+          unsigned r = rank - s_base[j], w = 0;
+          const uint32_t* sj = sel + j * mw;
+          uint32_t bits = sj[0];
+          while (static_cast<unsigned>(__popc(bits)) <= r) {
+            r -= __popc(bits);
+            bits = sj[++w];
+          }
+          for (; r > 0; --r) bits &= bits - 1u;
+          pair_terms(j, 32 * w + __ffs(bits) - 1, term + fit_terms * x);
+        }
+        __syncthreads();
+        // d. Sums in track order, eight reads in flight.
+        auto accumulate = [&](float& acc, unsigned aj, unsigned ak) {
+          const unsigned lo = max(s_base[aj], r0), hi = min(s_base[aj] + s_count[aj], r0 + rn);
+          const float* r = term + ak + fit_terms * (lo - r0);
+          const unsigned n = hi > lo ? hi - lo : 0u;
+          unsigned i = 0;
+          for (; i + 8 <= n; i += 8) {
+            float v[8];
+#pragma unroll
+            for (unsigned u = 0; u < 8; ++u) v[u] = r[fit_terms * (i + u)];
+#pragma unroll
+            for (unsigned u = 0; u < 8; ++u) acc = acc + v[u];
+          }
+          for (; i < n; ++i) acc = acc + r[fit_terms * i];
+        };
+        if (t < rows) accumulate(acc0, aj0, ak0);
+        if (t + block < rows) accumulate(acc1, aj1, ak1);
+        __syncthreads();
+      }
     }
-    if (t < rows) s_sum[t] = acc0;
-    if (t + block < rows) s_sum[t + block] = acc1;
+    if (t < rows) s_sum[n_sums * aj0 + ak0] = acc0;
+    if (t + block < rows) s_sum[n_sums * aj1 + ak1] = acc1;
+    if (t < S) s_sum[n_sums * t + fit_terms] = static_cast<float>(n_selected);
     __syncthreads();
     if (t < wave_size && !converged) {
-      const float* sum = s_sum + n_terms * my_j;
+      const float* sum = s_sum + n_sums * my_j;
       const float g0 = sum[0], g1 = sum[1], g2 = sum[2];
       const float a00 = sum[3], a11 = sum[4], a20 = sum[5], a21 = sum[6], a22 = sum[7];
       chi2tot = 0.f + sum[8];
