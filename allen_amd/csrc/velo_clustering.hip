@@ -52,9 +52,11 @@ __device__ __forceinline__ Grown grow_cluster(uint64_t pixel_map, uint64_t start
   const unsigned n = __popcll(cur);
   const unsigned x = col_lower * n + __popcll(cur & 0x00000000FFFF0000ull) + __popcll(cur & 0x0000FFFF00000000ull) * 2 +
                      __popcll(cur & 0xFFFF000000000000ull) * 3;
-  unsigned y = row_lower * n;
-#pragma unroll
-  for (unsigned r = 1; r < 15; ++r) y += __popcll(cur & (0x0001000100010001ull << r)) * r;
+  // Sum of the row offsets r (bit r of each 16-bit column) by bit planes of r. The reference sums
+  // r = 1..14; a pixel at r = 15 makes every caller drop the cluster, so the sums agree wherever
+  // they are used.
+  const unsigned y = row_lower * n + __popcll(cur & 0xAAAAAAAAAAAAAAAAull) + 2 * __popcll(cur & 0xCCCCCCCCCCCCCCCCull) +
+                     4 * __popcll(cur & 0xF0F0F0F0F0F0F0F0ull) + 8 * __popcll(cur & 0xFF00FF00FF00FF00ull);
   return {cur, x, y, n};
 }
 
@@ -895,6 +897,11 @@ namespace mp {
 constexpr unsigned waves_per_block = 4;
 constexpr unsigned word_cap = 384;
 constexpr unsigned cluster_cap = 224;
+constexpr unsigned hash_bits = 9;   // estimate: (bank, SP address) table of a resident module pair,
+constexpr unsigned hash_slots = 1u << hash_bits; // more slots than words, so an empty slot ends every probe
+constexpr uint32_t hash_empty = 0xFFFFFFFFu;
+constexpr unsigned hash_min_bank = 16; // largest bank (words) above which the table beats the scan (measured)
+static_assert(hash_slots > word_cap, "hash table load");
 
 // Banks of one module pair (at most 8), kept in the wave's LDS slice.
 struct Banks {
@@ -994,20 +1001,61 @@ for_words(const Banks& B, unsigned n_words, bool resident, uint32_t* lw, bool ac
 }
 } // namespace mp
 
-// K2 of one module pair (idx = event * 26 + module pair) by one wave.
+namespace mp {
+// Slot of a (bank, SP address) key in a table of 2^bits slots: multiplicative hashing.
+__device__ __forceinline__ unsigned hash_slot(uint32_t key, unsigned bits) { return (key * 2654435761u) >> (32 - bits); }
+} // namespace mp
+
+// K2 of one module pair (idx = event * 26 + module pair) by one wave. The neighbourhood of a
+// non-isolated SP is what the reference collects by scanning every word of its bank: the SP
+// bytes of the words at five neighbouring addresses, OR-ed (EstimateInputSize.cu:111-141). For a
+// module pair resident in LDS the words are first entered in a hash table keyed by (bank, SP
+// address) whose value is the OR of their SP bytes, and each SP looks its five neighbours up;
+// OR-ing is order-free, so the result is the scan's. Larger module pairs scan.
 __device__ __forceinline__ void estimate_item(
   const char* raw,
   const mp::DirFetch& dir,
   unsigned idx,
   mp::Banks& B,
   uint32_t* lw,
+  uint32_t* hkeys,
+  uint32_t* hvals,
   uint32_t* estimated_input_size,
   uint8_t* sp_flags)
 {
   mp::load_banks(raw, dir, B);
   const unsigned n_words = B.start[B.n];
   const bool resident = n_words <= mp::word_cap;
+  // A scan costs a wave about as many steps as its largest bank has words; the table, a fixed
+  // number of LDS round trips. The table serves resident module pairs whose largest bank has
+  // more than hash_min_bank words.
+  unsigned largest_bank = 0;
+  for (unsigned k = 0; k < B.n; ++k) largest_bank = max(largest_bank, B.start[k + 1] - B.start[k]);
+  const bool table = resident && largest_bank > mp::hash_min_bank;
+  // Table of about twice as many slots as words (64 to 512; always more slots than words, so an
+  // empty slot ends every probe).
+  const unsigned bits = min(mp::hash_bits, max(6u, 32u - __clz(2 * n_words - 1)));
+  const unsigned hmask = (1u << bits) - 1;
   if (resident) mp::load_words(B, 0, n_words, lw);
+  if (table) {
+    for (unsigned h = lane_id(); h <= hmask; h += wave_size) {
+      hkeys[h] = mp::hash_empty;
+      hvals[h] = 0u;
+    }
+    mp::wave_sync_lds();
+    for (unsigned w = lane_id(); w < n_words; w += wave_size) {
+      const uint32_t word = lw[w];
+      const uint32_t key = (mp::bank_of(B, w) << 15) | ((word & 0x007FFF00u) >> 8);
+      for (unsigned h = mp::hash_slot(key, bits);; h = (h + 1) & hmask) {
+        const uint32_t old = atomicCAS(&hkeys[h], mp::hash_empty, key);
+        if (old == mp::hash_empty || old == key) {
+          atomicOr(&hvals[h], word & 0xFFu);
+          break;
+        }
+      }
+    }
+    mp::wave_sync_lds();
+  }
   uint32_t count = 0;
   for (unsigned base = 0; base < n_words; base += wave_size) {
     const unsigned w = base + lane_id();
@@ -1018,8 +1066,43 @@ __device__ __forceinline__ void estimate_item(
     const uint32_t sp_addr = (sp_word & 0x007FFF00u) >> 8;
     const uint32_t sp_row = sp_addr & 0x3Fu, sp_col = sp_addr >> 6;
     uint32_t pixels = neighbourhood_init(sp_word);
-    mp::for_words(B, n_words, resident, lw, active && !isolated, B.start[bi], B.start[bi + 1],
-                  [&](uint32_t other) { neighbour_acc(sp_row, sp_col, other, pixels); });
+    if (table) {
+      if (active && !isolated) {
+        // neighbour_acc's five cases, each from the OR of the words at that address (rows and
+        // columns outside the SP grid match no word).
+        // The five first probes are issued together, then the rare longer probes, then the values.
+        const uint32_t kb = bi << 15;
+        const uint32_t cols[5] = {sp_col, sp_col + 1, sp_col + 1, sp_col + 1, sp_col};
+        const uint32_t rows[5] = {sp_row + 1, sp_row + 1, sp_row, sp_row - 1, sp_row - 1};
+        uint32_t key[5], got[5], v[5];
+        unsigned h[5];
+#pragma unroll
+        for (unsigned k = 0; k < 5; ++k) {
+          const bool valid = rows[k] < 64u && cols[k] < 512u;
+          key[k] = valid ? (kb | (cols[k] << 6) | rows[k]) : mp::hash_empty;
+          h[k] = mp::hash_slot(key[k], bits);
+          got[k] = valid ? hkeys[h[k]] : mp::hash_empty;
+        }
+#pragma unroll
+        for (unsigned k = 0; k < 5; ++k)
+          while (got[k] != key[k] && got[k] != mp::hash_empty) {
+            h[k] = (h[k] + 1) & hmask;
+            got[k] = hkeys[h[k]];
+          }
+#pragma unroll
+        for (unsigned k = 0; k < 5; ++k) v[k] = got[k] == key[k] && key[k] != mp::hash_empty ? hvals[h[k]] : 0u;
+        const uint32_t top = v[0], top_right = v[1], right = v[2], bot_right = v[3], bot = v[4];
+        pixels |= ((top & 0x01) | ((top & 0x10) << 2)) << 4;
+        pixels |= (top_right & 0x01) << 16;
+        pixels |= (right & 0x0F) << 12;
+        pixels |= (bot_right & 0x08) << 8;
+        pixels |= (bot & 0x80) >> 2;
+      }
+    }
+    else {
+      mp::for_words(B, n_words, resident, lw, active && !isolated, B.start[bi], B.start[bi + 1],
+                    [&](uint32_t other) { neighbour_acc(sp_row, sp_col, other, pixels); });
+    }
     if (active) {
       if (isolated) {
         count += isolated_cluster_count(sp_word & 0xFFu);
@@ -1051,6 +1134,8 @@ __global__ void __launch_bounds__(mp::waves_per_block* wave_size) k_estimate_mp(
   uint8_t* sp_flags)
 {
   __shared__ uint32_t s_words[mp::waves_per_block][mp::word_cap];
+  __shared__ uint32_t s_hkeys[mp::waves_per_block][mp::hash_slots];
+  __shared__ uint32_t s_hvals[mp::waves_per_block][mp::hash_slots];
   __shared__ mp::Banks s_banks[mp::waves_per_block];
   const unsigned wave = threadIdx.x / wave_size;
   const unsigned n_items = n_events * n_module_pairs;
@@ -1061,7 +1146,8 @@ __global__ void __launch_bounds__(mp::waves_per_block* wave_size) k_estimate_mp(
   for (; idx < n_items; idx += stride) {
     mp::DirFetch next = cur;
     if (idx + stride < n_items) next = mp::fetch_dir(directory, directory_count, idx + stride);
-    estimate_item(raw, cur, idx, s_banks[wave], s_words[wave], estimated_input_size, sp_flags);
+    estimate_item(raw, cur, idx, s_banks[wave], s_words[wave], s_hkeys[wave], s_hvals[wave], estimated_input_size,
+                  sp_flags);
     cur = next;
   }
 }
