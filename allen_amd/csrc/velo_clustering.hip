@@ -897,9 +897,10 @@ namespace mp {
 constexpr unsigned waves_per_block = 4;
 constexpr unsigned word_cap = 384;
 constexpr unsigned cluster_cap = 224;
-constexpr unsigned hash_bits = 9;   // estimate: (bank, SP address) table of a resident module pair,
+constexpr unsigned hash_bits = 10;  // estimate: (bank, SP address) table of a resident module pair,
 constexpr unsigned hash_slots = 1u << hash_bits; // more slots than words, so an empty slot ends every probe
-constexpr uint32_t hash_empty = 0xFFFFFFFFu;
+constexpr uint32_t hash_empty = 0xFFFFFFFFu; // an empty slot; keys have 18 bits, so no slot of a key is all ones
+constexpr uint32_t hash_no_key = 0x01000000u; // a lookup key that matches no slot (nor an empty one)
 constexpr unsigned hash_min_bank = 16; // largest bank (words) above which the table beats the scan (measured)
 static_assert(hash_slots > word_cap, "hash table load");
 
@@ -1010,16 +1011,16 @@ __device__ __forceinline__ unsigned hash_slot(uint32_t key, unsigned bits) { ret
 // non-isolated SP is what the reference collects by scanning every word of its bank: the SP
 // bytes of the words at five neighbouring addresses, OR-ed (EstimateInputSize.cu:111-141). For a
 // module pair resident in LDS the words are first entered in a hash table keyed by (bank, SP
-// address) whose value is the OR of their SP bytes, and each SP looks its five neighbours up;
-// OR-ing is order-free, so the result is the scan's. Larger module pairs scan.
+// address) whose value is the OR of their SP bytes (one word per slot: key << 8 | value), and
+// each SP looks its five neighbours up; OR-ing is order-free, so the result is the scan's.
+// Larger module pairs, and those whose banks are all small, scan.
 __device__ __forceinline__ void estimate_item(
   const char* raw,
   const mp::DirFetch& dir,
   unsigned idx,
   mp::Banks& B,
   uint32_t* lw,
-  uint32_t* hkeys,
-  uint32_t* hvals,
+  uint32_t* table_slots,
   uint32_t* estimated_input_size,
   uint8_t* sp_flags)
 {
@@ -1032,24 +1033,22 @@ __device__ __forceinline__ void estimate_item(
   unsigned largest_bank = 0;
   for (unsigned k = 0; k < B.n; ++k) largest_bank = max(largest_bank, B.start[k + 1] - B.start[k]);
   const bool table = resident && largest_bank > mp::hash_min_bank;
-  // Table of about twice as many slots as words (64 to 512; always more slots than words, so an
-  // empty slot ends every probe).
-  const unsigned bits = min(mp::hash_bits, max(6u, 32u - __clz(2 * n_words - 1)));
+  // Table of four times as many slots as words, 64 to 1024 (load at most 3/8: the wave waits for
+  // its longest probe, so the probes must stay short everywhere).
+  const unsigned bits = min(mp::hash_bits, max(6u, 32u - __clz(4 * n_words - 1)));
   const unsigned hmask = (1u << bits) - 1;
   if (resident) mp::load_words(B, 0, n_words, lw);
   if (table) {
-    for (unsigned h = lane_id(); h <= hmask; h += wave_size) {
-      hkeys[h] = mp::hash_empty;
-      hvals[h] = 0u;
-    }
+    for (unsigned h = lane_id(); h <= hmask; h += wave_size) table_slots[h] = mp::hash_empty;
     mp::wave_sync_lds();
     for (unsigned w = lane_id(); w < n_words; w += wave_size) {
       const uint32_t word = lw[w];
       const uint32_t key = (mp::bank_of(B, w) << 15) | ((word & 0x007FFF00u) >> 8);
       for (unsigned h = mp::hash_slot(key, bits);; h = (h + 1) & hmask) {
-        const uint32_t old = atomicCAS(&hkeys[h], mp::hash_empty, key);
-        if (old == mp::hash_empty || old == key) {
-          atomicOr(&hvals[h], word & 0xFFu);
+        const uint32_t old = atomicCAS(&table_slots[h], mp::hash_empty, (key << 8) | (word & 0xFFu));
+        if (old == mp::hash_empty) break;
+        if ((old >> 8) == key) {
+          atomicOr(&table_slots[h], word & 0xFFu);
           break;
         }
       }
@@ -1070,7 +1069,7 @@ __device__ __forceinline__ void estimate_item(
       if (active && !isolated) {
         // neighbour_acc's five cases, each from the OR of the words at that address (rows and
         // columns outside the SP grid match no word).
-        // The five first probes are issued together, then the rare longer probes, then the values.
+        // The five first probes are issued together, then the rare longer probes.
         const uint32_t kb = bi << 15;
         const uint32_t cols[5] = {sp_col, sp_col + 1, sp_col + 1, sp_col + 1, sp_col};
         const uint32_t rows[5] = {sp_row + 1, sp_row + 1, sp_row, sp_row - 1, sp_row - 1};
@@ -1079,18 +1078,18 @@ __device__ __forceinline__ void estimate_item(
 #pragma unroll
         for (unsigned k = 0; k < 5; ++k) {
           const bool valid = rows[k] < 64u && cols[k] < 512u;
-          key[k] = valid ? (kb | (cols[k] << 6) | rows[k]) : mp::hash_empty;
+          key[k] = valid ? (kb | (cols[k] << 6) | rows[k]) : mp::hash_no_key;
           h[k] = mp::hash_slot(key[k], bits);
-          got[k] = valid ? hkeys[h[k]] : mp::hash_empty;
+          got[k] = valid ? table_slots[h[k]] : mp::hash_empty;
         }
 #pragma unroll
         for (unsigned k = 0; k < 5; ++k)
-          while (got[k] != key[k] && got[k] != mp::hash_empty) {
+          while ((got[k] >> 8) != key[k] && got[k] != mp::hash_empty) {
             h[k] = (h[k] + 1) & hmask;
-            got[k] = hkeys[h[k]];
+            got[k] = table_slots[h[k]];
           }
 #pragma unroll
-        for (unsigned k = 0; k < 5; ++k) v[k] = got[k] == key[k] && key[k] != mp::hash_empty ? hvals[h[k]] : 0u;
+        for (unsigned k = 0; k < 5; ++k) v[k] = (got[k] >> 8) == key[k] ? (got[k] & 0xFFu) : 0u;
         const uint32_t top = v[0], top_right = v[1], right = v[2], bot_right = v[3], bot = v[4];
         pixels |= ((top & 0x01) | ((top & 0x10) << 2)) << 4;
         pixels |= (top_right & 0x01) << 16;
@@ -1134,8 +1133,7 @@ __global__ void __launch_bounds__(mp::waves_per_block* wave_size) k_estimate_mp(
   uint8_t* sp_flags)
 {
   __shared__ uint32_t s_words[mp::waves_per_block][mp::word_cap];
-  __shared__ uint32_t s_hkeys[mp::waves_per_block][mp::hash_slots];
-  __shared__ uint32_t s_hvals[mp::waves_per_block][mp::hash_slots];
+  __shared__ uint32_t s_table[mp::waves_per_block][mp::hash_slots];
   __shared__ mp::Banks s_banks[mp::waves_per_block];
   const unsigned wave = threadIdx.x / wave_size;
   const unsigned n_items = n_events * n_module_pairs;
@@ -1146,7 +1144,7 @@ __global__ void __launch_bounds__(mp::waves_per_block* wave_size) k_estimate_mp(
   for (; idx < n_items; idx += stride) {
     mp::DirFetch next = cur;
     if (idx + stride < n_items) next = mp::fetch_dir(directory, directory_count, idx + stride);
-    estimate_item(raw, cur, idx, s_banks[wave], s_words[wave], s_hkeys[wave], s_hvals[wave], estimated_input_size,
+    estimate_item(raw, cur, idx, s_banks[wave], s_words[wave], s_table[wave], estimated_input_size,
                   sp_flags);
     cur = next;
   }
