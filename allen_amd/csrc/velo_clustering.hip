@@ -986,9 +986,20 @@ template<typename F>
 __device__ __forceinline__ void
 for_words(const Banks& B, unsigned n_words, bool resident, uint32_t* lw, bool active, unsigned lo, unsigned hi, F&& f)
 {
+  // Words [a, b) of the slice in order, eight LDS reads in flight at a time.
+  auto scan = [&](const uint32_t* w, unsigned a, unsigned b) {
+    unsigned k = a;
+    for (; k + 8 <= b; k += 8) {
+      uint32_t v[8];
+#pragma unroll
+      for (unsigned u = 0; u < 8; ++u) v[u] = w[k + u];
+#pragma unroll
+      for (unsigned u = 0; u < 8; ++u) f(v[u]);
+    }
+    for (; k < b; ++k) f(w[k]);
+  };
   if (resident) {
-    if (active)
-      for (unsigned k = lo; k < hi; ++k) f(lw[k]);
+    if (active) scan(lw, lo, hi);
     return;
   }
   for (unsigned t0 = 0; t0 < n_words; t0 += word_cap) {
@@ -996,7 +1007,7 @@ for_words(const Banks& B, unsigned n_words, bool resident, uint32_t* lw, bool ac
     load_words(B, t0, tn, lw);
     if (active) {
       const unsigned a = max(lo, t0), b = min(hi, t0 + tn);
-      for (unsigned k = a; k < b; ++k) f(lw[k - t0]);
+      if (a < b) scan(lw, a - t0, b - t0);
     }
   }
 }
