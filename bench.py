@@ -67,7 +67,11 @@ def physics_of(seq, ev) -> dict:
 
 
 def algorithmic_bytes(ev, counts: dict) -> dict:
-    """Algorithmic HBM bytes of each stage for one step (definitions in DESIGN.md)."""
+    """Algorithmic HBM bytes of each stage for one step: clustering reads the raw banks, the size and
+    type arrays and the offsets, and writes 18 B per cluster and 4 B per module pair; the search reads
+    18 B per cluster and the module-pair offsets and counts, and writes a used flag per cluster, the
+    track hits, 6 B per three-hit track, 10 B per seed (tracklet and ring entry) and 16 B of counters
+    per event."""
     n = ev.n_events
     sizes16 = ev.sizes.view("<u2")
     n_banks = int(ev.sizes[n]) - int(ev.sizes[0])
@@ -123,7 +127,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the VELO path shards by events "
-              f"and a multi-GPU figure is not part of this build.", file=sys.stderr)
+              f"over independent GPUs, and this program measures one.", file=sys.stderr)
         sys.exit(2)
 
     import torch
