@@ -14,10 +14,12 @@ namespace velo {
 // ---------------------------------------------------------------------------------------
 // Prefix sum: out[0] = 0, out[i+1] = in[0] + ... + in[i] (u32 wrap), in three passes:
 // per-tile sums, a scan of the tile sums, then per-tile scans seeded with their offset.
-// Sizes up to one tile run as a single workgroup.
+// Sizes up to one_pass_items run as a single workgroup (a launch instead of three).
 constexpr unsigned scan_block = 1024;
 constexpr unsigned scan_items = 4;
 constexpr unsigned scan_tile = scan_block * scan_items;
+constexpr unsigned one_pass_items = 32;
+constexpr unsigned one_pass_max = scan_block * one_pass_items;
 
 __device__ __forceinline__ uint32_t block_scan_1024(uint32_t v, uint32_t* scratch, uint32_t* total)
 {
@@ -63,16 +65,17 @@ __global__ void __launch_bounds__(scan_block) k_scan_small_inplace(uint32_t* v, 
 
 // Per-tile scan: thread t owns items [t*items, (t+1)*items) of its tile (contiguous, so the
 // prefix within a thread is a short serial sum).
+template<unsigned items>
 __global__ void __launch_bounds__(scan_block)
   k_scan_tiles(const uint32_t* in, const uint32_t* n_dev, unsigned n_cap, uint32_t* out, const uint32_t* tile_offsets)
 {
   __shared__ uint32_t scratch[scan_block / wave_size + 1];
   const unsigned n = scan_length(n_dev, n_cap);
-  const size_t base = (size_t) blockIdx.x * scan_tile + (size_t) threadIdx.x * scan_items;
-  uint32_t v[scan_items];
+  const size_t base = (size_t) blockIdx.x * scan_block * items + (size_t) threadIdx.x * items;
+  uint32_t v[items];
   uint32_t s = 0;
 #pragma unroll
-  for (unsigned k = 0; k < scan_items; ++k) {
+  for (unsigned k = 0; k < items; ++k) {
     const size_t i = base + k;
     v[k] = i < n ? in[i] : 0u;
     s += v[k];
@@ -81,7 +84,7 @@ __global__ void __launch_bounds__(scan_block)
   uint32_t run = block_scan_1024(s, scratch, &total) + (tile_offsets ? tile_offsets[blockIdx.x] : 0u);
   if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = 0;
 #pragma unroll
-  for (unsigned k = 0; k < scan_items; ++k) {
+  for (unsigned k = 0; k < items; ++k) {
     const size_t i = base + k;
     run += v[k];
     if (i < n) out[i + 1] = run;
@@ -100,13 +103,17 @@ hipError_t launch_prefix_sum_dyn(
 {
   const unsigned tiles = (n_cap + scan_tile - 1) / scan_tile;
   if (tiles <= 1) {
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(scan_block), 0, s, in, n_dev, n_cap, out, nullptr);
+    hipLaunchKernelGGL(k_scan_tiles<scan_items>, dim3(1), dim3(scan_block), 0, s, in, n_dev, n_cap, out, nullptr);
+    return hipGetLastError();
+  }
+  if (n_cap <= one_pass_max) {
+    hipLaunchKernelGGL(k_scan_tiles<one_pass_items>, dim3(1), dim3(scan_block), 0, s, in, n_dev, n_cap, out, nullptr);
     return hipGetLastError();
   }
   if (!scratch) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_scan_tile_sums, dim3(tiles), dim3(scan_block), 0, s, in, n_dev, n_cap, scratch);
   hipLaunchKernelGGL(k_scan_small_inplace, dim3(1), dim3(scan_block), 0, s, scratch, tiles);
-  hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(scan_block), 0, s, in, n_dev, n_cap, out, scratch);
+  hipLaunchKernelGGL(k_scan_tiles<scan_items>, dim3(tiles), dim3(scan_block), 0, s, in, n_dev, n_cap, out, scratch);
   return hipGetLastError();
 }
 

@@ -253,7 +253,7 @@ def test_prefix_sum_iota(constants):
     np.testing.assert_array_equal(out.cpu().numpy(), expect)
 
 
-@pytest.mark.parametrize("n", [0, 1, 4095, 4096, 4097, 26 * 1000, 26 * 20000 + 3])
+@pytest.mark.parametrize("n", [0, 1, 4095, 4096, 4097, 26 * 1000, 32768, 32769, 26 * 20000 + 3])
 def test_prefix_sum_sizes(n, constants):
     import torch
     rng = np.random.default_rng(n)
