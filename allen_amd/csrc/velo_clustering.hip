@@ -339,10 +339,7 @@ __global__ void k_bank_directory(
   if (e >= n_events) return;
   const RawEvent ev(raw, offsets, sizes, types, e);
   uint32_t total = 0;
-  unsigned my_count = 0;
-  unsigned cnt[n_module_pairs];
-#pragma unroll
-  for (unsigned m = 0; m < n_module_pairs; ++m) cnt[m] = 0;
+  unsigned my_count = 0; // lane m: banks of module pair m so far
   for (unsigned base = 0; base < ev.n_banks; base += wave_size) {
     const unsigned b = base + lane_id();
     unsigned mp = n_module_pairs;
@@ -355,11 +352,14 @@ __global__ void k_bank_directory(
         mp = v23 ? pair / 8 : (pair << 1) / 8;
       }
     }
-#pragma unroll
-    for (unsigned m = 0; m < n_module_pairs; ++m) {
+    // One round per module pair present in this chunk: its banks take the next ranks in bank
+    // order (lane order within the chunk).
+    for (uint64_t todo = __ballot(mp < n_module_pairs); todo;) {
+      const unsigned m = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(mp), __ffsll(static_cast<unsigned long long>(todo)) - 1));
       const uint64_t bal = __ballot(mp == m);
+      const unsigned before = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(my_count), static_cast<int>(m)));
       if (mp == m) {
-        const unsigned rank = cnt[m] + __popcll(bal & lanes_below_mask());
+        const unsigned rank = before + __popcll(bal & lanes_below_mask());
         if (rank < max_banks_per_mp) {
           BankEntry en;
           en.word_offset = static_cast<uint32_t>((reinterpret_cast<const char*>(bank.word) - raw) / 4);
@@ -369,8 +369,8 @@ __global__ void k_bank_directory(
           directory[(static_cast<size_t>(e) * n_module_pairs + m) * max_banks_per_mp + rank] = en;
         }
       }
-      cnt[m] += __popcll(bal);
-      if (lane_id() == m) my_count = cnt[m];
+      if (lane_id() == m) my_count += __popcll(bal);
+      todo &= ~bal;
     }
   }
 #pragma unroll
