@@ -214,7 +214,7 @@ def main():
         # search keeps its fixed workgroup of 256.
         from allen_amd import autotune
         t0 = time.time()
-        res = autotune.tune_throughput(seqs, lambda: timed(10, 3, None) / 10, ev.n_events, log=log)
+        res = autotune.tune_throughput(seqs, lambda: timed(20, 3, None) / 20, ev.n_events, log=log)
         tuned = res.knobs
         knobs.update(tuned)
         tuning = {"default_ms_per_step": round(res.default_search_ms, 4), "best_ms_per_step": round(res.search_ms, 4),
