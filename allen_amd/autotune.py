@@ -51,7 +51,7 @@ class Result:
 # Candidates of the throughput tuning: workgroup size x LDS hit budget (the budget sets how many
 # workgroups fit a CU's LDS, the size how many lanes an event gets).
 THROUGHPUT_CANDIDATES = tuple({"search_block": b, "search_lds_hits": h}
-                              for b in (128, 192, 256) for h in (512, 640, 800, 960, 1200, 1760))
+                              for b in (128, 192, 256) for h in (512, 640, 800, 960, 1200, 1760, 2048))
 
 
 def default_knobs() -> dict:

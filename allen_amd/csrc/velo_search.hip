@@ -23,6 +23,10 @@ constexpr unsigned search_bytes_per_hit = 15;
 // Capacity of the list of unused hits of the previous module pair used by seeding step 1; a
 // larger module pair takes the pendulum walk.
 constexpr unsigned unused_list_cap = 1024;
+// Tracks to follow kept in LDS: the last ring_window positions of the ring (written through to
+// the ring in HBM, which holds all max_tracks_to_follow).
+constexpr unsigned ring_window = 1024;
+static_assert(max_tracks_to_follow % ring_window == 0, "window positions follow ring positions");
 
 // Three 0/1 flags scanned at once over the block. Exclusive ranks in r[], totals in t[].
 // One barrier: consecutive scans must alternate between two scratch buffers (a scan's reads of
@@ -247,7 +251,8 @@ struct EventOutputs {
   TrackHits* tracks;
   TrackletHits* three_hit_tracks;
   TrackletHits* tracklets;
-  uint32_t* ttf;        // the ring of tracks to follow, in LDS
+  uint32_t* ttf;        // the ring of tracks to follow in HBM (max_tracks_to_follow entries)
+  uint32_t* ttf_window; // its last ring_window positions, in LDS (position p at p % ring_window)
   uint16_t* h1_indices;
   unsigned track_capacity;
   unsigned event_number;
@@ -740,6 +745,7 @@ __device__ __forceinline__ void track_seeding(
         t.hits[2] = best_h2;
         o.tracklets[tn] = t;
         o.ttf[tn] = bit_seed | tn;
+        o.ttf_window[tn % ring_window] = bit_seed | tn;
       }
       cn.ttf += s.t[0];
       phase_mark<count>(wc, phase_seed3);
@@ -759,6 +765,7 @@ __device__ __forceinline__ void track_forwarding(
   const ModulePairData next,
   unsigned prev_ttf,
   unsigned diff_ttf,
+  bool window,
   int16_t phi_tolerance,
   float max_scatter,
   unsigned max_skipped_modules)
@@ -774,7 +781,8 @@ __device__ __forceinline__ void track_forwarding(
     bool valid = false;
     const uint16_t* t_hits = nullptr;
     if (e < diff_ttf) {
-      full = o.ttf[(prev_ttf + e) % max_tracks_to_follow];
+      const unsigned q = (prev_ttf + e) % max_tracks_to_follow;
+      full = window ? o.ttf_window[q % ring_window] : o.ttf[q];
       track_number = full & bit_track_number;
       if ((full & bit_seed) == bit_seed) {
         valid = track_number < max_tracks_to_follow;
@@ -875,6 +883,7 @@ __device__ __forceinline__ void track_forwarding(
         const unsigned p = (cn.ttf + s.r[1]) % max_tracks_to_follow;
         // A promoted track beyond the slots of its event is skipped when read back.
         o.ttf[p] = track_number;
+        o.ttf_window[p % ring_window] = track_number;
       }
       if (three) {
         const unsigned p = cn.three + s.r[2];
@@ -927,11 +936,17 @@ __device__ __forceinline__ void search_event(
     last_ttf = cn.ttf;
     const unsigned diff_ttf = last_ttf - prev_ttf;
     if (diff_ttf >= max_tracks_to_follow) cn.status |= status_ttf_ring_overflow;
+    // The positions read here, [prev_ttf, last_ttf), and every position written from their
+    // writing to their reading (this forwarding reinserts at most diff_ttf entries) span at most
+    // 2 diff_ttf positions: within the LDS window they are all distinct. Beyond it the ring is
+    // read back from HBM, after a fence that makes this workgroup's ring writes visible to it.
+    const bool window = 2 * diff_ttf <= ring_window;
+    if (!window) __threadfence_block();
     // Orders the ring slots and tracklets written by the last seeding chunk before they are
     // read (seeding chunks end right after their writes, without a barrier).
     __syncthreads();
     track_forwarding<count>(
-      wc, hits, sh, cn, o, sh.mp[first_module_pair - 2], prev_ttf, diff_ttf, phi_tolerance, max_scatter,
+      wc, hits, sh, cn, o, sh.mp[first_module_pair - 2], prev_ttf, diff_ttf, window, phi_tolerance, max_scatter,
       max_skipped_modules);
     __syncthreads();
     phase_mark<count>(wc, phase_forwarding);
@@ -958,12 +973,18 @@ __device__ __forceinline__ void search_event(
   last_ttf = cn.ttf;
   const unsigned diff_ttf = last_ttf - prev_ttf;
   if (diff_ttf >= max_tracks_to_follow) cn.status |= status_ttf_ring_overflow;
+  const bool window = diff_ttf <= ring_window; // nothing is written while these are read
+  if (!window) {
+    __threadfence_block();
+    __syncthreads();
+  }
   for (unsigned c = 0; c < diff_ttf; c += blockDim.x) {
     const unsigned e = c + threadIdx.x;
     bool seed = false;
     uint32_t full = 0;
     if (e < diff_ttf) {
-      full = o.ttf[(prev_ttf + e) % max_tracks_to_follow];
+      const unsigned q = (prev_ttf + e) % max_tracks_to_follow;
+      full = window ? o.ttf_window[q % ring_window] : o.ttf[q];
       seed = (full & bit_seed) == bit_seed;
     }
     const Scan3 s = scan3(sh, cn, seed, false, false);
@@ -978,7 +999,8 @@ __device__ __forceinline__ void search_event(
 
 // One workgroup per event, or (work_counter set) a resident grid whose workgroups take the
 // next event slot from the counter until the slots run out. Dynamic LDS: the seeding scratch
-// (13 B per lane), the ring of tracks to follow (2048 x u32), the list of unused hits of the
+// (13 B per lane), the LDS window of the ring of tracks to follow (ring_window x u32; the ring
+// itself, 2048 x u32, is written through to HBM), the list of unused hits of the
 // previous module pair (1024 x u16), then `lds_hits` hits as x[], y[], z[] (f32), phi[] (i16)
 // and a flags byte (used, module parity). Events with up to 5 x lds_hits hits keep phi and the
 // flags in LDS and read coordinates from global memory; larger events run on global memory.
@@ -1027,7 +1049,7 @@ __global__ void k_search_by_triplet(
   ss.h1_chunk = ss.pair_h2 + blockDim.x;
   ss.n_h0 = reinterpret_cast<uint8_t*>(ss.h1_chunk + blockDim.x);
   uint32_t* ring = reinterpret_cast<uint32_t*>(dyn_all + ((13 * blockDim.x + 15) & ~15u));
-  ss.unused = reinterpret_cast<uint16_t*>(ring + max_tracks_to_follow);
+  ss.unused = reinterpret_cast<uint16_t*>(ring + ring_window);
   char* dyn = reinterpret_cast<char*>(ss.unused + unused_list_cap);
   ScorerLds* scorer = nullptr;
   if constexpr (seed_mode == seed_mode_scored) {
@@ -1068,7 +1090,8 @@ __global__ void k_search_by_triplet(
     o.tracks = tracks_all + toff;
     o.three_hit_tracks = three_hit_all + toff;
     o.tracklets = tracklets_all + (size_t) event_number * max_tracks_to_follow;
-    o.ttf = ring;
+    o.ttf = ttf_all + (size_t) event_number * max_tracks_to_follow;
+    o.ttf_window = ring;
     o.h1_indices = rel_indices_all + hit_offset;
     o.event_number = event_number;
     o.seed_log = seed_log;
@@ -1133,10 +1156,6 @@ __global__ void k_search_by_triplet(
       search_event<count, seed_mode>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
       __syncthreads();
     }
-    // The ring of tracks to follow as far as it was written (dev_tracks_to_follow).
-    uint32_t* gttf = ttf_all + (size_t) event_number * max_tracks_to_follow;
-    const unsigned ring_written = min(cn.ttf, max_tracks_to_follow);
-    for (unsigned i = threadIdx.x; i < ring_written; i += blockDim.x) gttf[i] = ring[i];
     if constexpr (count) {
       // Wave reduction, then one atomic per wave.
       unsigned long long v[4] = {wc.seed_eval, wc.fwd_eval, wc.extrap, wc.steps};
@@ -1320,10 +1339,10 @@ __global__ void k_three_hit_filter(
   if (threadIdx.x == 0) number_of_output[event_number] = base;
 }
 
-// Launch geometry: the seeding scratch, the ring, the unused-hit list, then the hits.
+// Launch geometry: the seeding scratch, the ring window, the unused-hit list, then the hits.
 size_t search_dynamic_lds(unsigned block, unsigned lds_hits)
 {
-  return ((13 * (size_t) block + 15) & ~size_t(15)) + 4 * (size_t) max_tracks_to_follow + 2 * unused_list_cap +
+  return ((13 * (size_t) block + 15) & ~size_t(15)) + 4 * (size_t) ring_window + 2 * unused_list_cap +
          (size_t) lds_hits * search_bytes_per_hit;
 }
 
