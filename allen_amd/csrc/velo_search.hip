@@ -25,8 +25,10 @@ constexpr unsigned search_bytes_per_hit = 15;
 constexpr unsigned unused_list_cap = 1024;
 // Tracks to follow kept in LDS: the last ring_window positions of the ring (written through to
 // the ring in HBM, which holds all max_tracks_to_follow).
-constexpr unsigned ring_window = 1024;
+constexpr unsigned ring_window = 512;
+constexpr unsigned ring_window_bytes = 4 + 4 + 2 + 1; // entry, last two hits, first hit, number of hits
 static_assert(max_tracks_to_follow % ring_window == 0, "window positions follow ring positions");
+static_assert(ring_window % 16 == 0, "the arrays after the window stay aligned");
 
 // Three 0/1 flags scanned at once over the block. Exclusive ranks in r[], totals in t[].
 // One barrier: consecutive scans must alternate between two scratch buffers (a scan's reads of
@@ -253,6 +255,11 @@ struct EventOutputs {
   TrackletHits* tracklets;
   uint32_t* ttf;        // the ring of tracks to follow in HBM (max_tracks_to_follow entries)
   uint32_t* ttf_window; // its last ring_window positions, in LDS (position p at p % ring_window)
+  // Beside each window entry, what forwarding reads of its track: the last two hits (low and
+  // high 16 bits), the first hit (read for a three-hit seed) and the number of hits.
+  uint32_t* ttf_last2;
+  uint16_t* ttf_first;
+  uint8_t* ttf_n;
   uint16_t* h1_indices;
   unsigned track_capacity;
   unsigned event_number;
@@ -745,7 +752,11 @@ __device__ __forceinline__ void track_seeding(
         t.hits[2] = best_h2;
         o.tracklets[tn] = t;
         o.ttf[tn] = bit_seed | tn;
-        o.ttf_window[tn % ring_window] = bit_seed | tn;
+        const unsigned wslot = tn % ring_window;
+        o.ttf_window[wslot] = bit_seed | tn;
+        o.ttf_last2[wslot] = static_cast<uint32_t>(h1_index) | (static_cast<uint32_t>(best_h2) << 16);
+        o.ttf_first[wslot] = best_h0;
+        o.ttf_n[wslot] = 3;
       }
       cn.ttf += s.t[0];
       phase_mark<count>(wc, phase_seed3);
@@ -779,30 +790,51 @@ __device__ __forceinline__ void track_forwarding(
     uint16_t th[3] = {0, 0, 0};
     unsigned track_number = 0;
     bool valid = false;
-    const uint16_t* t_hits = nullptr;
+    unsigned h0_num = 0, h1_num = 0;
     if (e < diff_ttf) {
       const unsigned q = (prev_ttf + e) % max_tracks_to_follow;
-      full = window ? o.ttf_window[q % ring_window] : o.ttf[q];
-      track_number = full & bit_track_number;
-      if ((full & bit_seed) == bit_seed) {
-        valid = track_number < max_tracks_to_follow;
-        if (valid) t_hits = o.tracklets[track_number].hits;
-        number_of_hits = 3;
-      }
-      else if (track_number < o.track_capacity) {
+      if (window) {
+        // The entry and its track's last hits from LDS (written with the entry).
+        const unsigned wslot = q % ring_window;
+        full = o.ttf_window[wslot];
+        const uint32_t last2 = o.ttf_last2[wslot];
+        h0_num = last2 & 0xFFFFu;
+        h1_num = last2 >> 16;
+        number_of_hits = o.ttf_n[wslot];
+        th[0] = o.ttf_first[wslot];
+        th[1] = static_cast<uint16_t>(h0_num);
+        th[2] = static_cast<uint16_t>(h1_num);
+        track_number = full & bit_track_number;
+        if ((full & bit_seed) == bit_seed) valid = track_number < max_tracks_to_follow;
         // Guards that only matter once the slots of an event have overflowed (status raised).
-        t_hits = o.tracks[track_number].hits;
-        number_of_hits = o.tracks[track_number].hitsNum;
-        valid = number_of_hits >= 3 && number_of_hits < max_track_size;
+        else valid = track_number < o.track_capacity && number_of_hits >= 3 && number_of_hits < max_track_size;
+      }
+      else {
+        full = o.ttf[q];
+        track_number = full & bit_track_number;
+        const uint16_t* t_hits = nullptr;
+        if ((full & bit_seed) == bit_seed) {
+          valid = track_number < max_tracks_to_follow;
+          if (valid) t_hits = o.tracklets[track_number].hits;
+          number_of_hits = 3;
+        }
+        else if (track_number < o.track_capacity) {
+          // Guards that only matter once the slots of an event have overflowed (status raised).
+          t_hits = o.tracks[track_number].hits;
+          number_of_hits = o.tracks[track_number].hitsNum;
+          valid = number_of_hits >= 3 && number_of_hits < max_track_size;
+        }
+        if (valid) {
+          h0_num = t_hits[number_of_hits - 2];
+          h1_num = t_hits[number_of_hits - 1];
+          th[0] = t_hits[0];
+          th[1] = t_hits[1];
+          th[2] = t_hits[2];
+        }
       }
     }
     if (valid) {
       const unsigned skipped = (full & bit_skipped) >> skipped_pos;
-      const unsigned h0_num = t_hits[number_of_hits - 2];
-      const unsigned h1_num = t_hits[number_of_hits - 1];
-      th[0] = t_hits[0];
-      th[1] = t_hits[1];
-      th[2] = t_hits[2];
       const float h0x = hits.x(h0_num), h0y = hits.y(h0_num), h0z = hits.z(h0_num);
       const float h1x = hits.x(h1_num), h1y = hits.y(h1_num), h1z = hits.z(h1_num);
       const float td = 1.0f / (h1z - h0z);
@@ -883,7 +915,17 @@ __device__ __forceinline__ void track_forwarding(
         const unsigned p = (cn.ttf + s.r[1]) % max_tracks_to_follow;
         // A promoted track beyond the slots of its event is skipped when read back.
         o.ttf[p] = track_number;
-        o.ttf_window[p % ring_window] = track_number;
+        const unsigned wslot = p % ring_window;
+        o.ttf_window[wslot] = track_number;
+        if (best_h2 != -1) {
+          o.ttf_last2[wslot] = h1_num | (static_cast<uint32_t>(best_h2) << 16);
+          o.ttf_n[wslot] = static_cast<uint8_t>(number_of_hits + 1);
+        }
+        else {
+          o.ttf_last2[wslot] = h0_num | (h1_num << 16);
+          o.ttf_n[wslot] = static_cast<uint8_t>(number_of_hits);
+        }
+        o.ttf_first[wslot] = th[0];
       }
       if (three) {
         const unsigned p = cn.three + s.r[2];
@@ -1049,7 +1091,10 @@ __global__ void k_search_by_triplet(
   ss.h1_chunk = ss.pair_h2 + blockDim.x;
   ss.n_h0 = reinterpret_cast<uint8_t*>(ss.h1_chunk + blockDim.x);
   uint32_t* ring = reinterpret_cast<uint32_t*>(dyn_all + ((13 * blockDim.x + 15) & ~15u));
-  ss.unused = reinterpret_cast<uint16_t*>(ring + ring_window);
+  uint32_t* ring_last2 = ring + ring_window;
+  uint16_t* ring_first = reinterpret_cast<uint16_t*>(ring_last2 + ring_window);
+  uint8_t* ring_n = reinterpret_cast<uint8_t*>(ring_first + ring_window);
+  ss.unused = reinterpret_cast<uint16_t*>(ring_n + ring_window);
   char* dyn = reinterpret_cast<char*>(ss.unused + unused_list_cap);
   ScorerLds* scorer = nullptr;
   if constexpr (seed_mode == seed_mode_scored) {
@@ -1092,6 +1137,9 @@ __global__ void k_search_by_triplet(
     o.tracklets = tracklets_all + (size_t) event_number * max_tracks_to_follow;
     o.ttf = ttf_all + (size_t) event_number * max_tracks_to_follow;
     o.ttf_window = ring;
+    o.ttf_last2 = ring_last2;
+    o.ttf_first = ring_first;
+    o.ttf_n = ring_n;
     o.h1_indices = rel_indices_all + hit_offset;
     o.event_number = event_number;
     o.seed_log = seed_log;
@@ -1342,7 +1390,7 @@ __global__ void k_three_hit_filter(
 // Launch geometry: the seeding scratch, the ring window, the unused-hit list, then the hits.
 size_t search_dynamic_lds(unsigned block, unsigned lds_hits)
 {
-  return ((13 * (size_t) block + 15) & ~size_t(15)) + 4 * (size_t) ring_window + 2 * unused_list_cap +
+  return ((13 * (size_t) block + 15) & ~size_t(15)) + ring_window_bytes * (size_t) ring_window + 2 * unused_list_cap +
          (size_t) lds_hits * search_bytes_per_hit;
 }
 
