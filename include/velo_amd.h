@@ -394,7 +394,7 @@ int velo_sequence_status(velo_sequence* s, hipStream_t stream, unsigned* status)
 /* Knobs applied to subsequent runs:
  *   "search_block"      workgroup size of search_by_triplet (multiple of 64, <= 1024; 0 = 256)
  *   "search_lds_hits"   hits of an event staged in LDS by search_by_triplet (<= 8192, within the
- *                       device's LDS per workgroup; 0 = 1760); up to 4x that many hits keep phi
+ *                       device's LDS per workgroup; 0 = 1760); up to 5x that many hits keep phi
  *                       and flags in LDS and coordinates in global memory, larger events run on
  *                       global memory
  *   "search_largest_first"  1 (default): search_by_triplet takes the events in descending order
