@@ -172,6 +172,8 @@ __global__ void k_copy_track_hit_number(
 }
 
 // K8: one thread per (track, hit); the event's tracks are [main tracks][filtered 3-hit tracks].
+constexpr unsigned consolidate_lds_tracks = 2047; // tracks of an event whose offsets are staged in LDS
+
 __global__ void k_consolidate_tracks(
   const uint32_t* event_list,
   const uint32_t* number_of_events,
@@ -201,15 +203,23 @@ __global__ void k_consolidate_tracks(
   const unsigned h_end = offsets_velo_track_hit_number[t0 + n_total];
   for (unsigned i = threadIdx.x; i < n_total; i += blockDim.x)
     if (accepted) accepted[t0 + i] = 1;
+  // The event's slice of the hit-number offsets, in LDS when it fits: each hit finds its track by
+  // a binary search over it.
+  __shared__ uint32_t s_off[consolidate_lds_tracks + 1];
+  const bool in_lds = n_total <= consolidate_lds_tracks;
+  if (in_lds)
+    for (unsigned i = threadIdx.x; i <= n_total; i += blockDim.x) s_off[i] = offsets_velo_track_hit_number[t0 + i];
+  __syncthreads();
+  const uint32_t* off = in_lds ? s_off : offsets_velo_track_hit_number + t0;
   for (unsigned h = h_begin + threadIdx.x; h < h_end; h += blockDim.x) {
     // track of hit h: binary search in the event's slice of the hit-number offsets
     unsigned lo = 0, hi = n_total;
     while (hi - lo > 1) {
       const unsigned mid = (lo + hi) / 2;
-      if (offsets_velo_track_hit_number[t0 + mid] <= h) lo = mid;
+      if (off[mid] <= h) lo = mid;
       else hi = mid;
     }
-    const unsigned k = h - offsets_velo_track_hit_number[t0 + lo];
+    const unsigned k = h - off[lo];
     const unsigned hit = lo < n_main ? tracks_all[toff + lo].hits[k] : three_hit_output_all[toff + lo - n_main].hits[k];
     const size_t src = (size_t) hit_offset + hit;
     out.ids()[h] = in.ids()[src];
