@@ -1176,6 +1176,22 @@ __device__ __forceinline__ uint64_t phi_id_key(int16_t phi, uint32_t id)
   return (static_cast<uint64_t>(static_cast<uint16_t>(phi) ^ 0x8000u) << 32) | id;
 }
 
+// The number of keys[0 .. n) below `key` (n is wave-uniform: every lane reads the same key,
+// eight reads in flight at a time).
+__device__ __forceinline__ unsigned keys_below(const uint64_t* keys, unsigned n, uint64_t key)
+{
+  unsigned below = 0, j = 0;
+  for (; j + 8 <= n; j += 8) {
+    uint64_t v[8];
+#pragma unroll
+    for (unsigned u = 0; u < 8; ++u) v[u] = keys[j + u];
+#pragma unroll
+    for (unsigned u = 0; u < 8; ++u) below += v[u] < key;
+  }
+  for (; j < n; ++j) below += keys[j] < key;
+  return below;
+}
+
 // K3 + K4 of one module pair (idx = event * 26 + module pair) by one wave.
 template<int version>
 __device__ __forceinline__ void cluster_item(
@@ -1292,8 +1308,7 @@ __device__ __forceinline__ void cluster_item(
       c.x = lxyz[3 * i];
       c.y = lxyz[3 * i + 1];
       c.z = lxyz[3 * i + 2];
-      unsigned rank = 0;
-      for (unsigned j = 0; j < n; ++j) rank += lkey[j] < key;
+      const unsigned rank = keys_below(lkey, n, key);
       store_cluster(out, start + rank, c);
     }
     return;
@@ -1321,7 +1336,7 @@ __device__ __forceinline__ void cluster_item(
       mp::wave_sync_lds();
       if (mine) {
         const uint64_t key = phi_id_key(c.phi, c.id);
-        for (unsigned j = 0; j < tn; ++j) rank += lkey[j] < key;
+        rank += keys_below(lkey, tn, key);
       }
     }
     if (mine) store_cluster(out, start + rank, c);
