@@ -954,12 +954,14 @@ __device__ __forceinline__ void load_banks(const char* raw, const DirFetch& f, B
   wave_sync_lds();
 }
 
+// The bank holding word w: the starts are non-decreasing, so the last bank starting at or before
+// w is the number of banks after the first that do (all starts read at once, one LDS round trip).
 __device__ __forceinline__ unsigned bank_of(const Banks& B, unsigned w)
 {
-  unsigned bi = 0;
   const unsigned n = B.n;
-  for (unsigned k = 1; k < n; ++k)
-    if (B.start[k] <= w) bi = k;
+  unsigned bi = 0;
+#pragma unroll
+  for (unsigned k = 1; k < max_banks_per_mp; ++k) bi += (k < n && B.start[k] <= w) ? 1u : 0u;
   return bi;
 }
 } // namespace mp
