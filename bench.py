@@ -339,9 +339,16 @@ def main():
     # ~12 per extrapolation (hit_phi_16), 3 per binary-search iteration.
     ops = 9 * (int(work[0]) + int(work[1])) + 12 * int(work[2]) + 3 * int(work[3])
     search_tops = ops / (stage_ms["search"] / 1e3) / 1e12
+    # Clustering (or retina decoding) against HBM: its algorithmic bytes over its one-stream launch time.
+    cl_gbs = alg["clustering"] / (stage_ms["clustering"] / 1e3) / 1e9
     extra = {"search_valu": {"ops_per_step": ops, "achieved_tops": round(search_tops, 4),
                              "peak_tops": round(VALU_PEAK_TOPS, 2), "frac": round(search_tops / VALU_PEAK_TOPS, 5),
                              "work": [int(x) for x in work]},
+             "clustering_hbm": {"achieved": round(cl_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": round(cl_gbs / HBM_PEAK_GBS, 5), "algorithmic_bytes": alg["clustering"],
+                                "avg_launch_ms": round(stage_ms["clustering"], 4),
+                                "traffic": measured_traffic(
+                                    args.workload + ("_retina" if args.decoding == "retina" else ""), "clustering")},
              "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
              "search_events": search_events,
              "physics": physics}
