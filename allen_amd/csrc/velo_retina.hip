@@ -230,9 +230,21 @@ __global__ void __launch_bounds__(waves_per_block* wave_size) k_retina_decode(
   for (unsigned i = l; i < n; i += wave_size) {
     const Decoded d = decode_at(i);
     if (!d.valid) continue;
+    // Every lane reads the same keys (n is wave-uniform), eight reads in flight at a time.
     unsigned position = 0;
     bool last = true;
-    for (unsigned j = 0; j < n; ++j) {
+    unsigned j = 0;
+    for (; j + 8 <= n; j += 8) {
+      int64_t o[8];
+#pragma unroll
+      for (unsigned u = 0; u < 8; ++u) o[u] = kk[j + u];
+#pragma unroll
+      for (unsigned u = 0; u < 8; ++u) {
+        position += d.key > o[u];
+        last = last && !(j + u > i && o[u] == d.key);
+      }
+    }
+    for (; j < n; ++j) {
       const int64_t o = kk[j];
       position += d.key > o;
       last = last && !(j > i && o == d.key);
