@@ -203,8 +203,10 @@ __global__ void __launch_bounds__(waves_per_block* wave_size) k_retina_decode(
     return decode<version>(g, bank, bank.word[w]);
   };
   uint32_t nonzero = 0;
+  Decoded first {}; // the lane's first word, kept for the ranking pass
   for (unsigned k = l; k < n; k += wave_size) {
     const Decoded d = decode_at(k);
+    if (k == l) first = d;
     // A word of another bank type has no key in the reference (left unset there); it ranks
     // last here and is not written.
     const int64_t key = d.valid ? d.key : 0x7FFFFFFFFFFFFFFFll;
@@ -228,7 +230,7 @@ __global__ void __launch_bounds__(waves_per_block* wave_size) k_retina_decode(
   // Rank by key; the last word of a key's run writes the slot.
   const Clusters out {cluster_container, total};
   for (unsigned i = l; i < n; i += wave_size) {
-    const Decoded d = decode_at(i);
+    const Decoded d = i == l ? first : decode_at(i);
     if (!d.valid) continue;
     // Every lane reads the same keys (n is wave-uniform), eight reads in flight at a time.
     unsigned position = 0;
