@@ -277,10 +277,21 @@ int velo_decode_retinaclusters(const velo_decode_retinaclusters_params* p, hipSt
     return VELO_ERR_INVALID_ARGUMENT;
   if (p->mep_layout) return VELO_ERR_UNSUPPORTED_LAYOUT;
   const size_t items = (size_t) p->host_number_of_events * velo::n_module_pairs;
+  const size_t T = p->host_total_number_of_velo_clusters;
+  // The reference's memsets (DecodeRetinaClusters.cu:428-447).
   int rc = hip_status(velo::launch_fill_u32(p->dev_module_cluster_num, items, 0, stream));
   if (rc == VELO_OK) rc = hip_status(velo::launch_fill_u32(p->dev_offsets_module_pair_cluster, items + 1, 0, stream));
-  if (rc != VELO_OK || p->host_raw_bank_version < 0) return rc;
+  if (rc == VELO_OK && p->host_raw_bank_version < 0) {
+    rc = hip_status(velo::launch_fill_u8(reinterpret_cast<uint8_t*>(p->dev_velo_cluster_container), 18 * T, 0, stream));
+    if (rc == VELO_OK)
+      rc = hip_status(velo::launch_fill_u32(reinterpret_cast<uint32_t*>(p->dev_hit_sorting_key), 2 * T, 0, stream));
+    return rc;
+  }
+  if (rc != VELO_OK) return rc;
   if (!version_ok(p->host_raw_bank_version)) return VELO_ERR_BANK_VERSION;
+  if (items > 0xFFFFFFFFull / 2) return VELO_ERR_INVALID_ARGUMENT;
+  rc = hip_status(velo::launch_fill_u32(p->dev_hit_permutations, T, 0, stream));
+  if (rc != VELO_OK) return rc;
   velo::RetinaDecodeLaunch a;
   a.version = p->host_raw_bank_version;
   a.raw = p->dev_velo_retina_raw_input;
@@ -299,6 +310,7 @@ int velo_decode_retinaclusters(const velo_decode_retinaclusters_params* p, hipSt
   a.cluster_container = p->dev_velo_cluster_container;
   a.hit_permutations = p->dev_hit_permutations;
   a.hit_sorting_key = p->dev_hit_sorting_key;
+  a.populate_items = static_cast<unsigned>(items);
   return hip_status(velo::launch_retina_decode(a, stream));
 }
 
@@ -439,8 +451,15 @@ int velo_search_by_triplet(const velo_search_by_triplet_params* p, hipStream_t s
   a.block_dim = p->block_dim_x;
   a.lds_hits = 0;
   a.work_stats = nullptr;
-  // The kernel writes every per-event output, so the reference's memsets are not needed.
-  return hip_status(velo::launch_search_by_triplet(a, stream));
+  // The reference's memsets (SearchByTriplet.cu:47-49): events left out of dev_event_list keep
+  // zero track counts and atomics, and no hit is marked used.
+  int rc = hip_status(velo::launch_fill_u32(
+    p->dev_atomics_velo, (size_t) p->host_number_of_events * velo::num_atomics, 0, stream));
+  if (rc == VELO_OK) rc = hip_status(velo::launch_fill_u8(p->dev_hit_used, p->host_total_number_of_velo_clusters, 0, stream));
+  if (rc == VELO_OK)
+    rc = hip_status(velo::launch_fill_u32(p->dev_number_of_velo_tracks, p->host_number_of_events, 0, stream));
+  if (rc == VELO_OK) rc = hip_status(velo::launch_search_by_triplet(a, stream));
+  return rc;
 }
 
 int velo_three_hit_tracks_filter(const velo_three_hit_tracks_filter_params* p, hipStream_t stream)
@@ -503,7 +522,13 @@ int velo_consolidate_tracks(const velo_consolidate_tracks_params* p, hipStream_t
   a.offsets_three_hit = p->dev_offsets_number_of_three_hit_tracks_filtered;
   a.velo_track_hits = p->dev_velo_track_hits;
   a.accepted_velo_tracks = p->dev_accepted_velo_tracks;
-  return hip_status(velo::launch_consolidate_tracks(a, stream));
+  // Every track is accepted (VeloConsolidateTracks.cu:84), listed event or not.
+  int rc = VELO_OK;
+  if (p->dev_accepted_velo_tracks)
+    rc = hip_status(
+      velo::launch_fill_u8(p->dev_accepted_velo_tracks, p->host_number_of_reconstructed_velo_tracks, 1, stream));
+  if (rc == VELO_OK) rc = hip_status(velo::launch_consolidate_tracks(a, stream));
+  return rc;
 }
 
 int velo_kalman_filter(const velo_kalman_filter_params* p, hipStream_t stream)
@@ -865,6 +890,7 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream
     r2.cluster_container = sorted;
     r2.hit_permutations = s->get<uint32_t>("hit_permutations");
     r2.hit_sorting_key = s->get<int64_t>("hit_sorting_key");
+    r2.populate_items = E * velo::n_module_pairs;
     VELO_MARK(mark_begin(VELO_STAGE_CLUSTERING));
     VELO_TRY(velo::launch_retina_decode(r2, stream));
     VELO_MARK(mark_end(VELO_STAGE_CLUSTERING));

@@ -1397,37 +1397,25 @@ __global__ void __launch_bounds__(mp::waves_per_block* wave_size, 6) k_cluster_m
   }
 }
 
-// Blocks of `kernel` that stay resident on the whole device (0 if the runtime cannot tell).
-static unsigned resident_blocks(const void* kernel, unsigned block)
-{
-  int dev = 0, cus = 0, per_cu = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return 0;
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, static_cast<int>(block), 0) != hipSuccess)
-    return 0;
-  return static_cast<unsigned>(cus) * static_cast<unsigned>(per_cu);
-}
-
 // One wave per item, or (resident) at most the blocks that stay resident, whose waves loop
 // over the items. One wave per item balances uneven module pairs better (the dispatcher starts
 // a new wave wherever one ends), and is the default.
-static unsigned mp_grid(const void* kernel, unsigned n_items, bool resident, unsigned* cache)
+static unsigned mp_grid(const void* kernel, unsigned n_items, bool resident)
 {
   const unsigned need = (n_items + mp::waves_per_block - 1) / mp::waves_per_block;
   if (!resident) return need;
-  if (!*cache) *cache = resident_blocks(kernel, mp::waves_per_block * wave_size);
-  return *cache ? min(need, *cache) : need;
+  const unsigned blocks = device_resident_blocks(kernel, mp::waves_per_block * wave_size, 0);
+  return blocks ? min(need, blocks) : need;
 }
 
 hipError_t launch_estimate_mp(const EstimateMpLaunch& a, hipStream_t s)
 {
   if (!a.n_events) return hipSuccess;
-  static unsigned cache[5] = {0, 0, 0, 0, 0};
   const unsigned v = a.version == 2 || a.version == 3 ? a.version : 4;
   const void* kernel = v == 2 ? reinterpret_cast<const void*>(&k_estimate_mp<2>)
                      : v == 3 ? reinterpret_cast<const void*>(&k_estimate_mp<3>)
                               : reinterpret_cast<const void*>(&k_estimate_mp<4>);
-  const unsigned grid = mp_grid(kernel, a.n_events * n_module_pairs, a.resident_grid, &cache[v]);
+  const unsigned grid = mp_grid(kernel, a.n_events * n_module_pairs, a.resident_grid);
 #define VELO_LAUNCH_EMP(V)                                                                                          \
   hipLaunchKernelGGL(                                                                                               \
     k_estimate_mp<V>, dim3(grid), dim3(mp::waves_per_block * wave_size), 0, s, a.raw,                               \
@@ -1444,12 +1432,11 @@ hipError_t launch_estimate_mp(const EstimateMpLaunch& a, hipStream_t s)
 hipError_t launch_cluster_mp(const ClusterMpLaunch& a, hipStream_t s)
 {
   if (!a.n_events) return hipSuccess;
-  static unsigned cache[5] = {0, 0, 0, 0, 0};
   const unsigned v = a.version == 2 || a.version == 3 ? a.version : 4;
   const void* kernel = v == 2 ? reinterpret_cast<const void*>(&k_cluster_mp<2>)
                      : v == 3 ? reinterpret_cast<const void*>(&k_cluster_mp<3>)
                               : reinterpret_cast<const void*>(&k_cluster_mp<4>);
-  const unsigned grid = mp_grid(kernel, a.n_events * n_module_pairs, a.resident_grid, &cache[v]);
+  const unsigned grid = mp_grid(kernel, a.n_events * n_module_pairs, a.resident_grid);
 #define VELO_LAUNCH_CMP(V)                                                                                          \
   hipLaunchKernelGGL(                                                                                               \
     k_cluster_mp<V>, dim3(grid), dim3(mp::waves_per_block * wave_size), 0, s, a.raw,                                \

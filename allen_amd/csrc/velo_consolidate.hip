@@ -8,6 +8,9 @@
 #include "velo_kernels.h"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 namespace velo {
 
@@ -146,6 +149,43 @@ hipError_t launch_fill_u8(uint8_t* p, size_t n, uint8_t value, hipStream_t s)
   const unsigned grid = (unsigned) std::min<size_t>((n + 255) / 256, 4096);
   hipLaunchKernelGGL(k_fill_u8, dim3(grid), dim3(256), 0, s, p, n, value);
   return hipGetLastError();
+}
+
+namespace {
+std::mutex g_query_mutex;
+std::map<std::tuple<int, const void*, unsigned, size_t>, unsigned> g_resident; // (device, kernel, block, lds)
+std::map<int, int> g_max_lds;                                                   // device -> bytes
+} // namespace
+
+int device_max_lds_per_block()
+{
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  std::lock_guard<std::mutex> lock(g_query_mutex);
+  auto it = g_max_lds.find(dev);
+  if (it != g_max_lds.end()) return it->second;
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
+  g_max_lds.emplace(dev, v);
+  return v;
+}
+
+unsigned device_resident_blocks(const void* kernel, unsigned block, size_t dyn_lds)
+{
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  const auto key = std::make_tuple(dev, kernel, block, dyn_lds);
+  std::lock_guard<std::mutex> lock(g_query_mutex);
+  auto it = g_resident.find(key);
+  if (it != g_resident.end()) return it->second;
+  int cus = 0, per_cu = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, static_cast<int>(block), dyn_lds) != hipSuccess ||
+      cus <= 0 || per_cu <= 0)
+    return 0;
+  const unsigned v = static_cast<unsigned>(cus) * static_cast<unsigned>(per_cu);
+  g_resident.emplace(key, v);
+  return v;
 }
 
 // ---------------------------------------------------------------------------------------

@@ -56,6 +56,9 @@ struct RetinaDecodeLaunch {
   char* cluster_container;
   uint32_t* hit_permutations;
   int64_t* hit_sorting_key;
+  // Entries of offsets_module_pair_cluster written, by (event, module pair) index over all
+  // events whether listed or not (DecodeRetinaClusters.cu:11-36): 26 x host_number_of_events.
+  unsigned populate_items = 0;
 };
 unsigned retina_sensor_slots(int version);
 hipError_t launch_retina_sensor_sizes(const RetinaSizesLaunch& a, hipStream_t s);
@@ -358,5 +361,13 @@ hipError_t launch_prefix_sum_dyn(
 // Fill `n` u32 with `value` (a kernel, so the work can be captured in a graph).
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t value, hipStream_t s);
 hipError_t launch_fill_u8(uint8_t* p, size_t n, uint8_t value, hipStream_t s);
+
+// Launch-geometry queries of the current device, computed once per (device, query) and shared by
+// every host thread (the lookups are serialised by a mutex).
+// Dynamic LDS a workgroup may use (hipDeviceAttributeMaxSharedMemoryPerBlock), 0 on failure.
+int device_max_lds_per_block();
+// Workgroups of `kernel` with `block` lanes and `dyn_lds` bytes of dynamic LDS that stay resident
+// on the whole device (CUs x occupancy), 0 if the runtime cannot tell.
+unsigned device_resident_blocks(const void* kernel, unsigned block, size_t dyn_lds);
 
 } // namespace velo

@@ -154,20 +154,22 @@ __global__ void __launch_bounds__(waves_per_block* wave_size) k_retina_decode(
   uint32_t* offsets_module_pair_cluster,
   char* cluster_container,
   uint32_t* hit_permutations,
-  int64_t* hit_sorting_key)
+  int64_t* hit_sorting_key,
+  unsigned populate_items)
 {
   constexpr unsigned S = slots<version>();
   constexpr unsigned step = S / n_module_pairs;
   __shared__ int64_t s_key[waves_per_block][key_cap];
   const unsigned wave = threadIdx.x / wave_size, l = lane_id();
   const unsigned item = blockIdx.x * waves_per_block + wave;
-  if (item >= n_list * n_module_pairs) return;
-  const unsigned b = item / n_module_pairs, mp = item % n_module_pairs;
-  // populate_module_pair_offsets_and_sizes (:11-36), element by element.
-  if (l == 0) {
+  // populate_module_pair_offsets_and_sizes (:11-36), element by element over every event's
+  // module pairs (listed or not); the grid covers max(populate_items, listed items).
+  if (l == 0 && item < populate_items) {
     offsets_module_pair_cluster[item + 1] = offsets_each_sensor_pair_size[static_cast<size_t>(item + 1) * step];
     if (item == 0) offsets_module_pair_cluster[0] = 0;
   }
+  if (item >= n_list * n_module_pairs) return;
+  const unsigned b = item / n_module_pairs, mp = item % n_module_pairs;
   const unsigned event_number = event_list ? event_list[b] : b;
   const uint32_t* sp_off = offsets_each_sensor_pair_size + static_cast<size_t>(event_number) * S;
   const uint32_t lo = sp_off[mp * step], hi = sp_off[(mp + 1) * step];
@@ -281,15 +283,15 @@ hipError_t launch_retina_sensor_sizes(const RetinaSizesLaunch& a, hipStream_t s)
 
 hipError_t launch_retina_decode(const RetinaDecodeLaunch& a, hipStream_t s)
 {
-  if (!a.n_list) return hipSuccess;
-  const unsigned items = a.n_list * n_module_pairs;
+  const unsigned items = max(a.n_list * n_module_pairs, a.populate_items);
+  if (!items) return hipSuccess;
   const unsigned grid = (items + waves_per_block - 1) / waves_per_block;
 #define VELO_LAUNCH_RD(V)                                                                                           \
   hipLaunchKernelGGL(                                                                                               \
     k_retina_decode<V>, dim3(grid), dim3(waves_per_block * wave_size), 0, s, a.raw, a.offsets, a.sizes, a.types,    \
     a.event_list, a.n_list, a.event_start, reinterpret_cast<const Geometry*>(a.geometry),                         \
     a.offsets_each_sensor_pair_size, a.retina_bank_index, a.number_of_events, a.module_cluster_num,              \
-    a.offsets_module_pair_cluster, a.cluster_container, a.hit_permutations, a.hit_sorting_key)
+    a.offsets_module_pair_cluster, a.cluster_container, a.hit_permutations, a.hit_sorting_key, a.populate_items)
   switch (a.version) {
   case 2: VELO_LAUNCH_RD(2); break;
   case 3: VELO_LAUNCH_RD(3); break;

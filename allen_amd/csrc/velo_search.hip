@@ -1399,13 +1399,8 @@ hipError_t launch_search_by_triplet(const SearchLaunch& a, hipStream_t s)
   if (!a.n_list) return hipSuccess;
   const unsigned block = a.block_dim ? a.block_dim : 256;
   // Dynamic LDS stays within the device's per-workgroup limit (less the static part).
-  static int max_lds = 0;
-  if (!max_lds) {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
-      return hipErrorInvalidDevice;
-  }
+  const int max_lds = device_max_lds_per_block();
+  if (!max_lds) return hipErrorInvalidDevice;
   // Scored mode: the seeds of a sub-chunk must fit wave 0 and the hidden layer the seeding
   // scratch, which holds for 256 lanes; its weights take LDS from the hits (same residency).
   const bool scored = a.scorer_params != nullptr;
@@ -1422,17 +1417,13 @@ hipError_t launch_search_by_triplet(const SearchLaunch& a, hipStream_t s)
   // Persistent mode: as many workgroups as stay resident (they take events from the counter).
   unsigned grid = a.n_list;
   if (a.work_counter) {
-    int dev = 0, cus = 0, per_cu = 0;
     const void* kernel = a.work_stats      ? reinterpret_cast<const void*>(&k_search_by_triplet<true, seed_mode_plain>)
                          : a.scorer_params ? reinterpret_cast<const void*>(&k_search_by_triplet<false, seed_mode_scored>)
                          : a.seed_log.counter
                            ? reinterpret_cast<const void*>(&k_search_by_triplet<false, seed_mode_log>)
                            : reinterpret_cast<const void*>(&k_search_by_triplet<false, seed_mode_plain>);
-    if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, static_cast<int>(block), dyn) == hipSuccess &&
-        cus > 0 && per_cu > 0)
-      grid = min(grid, static_cast<unsigned>(cus * per_cu));
+    const unsigned resident = device_resident_blocks(kernel, block, dyn);
+    if (resident) grid = min(grid, resident);
   }
 #define VELO_LAUNCH_SEARCH(C, L)                                                                                    \
   hipLaunchKernelGGL(                                                                                               \

@@ -233,7 +233,10 @@ def compare(gpu: dict, ref, fused: bool) -> list[str]:
         if m:
             errs.append(m)
             break
-    eq("velo_track_hits")
+    # Consolidation writes id, x, y, z of each track hit (VeloConsolidateTracks.cu:180-190); the
+    # phi words of the container are never written by the reference.
+    n_hits = int(ref["offsets_velo_track_hit_number"][-1])
+    eq("velo_track_hits", gpu["velo_track_hits"][:16 * n_hits], ref["velo_track_hits"][:16 * n_hits])
     if not np.all(gpu["accepted_velo_tracks"] == 1):
         errs.append("accepted_velo_tracks not all 1")
     n_tracks = int(ref["offsets_all_velo_tracks"][-1])
