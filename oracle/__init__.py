@@ -14,7 +14,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "build", "libvelo_oracle.so")
+# VELO_ORACLE_LIB selects another build of the oracle (the planted-mistake variants of
+# tests/test_pins_mutants.py); the default is the oracle itself.
+LIB_PATH = os.environ.get("VELO_ORACLE_LIB") or os.path.join(_HERE, "build", "libvelo_oracle.so")
 
 
 class VoInputs(ctypes.Structure):
@@ -39,6 +41,12 @@ class VoInputs(ctypes.Structure):
         ("collect_seeds", ctypes.c_int),
         ("retina", ctypes.c_int),
         ("pv_finder", ctypes.c_int),
+        ("event_list", ctypes.c_void_p),
+        ("event_list_size", ctypes.c_uint),
+        ("vectorized_seeding", ctypes.c_int),
+        ("sorted_clusters", ctypes.c_void_p),
+        ("clusters_offsets", ctypes.c_void_p),
+        ("clusters_counts", ctypes.c_void_p),
     ]
 
 
@@ -144,9 +152,39 @@ def make_inputs(events, geometry: bytes, beamline: np.ndarray, **props):
     a.geometry_bytes = len(geometry)
     a.beamline = _ptr(beamline)
     a.retina = int(getattr(events, "retina", False))
+    event_list = props.pop("event_list", None)
+    if event_list is not None:
+        event_list = np.ascontiguousarray(event_list, dtype=np.uint32)
+        keep.append(event_list)
+        a.event_list = _ptr(event_list)
+        a.event_list_size = event_list.size
     for k, v in props.items():
         setattr(a, k, v)
     return a, keep
+
+
+def run_clusters(sorted_container, offsets, counts, n_events, geometry: bytes, beamline: np.ndarray, **props):
+    """The sequence from velo_search_by_triplet on, over pre-built sorted clusters (Velo::Clusters
+    layout, uint8) with their module-pair offsets (26 n + 1) and counts (26 n)."""
+    a = VoInputs()
+    lib().vo_default_inputs(ctypes.byref(a))
+    sorted_container = np.ascontiguousarray(sorted_container, dtype=np.uint8)
+    offsets = np.ascontiguousarray(offsets, dtype=np.uint32)
+    counts = np.ascontiguousarray(counts, dtype=np.uint32)
+    assert offsets.size == 26 * n_events + 1 and counts.size == 26 * n_events
+    assert sorted_container.size >= 18 * int(offsets[-1])
+    keep = [sorted_container, offsets, counts, beamline, geometry]
+    a.sorted_clusters, a.clusters_offsets, a.clusters_counts = _ptr(sorted_container), _ptr(offsets), _ptr(counts)
+    a.n_events = n_events
+    a.geometry = geometry
+    a.geometry_bytes = len(geometry)
+    a.beamline = _ptr(beamline)
+    for k, v in props.items():
+        setattr(a, k, v)
+    h = lib().vo_run(ctypes.byref(a))
+    if not h:
+        raise ValueError("oracle run failed")
+    return Result(h, keep)
 
 
 def run(events, geometry: bytes, beamline: np.ndarray, **props) -> Result:

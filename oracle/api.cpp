@@ -61,8 +61,15 @@ struct Sequence {
   }
 };
 
+struct PrebuiltClusters {
+  const char* sorted = nullptr;
+  const uint32_t* offsets = nullptr;
+  const uint32_t* counts = nullptr;
+};
+
 struct Params {
   SearchParams search;
+  PrebuiltClusters clusters;
   float max_chi2 = 20.f;
   unsigned block_dim_x = 256;
   bool f64 = true;
@@ -80,18 +87,33 @@ void run_sequence(
   const SpTables& sp,
   const float* beamline,
   const Params& prm,
-  SeedSink* sink)
+  SeedSink* sink,
+  const uint32_t* list = nullptr,
+  unsigned list_size = 0)
 {
+  // MASK_INPUT dev_event_list: the events the kernels visit, in list order (the identity when
+  // none is given). Buffers span all n_events; what an unlisted event owns stays as the
+  // reference's memsets leave it (every buffer here starts zeroed).
   std::vector<uint32_t> event_list(n_events);
   std::iota(event_list.begin(), event_list.end(), 0u);
-  const uint32_t* el = event_list.data();
-  const unsigned nl = n_events;
+  const uint32_t* el = list ? list : event_list.data();
+  const unsigned nl = list ? list_size : n_events;
 
   uint32_t total_clusters = 0;
   uint32_t* est_off = nullptr;
   uint32_t* mcnum = nullptr;
   char* sorted = nullptr;
-  if (prm.retina) {
+  if (prm.clusters.sorted) {
+    // Pre-built sorted clusters: the sequence starts at velo_search_by_triplet.
+    mcnum = s.make<uint32_t>("module_cluster_num", n_events * n_module_pairs);
+    std::copy(prm.clusters.counts, prm.clusters.counts + (size_t) n_events * n_module_pairs, mcnum);
+    est_off = s.make<uint32_t>("offsets_estimated_input_size", n_events * n_module_pairs + 1);
+    std::copy(prm.clusters.offsets, prm.clusters.offsets + (size_t) n_events * n_module_pairs + 1, est_off);
+    total_clusters = est_off[(size_t) n_events * n_module_pairs];
+    sorted = s.make<char>("sorted_velo_cluster_container", (size_t) total_clusters * cluster_element_size);
+    std::copy(prm.clusters.sorted, prm.clusters.sorted + (size_t) total_clusters * cluster_element_size, sorted);
+  }
+  else if (prm.retina) {
     // velo_reconstruction.py:18-71: calculate_number_of_retinaclusters_each_sensor_pair,
     // prefix_sum_offsets_estimated_input_size, decode_retinaclusters
     const unsigned slots = retina_sensor_slots(in.version);
@@ -241,6 +263,11 @@ bool make_inputs(const vo_inputs* a, vo::RawInput& in, vo::Geometry& g, vo::SpTa
   p.kalman_fp64_fallback = a->kalman_fp64_fallback != 0;
   p.retina = a->retina != 0;
   p.pv_finder = a->pv_finder != 0;
+  p.search.vectorized_seeding = a->vectorized_seeding != 0;
+  p.clusters.sorted = a->sorted_clusters;
+  p.clusters.offsets = a->clusters_offsets;
+  p.clusters.counts = a->clusters_counts;
+  if (p.clusters.sorted && (!p.clusters.offsets || !p.clusters.counts)) return false;
   return true;
 }
 } // namespace
@@ -261,7 +288,7 @@ void vo_default_inputs(vo_inputs* a)
 
 void* vo_run(const vo_inputs* a)
 {
-  if (a->bank_version != 2 && a->bank_version != 3 && a->bank_version != 4) return nullptr;
+  if (!a->sorted_clusters && a->bank_version != 2 && a->bank_version != 3 && a->bank_version != 4) return nullptr;
   vo::RawInput in;
   vo::Geometry g;
   vo::SpTables sp;
@@ -269,7 +296,15 @@ void* vo_run(const vo_inputs* a)
   if (!make_inputs(a, in, g, sp, p)) return nullptr;
   auto* h = new Handle;
   SeedLog log;
-  vo::run_sequence(h->seq, in, a->n_events, a->event_start, g, sp, a->beamline, p, a->collect_seeds ? &log : nullptr);
+  if (a->event_list)
+    for (unsigned i = 0; i < a->event_list_size; ++i)
+      if (a->event_list[i] >= a->n_events) {
+        delete h;
+        return nullptr;
+      }
+  vo::run_sequence(
+    h->seq, in, a->n_events, a->event_start, g, sp, a->beamline, p, a->collect_seeds ? &log : nullptr,
+    a->event_list, a->event_list_size);
   if (a->collect_seeds) {
     auto* rec = h->seq.make<uint32_t>("seed_log", log.records.size());
     std::copy(log.records.begin(), log.records.end(), rec);

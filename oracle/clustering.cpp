@@ -135,7 +135,14 @@ static void estimate_raw_bank_size(
         const uint32_t other = bank.word[k];
         // EstimateInputSize.cu:113 tests the *current* word's isolation flag, which is
         // always clear on this branch, so every other SP of the bank (either sensor) is scanned.
+#if VO_MUTANT == 1
+        const uint32_t other_no_sp_neighbours = other & 0x80000000u;
+#else
         const uint32_t other_no_sp_neighbours = sp_word & 0x80000000u;
+#endif
+#if VO_MUTANT == 2
+        if (version > 3 && ((other >> 23) & 1u) != ((sp_word >> 23) & 1u)) continue;
+#endif
         if (!other_no_sp_neighbours) {
           const uint32_t other_addr = (other & 0x007FFF00u) >> 8;
           const uint32_t orow = other_addr & 0x3Fu;
@@ -164,7 +171,11 @@ static void estimate_raw_bank_size(
       const uint32_t first = consolidated & 0x33;
       const uint32_t second = consolidated & 0xCC;
       if (first) {
+#if VO_MUTANT == 3
+        const uint32_t pixel = 31 - __builtin_clz(first);
+#else
         const uint32_t pixel = __builtin_clz(first) - 24;
+#endif
         const uint32_t c = (*event_candidate_num)++;
         cluster_candidates[c] = (sp_index << 11) | (sensor_number << 3) | pixel;
         ++found;
@@ -208,9 +219,15 @@ void estimate_input_size(
 // MaskedVeloClustering.cu:21-26
 static inline uint64_t make_8con_mask(uint64_t c)
 {
+#if VO_MUTANT == 6
+  return c | ((c & 0x7FFF7FFF7FFF7FFFull) << 1) | ((c & 0xFFFEFFFEFFFEFFFEull) << 15) | (c << 16) |
+         ((c & 0x7FFF7FFF7FFF7FFFull) << 17) | ((c & 0xFFFEFFFEFFFEFFFEull) >> 1) |
+         ((c & 0x7FFF7FFF7FFF7FFFull) >> 15) | (c >> 16) | ((c & 0xFFFEFFFEFFFEFFFEull) >> 17);
+#else
   return c | ((c & 0x7FFF7FFF7FFF7FFFull) << 1) | ((c & 0xFFFEFFFEFFFEFFFEull) << 15) | (c << 16) |
          ((c & 0xFFFCFFFCFFFCFFFCull) << 17) | ((c & 0xFFFEFFFEFFFEFFFEull) >> 1) |
          ((c & 0x7FFF7FFF7FFF7FFFull) >> 15) | (c >> 16) | ((c & 0x3FFF3FFF3FFF3FFFull) >> 17);
+#endif
 }
 // MaskedVeloClustering.cu:31-44
 static inline uint64_t mask_east(uint64_t c)
@@ -330,7 +347,9 @@ static void rest_of_clusters(
       const uint32_t other_sensor_bit = (other & 0x800000u) ? 1u : 0u;
       if (other_sensor_bit != (sensor_number & 0x1u)) continue;
     }
+#if VO_MUTANT != 8
     if (other & 0x80000000u) continue;
+#endif
     const uint32_t oaddr = (other & 0x007FFF00u) >> 8;
     const int32_t orow = oaddr & 0x3Fu;
     const int32_t ocol = oaddr >> 6;
@@ -348,9 +367,17 @@ static void rest_of_clusters(
   const uint64_t pixel_map = (((uint64_t) pixel_array[3]) << 32) | pixel_array[2];
   const ClusterGrow cl = make_cluster(pixel_map, starting_pixel, col_lower_limit + 4, row_lower_limit);
   const uint64_t precedence =
+#if VO_MUTANT == 7
+    (cl.pixels & (starting_pixel ^ (0 - starting_pixel)));
+#else
     (mask_east(cl.pixels) & pixel_array[4]) | (cl.pixels & (starting_pixel ^ (0 - starting_pixel)));
+#endif
+#if VO_MUTANT == 4
+  bool keep = precedence == 0 && !popc64(cl.pixels & 0x8000800080008000ull);
+#else
   bool keep = precedence == 0 && !popc64(cl.pixels & 0x8000800080008000ull) &&
               !popc64(cl.pixels & 0x0001000100010001ull);
+#endif
   if (!keep) return;
   unsigned x = cl.x, y = cl.y, n = cl.n;
   if (popc64(cl.pixels & 0xFFFFull)) {
@@ -360,6 +387,9 @@ static void rest_of_clusters(
     x += ext.x;
     y += ext.y;
     n += ext.n;
+#if VO_MUTANT == 5
+    keep &= !popc64(ext.pixels & 0x0001000100010001ull);
+#endif
     keep &= !popc64(ext.pixels & 0x8000800080008000ull) && !popc64(ext.pixels & 0xFFFFull) &&
             !popc64(cl.pixels & 0x0001000100010001ull);
   }

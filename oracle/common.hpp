@@ -30,6 +30,13 @@
 #include <cstddef>
 #include <vector>
 
+// Planted mistakes for the pin tests (tests/test_pins_mutants.py): VO_MUTANT = n selects one
+// wrong variant of a rule (see each "#if VO_MUTANT" site). Only `make mutants` builds them, into
+// separate libraries under build/mutants/; the oracle itself is built with none.
+#ifndef VO_MUTANT
+#define VO_MUTANT 0
+#endif
+
 namespace vo {
 
 // device/event_model/velo/include/VeloDefinitions.cuh:16-57

@@ -9,6 +9,9 @@ struct SearchParams {
   float phi_tolerance = 0.045f;   // SearchByTriplet.cuh:129
   float max_scatter = 0.08f;      // SearchByTriplet.cuh:130
   unsigned max_skipped_modules = 1; // velo_reconstruction.py:188
+  // Seeding as the CPU target dispatches it (track_seeding_vectorized, SearchByTriplet.cu:616-818)
+  // instead of the default track_seeding (:298-446).
+  bool vectorized_seeding = false;
 };
 
 // Exact work counts of search_by_triplet, used as the algorithmic op count of the kernel.

@@ -148,7 +148,11 @@ void track_seeding(
     unsigned found = 0;
     for (unsigned i = 0; i < prev.hit_num && found < h0_candidates_to_consider; ++i) {
       const unsigned sign = i & 0x01;
+#if VO_MUTANT == 12
+      const int index_diff = sign ? -static_cast<int>(i) : static_cast<int>(i);
+#else
       const int index_diff = sign ? i : -static_cast<int>(i);
+#endif
       phi_index += index_diff;
       const int in_bounds = phi_index < 0 ? phi_index + static_cast<int>(prev.hit_num) :
                                             (phi_index >= static_cast<int>(prev.hit_num) ?
@@ -166,7 +170,11 @@ void track_seeding(
       const float tx = txn * td;
       const float ty = tyn * td;
       int16_t cand, ephi;
+#if VO_MUTANT == 16
+      find_forward_candidate(next, cc.phi_begin(), h0, tx, ty, next.z[1] - prev.z[0], phi_tolerance, cand, ephi, st);
+#else
       find_forward_candidate(next, cc.phi_begin(), h0, tx, ty, next.z[0] - prev.z[0], phi_tolerance, cand, ephi, st);
+#endif
       for (unsigned j = 0; j < next.hit_num; ++j) {
         const unsigned in_bounds = (cand + j) % next.hit_num;
         const unsigned h2_index = next.hit_start + in_bounds;
@@ -183,7 +191,11 @@ void track_seeding(
           const float dy = predy - h2.y;
           const float scatter = dx * dx + dy * dy;
           if (sink) sink->candidate(h0_index, h1_index, h2_index, scatter);
+#if VO_MUTANT == 15
+          if (scatter <= best_fit && scatter < max_scatter) {
+#else
           if (scatter < best_fit) {
+#endif
             best_fit = scatter;
             best_h0 = h0_index;
             best_h2 = h2_index;
@@ -197,6 +209,134 @@ void track_seeding(
       sink->seed(module_pair, best_h0, h1_index, best_h2, f);
     }
     if (best_fit < max_scatter) {
+#if VO_MUTANT == 18
+      if (atomics[atomic_ttf] >= max_tracks_to_follow) continue;
+#endif
+      const unsigned tn = atomics[atomic_ttf]++ % max_tracks_to_follow;
+      tracklets[tn] = TrackletHits {{best_h0, static_cast<uint16_t>(h1_index), best_h2}};
+      tracks_to_follow[tn] = bit_seed | tn;
+    }
+  }
+}
+
+// SearchByTriplet.cu:616-818: the CPU target's seeding. The h1 hits are visited in order without
+// the compaction into h1_indices (so neither dev_rel_indices nor the local-hits atomic is
+// written), and the h0 candidates are scored four at a time in lock step: in each round every
+// active lane takes its next unused h2 of its phi window, and the lane of least scatter among
+// those below the best so far takes the best. Among equal scatters of one round the lowest lane
+// wins (umesimd's imin over the mask; its tie rule is not pinned by anything in the reference).
+void track_seeding_vectorized(
+  const Clusters& cc,
+  const ModulePair* mpd,
+  const bool* hit_used,
+  TrackletHits* tracklets,
+  uint32_t* tracks_to_follow,
+  uint32_t* atomics,
+  float max_scatter,
+  int16_t phi_tolerance,
+  unsigned h0_candidates_to_consider,
+  unsigned module_pair,
+  SearchStats& st,
+  SeedSink* sink)
+{
+  constexpr unsigned lanes = 4; // vector128_length() for float
+  const ModulePair& prev = mpd[0];
+  const ModulePair& cur = mpd[1];
+  const ModulePair& next = mpd[2];
+  for (unsigned r = 0; r < cur.hit_num; ++r) {
+    const unsigned h1_index = cur.hit_start + r;
+    if (hit_used[h1_index]) continue;
+    uint16_t best_h0 = 0, best_h2 = 0;
+    float best_fit = max_scatter;
+    const HitBase h1 {cc.x(h1_index), cc.y(h1_index), cc.z(h1_index)};
+    unsigned best_h0s[max_h0_candidates];
+    int phi_index =
+      binary_search_leftmost(cc.phi_begin() + prev.hit_start, prev.hit_num, cc.phi(h1_index), &st.search_steps);
+    unsigned found = 0;
+    for (unsigned i = 0; i < prev.hit_num && found < h0_candidates_to_consider; ++i) {
+      phi_index += (i & 1) ? static_cast<int>(i) : -static_cast<int>(i);
+      const int n = static_cast<int>(prev.hit_num);
+      const int in_bounds = phi_index < 0 ? phi_index + n : (phi_index >= n ? phi_index - n : phi_index);
+      const unsigned h0_index = prev.hit_start + in_bounds;
+      if (!hit_used[h0_index]) best_h0s[found++] = h0_index;
+    }
+    for (unsigned batch = 0; batch < found; batch += lanes) {
+      const unsigned len = batch + lanes < found ? lanes : found - batch;
+      HitBase h0[lanes];
+      float tx[lanes], ty[lanes];
+      int16_t ephi[lanes];
+      int cand[lanes];
+      unsigned iter[lanes];
+      bool active[lanes];
+      for (unsigned l = 0; l < len; ++l) {
+        const unsigned h0_index = best_h0s[batch + l];
+        h0[l] = HitBase {cc.x(h0_index), cc.y(h0_index), cc.z(h0_index)};
+        const float td = 1.0f / (h1.z - h0[l].z);
+        tx[l] = (h1.x - h0[l].x) * td;
+        ty[l] = (h1.y - h0[l].y) * td;
+        int16_t c;
+        find_forward_candidate(next, cc.phi_begin(), h0[l], tx[l], ty[l], next.z[0] - prev.z[0], phi_tolerance, c, ephi[l], st);
+        cand[l] = c;
+        iter[l] = ~0u;
+        active[l] = true;
+      }
+      for (;;) {
+        unsigned h2_of[lanes];
+        bool any = false;
+        for (unsigned l = 0; l < len; ++l) {
+          while (active[l]) {
+            ++iter[l];
+            if (iter[l] == next.hit_num) {
+              active[l] = false;
+              break;
+            }
+            const unsigned h2_index = next.hit_start + (cand[l] + iter[l]) % next.hit_num;
+            const int16_t d = static_cast<int16_t>(cc.phi(h2_index) - ephi[l]);
+            if ((d < 0 ? -d : d) > phi_tolerance) {
+              active[l] = false;
+              break;
+            }
+            if (!hit_used[h2_index]) {
+              h2_of[l] = h2_index;
+              break;
+            }
+          }
+          any = any || active[l];
+        }
+        if (!any) break;
+        int pick = -1;
+        float pick_scatter = 0.f;
+        for (unsigned l = 0; l < len; ++l) {
+          if (!active[l]) continue;
+          ++st.seed_evaluations;
+          const HitBase h2 {cc.x(h2_of[l]), cc.y(h2_of[l]), cc.z(h2_of[l])};
+          const float dz = h2.z - h0[l].z;
+          const float dx = (h0[l].x + tx[l] * dz) - h2.x;
+          const float dy = (h0[l].y + ty[l] * dz) - h2.y;
+          const float scatter = dx * dx + dy * dy;
+          if (sink) sink->candidate(best_h0s[batch + l], h1_index, h2_of[l], scatter);
+#if VO_MUTANT == 19
+          if (scatter < best_fit && (pick < 0 || scatter <= pick_scatter)) {
+#else
+          if (scatter < best_fit && (pick < 0 || scatter < pick_scatter)) {
+#endif
+            pick = static_cast<int>(l);
+            pick_scatter = scatter;
+          }
+        }
+        if (pick >= 0) {
+          best_fit = pick_scatter;
+          best_h0 = best_h0s[batch + pick];
+          best_h2 = h2_of[pick];
+        }
+      }
+    }
+    if (best_fit < max_scatter) {
+      if (sink) {
+        float f[n_seed_features];
+        seed_features(cc, best_h0, h1_index, best_h2, module_pair, f);
+        sink->seed(module_pair, best_h0, h1_index, best_h2, f);
+      }
       const unsigned tn = atomics[atomic_ttf]++ % max_tracks_to_follow;
       tracklets[tn] = TrackletHits {{best_h0, static_cast<uint16_t>(h1_index), best_h2}};
       tracks_to_follow[tn] = bit_seed | tn;
@@ -241,7 +381,11 @@ void track_forwarding(
     const unsigned h0_num = t->hits[number_of_hits - 2];
     const unsigned h1_num = t->hits[number_of_hits - 1];
     const HitBase h0 {cc.x(h0_num), cc.y(h0_num), cc.z(h0_num)};
+#if VO_MUTANT == 17
+    const unsigned h0_module = ((cc.id(h1_num) & 0x3fc0000u) >> 18) / 4;
+#else
     const unsigned h0_module = ((cc.id(h0_num) & 0x3fc0000u) >> 18) / 4;
+#endif
     const HitBase h1 {cc.x(h1_num), cc.y(h1_num), cc.z(h1_num)};
     const float td = 1.0f / (h1.z - h0.z);
     const float txn = h1.x - h0.x;
@@ -259,6 +403,9 @@ void track_forwarding(
       const int16_t phi_diff = cc.phi(h2_index) - ephi;
       const int16_t abs_phi_diff = phi_diff < 0 ? -phi_diff : phi_diff;
       if (abs_phi_diff > phi_tolerance) break;
+#if VO_MUTANT == 14
+      if (hit_used[h2_index]) continue;
+#endif
       ++st.forward_evaluations;
       const HitBase h2 {cc.x(h2_index), cc.y(h2_index), cc.z(h2_index)};
       const float dz = h2.z - h0.z;
@@ -293,7 +440,11 @@ void track_forwarding(
         tracks_to_follow[p] = track_number;
       }
     }
+#if VO_MUTANT == 13
+    else if (skipped <= max_skipped_modules) {
+#else
     else if (skipped < max_skipped_modules) {
+#endif
       track_number = ((skipped + 1) << skipped_pos) | (full & (bit_seed | bit_track_number));
       const unsigned p = atomics[atomic_ttf]++ % max_tracks_to_follow;
       tracks_to_follow[p] = track_number;
@@ -339,9 +490,22 @@ void search_by_triplet_event(
   unsigned first_module_pair = n_module_pairs - 1;
   load(first_module_pair);
   const int16_t phi_tolerance_i16 = hit_phi_float_to_16(p.phi_tolerance);
-  track_seeding(
-    cc, mpd, hit_used, tracklets, tracks_to_follow, h1_rel_indices, atomics, p.max_scatter, phi_tolerance_i16,
-    initial_seeding_h0_candidates, first_module_pair - 1, st, sink);
+  // SearchByTriplet.cu:163,229: the CPU target dispatches to the vectorized seeding.
+  auto seeding = [&](unsigned h0_candidates, unsigned module_pair) {
+    if (p.vectorized_seeding)
+      track_seeding_vectorized(
+        cc, mpd, hit_used, tracklets, tracks_to_follow, atomics, p.max_scatter, phi_tolerance_i16, h0_candidates,
+        module_pair, st, sink);
+    else
+      track_seeding(
+        cc, mpd, hit_used, tracklets, tracks_to_follow, h1_rel_indices, atomics, p.max_scatter, phi_tolerance_i16,
+        h0_candidates, module_pair, st, sink);
+  };
+#if VO_MUTANT == 10
+  seeding(seeding_h0_candidates, first_module_pair - 1);
+#else
+  seeding(initial_seeding_h0_candidates, first_module_pair - 1);
+#endif
   unsigned last_ttf = 0;
   --first_module_pair;
   while (first_module_pair > 1) {
@@ -354,9 +518,11 @@ void search_by_triplet_event(
     track_forwarding(
       cc, hit_used, mpd, diff_ttf, tracks_to_follow, three_hit_tracks, prev_ttf, tracklets, tracks, atomics,
       number_of_velo_tracks_event, phi_tolerance_i16, p.max_scatter, p.max_skipped_modules, st);
-    track_seeding(
-      cc, mpd, hit_used, tracklets, tracks_to_follow, h1_rel_indices, atomics, p.max_scatter, phi_tolerance_i16,
-      seeding_h0_candidates, first_module_pair - 1, st, sink);
+#if VO_MUTANT == 11
+    seeding(initial_seeding_h0_candidates, first_module_pair - 1);
+#else
+    seeding(seeding_h0_candidates, first_module_pair - 1);
+#endif
     --first_module_pair;
   }
   const unsigned prev_ttf = last_ttf;

@@ -33,6 +33,14 @@ typedef struct {
   int retina;                 /* the banks are retina clusters (type 73): decode_retinaclusters replaces
                                  the candidates, estimate, masked clustering and phi sort */
   int pv_finder;              /* also run the beamline PV finder on the beamline states */
+  const uint32_t* event_list; /* MASK_INPUT dev_event_list (event numbers < n_events), or NULL for all */
+  unsigned event_list_size;
+  int vectorized_seeding;     /* seeding as the CPU target dispatches it (track_seeding_vectorized) */
+  /* Optional: start at search_by_triplet from these sorted clusters (Velo::Clusters layout over
+   * clusters_offsets[26 n_events] slots) instead of decoding raw banks. */
+  const char* sorted_clusters;
+  const uint32_t* clusters_offsets; /* offsets_estimated_input_size, 26 n_events + 1 */
+  const uint32_t* clusters_counts;  /* module_cluster_num, 26 n_events */
 } vo_inputs;
 
 void vo_default_inputs(vo_inputs* a);

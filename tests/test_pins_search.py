@@ -1,0 +1,286 @@
+"""Pins of the oracle's search_by_triplet (runs without a GPU).
+
+The oracle restates SearchByTriplet.cu; tests/textbook_search.py states the same rules with a
+brute-force choice over all candidate pairs. They are compared on constructed events whose hits
+are placed by hand (the sequence starts from a sorted cluster container): tiny random events of
+at most three hits per module pair, the pendulum order and the 8 / 4 h0 candidate limits, tracks
+with one and two missing module pairs, exact scatter ties (including one where the CPU target's
+lock-step seeding picks another seed than the scalar seeding) and a ring of tracks to follow
+that overflows its 2048 slots. The same comparison on generated pile-up events, and the count of
+tracks that differ between the two seeding orders on the 200- and 1000-event workloads, close it.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from allen_amd import synth
+from tests import layout, textbook, textbook_search as ts
+
+
+@pytest.fixture(scope="module")
+def zs(geometry):
+    return textbook.parse_geometry(geometry)["module_zs"]
+
+
+def phi16(x, y):
+    return int(oracle.lib().vo_hit_phi_16(float(x), float(y)))
+
+
+def run_both(events, geometry, beamline, zs, gaps=None, vectorized=False, **props):
+    """Oracle and statement on the same constructed events; asserts equality event by event and
+    returns (oracle result, list of statement results)."""
+    container, offsets, counts = layout.sorted_clusters(events, phi16, gaps)
+    n = len(events)
+    r = oracle.run_clusters(container, offsets, counts, n, geometry, beamline, compute_f64=0,
+                            vectorized_seeding=int(vectorized), **props)
+    out = []
+    sprops = {k: v for k, v in props.items() if k in ("phi_tolerance", "max_scatter", "max_skipped_modules")}
+    for e in range(n):
+        E = ts.event_from_container(container, offsets, counts, e, zs)
+        got = ts.search(E, vectorized=vectorized, **sprops)
+        toff = int(np.float32(offsets[26 * e]) * np.float32(0.34))
+        nt = int(r["number_of_velo_tracks"][e])
+        o_tracks = [t["hits"][:t["hitsNum"]].tolist() for t in r["tracks"][toff:toff + nt]]
+        n3 = int(r["atomics_velo"][3 * e])
+        o_three = [t["hits"].tolist() for t in r["three_hit_tracks"][toff:toff + n3]]
+        h0, h1 = int(offsets[26 * e]), int(offsets[26 * (e + 1)])
+        assert o_tracks == got["tracks"], f"tracks of event {e}"
+        assert o_three == got["three_hit"], f"three-hit tracks of event {e}"
+        assert np.array_equal(r["hit_used"][h0:h1], got["used"]), f"hit_used of event {e}"
+        assert int(r["atomics_velo"][3 * e + 1]) == got["ttf"], f"tracks to follow of event {e}"
+        written = min(got["ttf"], ts.RING)
+        ring = r["tracks_to_follow"][e * ts.RING:(e + 1) * ts.RING]
+        assert ring[:written].tolist() == got["ring"][:written], f"ring of event {e}"
+        out.append(got)
+    return r, out
+
+
+def line_hits(x0, y0, z0, tx, ty, pairs, zs, low, side=None):
+    """Hits of a straight line at the module planes of the given module pairs (the module of the
+    pair on the line's side of x, or `side`), as (module pair, x, y, z, LHCbID)."""
+    out = []
+    for k, m in enumerate(pairs):
+        s = (0 if x0 + tx * (zs[2 * m] - z0) < 0 else 1) if side is None else side
+        z = np.float32(zs[2 * m + s])
+        dz = np.float32(z - np.float32(z0))
+        x = np.float32(np.float32(x0) + np.float32(tx) * dz)
+        y = np.float32(np.float32(y0) + np.float32(ty) * dz)
+        out.append((m, x, y, z, layout.velo_lhcb_id(2 * m + s, k % 4, low + k)))
+    return out
+
+
+# ---------------------------------------------------------------------------- random tiny events
+@pytest.mark.parametrize("vectorized", [False, True], ids=["scalar", "vectorized"])
+def test_tiny_random_events(vectorized, geometry, beamline, zs):
+    """Up to three hits per module pair: a few straight tracks from a vertex, some with hits
+    missing, plus noise; both seeding orders."""
+    rng = np.random.default_rng(31)
+    events, gaps = [], []
+    for e in range(150):
+        hits = []
+        for t in range(int(rng.integers(1, 4))):
+            phi, th = rng.uniform(-np.pi, np.pi), rng.uniform(0.02, 0.25)
+            pairs = [m for m in range(26) if rng.random() < 0.9]
+            hits += line_hits(rng.normal(0, 0.05), rng.normal(0, 0.05), rng.normal(0, 40),
+                              th * np.cos(phi), th * np.sin(phi), pairs, zs, 64 * t)
+        for k in range(int(rng.integers(0, 12))):
+            m = int(rng.integers(0, 26))
+            r, p = rng.uniform(6, 40), rng.uniform(-np.pi, np.pi)
+            hits.append((m, r * np.cos(p), r * np.sin(p), zs[2 * m], layout.velo_lhcb_id(2 * m, 0, 4000 + k)))
+        per = {}
+        kept = []
+        for h in hits:
+            if per.get(h[0], 0) < 3:
+                per[h[0]] = per.get(h[0], 0) + 1
+                kept.append(h)
+        events.append(kept)
+        gaps.append(rng.integers(0, 3, 26))
+    r, got = run_both(events, geometry, beamline, zs, gaps, vectorized)
+    assert sum(len(g["tracks"]) for g in got) > 100
+    assert sum(len(g["three_hit"]) for g in got) > 0
+
+
+# ---------------------------------------------------------------------------- pendulum and K
+def _pendulum_event(zs, c, rank):
+    """h1 = (0, 30) on module pair c and 20 hits on pair c + 1 at y = 30, x = 0.35 k for
+    k = -9..10 (phi falls as x grows, so h1's insertion point is the hit at x = 0 and the hit at
+    x = -0.35 k has pendulum rank 2 k - 1, the one at x = +0.35 k rank 2 k). The hit of the given
+    rank lies on a line through h1 and an h2 on pair c - 1; every other hit misses that h2 by at
+    least 0.35 mm (scatter >= 0.12 > max_scatter)."""
+    z0, z1, z2 = zs[2 * (c + 1)], zs[2 * c], zs[2 * (c - 1)]
+    k = (rank + 1) // 2
+    x0 = -0.35 * k if rank % 2 else 0.35 * k
+    # the line through (x0, z0) and (0, z1) reaches z2 at x0 (z2 - z1) / (z0 - z1)
+    x2 = np.float32(x0) * (np.float32(z2) - np.float32(z1)) / (np.float32(z0) - np.float32(z1))
+    hits = [(c, 0.0, 30.0, z1, layout.velo_lhcb_id(2 * c, 0, 1)),
+            (c - 1, float(x2), 30.0, z2, layout.velo_lhcb_id(2 * (c - 1), 0, 2))]
+    for j in range(-9, 11):
+        hits.append((c + 1, 0.35 * j, 30.0, z0, layout.velo_lhcb_id(2 * (c + 1), 1, 100 + j)))
+    return hits, np.float32(x0)
+
+
+def test_pendulum_order_and_h0_limits(geometry, beamline, zs):
+    """The true h0 is found exactly when it is among the first K unused hits of the previous
+    module pair in pendulum order (p, p+1, p-1, p+2, ...): K = 8 at the first step (h1 on pair
+    24), 4 after (h1 on pair 23)."""
+    cases = [(24, r, r < 8) for r in (5, 6, 7, 8, 9)] + [(23, r, r < 4) for r in (1, 2, 3, 4, 5)]
+    events, true_x = [], []
+    for c, rank, _ in cases:
+        hits, x0 = _pendulum_event(zs, c, rank)
+        events.append(hits)
+        true_x.append(x0)
+    container, offsets, counts = layout.sorted_clusters(events, phi16)
+    _, got = run_both(events, geometry, beamline, zs)
+    for e, ((c, rank, found), x0) in enumerate(zip(cases, true_x)):
+        E = ts.event_from_container(container, offsets, counts, e, zs)
+        seeds = got[e]["three_hit"] + [t[:3] for t in got[e]["tracks"]]
+        hit_x = [float(E.x[t[0]]) for t in seeds]
+        assert (x0 in hit_x) == found, (c, rank, hit_x)
+
+
+def test_skipped_module_pairs(geometry, beamline, zs):
+    """A track missing one module pair continues (max_skipped_modules = 1); missing two in a row,
+    it stops there, and its remaining hits seed a new track."""
+    events = []
+    for gap in (1, 2, 3):
+        pairs = [m for m in range(26) if not (12 <= m < 12 + gap)]
+        events.append(line_hits(0.03, -0.02, 10.0, 0.06, 0.11, pairs, zs, 0))
+    r, got = run_both(events, geometry, beamline, zs)
+    # one skip: a single track with every hit
+    assert [len(t) for t in got[0]["tracks"]] == [25]
+    # two or three skips in a row: the track stops and another starts after the gap
+    assert sorted(len(t) for t in got[1]["tracks"]) == [12, 12]
+    assert sorted(len(t) for t in got[2]["tracks"]) == [11, 12]
+
+
+# ---------------------------------------------------------------------------- exact ties
+def _tie_event(zs, e_off=0.1, y0=20.0):
+    """Exact scatter ties in the first seeding step (h0 on pair 25, h1 on 24, h2 on 23).
+
+    h0a = (-e, y0) and h0b = (+e, y0) on the same module, h1 = (0, y0): their slopes are exact
+    negatives, so their predictions at z(h2) mirror each other and the h2 at x = 0 has the same
+    scatter from both. One more hit on pair 23, out of scatter reach, lies in the phi window of
+    h0a's extrapolation ahead of the tie hit but below h0b's window. h0a comes first in pendulum
+    order. The scalar seeding keeps h0a (first strict minimum); the lock-step seeding meets the
+    tie in h0b's lane one round earlier and keeps h0b."""
+    z0, z1, z2 = zs[51], zs[49], zs[47]
+    h0a = (25, -e_off, y0, z0, layout.velo_lhcb_id(51, 0, 1))
+    h0b = (25, e_off, y0, z0, layout.velo_lhcb_id(51, 0, 2))
+    h1 = (24, 0.0, y0, z1, layout.velo_lhcb_id(49, 0, 3))
+    tie = (23, 0.0, y0, z2, layout.velo_lhcb_id(47, 0, 4))
+    tol = int(oracle.lib().vo_hit_phi_float_to_16(0.045))
+    # the window of each lane starts at its extrapolated phi - tol; find an x whose phi is at
+    # least h0a's start and below h0b's
+    xa = np.float32(e_off)  # h0a's prediction lands at about +e
+    pa, pb = phi16(xa, y0), phi16(-xa, y0)
+    lo, hi = min(pa, pb) - tol, max(pa, pb) - tol
+    for x in np.linspace(e_off + 0.3, 3.0, 2000, dtype=np.float32):  # beyond scatter reach of h0a
+        if lo <= phi16(x, y0) < hi:
+            extra = (23, float(x), y0, z2, layout.velo_lhcb_id(47, 1, 5))
+            return [h0a, h0b, h1, tie, extra]
+    raise AssertionError("no hit fits between the two windows")
+
+
+def test_exact_scatter_ties_scalar_and_lockstep(geometry, beamline, zs):
+    events = [_tie_event(zs, *a) for a in ((0.1, 20.0), (0.05, 25.0), (0.15, 15.0), (0.08, 30.0), (0.2, 22.0))]
+    _, scalar = run_both(events, geometry, beamline, zs, vectorized=False)
+    _, lockstep = run_both(events, geometry, beamline, zs, vectorized=True)
+    for s, v in zip(scalar, lockstep):
+        assert len(s["three_hit"]) == len(v["three_hit"]) == 1
+        # same h1 and h2; h0a (x = -e, the higher phi, last of pair 25) against h0b
+        assert s["three_hit"][0][1:] == v["three_hit"][0][1:]
+        assert s["three_hit"][0][0] != v["three_hit"][0][0]
+
+
+def test_equal_scatter_in_one_lockstep_round(geometry, beamline, zs):
+    """The mirrored h0 pair without the extra hit: both lanes meet the tie hit in the same round,
+    and the lower lane (h0a, first in pendulum order) keeps it, as does the scalar seeding. This
+    is the assumed tie rule of umesimd's imin (lowest index), which no reference file pins."""
+    events = [_tie_event(zs, *a)[:4] for a in ((0.1, 20.0), (0.05, 25.0), (0.2, 22.0))]
+    _, scalar = run_both(events, geometry, beamline, zs, vectorized=False)
+    _, lockstep = run_both(events, geometry, beamline, zs, vectorized=True)
+    for s, v in zip(scalar, lockstep):
+        assert s["three_hit"] == v["three_hit"] and len(s["three_hit"]) == 1
+
+
+def test_two_h2_at_equal_scatter(geometry, beamline, zs):
+    """Two h2 hits mirrored about the prediction of a track parallel to the beam (slope 0 and
+    the predictions are exact): the first of them in the phi window (x = 7 + d, the lower phi at
+    y > 0) wins, in seeding (mirrored hits on pair 23) and in forwarding (on pair 22)."""
+    events, where = [], []
+    for d in (0.05, 0.125, 0.2):
+        for tie_pair in (23, 22):
+            hits = []
+            for m in range(25, tie_pair - 1, -1):
+                if m == tie_pair:
+                    hits.append((m, 7.0 + d, 9.0, zs[2 * m], layout.velo_lhcb_id(2 * m, 1, 100 + m)))
+                    hits.append((m, 7.0 - d, 9.0, zs[2 * m], layout.velo_lhcb_id(2 * m, 2, 200 + m)))
+                else:
+                    hits.append((m, 7.0, 9.0, zs[2 * m], layout.velo_lhcb_id(2 * m, 0, m)))
+            events.append(hits)
+            where.append(d)
+    container, offsets, counts = layout.sorted_clusters(events, phi16)
+    for vec in (False, True):
+        _, got = run_both(events, geometry, beamline, zs, vectorized=vec)
+        for e, d in enumerate(where):
+            E = ts.event_from_container(container, offsets, counts, e, zs)
+            t = (got[e]["tracks"] + got[e]["three_hit"])[0]
+            assert float(E.x[t[-1]]) == np.float32(7.0 + d), (e, d, vec)
+
+
+# ---------------------------------------------------------------------------- ring overflow
+def test_ring_of_tracks_to_follow_overflows(geometry, beamline, zs):
+    """2300 tracks from one vertex, evenly spread in phi, with a phi tolerance that keeps the
+    windows a few hits wide: more than 2048 tracks to follow are live at every step, so slots of
+    the ring (and tracklets in them) are overwritten before they are read. The oracle raises its
+    overflow status and otherwise agrees with the literal ring of the statement."""
+    n = 2300
+    phis = np.arange(n) * (2 * np.pi / n)
+    hits = []
+    for k, p in enumerate(phis):
+        hits += line_hits(0.0, 0.0, -400.0, 0.1 * np.cos(p), 0.1 * np.sin(p), list(range(26)), zs, 0)
+    # unique LHCbIDs within a module pair
+    hits = [(m, x, y, z, (i & ~0x3FFFF) | (j & 0x3FFFF)) for j, (m, x, y, z, i) in enumerate(hits)]
+    r, got = run_both([hits], geometry, beamline, zs, phi_tolerance=0.004)
+    assert int(r["status"][0]) & 1
+    assert got[0]["ttf"] > 4 * 2048
+
+
+# ---------------------------------------------------------------------------- generated events
+@pytest.mark.parametrize("vectorized", [False, True], ids=["scalar", "vectorized"])
+def test_generated_events_against_statement(vectorized, geometry, beamline, zs):
+    ev = synth.generate(n_events=4, pileup=1, nu=7.6, seed=9)
+    r = oracle.run(ev, geometry, beamline, compute_f64=0, vectorized_seeding=int(vectorized))
+    off, cnt = r["offsets_estimated_input_size"], r["module_cluster_num"]
+    for e in range(ev.n_events):
+        E = ts.event_from_container(r["sorted_velo_cluster_container"], off, cnt, e, zs)
+        got = ts.search(E, vectorized=vectorized)
+        toff = int(np.float32(off[26 * e]) * np.float32(0.34))
+        nt = int(r["number_of_velo_tracks"][e])
+        assert [t["hits"][:t["hitsNum"]].tolist() for t in r["tracks"][toff:toff + nt]] == got["tracks"]
+        assert int(r["atomics_velo"][3 * e + 1]) == got["ttf"]
+
+
+def _track_sets(r):
+    off, hoff = r["offsets_all_velo_tracks"], r["offsets_velo_track_hit_number"]
+    ids = r["velo_track_hits"][:4 * int(hoff[-1])].view("<u4")
+    return [tuple(ids[hoff[t]:hoff[t + 1]].tolist()) for t in range(int(off[-1]))]
+
+
+@pytest.mark.parametrize("preset", ["single_pv_200", "pileup_1000"])
+def test_seeding_orders_agree_on_the_workloads(preset, geometry, beamline):
+    """The CPU target's lock-step seeding and the scalar seeding give the same tracks on the
+    200-event single-vertex and 1000-event pile-up workloads: not one hit list differs (the
+    criterion is 99.9 % identical). They differ only in scratch the CPU target leaves unwritten
+    (the local-hits atomic and dev_rel_indices)."""
+    ev = synth.preset(preset)
+    a = oracle.run(ev, geometry, beamline, compute_f64=0)
+    b = oracle.run(ev, geometry, beamline, compute_f64=0, vectorized_seeding=1)
+    ta, tb = _track_sets(a), _track_sets(b)
+    assert len(ta) == len(tb) > 20000
+    same = sum(x == y for x, y in zip(ta, tb))
+    print(f"{preset}: {same} of {len(ta)} tracks identical")
+    assert same == len(ta)
+    for k in ["tracks_to_follow", "tracklets", "hit_used", "three_hit_tracks_output", "velo_kalman_beamline_states"]:
+        assert np.array_equal(a[k], b[k]), k
+    assert np.all(b["atomics_velo"].reshape(-1, 3)[:, 2] == 0)
