@@ -46,6 +46,7 @@ PARTICLE_DTYPE = np.dtype([
     ("event", "<u4"), ("pv", "<u4"), ("x0", "<f4"), ("y0", "<f4"), ("z0", "<f4"),
     ("tx", "<f4"), ("ty", "<f4"), ("eta", "<f4"), ("phi", "<f4"),
     ("n_hits", "<u4"), ("n_modules", "<u4"), ("first_module", "<u4"), ("last_module", "<u4"),
+    ("p", "<f4"), ("pt", "<f4"),
 ])
 HIT_DTYPE = np.dtype([
     ("event", "<u4"), ("particle", "<u4"), ("lhcb_id", "<u4"), ("sensor", "<u2"),

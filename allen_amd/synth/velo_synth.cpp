@@ -83,6 +83,17 @@ double neg_binomial(std::mt19937_64& rng, double mean, double k)
   return po(rng);
 }
 
+// Uniform (0, 1] from a 64-bit key (splitmix64 finaliser), for draws that must not disturb the
+// event's main random stream.
+double keyed_uniform(uint64_t key)
+{
+  key += 0x9E3779B97F4A7C15ull;
+  key = (key ^ (key >> 30)) * 0xBF58476D1CE4E5B9ull;
+  key = (key ^ (key >> 27)) * 0x94D049BB133111EBull;
+  key ^= key >> 31;
+  return (static_cast<double>(key >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+}
+
 void generate_event(const velo_synth_config& c, const Geometry& g, unsigned ev, EventOut& out)
 {
   // Per-event stream: reproducible whatever the thread partition.
@@ -160,6 +171,12 @@ void generate_event(const velo_synth_config& c, const Geometry& g, unsigned ev, 
       p.phi = phi;
       p.first_module = 0xFFFFFFFFu;
       p.last_module = 0;
+      {
+        const uint64_t key = (c.seed * 0xD1342543DE82EF95ull) ^ (static_cast<uint64_t>(ev) << 32) ^ out.particles.size();
+        const double pt = -500.0 * std::log(keyed_uniform(key));
+        p.pt = static_cast<float>(pt);
+        p.p = static_cast<float>(pt * std::cosh(eta));
+      }
       const unsigned pidx = out.particles.size();
       std::vector<uint8_t> module_hit(n_modules, 0);
       for (unsigned sensor = 0; sensor < n_sensors; ++sensor) {

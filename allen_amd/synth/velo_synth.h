@@ -52,6 +52,9 @@ typedef struct {
   uint32_t n_hits;               /* digitised hits (clusters) */
   uint32_t n_modules;            /* distinct modules with a hit */
   uint32_t first_module, last_module;
+  float p, pt;                   /* momentum and transverse momentum (MeV): pT ~ exponential of mean
+                                    500 MeV, p = pT cosh(eta); drawn from a stream of their own, so
+                                    they change nothing else of the event */
 } velo_synth_particle;
 
 typedef struct {

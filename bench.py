@@ -36,6 +36,18 @@ WORKLOADS = {
 
 HBM_PEAK_GBS = 8000.0                    # MI355X HBM3E
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 lanes x 2.4 GHz, non-packed lane ops/s
+LDS_PEAK_GBS = 256 * 128 * 2.4e9 / 1e9    # 256 CUs x 128 B/clk x 2.4 GHz
+CPU_SHARE = 16                            # host cores that belong to one GPU of the box
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def log(*a):
@@ -96,7 +108,9 @@ def main():
                     help="also run the beamline primary-vertex finder on the beamline states (the path's consumer; "
                          "the cpu baseline then runs it too)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores (capped at 16)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the CPU baseline; 0: the host cores this process may use, at most the 16 "
+                         "that are one GPU's share of the box (both figures are reported)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="CPU work of the all-core baseline sample (the single-thread sample takes half)")
     ap.add_argument("--search-block", type=int, default=0)
@@ -114,10 +128,12 @@ def main():
                          "fastest (outputs are checked to be unchanged)")
     ap.add_argument("--graph", action="store_true",
                     help="time replays of the whole sequence captured as one HIP graph instead of eager runs")
-    ap.add_argument("--scorer", default="",
-                    help="NN-pruned seeding with the scorer parameters in this .npz (allen_amd.scorer), or 'shipped' "
-                         "for the package's trained scorer; the plain search is timed on the same events first and "
-                         "reported beside it")
+    ap.add_argument("--nn-pruned", action=argparse.BooleanOptionalAction, default=True,
+                    help="after the plain path, time the NN-pruned path (learned seed scorer) on the same events and "
+                         "streams and report it beside the plain one, as the metric names both")
+    ap.add_argument("--scorer", default="shipped",
+                    help="scorer of the NN-pruned leg: 'shipped' (the package's trained scorer) or an .npz of "
+                         "parameters (allen_amd.scorer)")
     ap.add_argument("--threshold", type=float, default=None,
                     help="logit below which a seed is pruned (default: the shipped scorer's operating threshold)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -208,7 +224,7 @@ def main():
 
     tuned = None
     tuning = None
-    if args.autotune and not args.scorer:
+    if args.autotune:
         # Workgroup size and LDS hit budget of search_by_triplet, chosen by timing whole steps in
         # this run's own regime (streams, graph, workload); outputs must not change. The scored
         # search keeps its fixed workgroup of 256.
@@ -221,24 +237,6 @@ def main():
                   "trials": res.trials, "seconds": round(time.time() - t0, 1)}
         log(f"[bench] autotune ({time.time() - t0:.1f}s, {len(res.trials)} trials): {res.knobs} "
             f"{res.search_ms:.4f} ms per step (block 256: {res.default_search_ms:.4f} ms)")
-
-    plain = None
-    params = None
-    if args.scorer:
-        from allen_amd import scorer
-        plain_ms = timed(args.steps, args.warmup, None) / args.steps
-        seq.run(dev)
-        plain = {"ms_per_step": round(plain_ms, 4), "events_per_s": round(ev.n_events / (plain_ms / 1e3), 1),
-                 "physics": physics_of(seq, ev)}
-        log(f"[bench] plain search: {plain}")
-        if args.scorer == "shipped":
-            params, shipped_threshold = scorer.trained()
-        else:
-            params, shipped_threshold = scorer.load_params(args.scorer), 0.0
-        if args.threshold is None:
-            args.threshold = shipped_threshold
-        for sq in seqs:
-            sq.set_scorer(params, args.threshold)
 
     for _ in range(args.warmup):
         seq.run(dev)
@@ -330,50 +328,119 @@ def main():
     dom_bytes = alg.get(dominant)
     if dom_bytes is None:
         dom_bytes = alg["clustering"]
-    achieved = dom_bytes / (kern_avg_ms / 1e3) / 1e9
+    # The dominant kernel's launch time is its one-stream time (HIP events around the stage on the
+    # run's stream, nothing else on the GPU), which the kernel trace of the same command reproduces;
+    # in the multi-stream timed steps its launches share the CUs with other slices' kernels.
+    launch_ms = stage_ms[dominant]
+    achieved = dom_bytes / (launch_ms / 1e3) / 1e9
+    traffic_name = args.workload + ("_retina" if args.decoding == "retina" else "")
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": measured_traffic(args.workload + ("_retina" if args.decoding == "retina" else ""), dominant),
-                "algorithmic_bytes": dom_bytes, "avg_launch_ms": round(kern_avg_ms, 4)}
-    # search_by_triplet is bounded by lane arithmetic, not bytes: 9 VALU ops per scored candidate,
-    # ~12 per extrapolation (hit_phi_16), 3 per binary-search iteration.
-    ops = 9 * (int(work[0]) + int(work[1])) + 12 * int(work[2]) + 3 * int(work[3])
-    search_tops = ops / (stage_ms["search"] / 1e3) / 1e12
-    # Clustering (or retina decoding) against HBM: its algorithmic bytes over its one-stream launch time.
-    cl_gbs = alg["clustering"] / (stage_ms["clustering"] / 1e3) / 1e9
-    extra = {"search_valu": {"ops_per_step": ops, "achieved_tops": round(search_tops, 4),
-                             "peak_tops": round(VALU_PEAK_TOPS, 2), "frac": round(search_tops / VALU_PEAK_TOPS, 5),
-                             "work": [int(x) for x in work]},
-             "clustering_hbm": {"achieved": round(cl_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "traffic": measured_traffic(traffic_name, dominant),
+                "algorithmic_bytes": dom_bytes, "avg_launch_ms": round(launch_ms, 4),
+                "launch_ms_in_timed_steps": round(kern_avg_ms, 4)}
+    # search_by_triplet's own bounds are the vector ALUs and LDS, not HBM: per scored candidate
+    # (seeding pair or forwarding h2) 9 VALU lane-ops and 15 B of hit data from LDS (x, y, z, phi,
+    # flags); per extrapolation ~12 lane-ops (hit_phi_16 with its division); per binary-search probe
+    # 3 lane-ops and a 2 B phi from LDS.
+    evals = int(work[0]) + int(work[1])
+    ops = 9 * evals + 12 * int(work[2]) + 3 * int(work[3])
+    lds_bytes = 15 * evals + 2 * int(work[3])
+    search_s = stage_ms["search"] / 1e3
+    search_tops = ops / search_s / 1e12
+    search_lds = lds_bytes / search_s / 1e9
+    search_alu_lds = {"valu": {"ops_per_launch": ops, "achieved": round(search_tops, 4),
+                               "peak": round(VALU_PEAK_TOPS, 2), "unit": "T lane-ops/s",
+                               "frac": round(search_tops / VALU_PEAK_TOPS, 5)},
+                      "lds": {"bytes_per_launch": lds_bytes, "achieved": round(search_lds, 2),
+                              "peak": round(LDS_PEAK_GBS, 1), "unit": "GB/s",
+                              "frac": round(search_lds / LDS_PEAK_GBS, 6)},
+                      "hbm": {"achieved": round(alg["search"] / search_s / 1e9, 2), "peak": HBM_PEAK_GBS,
+                              "unit": "GB/s", "frac": round(alg["search"] / search_s / 1e9 / HBM_PEAK_GBS, 5)},
+                      "avg_launch_ms": round(stage_ms["search"], 4),
+                      "work": {"seed_evaluations": int(work[0]), "forward_evaluations": int(work[1]),
+                               "extrapolations": int(work[2]), "search_probes": int(work[3])}}
+    if dominant == "search":
+        roofline["search_alu_lds"] = search_alu_lds
+    # Decoding and clustering against HBM: the algorithmic bytes of K1 + K2 + K3 over the sum of
+    # their one-stream launch times.
+    cl_ms = stage_ms["candidates"] + stage_ms["estimate"] + stage_ms["clustering"]
+    cl_gbs = alg["clustering"] / (cl_ms / 1e3) / 1e9
+    extra = {"clustering_hbm": {"achieved": round(cl_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": round(cl_gbs / HBM_PEAK_GBS, 5), "algorithmic_bytes": alg["clustering"],
-                                "avg_launch_ms": round(stage_ms["clustering"], 4),
-                                "traffic": measured_traffic(
-                                    args.workload + ("_retina" if args.decoding == "retina" else ""), "clustering")},
+                                "launch_ms": round(cl_ms, 4),
+                                "stages": ["candidates", "estimate", "clustering"],
+                                "traffic": measured_traffic(traffic_name, "clustering")},
              "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
              "search_events": search_events,
              "physics": physics}
+    if dominant != "search":
+        extra["search_alu_lds"] = search_alu_lds
+
+    # The NN-pruned leg: the same events and streams with the learned seed scorer pruning seeds
+    # (its in-search products need 256-lane workgroups), timed the same way.
+    nn = None
+    if args.nn_pruned:
+        from allen_amd import scorer
+        if args.scorer == "shipped":
+            params, shipped_threshold = scorer.trained()
+        else:
+            params, shipped_threshold = scorer.load_params(args.scorer), 0.0
+        threshold = shipped_threshold if args.threshold is None else args.threshold
+        for sq in seqs:
+            sq.set_tuning("search_block", 256)
+            sq.set_tuning("search_lds_hits", 0)
+            sq.set_scorer(params, threshold)
+        nn_ms = timed(args.steps, args.warmup, None) / args.steps
+        seq.run(dev)
+        nn_physics = physics_of(seq, ev)
+        nn = {"events_per_s": round(n / (nn_ms / 1e3), 1), "ms_per_step": round(nn_ms, 4),
+              "speedup_vs_plain": round(ms_per_step / nn_ms, 4), "threshold": threshold,
+              "scorer": os.path.basename(args.scorer), "search_knobs": {"search_block": 256},
+              "physics": nn_physics,
+              "long_efficiency_loss": round(physics["02_long_efficiency"] - nn_physics["02_long_efficiency"], 6),
+              "ghost_rate_change": round(nn_physics["ghost_rate"] - physics["ghost_rate"], 6)}
+        log(f"[bench] nn-pruned: {nn}")
+        for sq in seqs:
+            sq.set_scorer(None)
 
     cpu = None
     if not args.no_cpu_baseline:
         import oracle
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        threads = args.cpu_threads or min(CPU_SHARE, usable)
         sample = synth.generate(**dict(wl, n_events=min(n, 1000)), keep_truth=False)
+        config1 = synth.generate(**WORKLOADS["single_pv_200"], keep_truth=False)
 
-        def timed(n_threads, seconds):
+        def cpu_timed(events, n_threads, seconds):
             # One calibration pass, then enough repetitions of the sample for about `seconds` of CPU work.
-            _, s1 = oracle.throughput(sample, geometry, beamline, n_threads, 1, pv_finder=int(args.pv_finder))
+            _, s1 = oracle.throughput(events, geometry, beamline, n_threads, 1, native=True,
+                                      pv_finder=int(args.pv_finder))
             reps = max(1, int(round(seconds / max(s1, 1e-3))))
-            eps, secs = oracle.throughput(sample, geometry, beamline, n_threads, reps, pv_finder=int(args.pv_finder))
+            eps, secs = oracle.throughput(events, geometry, beamline, n_threads, reps, native=True,
+                                          pv_finder=int(args.pv_finder))
             return eps, secs, reps
 
-        eps, secs, reps = timed(threads, args.cpu_seconds)
-        eps1, secs1, reps1 = timed(1, args.cpu_seconds / 2)
-        log(f"[bench] cpu baseline: {eps:.1f} events/s on {threads} threads, {eps1:.1f} on one")
+        eps, secs, reps = cpu_timed(sample, threads, args.cpu_seconds)
+        eps1, secs1, reps1 = cpu_timed(sample, 1, args.cpu_seconds / 2)
+        c1, c1s, c1r = cpu_timed(config1, threads, args.cpu_seconds / 4)
+        c1_1, c1_1s, c1_1r = cpu_timed(config1, 1, args.cpu_seconds / 4)
+        log(f"[bench] cpu baseline: {eps:.1f} events/s on {threads} threads, {eps1:.1f} on one; "
+            f"single_pv_200: {c1:.1f} / {c1_1:.1f}")
         cpu = {"value": round(eps, 1), "unit": "events/s", "cores": threads, "kind": "port",
+               "build": "oracle/Makefile native: g++ -O3 -march=x86-64-v3 (Allen's CPU-target optimisation and CI "
+                        "platform level), default contraction",
+               "cpu_model": cpu_model(), "host_cores_usable": usable,
+               "cores_note": f"{threads} threads: the host cores usable here, at most the {CPU_SHARE} that are one "
+                             f"GPU's share of the box" if not args.cpu_threads else "set by --cpu-threads",
                "sample": f"{sample.n_events} events of the same workload x {reps} repetitions, one event per task "
                          f"on {threads} std::threads ({secs:.1f} s); single thread: {sample.n_events} events x "
                          f"{reps1} repetitions ({secs1:.1f} s) -> {eps1:.1f} events/s",
-               "single_thread_events_per_s": round(eps1, 1)}
+               "single_thread_events_per_s": round(eps1, 1),
+               "single_pv_200": {"events_per_s": round(c1, 1), "threads": threads,
+                                 "single_thread_events_per_s": round(c1_1, 1),
+                                 "sample": f"200 events x {c1r} repetitions ({c1s:.1f} s); single thread x "
+                                           f"{c1_1r} ({c1_1s:.1f} s)"}}
 
     out = {
         "metric": "VELO events/s on 1 MI355X, plain vs NN-pruned, at fixed long-track efficiency",
@@ -389,9 +456,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded generator, real VELO geometry, v4 super-pixel banks)",
         "config": {"workload": args.workload, "events_per_step": n, "decoding": args.decoding,
-                   "pv_finder": bool(args.pv_finder),
-                   "path": "nn_pruned" if args.scorer else "plain",
-                   **({"scorer": os.path.basename(args.scorer), "threshold": args.threshold} if args.scorer else {}),
+                   "pv_finder": bool(args.pv_finder), "path": "plain",
                    "launch": "hip_graph" if args.graph else "eager", "streams": n_streams,
                    "search_grid": "persistent" if args.persistent else "one_workgroup_per_event",
                    **({"search_knobs": tuned} if tuned else {}),
@@ -399,7 +464,8 @@ def main():
         "status": status,
         "roofline": roofline,
         "cpu_baseline": cpu,
-        **({"plain": plain, "speedup_vs_plain": round(plain["ms_per_step"] / ms_per_step, 4)} if plain else {}),
+        "long_track_efficiency": physics["02_long_efficiency"],
+        **({"nn_pruned": nn} if nn else {}),
         **({"autotune": tuning} if tuning else {}),
         **({"single_stream": {"ms_per_step": round(single_ms, 4), "events_per_s": round(n / (single_ms / 1e3), 1)}}
            if single_ms else {}),

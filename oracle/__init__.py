@@ -195,10 +195,34 @@ def run(events, geometry: bytes, beamline: np.ndarray, **props) -> Result:
     return Result(h, keep)
 
 
-def throughput(events, geometry: bytes, beamline: np.ndarray, n_threads: int, repetitions: int = 1, **props):
+NATIVE_LIB_PATH = os.path.join(_HERE, "build", "libvelo_oracle_native.so")
+_native = None
+
+
+def native_lib():
+    """The -O3 -march=x86-64-v3 build of the same sources (oracle/Makefile `native`), used only to
+    time the CPU baseline."""
+    global _native
+    if _native is None:
+        if not os.path.exists(NATIVE_LIB_PATH):
+            raise RuntimeError(f"{NATIVE_LIB_PATH} is not built; run `make -C oracle native`")
+        L = ctypes.CDLL(NATIVE_LIB_PATH)
+        L.vo_default_inputs.argtypes = [ctypes.POINTER(VoInputs)]
+        L.vo_throughput.restype = ctypes.c_double
+        L.vo_throughput.argtypes = [ctypes.POINTER(VoInputs), ctypes.c_uint, ctypes.c_uint,
+                                    ctypes.POINTER(ctypes.c_double)]
+        _native = L
+    return _native
+
+
+def throughput(events, geometry: bytes, beamline: np.ndarray, n_threads: int, repetitions: int = 1,
+               native: bool = False, **props):
+    """Events per second of the sequence over the events, one event per task on n_threads
+    std::threads, `repetitions` passes; native selects the -O3 x86-64-v3 build."""
     a, keep = make_inputs(events, geometry, beamline, **props)
     secs = ctypes.c_double()
-    eps = lib().vo_throughput(ctypes.byref(a), n_threads, repetitions, ctypes.byref(secs))
+    L = native_lib() if native else lib()
+    eps = L.vo_throughput(ctypes.byref(a), n_threads, repetitions, ctypes.byref(secs))
     del keep
     return eps, secs.value
 
