@@ -129,6 +129,8 @@ def velo_masked_clustering_t(args, events, constants):
     p.dev_velo_cluster_container = args.ptr("velo_cluster_container")
     p.constants = constants.handle
     p.event_start, p.mep_layout = args.host["event_start"], 0
+    args.make("velo_clusters", n * velo.CLUSTERS_VIEW_DTYPE.itemsize, torch.uint8)
+    p.dev_velo_clusters = args.ptr("velo_clusters")
     velo.check(velo.lib().velo_masked_clustering(ctypes.byref(p), None), "velo_masked_clustering")
 
 
@@ -228,7 +230,7 @@ def velo_copy_track_hit_number_t(args):
 
 
 def velo_consolidate_tracks_t(args):
-    """VeloConsolidateTracks.cu:48-110 (the track hits; the view objects are the caller's)."""
+    """VeloConsolidateTracks.cu:48-110: the track hits, then the view objects of create_velo_views."""
     torch = velo._torch()
     n = args.host["number_of_events"]
     nt = args.host["number_of_reconstructed_velo_tracks"]
@@ -248,11 +250,20 @@ def velo_consolidate_tracks_t(args):
     p.dev_offsets_number_of_three_hit_tracks_filtered = args.ptr("offsets_three_hit_tracks_filtered")
     p.dev_number_of_events = args.ptr("number_of_events")
     p.dev_velo_track_hits, p.dev_accepted_velo_tracks = args.ptr("velo_track_hits"), args.ptr("accepted_velo_tracks")
+    args.make("velo_hits_view", n * velo.HITS_VIEW_DTYPE.itemsize, torch.uint8)
+    args.make("velo_track_view", nt * velo.TRACK_VIEW_DTYPE.itemsize, torch.uint8)
+    args.make("velo_tracks_view", n * velo.TRACKS_VIEW_DTYPE.itemsize, torch.uint8)
+    args.make("velo_multi_event_tracks_view", velo.MULTI_EVENT_TRACKS_VIEW_DTYPE.itemsize, torch.uint8)
+    args.make("imec_velo_tracks", 8, torch.uint8)
+    for name in ["velo_hits_view", "velo_track_view", "velo_tracks_view", "velo_multi_event_tracks_view",
+                 "imec_velo_tracks"]:
+        setattr(p, "dev_" + name, args.ptr(name))
     velo.check(velo.lib().velo_consolidate_tracks(ctypes.byref(p), None), "velo_consolidate_tracks")
 
 
-def velo_kalman_filter_t(args, constants, covariance_fp64_fallback=0):
-    """VeloKalmanFilter.cu:9-31, 152-228."""
+def velo_kalman_filter_t(args, constants, covariance_fp64_fallback=0, from_views=True):
+    """VeloKalmanFilter.cu:9-31, 152-228. from_views: the tracks come through dev_velo_tracks_view,
+    as in the reference; otherwise through the hit-number offsets and the track hits."""
     torch = velo._torch()
     n = args.host["number_of_events"]
     nt = args.host["number_of_reconstructed_velo_tracks"]
@@ -264,10 +275,17 @@ def velo_kalman_filter_t(args, constants, covariance_fp64_fallback=0):
     _list_fields(p, args)
     p.dev_number_of_events = args.ptr("number_of_events")
     p.dev_offsets_all_velo_tracks = args.ptr("offsets_all_velo_tracks")
-    p.dev_offsets_velo_track_hit_number = args.ptr("offsets_velo_track_hit_number")
-    p.dev_velo_track_hits, p.dev_is_backward = args.ptr("velo_track_hits"), args.ptr("is_backward")
+    if from_views:
+        p.dev_velo_tracks_view = args.ptr("velo_tracks_view")
+    else:
+        p.dev_offsets_velo_track_hit_number = args.ptr("offsets_velo_track_hit_number")
+        p.dev_velo_track_hits = args.ptr("velo_track_hits")
+    p.dev_is_backward = args.ptr("is_backward")
     p.dev_velo_kalman_beamline_states = args.ptr("velo_kalman_beamline_states")
     p.dev_velo_kalman_endvelo_states = args.ptr("velo_kalman_endvelo_states")
+    for name in ["velo_kalman_beamline_states_view", "velo_kalman_endvelo_states_view"]:
+        args.make(name, n * velo.KALMAN_STATES_VIEW_DTYPE.itemsize, torch.uint8)
+        setattr(p, "dev_" + name, args.ptr(name))
     p.constants, p.covariance_fp64_fallback = constants.handle, covariance_fp64_fallback
     velo.check(velo.lib().velo_kalman_filter(ctypes.byref(p), None), "velo_kalman_filter")
 
@@ -306,5 +324,5 @@ def velo_sp_sequence(constants, events, number_of_events, event_list=None, event
     args.host["accumulated_number_of_hits_in_velo_tracks"] = host_prefix_sum(
         args, "velo_track_hit_number", "offsets_velo_track_hit_number", nt)
     velo_consolidate_tracks_t(args)
-    velo_kalman_filter_t(args, constants, props.get("kalman_fp64_fallback", 0))
+    velo_kalman_filter_t(args, constants, props.get("kalman_fp64_fallback", 0), props.get("kalman_from_views", True))
     return args

@@ -156,15 +156,18 @@ def track_truth(events, offsets_all_velo_tracks, offsets_velo_track_hit_number, 
     weight = np.zeros(n_tracks)
     sel = hit_particle >= 0
     if sel.any():
-        pairs = np.stack([hit_track[sel], hit_particle[sel]], axis=1)
-        uniq, counts = np.unique(pairs, axis=0, return_counts=True)
-        # Per track the largest count (ties: the lowest particle index, as np.unique sorts).
-        best = np.lexsort((-counts, uniq[:, 0]))
+        # (track, particle) pairs as one sortable key.
+        n_part = int(hit_particle[sel].max()) + 1
+        ukey, counts = np.unique(hit_track[sel] * n_part + hit_particle[sel], return_counts=True)
+        u_track, u_part = ukey // n_part, ukey % n_part
+        # Per track the largest count (ties: the lowest particle index; lexsort is stable and the
+        # keys are sorted by particle within a track).
+        best = np.lexsort((-counts, u_track))
         first = np.ones(best.size, bool)
-        first[1:] = uniq[best[1:], 0] != uniq[best[:-1], 0]
+        first[1:] = u_track[best[1:]] != u_track[best[:-1]]
         b = best[first]
-        particle[uniq[b, 0]] = uniq[b, 1]
-        weight[uniq[b, 0]] = counts[b] / n_hits[uniq[b, 0]]
+        particle[u_track[b]] = u_part[b]
+        weight[u_track[b]] = counts[b] / n_hits[u_track[b]]
     return particle, weight, n_hits
 
 

@@ -311,7 +311,23 @@ int velo_decode_retinaclusters(const velo_decode_retinaclusters_params* p, hipSt
   a.hit_permutations = p->dev_hit_permutations;
   a.hit_sorting_key = p->dev_hit_sorting_key;
   a.populate_items = static_cast<unsigned>(items);
-  return hip_status(velo::launch_retina_decode(a, stream));
+  rc = hip_status(velo::launch_retina_decode(a, stream));
+  if (rc == VELO_OK && p->dev_velo_clusters) {
+    // DecodeRetinaClusters.cu:331-346,442: zeroed, then {container, total of the sensor-pair
+    // offsets, 0} per listed event.
+    velo::ClustersViewLaunch v;
+    v.event_list = p->dev_event_list;
+    v.n_list = p->event_list_size;
+    v.n_events = p->host_number_of_events;
+    v.zero_all = true;
+    v.number_of_events = p->dev_number_of_events;
+    v.totals = p->dev_offsets_each_sensor_pair_size;
+    v.totals_stride = velo::retina_sensor_slots(p->host_raw_bank_version);
+    v.container = p->dev_velo_cluster_container;
+    v.views = p->dev_velo_clusters;
+    rc = hip_status(velo::launch_clusters_views(v, stream));
+  }
+  return rc;
 }
 
 int velo_pv_beamline(const velo_pv_beamline_params* p, hipStream_t stream)
@@ -370,6 +386,8 @@ int velo_masked_clustering(const velo_masked_clustering_params* p, hipStream_t s
 {
   if (!p || !p->constants || !p->dev_module_cluster_num || !p->dev_velo_cluster_container)
     return VELO_ERR_INVALID_ARGUMENT;
+  if (p->dev_velo_clusters && (!p->dev_number_of_events || !p->dev_offsets_estimated_input_size))
+    return VELO_ERR_INVALID_ARGUMENT;
   if (p->mep_layout) return VELO_ERR_UNSUPPORTED_LAYOUT;
   int rc = hip_status(velo::launch_fill_u32(
     p->dev_module_cluster_num, (size_t) p->host_number_of_events * velo::n_module_pairs, 0, stream));
@@ -396,7 +414,22 @@ int velo_masked_clustering(const velo_masked_clustering_params* p, hipStream_t s
   a.sp_fy = p->constants->dev_sp_fy;
   a.module_cluster_num = p->dev_module_cluster_num;
   a.cluster_container = p->dev_velo_cluster_container;
-  return hip_status(velo::launch_masked_clustering(a, stream));
+  rc = hip_status(velo::launch_masked_clustering(a, stream));
+  if (rc == VELO_OK && p->dev_velo_clusters) {
+    // MaskedVeloClustering.cu:519-522: {container, estimated total, 0} per listed event.
+    velo::ClustersViewLaunch v;
+    v.event_list = p->dev_event_list;
+    v.n_list = p->event_list_size;
+    v.n_events = p->host_number_of_events;
+    v.zero_all = false;
+    v.number_of_events = p->dev_number_of_events;
+    v.totals = p->dev_offsets_estimated_input_size;
+    v.totals_stride = velo::n_module_pairs;
+    v.container = p->dev_velo_cluster_container;
+    v.views = p->dev_velo_clusters;
+    rc = hip_status(velo::launch_clusters_views(v, stream));
+  }
+  return rc;
 }
 
 int velo_sort_by_phi(const velo_sort_by_phi_params* p, hipStream_t stream)
@@ -523,18 +556,39 @@ int velo_consolidate_tracks(const velo_consolidate_tracks_params* p, hipStream_t
   a.velo_track_hits = p->dev_velo_track_hits;
   a.accepted_velo_tracks = p->dev_accepted_velo_tracks;
   // Every track is accepted (VeloConsolidateTracks.cu:84), listed event or not.
+  const int n_views = !!p->dev_velo_hits_view + !!p->dev_velo_track_view + !!p->dev_velo_tracks_view +
+                      !!p->dev_velo_multi_event_tracks_view + !!p->dev_imec_velo_tracks;
+  if (n_views != 0 && n_views != 5) return VELO_ERR_INVALID_ARGUMENT;
+  if (n_views && (!p->dev_number_of_events || !p->dev_offsets_all_velo_tracks || !p->dev_offsets_velo_track_hit_number))
+    return VELO_ERR_INVALID_ARGUMENT;
   int rc = VELO_OK;
   if (p->dev_accepted_velo_tracks)
     rc = hip_status(
       velo::launch_fill_u8(p->dev_accepted_velo_tracks, p->host_number_of_reconstructed_velo_tracks, 1, stream));
   if (rc == VELO_OK) rc = hip_status(velo::launch_consolidate_tracks(a, stream));
+  if (rc == VELO_OK && n_views) {
+    velo::VeloViewsLaunch v;
+    v.n_events = p->host_number_of_events;
+    v.number_of_events = p->dev_number_of_events;
+    v.offsets_all_velo_tracks = p->dev_offsets_all_velo_tracks;
+    v.offsets_velo_track_hit_number = p->dev_offsets_velo_track_hit_number;
+    v.velo_track_hits = p->dev_velo_track_hits;
+    v.hits_view = p->dev_velo_hits_view;
+    v.track_view = p->dev_velo_track_view;
+    v.tracks_view = p->dev_velo_tracks_view;
+    v.multi_event_tracks_view = p->dev_velo_multi_event_tracks_view;
+    v.imec_velo_tracks = p->dev_imec_velo_tracks;
+    rc = hip_status(velo::launch_create_velo_views(v, stream));
+  }
   return rc;
 }
 
 int velo_kalman_filter(const velo_kalman_filter_params* p, hipStream_t stream)
 {
   if (!p || !p->constants || !p->dev_velo_kalman_beamline_states || !p->dev_velo_kalman_endvelo_states ||
-      !p->dev_is_backward)
+      !p->dev_is_backward || !p->dev_number_of_events || !p->dev_offsets_all_velo_tracks)
+    return VELO_ERR_INVALID_ARGUMENT;
+  if (!p->dev_velo_tracks_view && (!p->dev_offsets_velo_track_hit_number || !p->dev_velo_track_hits))
     return VELO_ERR_INVALID_ARGUMENT;
   velo::KalmanLaunch a;
   a.event_list = p->dev_event_list;
@@ -548,6 +602,9 @@ int velo_kalman_filter(const velo_kalman_filter_params* p, hipStream_t stream)
   a.beamline_states = p->dev_velo_kalman_beamline_states;
   a.endvelo_states = p->dev_velo_kalman_endvelo_states;
   a.fp64_fallback = p->covariance_fp64_fallback != 0;
+  a.tracks_view = p->dev_velo_tracks_view;
+  a.beamline_states_view = p->dev_velo_kalman_beamline_states_view;
+  a.endvelo_states_view = p->dev_velo_kalman_endvelo_states_view;
   return hip_status(velo::launch_kalman_filter(a, stream));
 }
 

@@ -3,7 +3,10 @@
 //   k_copy_track_hit_number   VeloCopyTrackHitNumber.cu:49-78
 //   k_consolidate_tracks      VeloConsolidateTracks.cu:120-192 (one thread per track hit)
 //   k_kalman_filter           VeloKalmanFilter.cu:335-418, VeloKalmanFilter.cuh:25-158 (one thread per track)
+//   k_create_velo_views       VeloConsolidateTracks.cu:11-46 (Allen's view objects, byte for byte)
+//   k_clusters_views          MaskedVeloClustering.cu:519-522, DecodeRetinaClusters.cu:331-346
 //   k_scan_*                  replaces host_prefix_sum_t (HostPrefixSum.cpp:76-107) by a device scan
+#include "velo_amd.h"
 #include "velo_common.cuh"
 #include "velo_kernels.h"
 
@@ -389,50 +392,196 @@ __device__ __forceinline__ void store_state(float* base, unsigned total_tracks, 
   cv[5] = s.c33;
 }
 
+// One track: linear_fit, the upstream fit to the beamline and the downstream fit to the end of
+// the VELO (VeloKalmanFilter.cu:384-399); xyz holds its nh hits' x, y, z.
+__device__ __forceinline__ void fit_track(
+  const float* xyz,
+  unsigned nh,
+  unsigned t,
+  unsigned total_tracks,
+  const float* bl,
+  uint8_t* is_backward,
+  float* beamline_states,
+  float* endvelo_states,
+  unsigned fp64_fallback)
+{
+  // linear_fit (VeloKalmanFilter.cu:335-355)
+  const float fx = xyz[0], fy = xyz[1], fz = xyz[2];
+  const float lx = xyz[3 * (nh - 1)], ly = xyz[3 * (nh - 1) + 1], lz = xyz[3 * (nh - 1) + 2];
+  const float ltx = (lx - fx) / (lz - fz);
+  const float lty = (ly - fy) / (lz - fz);
+  const float ldz = (ltx * (bl[0] - lx) + lty * (bl[1] - ly)) / (ltx * ltx + lty * lty);
+  const float lin_z = lz + ldz;
+  const bool backward = lin_z > fz;
+  is_backward[t] = backward;
+  if (fp64_fallback) {
+    const KState b = simplified_fit<true, true>(xyz, nh, ltx, lty, bl, backward);
+    const KState v = simplified_fit<false, true>(xyz, nh, b.tx, b.ty, bl, backward);
+    store_state(beamline_states, total_tracks, t, b);
+    store_state(endvelo_states, total_tracks, t, v);
+  }
+  else {
+    const KState b = simplified_fit<true, false>(xyz, nh, ltx, lty, bl, backward);
+    const KState v = simplified_fit<false, false>(xyz, nh, b.tx, b.ty, bl, backward);
+    store_state(beamline_states, total_tracks, t, b);
+    store_state(endvelo_states, total_tracks, t, v);
+  }
+}
+
+// KalmanStates view of event e (VeloKalmanFilter.cu:370-374).
+__device__ __forceinline__ void write_states_view(
+  velo_kalman_states_view* views, const float* states, const uint32_t* offsets_all_velo_tracks, unsigned e,
+  unsigned n_events)
+{
+  velo_kalman_states_view v;
+  v.base_pointer = reinterpret_cast<const char*>(states);
+  v.offset = offsets_all_velo_tracks[e];
+  v.size = offsets_all_velo_tracks[e + 1] - offsets_all_velo_tracks[e];
+  v.total_number_of_tracks = offsets_all_velo_tracks[n_events];
+  v.pad_ = 0;
+  views[e] = v;
+}
+
+// One workgroup per listed event, one thread per track. views: the tracks and their hits come
+// through the event's Tracks view (Track -> Hits -> the consolidated container), as the
+// reference reads them; otherwise from the hit-number offsets and the container.
+template<bool views>
 __global__ void k_kalman_filter(
   const uint32_t* event_list,
   const uint32_t* number_of_events,
   const uint32_t* offsets_all_velo_tracks,
   const uint32_t* offsets_velo_track_hit_number,
   const char* velo_track_hits,
+  const velo_tracks_view* tracks_view,
   const float* beamline,
   uint8_t* is_backward,
   float* beamline_states,
   float* endvelo_states,
+  velo_kalman_states_view* beamline_states_view,
+  velo_kalman_states_view* endvelo_states_view,
   unsigned fp64_fallback)
 {
   const unsigned e = event_list ? event_list[blockIdx.x] : blockIdx.x;
   const unsigned n_events = number_of_events[0];
   const unsigned total_tracks = offsets_all_velo_tracks[n_events];
-  const unsigned total_hits = offsets_velo_track_hit_number[total_tracks];
-  const float* xyz_all = reinterpret_cast<const float*>(velo_track_hits) + total_hits;
   const float bl[2] = {beamline[0], beamline[1]};
-  for (unsigned t = offsets_all_velo_tracks[e] + threadIdx.x; t < offsets_all_velo_tracks[e + 1]; t += blockDim.x) {
-    const unsigned h0 = offsets_velo_track_hit_number[t];
-    const unsigned nh = offsets_velo_track_hit_number[t + 1] - h0;
-    if (nh < 2) continue; // every VELO track has >= 3 hits
-    const float* xyz = xyz_all + 3 * (size_t) h0;
-    // linear_fit (VeloKalmanFilter.cu:335-355)
-    const float fx = xyz[0], fy = xyz[1], fz = xyz[2];
-    const float lx = xyz[3 * (nh - 1)], ly = xyz[3 * (nh - 1) + 1], lz = xyz[3 * (nh - 1) + 2];
-    const float ltx = (lx - fx) / (lz - fz);
-    const float lty = (ly - fy) / (lz - fz);
-    const float ldz = (ltx * (bl[0] - lx) + lty * (bl[1] - ly)) / (ltx * ltx + lty * lty);
-    const float lin_z = lz + ldz;
-    const bool backward = lin_z > fz;
-    is_backward[t] = backward;
-    if (fp64_fallback) {
-      const KState b = simplified_fit<true, true>(xyz, nh, ltx, lty, bl, backward);
-      const KState v = simplified_fit<false, true>(xyz, nh, b.tx, b.ty, bl, backward);
-      store_state(beamline_states, total_tracks, t, b);
-      store_state(endvelo_states, total_tracks, t, v);
+  if (threadIdx.x == 0) {
+    if (beamline_states_view)
+      write_states_view(beamline_states_view, beamline_states, offsets_all_velo_tracks, e, n_events);
+    if (endvelo_states_view)
+      write_states_view(endvelo_states_view, endvelo_states, offsets_all_velo_tracks, e, n_events);
+  }
+  if constexpr (views) {
+    const velo_tracks_view tv = tracks_view[e];
+    for (unsigned i = threadIdx.x; i < tv.size; i += blockDim.x) {
+      const velo_track_view tr = tv.track[i];
+      const velo_hits_view h = *tr.hits;
+      if (tr.number_of_hits < 2) continue;
+      const float* xyz = reinterpret_cast<const float*>(h.base_pointer) + h.total_number_of_hits +
+                         3 * (size_t) (h.offset + tr.offset);
+      fit_track(xyz, tr.number_of_hits, tv.offset + i, total_tracks, bl, is_backward, beamline_states,
+                endvelo_states, fp64_fallback);
     }
-    else {
-      const KState b = simplified_fit<true, false>(xyz, nh, ltx, lty, bl, backward);
-      const KState v = simplified_fit<false, false>(xyz, nh, b.tx, b.ty, bl, backward);
-      store_state(beamline_states, total_tracks, t, b);
-      store_state(endvelo_states, total_tracks, t, v);
+  }
+  else {
+    const unsigned total_hits = offsets_velo_track_hit_number[total_tracks];
+    const float* xyz_all = reinterpret_cast<const float*>(velo_track_hits) + total_hits;
+    for (unsigned t = offsets_all_velo_tracks[e] + threadIdx.x; t < offsets_all_velo_tracks[e + 1];
+         t += blockDim.x) {
+      const unsigned h0 = offsets_velo_track_hit_number[t];
+      const unsigned nh = offsets_velo_track_hit_number[t + 1] - h0;
+      if (nh < 2) continue; // every VELO track has >= 3 hits
+      fit_track(xyz_all + 3 * (size_t) h0, nh, t, total_tracks, bl, is_backward, beamline_states, endvelo_states,
+                fp64_fallback);
     }
+  }
+}
+
+// create_velo_views (VeloConsolidateTracks.cu:11-46): event e's Hits and Tracks views and one
+// Track view per track; the first workgroup also writes the multi-event view and its address.
+__global__ void k_create_velo_views(
+  const uint32_t* number_of_events,
+  const uint32_t* offsets_all_velo_tracks,
+  const uint32_t* offsets_velo_track_hit_number,
+  const char* velo_track_hits,
+  velo_hits_view* hits_view,
+  velo_track_view* track_view,
+  velo_tracks_view* tracks_view,
+  velo_multi_event_tracks_view* multi_view,
+  const velo_multi_event_tracks_view** imec)
+{
+  const unsigned n_events = number_of_events[0];
+  const unsigned e = blockIdx.x;
+  const unsigned first_track = offsets_all_velo_tracks[e];
+  const unsigned n_tracks = offsets_all_velo_tracks[e + 1] - first_track;
+  const uint32_t* hit_offsets = offsets_velo_track_hit_number + first_track;
+  const unsigned first_hit = hit_offsets[0];
+  for (unsigned i = threadIdx.x; i < n_tracks; i += blockDim.x) {
+    velo_track_view t;
+    t.hits = hits_view + e;
+    t.track_index = i;
+    t.track_container_offset = first_track;
+    t.offset = hit_offsets[i] - first_hit;
+    t.number_of_hits = hit_offsets[i + 1] - hit_offsets[i];
+    track_view[first_track + i] = t;
+  }
+  if (threadIdx.x == 0) {
+    velo_hits_view h;
+    h.base_pointer = velo_track_hits;
+    h.offset = first_hit;
+    h.size = offsets_velo_track_hit_number[first_track + n_tracks] - first_hit;
+    h.total_number_of_hits = offsets_velo_track_hit_number[offsets_all_velo_tracks[n_events]];
+    h.pad_ = 0;
+    hits_view[e] = h;
+    velo_tracks_view ts;
+    ts.track = track_view + first_track;
+    ts.size = n_tracks;
+    ts.offset = first_track;
+    tracks_view[e] = ts;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    velo_multi_event_tracks_view m;
+    m.type_id = VELO_TYPE_ID_VELO_TRACKS;
+    m.pad_ = 0;
+    m.container = tracks_view;
+    m.number_of_events = n_events;
+    m.pad2_ = 0;
+    *multi_view = m;
+    imec[0] = multi_view;
+  }
+}
+
+// The Velo::Clusters view of each listed event (one thread per list entry); with zero_all every
+// entry is cleared first by the same threads (the reference's memset of the retina decoding's
+// views, DecodeRetinaClusters.cu:442; a single workgroup keeps the clear before the writes).
+__global__ void k_clusters_views(
+  const uint32_t* event_list,
+  unsigned n_list,
+  unsigned n_events,
+  unsigned zero_all,
+  const uint32_t* number_of_events,
+  const uint32_t* totals,
+  unsigned totals_stride,
+  char* container,
+  velo_clusters_view* views)
+{
+  if (zero_all) {
+    for (unsigned i = threadIdx.x; i < n_events; i += blockDim.x) {
+      velo_clusters_view z;
+      z.base_pointer = nullptr;
+      z.total_number_of_hits = 0;
+      z.offset = 0;
+      views[i] = z;
+    }
+    __syncthreads();
+  }
+  const unsigned total = totals[(size_t) totals_stride * number_of_events[0]];
+  for (unsigned i = threadIdx.x; i < n_list; i += blockDim.x) {
+    velo_clusters_view v;
+    v.base_pointer = container;
+    v.total_number_of_hits = total;
+    v.offset = 0;
+    views[event_list ? event_list[i] : i] = v;
   }
 }
 
@@ -460,10 +609,42 @@ hipError_t launch_consolidate_tracks(const ConsolidateLaunch& a, hipStream_t s)
 hipError_t launch_kalman_filter(const KalmanLaunch& a, hipStream_t s)
 {
   if (!a.n_list) return hipSuccess;
+  auto* bsv = static_cast<velo_kalman_states_view*>(a.beamline_states_view);
+  auto* esv = static_cast<velo_kalman_states_view*>(a.endvelo_states_view);
+  const auto* tv = static_cast<const velo_tracks_view*>(a.tracks_view);
+  if (tv)
+    hipLaunchKernelGGL(
+      k_kalman_filter<true>, dim3(a.n_list), dim3(256), 0, s, a.event_list, a.number_of_events,
+      a.offsets_all_velo_tracks, a.offsets_velo_track_hit_number, a.velo_track_hits, tv, a.beamline, a.is_backward,
+      reinterpret_cast<float*>(a.beamline_states), reinterpret_cast<float*>(a.endvelo_states), bsv, esv,
+      a.fp64_fallback);
+  else
+    hipLaunchKernelGGL(
+      k_kalman_filter<false>, dim3(a.n_list), dim3(256), 0, s, a.event_list, a.number_of_events,
+      a.offsets_all_velo_tracks, a.offsets_velo_track_hit_number, a.velo_track_hits, tv, a.beamline, a.is_backward,
+      reinterpret_cast<float*>(a.beamline_states), reinterpret_cast<float*>(a.endvelo_states), bsv, esv,
+      a.fp64_fallback);
+  return hipGetLastError();
+}
+
+hipError_t launch_create_velo_views(const VeloViewsLaunch& a, hipStream_t s)
+{
+  if (!a.n_events) return hipSuccess;
   hipLaunchKernelGGL(
-    k_kalman_filter, dim3(a.n_list), dim3(256), 0, s, a.event_list, a.number_of_events, a.offsets_all_velo_tracks,
-    a.offsets_velo_track_hit_number, a.velo_track_hits, a.beamline, a.is_backward,
-    reinterpret_cast<float*>(a.beamline_states), reinterpret_cast<float*>(a.endvelo_states), a.fp64_fallback);
+    k_create_velo_views, dim3(a.n_events), dim3(256), 0, s, a.number_of_events, a.offsets_all_velo_tracks,
+    a.offsets_velo_track_hit_number, a.velo_track_hits, static_cast<velo_hits_view*>(a.hits_view),
+    static_cast<velo_track_view*>(a.track_view), static_cast<velo_tracks_view*>(a.tracks_view),
+    static_cast<velo_multi_event_tracks_view*>(a.multi_event_tracks_view),
+    static_cast<const velo_multi_event_tracks_view**>(a.imec_velo_tracks));
+  return hipGetLastError();
+}
+
+hipError_t launch_clusters_views(const ClustersViewLaunch& a, hipStream_t s)
+{
+  if (!a.n_list && !a.zero_all) return hipSuccess;
+  hipLaunchKernelGGL(
+    k_clusters_views, dim3(1), dim3(1024), 0, s, a.event_list, a.n_list, a.n_events, unsigned(a.zero_all),
+    a.number_of_events, a.totals, a.totals_stride, a.container, static_cast<velo_clusters_view*>(a.views));
   return hipGetLastError();
 }
 

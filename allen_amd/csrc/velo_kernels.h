@@ -290,6 +290,40 @@ struct KalmanLaunch {
   char* beamline_states;
   char* endvelo_states;
   unsigned fp64_fallback = 0; // FP64 covariance step where the FP32 update is ill-conditioned
+  // The reference's view input and outputs (velo_amd.h layouts): with tracks_view the tracks and
+  // their hits are read through the views; the KalmanStates views are written when set.
+  const void* tracks_view = nullptr;
+  void* beamline_states_view = nullptr;
+  void* endvelo_states_view = nullptr;
+};
+
+// create_velo_views (VeloConsolidateTracks.cu:11-46): one workgroup per event of the batch.
+struct VeloViewsLaunch {
+  unsigned n_events; // host_number_of_events
+  const uint32_t* number_of_events;
+  const uint32_t* offsets_all_velo_tracks;
+  const uint32_t* offsets_velo_track_hit_number;
+  const char* velo_track_hits;
+  void* hits_view;
+  void* track_view;
+  void* tracks_view;
+  void* multi_event_tracks_view;
+  void* imec_velo_tracks;
+};
+
+// The Velo::Clusters view of each listed event: {container, total, 0} with total =
+// totals[totals_stride * number_of_events[0]]; with zero_all every one of n_events entries is
+// zeroed first.
+struct ClustersViewLaunch {
+  const uint32_t* event_list;
+  unsigned n_list;
+  unsigned n_events;
+  bool zero_all;
+  const uint32_t* number_of_events;
+  const uint32_t* totals;
+  unsigned totals_stride;
+  char* container;
+  void* views;
 };
 
 hipError_t launch_number_of_candidates(const CandidatesLaunch& a, hipStream_t s);
@@ -359,6 +393,8 @@ hipError_t launch_prefix_sum_dyn(
   uint32_t* scratch,
   hipStream_t s);
 // Fill `n` u32 with `value` (a kernel, so the work can be captured in a graph).
+hipError_t launch_create_velo_views(const VeloViewsLaunch& a, hipStream_t s);
+hipError_t launch_clusters_views(const ClustersViewLaunch& a, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t value, hipStream_t s);
 hipError_t launch_fill_u8(uint8_t* p, size_t n, uint8_t value, hipStream_t s);
 

@@ -60,7 +60,8 @@ MaskedClusteringParams = _params("MaskedClusteringParams", [
     ("dev_velo_raw_input_types", P), ("dev_offsets_estimated_input_size", P), ("dev_module_candidate_num", P),
     ("dev_cluster_candidates", P), ("dev_event_list", P), ("event_list_size", U), ("dev_candidates_offsets", P),
     ("dev_number_of_events", P), ("dev_velo_bank_index", P), ("dev_module_cluster_num", P),
-    ("dev_velo_cluster_container", P), ("constants", P), ("event_start", U), ("mep_layout", I)])
+    ("dev_velo_cluster_container", P), ("constants", P), ("event_start", U), ("mep_layout", I),
+    ("dev_velo_clusters", P)])
 
 PVParams = _params("PVParams", [
     ("host_number_of_events", U), ("dev_event_list", P), ("event_list_size", U), ("dev_number_of_events", P),
@@ -87,7 +88,7 @@ RetinaDecodeParams = _params("RetinaDecodeParams", [
     ("dev_offsets_each_sensor_pair_size", P), ("dev_retina_bank_index", P), ("dev_event_list", P),
     ("event_list_size", U), ("dev_number_of_events", P), ("dev_module_cluster_num", P),
     ("dev_offsets_module_pair_cluster", P), ("dev_velo_cluster_container", P), ("dev_hit_permutations", P),
-    ("dev_hit_sorting_key", P), ("constants", P), ("event_start", U), ("mep_layout", I)])
+    ("dev_hit_sorting_key", P), ("constants", P), ("event_start", U), ("mep_layout", I), ("dev_velo_clusters", P)])
 
 SortParams = _params("SortParams", [
     ("host_number_of_events", U), ("host_total_number_of_velo_clusters", U), ("dev_event_list", P),
@@ -123,14 +124,32 @@ ConsolidateParams = _params("ConsolidateParams", [
     ("dev_offsets_velo_track_hit_number", P), ("dev_sorted_velo_cluster_container", P),
     ("dev_offsets_estimated_input_size", P), ("dev_three_hit_tracks_output", P),
     ("dev_offsets_number_of_three_hit_tracks_filtered", P), ("dev_number_of_events", P),
-    ("dev_velo_track_hits", P), ("dev_accepted_velo_tracks", P)])
+    ("dev_velo_track_hits", P), ("dev_accepted_velo_tracks", P), ("dev_velo_hits_view", P),
+    ("dev_velo_track_view", P), ("dev_velo_tracks_view", P), ("dev_velo_multi_event_tracks_view", P),
+    ("dev_imec_velo_tracks", P)])
 
 KalmanParams = _params("KalmanParams", [
     ("host_number_of_reconstructed_velo_tracks", U), ("host_number_of_events", U), ("dev_event_list", P),
     ("event_list_size", U), ("dev_number_of_events", P), ("dev_offsets_all_velo_tracks", P),
     ("dev_offsets_velo_track_hit_number", P), ("dev_velo_track_hits", P), ("dev_is_backward", P),
     ("dev_velo_kalman_beamline_states", P), ("dev_velo_kalman_endvelo_states", P), ("constants", P),
-    ("covariance_fp64_fallback", U)])
+    ("covariance_fp64_fallback", U), ("dev_velo_tracks_view", P), ("dev_velo_kalman_beamline_states_view", P),
+    ("dev_velo_kalman_endvelo_states_view", P)])
+
+# Allen's device view objects (velo_amd.h): the byte layouts of Velo::Clusters,
+# Allen::Views::Velo::Consolidated::{Hits, Track, Tracks}, MultiEventContainer<Tracks> and
+# Allen::Views::Physics::KalmanStates with 64-bit pointers; pads are zero.
+CLUSTERS_VIEW_DTYPE = np.dtype([("base_pointer", "<u8"), ("total_number_of_hits", "<u4"), ("offset", "<u4")])
+HITS_VIEW_DTYPE = np.dtype([("base_pointer", "<u8"), ("offset", "<u4"), ("size", "<u4"),
+                            ("total_number_of_hits", "<u4"), ("pad_", "<u4")])
+TRACK_VIEW_DTYPE = np.dtype([("hits", "<u8"), ("track_index", "<u4"), ("track_container_offset", "<u4"),
+                             ("offset", "<u4"), ("number_of_hits", "<u4")])
+TRACKS_VIEW_DTYPE = np.dtype([("track", "<u8"), ("size", "<u4"), ("offset", "<u4")])
+MULTI_EVENT_TRACKS_VIEW_DTYPE = np.dtype([("type_id", "<i4"), ("pad_", "<u4"), ("container", "<u8"),
+                                          ("number_of_events", "<u4"), ("pad2_", "<u4")])
+KALMAN_STATES_VIEW_DTYPE = np.dtype([("base_pointer", "<u8"), ("offset", "<u4"), ("size", "<u4"),
+                                     ("total_number_of_tracks", "<u4"), ("pad_", "<u4")])
+TYPE_ID_VELO_TRACKS = 1
 
 PrefixSumParams = _params("PrefixSumParams", [
     ("dev_input_buffer", P), ("input_size", U), ("dev_output_buffer", P), ("dev_scratch", P),

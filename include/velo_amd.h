@@ -137,6 +137,51 @@ typedef struct {
 } velo_estimate_input_size_params;
 int velo_estimate_input_size(const velo_estimate_input_size_params* p, hipStream_t stream);
 
+/* ---- Allen's device view objects of this path, byte for byte (half_t == float, 64-bit pointers):
+ * the objects the clustering kernels (MaskedVeloClustering.cu:519-522, DecodeRetinaClusters.cu:346),
+ * create_velo_views (VeloConsolidateTracks.cu:11-46) and velo_kalman_filter (VeloKalmanFilter.cu:370-374)
+ * construct in place. The reference's classes have no virtual functions, and their empty CRTP
+ * bases take no bytes, so each is its data members in declaration order. Pads are written as 0. */
+typedef struct {               /* Velo::Clusters = Clusters_t<char> (VeloEventModel.cuh:91-110), 16 B */
+  char* base_pointer;
+  unsigned total_number_of_hits;
+  unsigned offset;
+} velo_clusters_view;
+typedef struct {               /* Allen::Views::Velo::Consolidated::Hits (VeloConsolidated.cuh:60-99), 24 B */
+  const char* base_pointer;
+  unsigned offset;
+  unsigned size;
+  unsigned total_number_of_hits;
+  unsigned pad_;
+} velo_hits_view;
+typedef struct {               /* ...::Consolidated::Track (VeloConsolidated.cuh:199-259), 24 B */
+  const velo_hits_view* hits;  /* the event's Hits view */
+  unsigned track_index;
+  unsigned track_container_offset;
+  unsigned offset;             /* of the track's first hit within the event */
+  unsigned number_of_hits;
+} velo_track_view;
+typedef struct {               /* ...::Consolidated::Tracks (VeloConsolidated.cuh:261-294), 16 B */
+  const velo_track_view* track; /* the event's first Track view */
+  unsigned size;
+  unsigned offset;
+} velo_tracks_view;
+typedef struct {               /* Allen::MultiEventContainer<Tracks> (MultiEventContainer.cuh:38-65), 24 B */
+  int type_id;                 /* Identifiable::m_type_id: VELO_TYPE_ID_VELO_TRACKS */
+  unsigned pad_;
+  const velo_tracks_view* container;
+  unsigned number_of_events;
+  unsigned pad2_;
+} velo_multi_event_tracks_view;
+typedef struct {               /* Allen::Views::Physics::KalmanStates (States.cuh:191-220), 24 B */
+  const char* base_pointer;
+  unsigned offset;
+  unsigned size;
+  unsigned total_number_of_tracks;
+  unsigned pad_;
+} velo_kalman_states_view;
+#define VELO_TYPE_ID_VELO_TRACKS 1 /* Allen::TypeIDs::VeloTracks (backend/include/TypeID.h:23-34) */
+
 /* ---- velo_masked_clustering_t (device/velo/mask_clustering/include/MaskedVeloClustering.cuh:14-37) */
 typedef struct {
   unsigned host_total_number_of_velo_clusters;
@@ -158,6 +203,7 @@ typedef struct {
   const velo_constants* constants;
   unsigned event_start;
   int mep_layout;
+  velo_clusters_view* dev_velo_clusters;   /* optional, events: written for the listed events */
 } velo_masked_clustering_params;
 int velo_masked_clustering(const velo_masked_clustering_params* p, hipStream_t stream);
 
@@ -191,8 +237,7 @@ unsigned velo_retina_sensor_slots(int bank_version);
  * decodes every word, keys it by (phi as u16) << 48 | LHCbID (the largest key for a zero word)
  * and writes the clusters of each module pair sorted by key from dev_offsets_module_pair_cluster
  * on; dev_module_cluster_num counts the non-zero words. Where words share a key the slot holds
- * the last of them in word order. The reference's dev_velo_clusters views are built on the
- * caller's side. No scratch is needed. */
+ * the last of them in word order. No scratch is needed. */
 typedef struct {
   unsigned host_total_number_of_velo_clusters;
   int host_raw_bank_version;
@@ -213,6 +258,7 @@ typedef struct {
   const velo_constants* constants;
   unsigned event_start;
   int mep_layout;
+  velo_clusters_view* dev_velo_clusters;   /* optional, events: zeroed, then written for the listed events */
 } velo_decode_retinaclusters_params;
 int velo_decode_retinaclusters(const velo_decode_retinaclusters_params* p, hipStream_t stream);
 
@@ -307,6 +353,13 @@ typedef struct {
   const unsigned* dev_number_of_events;
   char* dev_velo_track_hits;                 /* 18 x accumulated hits bytes */
   uint8_t* dev_accepted_velo_tracks;         /* tracks; set to 1 */
+  /* create_velo_views (VeloConsolidateTracks.cu:11-46), for every one of host_number_of_events
+   * events: all five NULL (no views) or all five set. */
+  velo_hits_view* dev_velo_hits_view;                          /* events */
+  velo_track_view* dev_velo_track_view;                        /* tracks */
+  velo_tracks_view* dev_velo_tracks_view;                      /* events */
+  velo_multi_event_tracks_view* dev_velo_multi_event_tracks_view; /* 1 */
+  const velo_multi_event_tracks_view** dev_imec_velo_tracks;   /* 1: the multi-event view's address */
 } velo_consolidate_tracks_params;
 int velo_consolidate_tracks(const velo_consolidate_tracks_params* p, hipStream_t stream);
 
@@ -317,8 +370,8 @@ typedef struct {
   const unsigned* dev_event_list; unsigned event_list_size;
   const unsigned* dev_number_of_events;
   const unsigned* dev_offsets_all_velo_tracks;
-  const unsigned* dev_offsets_velo_track_hit_number; /* replaces dev_velo_tracks_view */
-  const char* dev_velo_track_hits;
+  const unsigned* dev_offsets_velo_track_hit_number; /* with dev_velo_track_hits: the tracks, when */
+  const char* dev_velo_track_hits;                   /* dev_velo_tracks_view is NULL */
   uint8_t* dev_is_backward;
   char* dev_velo_kalman_beamline_states;     /* 56 x tracks bytes */
   char* dev_velo_kalman_endvelo_states;      /* 56 x tracks bytes */
@@ -328,6 +381,11 @@ typedef struct {
    * covariance prediction and update computed in FP64 and rounded to FP32 (no reference
    * counterpart; covariance within ~1e-6 of an FP64 fit instead of up to ~50 %). */
   unsigned covariance_fp64_fallback;
+  /* The reference's view input: when set, tracks and their hits are read through these views
+   * (velo_consolidate_tracks writes them) and the two fields above may be NULL. */
+  const velo_tracks_view* dev_velo_tracks_view;             /* events */
+  velo_kalman_states_view* dev_velo_kalman_beamline_states_view; /* optional, events: listed ones written */
+  velo_kalman_states_view* dev_velo_kalman_endvelo_states_view;  /* optional, events: listed ones written */
 } velo_kalman_filter_params;
 int velo_kalman_filter(const velo_kalman_filter_params* p, hipStream_t stream);
 

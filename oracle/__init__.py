@@ -47,6 +47,9 @@ class VoInputs(ctypes.Structure):
         ("sorted_clusters", ctypes.c_void_p),
         ("clusters_offsets", ctypes.c_void_p),
         ("clusters_counts", ctypes.c_void_p),
+        ("seed_logits", ctypes.c_void_p),
+        ("seed_threshold", ctypes.c_float),
+        ("seed_block", ctypes.c_uint),
     ]
 
 
@@ -158,6 +161,11 @@ def make_inputs(events, geometry: bytes, beamline: np.ndarray, **props):
         keep.append(event_list)
         a.event_list = _ptr(event_list)
         a.event_list_size = event_list.size
+    seed_logits = props.pop("seed_logits", None)
+    if seed_logits is not None:
+        seed_logits = np.ascontiguousarray(seed_logits, dtype=np.float32)
+        keep.append(seed_logits)
+        a.seed_logits = _ptr(seed_logits)
     for k, v in props.items():
         setattr(a, k, v)
     return a, keep

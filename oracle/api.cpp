@@ -264,6 +264,10 @@ bool make_inputs(const vo_inputs* a, vo::RawInput& in, vo::Geometry& g, vo::SpTa
   p.retina = a->retina != 0;
   p.pv_finder = a->pv_finder != 0;
   p.search.vectorized_seeding = a->vectorized_seeding != 0;
+  p.search.seed_logits = a->seed_logits;
+  p.search.seed_threshold = a->seed_threshold;
+  p.search.seed_block = a->seed_block ? a->seed_block : 256;
+  if (a->seed_logits && (a->vectorized_seeding || p.search.seed_block % 8 != 0)) return false;
   p.clusters.sorted = a->sorted_clusters;
   p.clusters.offsets = a->clusters_offsets;
   p.clusters.counts = a->clusters_counts;

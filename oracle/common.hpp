@@ -256,6 +256,7 @@ enum StatusBits : uint32_t {
   status_track_slots_overflow = 1u << 1,
   status_hit_index_overflow = 1u << 2,
   status_three_hit_slots_overflow = 1u << 3,
+  status_missing_seed_logit = 1u << 4, // seed pruning: a seed had no logit (it was dropped)
 };
 
 } // namespace vo

@@ -12,6 +12,15 @@ struct SearchParams {
   // Seeding as the CPU target dispatches it (track_seeding_vectorized, SearchByTriplet.cu:616-818)
   // instead of the default track_seeding (:298-446).
   bool vectorized_seeding = false;
+  // Seed pruning by a learned scorer (no reference counterpart). seed_logits holds one logit per
+  // cluster slot of the event (the logit of the seed whose h1 is that hit; NaN where none was
+  // scored). A seed is kept when its logit >= seed_threshold. Seeds are decided in groups: the
+  // unused h1 hits of each run of seed_block positions of the module pair, compacted in order and
+  // cut into groups of seed_block / K (K = the h0 candidates considered); a group none of whose
+  // seeds reaches the threshold keeps them all (the plain search as fallback).
+  const float* seed_logits = nullptr;
+  float seed_threshold = 0.0f;
+  unsigned seed_block = 256;
 };
 
 // Exact work counts of search_by_triplet, used as the algorithmic op count of the kernel.

@@ -6,6 +6,8 @@
 //   K6 velo_three_hit_tracks_filter  device/velo/search_by_triplet/src/ThreeHitTracksFilter.cu:48-169
 #include "oracle.hpp"
 
+#include <vector>
+
 namespace vo {
 
 // SortByPhi.cu:49-153. Gaps of the container are left untouched (their content is unspecified).
@@ -127,17 +129,63 @@ void track_seeding(
   unsigned h0_candidates_to_consider,
   unsigned module_pair,
   SearchStats& st,
-  SeedSink* sink)
+  SeedSink* sink,
+  const SearchParams& sp,
+  const float* seed_logits,
+  uint32_t& status)
 {
   const ModulePair& prev = mpd[0];
   const ModulePair& cur = mpd[1];
   const ModulePair& next = mpd[2];
+  // With a scorer, the seeds of a group are decided together (see SearchParams::seed_logits), so
+  // they are held here until their group is complete: (rel, h0, h2) of each found seed.
+  struct Held {
+    unsigned rel;
+    uint16_t h0, h2;
+  };
+  std::vector<Held> held;
+  const unsigned group = seed_logits ? sp.seed_block / h0_candidates_to_consider : 0;
+  auto append = [&](unsigned h0, unsigned h1, unsigned h2) {
+    const unsigned tn = atomics[atomic_ttf]++ % max_tracks_to_follow;
+    tracklets[tn] = TrackletHits {{static_cast<uint16_t>(h0), static_cast<uint16_t>(h1), static_cast<uint16_t>(h2)}};
+    tracks_to_follow[tn] = bit_seed | tn;
+  };
+  auto decide = [&]() {
+    bool any = false;
+    for (const Held& s : held) {
+      const float logit = seed_logits[h1_indices[s.rel]];
+      if (logit != logit) status |= status_missing_seed_logit;
+      any = any || logit >= sp.seed_threshold;
+    }
+#if VO_MUTANT == 20
+    any = true;
+#endif
+    for (const Held& s : held)
+      if (!any || seed_logits[h1_indices[s.rel]] >= sp.seed_threshold) append(s.h0, h1_indices[s.rel], s.h2);
+    held.clear();
+  };
+  // Group number of each compacted h1: its run of seed_block positions and, within the run, its
+  // rank among the run's unused hits divided by the group size.
+  std::vector<unsigned> group_of;
   for (unsigned r = 0; r < cur.hit_num; ++r) {
     const unsigned h1_index = cur.hit_start + r;
-    if (!hit_used[h1_index]) h1_indices[atomics[atomic_local_hits]++] = h1_index;
+    if (!hit_used[h1_index]) {
+      if (seed_logits) {
+        const unsigned run = r / sp.seed_block;
+        unsigned in_run = 0;
+#if VO_MUTANT == 21
+        for (unsigned q = 0; q < r; ++q) in_run += !hit_used[cur.hit_start + q];
+#else
+        for (unsigned q = run * sp.seed_block; q < r; ++q) in_run += !hit_used[cur.hit_start + q];
+#endif
+        group_of.push_back(run << 16 | in_run / group);
+      }
+      h1_indices[atomics[atomic_local_hits]++] = h1_index;
+    }
   }
   const unsigned n_h1 = atomics[atomic_local_hits];
   for (unsigned rel = 0; rel < n_h1; ++rel) {
+    if (seed_logits && rel > 0 && group_of[rel] != group_of[rel - 1]) decide();
     const unsigned h1_index = h1_indices[rel];
     uint16_t best_h0 = 0, best_h2 = 0;
     float best_fit = max_scatter;
@@ -212,11 +260,13 @@ void track_seeding(
 #if VO_MUTANT == 18
       if (atomics[atomic_ttf] >= max_tracks_to_follow) continue;
 #endif
-      const unsigned tn = atomics[atomic_ttf]++ % max_tracks_to_follow;
-      tracklets[tn] = TrackletHits {{best_h0, static_cast<uint16_t>(h1_index), best_h2}};
-      tracks_to_follow[tn] = bit_seed | tn;
+      if (seed_logits)
+        held.push_back(Held {rel, best_h0, best_h2});
+      else
+        append(best_h0, h1_index, best_h2);
     }
   }
+  if (seed_logits) decide();
 }
 
 // SearchByTriplet.cu:616-818: the CPU target's seeding. The h1 hits are visited in order without
@@ -499,7 +549,7 @@ void search_by_triplet_event(
     else
       track_seeding(
         cc, mpd, hit_used, tracklets, tracks_to_follow, h1_rel_indices, atomics, p.max_scatter, phi_tolerance_i16,
-        h0_candidates, module_pair, st, sink);
+        h0_candidates, module_pair, st, sink, p, p.seed_logits, status);
   };
 #if VO_MUTANT == 10
   seeding(seeding_h0_candidates, first_module_pair - 1);
@@ -570,10 +620,12 @@ void search_by_triplet(
     const Clusters cc {const_cast<char*>(sorted_container), total, hit_offset};
     const unsigned toff = track_offset(offsets_estimated_input_size, event_number);
     if (sink) sink->begin_event(event_number);
+    SearchParams pe = p;
+    if (pe.seed_logits) pe.seed_logits += hit_offset;
     search_by_triplet_event(
       cc, mhs, mhn, g, tracks + toff, three_hit_tracks + toff, tracklets + event_number * max_tracks_to_follow,
       tracks_to_follow + event_number * max_tracks_to_follow, hit_used + hit_offset, rel_indices + hit_offset,
-      atomics_velo + event_number * num_atomics, number_of_velo_tracks + event_number, p, st, status, sink);
+      atomics_velo + event_number * num_atomics, number_of_velo_tracks + event_number, pe, st, status, sink);
     const unsigned cap = (event_number + 1 < n_events ? track_offset(offsets_estimated_input_size, event_number + 1) :
                                                         container) -
                          toff;

@@ -41,6 +41,14 @@ typedef struct {
   const char* sorted_clusters;
   const uint32_t* clusters_offsets; /* offsets_estimated_input_size, 26 n_events + 1 */
   const uint32_t* clusters_counts;  /* module_cluster_num, 26 n_events */
+  /* Optional seed pruning by a learned scorer (no reference counterpart): one logit per cluster
+   * slot (offsets_estimated_input_size indexing: the logit of the seed whose h1 is that hit, NaN
+   * where none), kept when >= seed_threshold, decided in groups of seed_block / K consecutive
+   * unused h1 hits of each seed_block positions of a module pair (K = h0 candidates considered);
+   * a group with no kept seed keeps all of its seeds. Scalar seeding only. */
+  const float* seed_logits;
+  float seed_threshold;
+  unsigned seed_block;      /* 0: 256 */
 } vo_inputs;
 
 void vo_default_inputs(vo_inputs* a);

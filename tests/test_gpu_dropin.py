@@ -102,6 +102,70 @@ def test_sp_drop_ins_with_event_list(case, geometry, beamline, constants):
     # The host totals the sequence read back equal the oracle's.
     assert args.host["number_of_reconstructed_velo_tracks"] == int(ref["offsets_all_velo_tracks"][-1])
     assert args.host["total_number_of_velo_clusters"] == int(ref["offsets_estimated_input_size"][-1])
+    check_views(args, ref, event_list)
+
+
+def _view(args, name, dtype, count):
+    return args[name].cpu().numpy()[:dtype.itemsize * count].view(dtype)
+
+
+def check_views(args, ref, event_list):
+    """The view objects against the layouts Allen's constructors give them, built here from the
+    oracle's offsets and the buffers' device addresses (VeloEventModel.cuh:91-110,
+    VeloConsolidateTracks.cu:11-46, VeloConsolidated.cuh:60-294, States.cuh:191-220)."""
+    n = ref["number_of_velo_tracks"].size
+    oat = ref["offsets_all_velo_tracks"].astype(np.int64)
+    ohn = ref["offsets_velo_track_hit_number"].astype(np.int64)
+    nt = int(oat[-1])
+    listed = np.asarray(event_list, dtype=np.int64)
+    cl = _view(args, "velo_clusters", velo.CLUSTERS_VIEW_DTYPE, n)[listed]
+    assert np.all(cl["base_pointer"].astype(np.int64) == args.ptr("velo_cluster_container"))
+    assert np.all(cl["total_number_of_hits"] == int(ref["offsets_estimated_input_size"][-1]))
+    assert np.all(cl["offset"] == 0)
+    hits = _view(args, "velo_hits_view", velo.HITS_VIEW_DTYPE, n)
+    assert np.all(hits["base_pointer"].astype(np.int64) == args.ptr("velo_track_hits"))
+    np.testing.assert_array_equal(hits["offset"], ohn[oat[:-1]])
+    np.testing.assert_array_equal(hits["size"], ohn[oat[1:]] - ohn[oat[:-1]])
+    assert np.all(hits["total_number_of_hits"] == ohn[nt]) and np.all(hits["pad_"] == 0)
+    tr = _view(args, "velo_track_view", velo.TRACK_VIEW_DTYPE, nt)
+    ev = np.repeat(np.arange(n), np.diff(oat))
+    t = np.arange(nt)
+    np.testing.assert_array_equal(tr["hits"].astype(np.int64), args.ptr("velo_hits_view") + velo.HITS_VIEW_DTYPE.itemsize * ev)
+    np.testing.assert_array_equal(tr["track_index"], t - oat[ev])
+    np.testing.assert_array_equal(tr["track_container_offset"], oat[ev])
+    np.testing.assert_array_equal(tr["offset"], ohn[t] - ohn[oat[ev]])
+    np.testing.assert_array_equal(tr["number_of_hits"], np.diff(ohn[:nt + 1]))
+    ts = _view(args, "velo_tracks_view", velo.TRACKS_VIEW_DTYPE, n)
+    np.testing.assert_array_equal(ts["track"].astype(np.int64), args.ptr("velo_track_view") + velo.TRACK_VIEW_DTYPE.itemsize * oat[:-1])
+    np.testing.assert_array_equal(ts["size"], np.diff(oat))
+    np.testing.assert_array_equal(ts["offset"], oat[:-1])
+    m = _view(args, "velo_multi_event_tracks_view", velo.MULTI_EVENT_TRACKS_VIEW_DTYPE, 1)[0]
+    assert m["type_id"] == velo.TYPE_ID_VELO_TRACKS and m["container"].astype(np.int64) == args.ptr("velo_tracks_view")
+    assert m["number_of_events"] == n and m["pad_"] == 0 and m["pad2_"] == 0
+    assert args["imec_velo_tracks"].cpu().numpy()[:8].view("<u8")[0] == args.ptr("velo_multi_event_tracks_view")
+    for name in ["velo_kalman_beamline_states", "velo_kalman_endvelo_states"]:
+        sv = _view(args, name + "_view", velo.KALMAN_STATES_VIEW_DTYPE, n)[listed]
+        assert np.all(sv["base_pointer"].astype(np.int64) == args.ptr(name))
+        np.testing.assert_array_equal(sv["offset"], oat[listed])
+        np.testing.assert_array_equal(sv["size"], oat[listed + 1] - oat[listed])
+        assert np.all(sv["total_number_of_tracks"] == nt) and np.all(sv["pad_"] == 0)
+
+
+def test_kalman_through_offsets_equals_views(geometry, beamline, constants):
+    """The Kalman drop-in reading tracks through dev_velo_tracks_view (the reference's input) and
+    through the hit-number offsets gives the same bytes."""
+    ev = synth.generate(n_events=20, pileup=1, nu=7.6, seed=66)
+    dev = velo.DeviceEvents(ev)
+    out = {}
+    for views in (True, False):
+        args = algorithms.velo_sp_sequence(constants, dev, ev.n_events, event_list=list(range(19, -1, -3)),
+                                           poison=True, kalman_from_views=views)
+        nt = args.host["number_of_reconstructed_velo_tracks"]
+        out[views] = [_host(args, k, np.uint8, n) for k, n in
+                      [("velo_kalman_beamline_states", 56 * nt), ("velo_kalman_endvelo_states", 56 * nt),
+                       ("is_backward", nt)]]
+    for a, b in zip(out[True], out[False]):
+        np.testing.assert_array_equal(a, b)
 
 
 def test_retina_drop_ins_with_event_list(geometry, beamline, constants):
@@ -141,6 +205,8 @@ def test_retina_drop_ins_with_event_list(geometry, beamline, constants):
     d.dev_module_cluster_num, d.dev_offsets_module_pair_cluster = mcn.data_ptr(), mp_off.data_ptr()
     d.dev_velo_cluster_container, d.dev_hit_permutations = container.data_ptr(), perm.data_ptr()
     d.dev_hit_sorting_key, d.constants, d.event_start = keys.data_ptr(), constants.handle, start
+    views = ones(n_run * velo.CLUSTERS_VIEW_DTYPE.itemsize, torch.uint8)
+    d.dev_velo_clusters = views.data_ptr()
     velo.check(velo.lib().velo_decode_retinaclusters(ctypes.byref(d), None), "decode")
     torch.cuda.synchronize()
     host = lambda t, dt, k: t.cpu().numpy().view(dt)[:k]
@@ -155,3 +221,10 @@ def test_retina_drop_ins_with_event_list(geometry, beamline, constants):
     r = parity.cluster_views(ref["sorted_velo_cluster_container"], total)
     for a, b in zip(g, r):
         np.testing.assert_array_equal(a[slots_f].view(np.uint8), b[slots_f].view(np.uint8))
+    # Velo::Clusters views (DecodeRetinaClusters.cu:331-346,442): zeroed, then the listed events'.
+    v = views.cpu().numpy().view(velo.CLUSTERS_VIEW_DTYPE)
+    listed = np.zeros(n_run, bool)
+    listed[event_list] = True
+    assert np.all(v["base_pointer"][listed].astype(np.int64) == container.data_ptr())
+    assert np.all(v["total_number_of_hits"][listed] == total) and np.all(v["offset"] == 0)
+    assert np.all(v["base_pointer"][~listed] == 0) and np.all(v["total_number_of_hits"][~listed] == 0)

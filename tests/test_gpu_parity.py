@@ -27,8 +27,8 @@ def constants(geometry, beamline):
     c.close()
 
 
-def run_both(name, geometry, beamline, constants, fused, tuning=None, oracle_props=None):
-    ev = synth.generate(**DATASETS[name])
+def run_both(name, geometry, beamline, constants, fused, tuning=None, oracle_props=None, events=None):
+    ev = events if events is not None else synth.generate(**DATASETS[name])
     print(f"[parity] dataset={name} events={ev.n_events} raw_bytes={ev.raw.size} fused={fused} tuning={tuning}",
           flush=True)
     ref = oracle.run(ev, geometry, beamline, **(oracle_props or {}))
@@ -54,6 +54,52 @@ def test_sequence_matches_oracle(name, fused, geometry, beamline, constants):
     assert not errs, "\n".join(errs[:20])
     if name != "empty_events":
         assert ref["offsets_all_velo_tracks"][-1] > 0
+
+
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "per_algorithm"])
+def test_real_sp_banks_match_oracle(fused, geometry, beamline, constants):
+    """The reference's bundled MDF file's real super-pixel banks (10 minimum-bias events, bank
+    version 2, 208 banks per event): pixel maps of real data, with clusters at the edges of the
+    masked window and larger than it, through the whole HIP path against the oracle."""
+    import os
+    from tests import textbook
+    ev = textbook.load_mdf_fixture(os.path.join(os.path.dirname(__file__), "golden", "mdf_minbias_vp.npz"))
+    assert ev.bank_version == 2 and ev.n_events == 10
+    ref, gpu, status = run_both("mdf_minbias_vp", geometry, beamline, constants, fused, events=ev)
+    assert int(ref["status"][0]) == 0
+    assert status & velo.STATUS_OVERFLOW_MASK == 0
+    errs = parity.compare(gpu, ref, fused)
+    assert not errs, "\n".join(errs[:20])
+    assert ref["module_cluster_num"].sum() > 15000 and ref["offsets_all_velo_tracks"][-1] > 1000
+
+
+def test_batched_persistent_graph_replays(geometry, beamline, constants):
+    """The batched workload's event mix (one vertex with Poisson(120) tracks, 5 % of the events at
+    nu = 20 pile-up) through the persistent work-stealing search, the whole sequence captured as
+    one HIP graph and replayed three times with eager runs of other events in between: every
+    replay equals the oracle bit for bit."""
+    import torch
+    cfg = dict(n_events=400, pileup=0, mean_tracks=120.0, high_occupancy_fraction=0.05, high_nu=20.0, seed=23)
+    ev = synth.generate(**cfg)
+    other = synth.generate(**dict(cfg, n_events=100, seed=24))
+    print(f"[batched-graph] events={ev.n_events} raw_bytes={ev.raw.size}", flush=True)
+    ref = oracle.run(ev, geometry, beamline)
+    assert int(ref["status"][0]) == 0
+    dev, dev_other = velo.DeviceEvents(ev), velo.DeviceEvents(other)
+    seq = velo.VeloSequence(constants, ev.n_events, max(ev.raw.size, other.raw.size))
+    seq.set_tuning("search_persistent", 1)
+    stream = torch.cuda.Stream()
+    seq.capture(dev, stream)
+    stream.synchronize()
+    for replay in range(3):
+        seq.run(dev_other)
+        torch.cuda.synchronize()
+        seq.replay(stream)
+        stream.synchronize()
+        assert seq.status() & velo.STATUS_OVERFLOW_MASK == 0
+        errs = parity.compare(parity.gpu_buffers(seq, ref), ref, True)
+        assert not errs, f"replay {replay}:\n" + "\n".join(errs[:20])
+    seq.close()
 
 
 TUNINGS = {

@@ -193,3 +193,74 @@ def test_trained_scorer_gain(geometry, beamline):
     print(f"plain {plain.summary()}\ntrained {trained.summary()}\nuntrained {untrained.summary()}")
     seq.close()
     constants.close()
+
+
+def _logits_by_slot(seq, ref, n_events):
+    """The scored search's logged logit of every seed, placed at its h1's cluster slot (NaN
+    elsewhere), the oracle's input for the pruning rule."""
+    rec, _, seen = seq.seed_log()
+    assert rec.shape[0] == seen, "the seed log overflowed"
+    logits = seq.seed_logits(seen)
+    off = ref["offsets_estimated_input_size"]
+    out = np.full(int(off[-1]), np.nan, np.float32)
+    slot = off[26 * rec[:, 0].astype(np.int64)].astype(np.int64) + rec[:, 3]
+    assert np.unique(slot).size == slot.size, "two seeds share an h1"
+    out[slot] = logits
+    return out
+
+
+@pytest.mark.parametrize("n_events,seed", [(300, 81), (120, 82)])
+def test_pruned_search_matches_oracle_at_shipped_threshold(n_events, seed, geometry, beamline):
+    """Config-4 events (one vertex, Poisson(120) tracks) through the NN-pruned search at the shipped
+    scorer's threshold: with the logits the search computed (logged beside it), the oracle's pruning
+    rule (groups of 256 / K unused h1 hits per 256 positions, keep-all fallback) gives every
+    buffer of the sequence bit for bit."""
+    import oracle
+    from allen_amd import scorer, synth, velo
+    from tests import parity
+    params, thr = scorer.trained()
+    ev = synth.generate(n_events=n_events, pileup=0, mean_tracks=120.0, seed=seed)
+    plain = oracle.run(ev, geometry, beamline, compute_f64=0)
+    n_seeds = int(plain["atomics_velo"][1::3].sum())
+    seq, constants = _scored_run(geometry, beamline, ev, params, thr, log_cap=2 * n_seeds + 1000)
+    assert seq.status() & velo.STATUS_OVERFLOW_MASK == 0
+    logits = _logits_by_slot(seq, plain, ev.n_events)
+    ref = oracle.run(ev, geometry, beamline, seed_logits=logits, seed_threshold=thr, seed_block=256)
+    assert int(ref["status"][0]) == 0, f"oracle status {int(ref['status'][0]):#x}"
+    pruned = int(ref["atomics_velo"][1::3].sum())
+    print(f"[pruned] {n_events} events: {n_seeds} plain seeds, {pruned} kept at threshold {thr}")
+    assert pruned < n_seeds
+    errs = parity.compare(parity.gpu_buffers(seq, ref), ref, True)
+    assert not errs, "\n".join(errs[:20])
+    seq.close()
+    constants.close()
+
+
+def test_pruned_search_physics_on_20k_events(geometry, beamline):
+    """Config 4's evaluation size: 20 000 one-vertex events, plain and NN-pruned at the shipped
+    threshold. The long-track efficiency loss stays within 0.2 % absolute and the ghost rate does
+    not rise."""
+    from allen_amd import checker, scorer, synth, velo
+    params, thr = scorer.trained()
+    ev = synth.generate(n_events=20000, pileup=0, mean_tracks=120.0, seed=2020, keep_truth=True, n_threads=16)
+    constants = velo.VeloConstants(geometry, beamline)
+    dev = velo.DeviceEvents(ev)
+    seq = velo.VeloSequence(constants, ev.n_events, ev.raw.size)
+
+    def physics():
+        off = seq.fetch("offsets_all_velo_tracks", np.uint32, ev.n_events + 1)
+        hn = seq.fetch("offsets_velo_track_hit_number", np.uint32, int(off[-1]) + 1)
+        return checker.check_buffers(ev, off, hn, seq.fetch("velo_track_hits", np.uint8, 4 * int(hn[-1]))).as_dict()
+
+    seq.run(dev)
+    assert seq.status() & velo.STATUS_OVERFLOW_MASK == 0
+    plain = physics()
+    seq.set_scorer(params, thr)
+    seq.run(dev)
+    assert seq.status() & velo.STATUS_OVERFLOW_MASK == 0
+    pruned = physics()
+    print(f"[pruned 20k] plain {plain}\n[pruned 20k] pruned {pruned}")
+    assert plain["02_long_efficiency"] - pruned["02_long_efficiency"] <= 0.002
+    assert pruned["ghost_rate"] <= plain["ghost_rate"]
+    seq.close()
+    constants.close()

@@ -82,3 +82,20 @@ def test_cache_sp_patterns_host_only():
     pat, sizes, fx, fy = velo.cache_sp_patterns()
     assert pat.shape == (256,) and fx.shape == (512,)
     assert pat[0x01] == 0 and pat[0x81] & 8
+
+
+def test_view_layouts_match_header():
+    """The view structs of the header have the reference classes' sizes (Velo::Clusters 16 B, Hits
+    24 B, Track 24 B, Tracks 16 B, MultiEventContainer<Tracks> 24 B, KalmanStates 24 B) and the
+    Python dtypes name the header's fields in order."""
+    text = open(os.path.join(ROOT, "include", "velo_amd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    structs = dict((m.group(2), m.group(1)) for m in re.finditer(r"typedef struct \{(.*?)\}\s*(\w+);", text, re.S))
+    views = {"velo_clusters_view": (velo.CLUSTERS_VIEW_DTYPE, 16), "velo_hits_view": (velo.HITS_VIEW_DTYPE, 24),
+             "velo_track_view": (velo.TRACK_VIEW_DTYPE, 24), "velo_tracks_view": (velo.TRACKS_VIEW_DTYPE, 16),
+             "velo_multi_event_tracks_view": (velo.MULTI_EVENT_TRACKS_VIEW_DTYPE, 24),
+             "velo_kalman_states_view": (velo.KALMAN_STATES_VIEW_DTYPE, 24)}
+    for cname, (dt, size) in views.items():
+        fields = [re.findall(r"(\w+)\s*$", d.strip())[0] for d in structs[cname].split(";") if d.strip()]
+        assert list(dt.names) == fields, cname
+        assert dt.itemsize == size, cname

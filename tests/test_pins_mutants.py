@@ -35,6 +35,8 @@ MUTANTS = {
     17: ("forwarding: module parity from the last hit", SEARCH),
     18: ("seeding: seeds past 2048 dropped instead of wrapping the ring", SEARCH),
     19: ("lock-step seeding: equal scatters of one round go to the higher lane", SEARCH),
+    20: ("pruned seeding: a group with no seed above the threshold keeps none", SEARCH),
+    21: ("pruned seeding: groups cut from the module pair's whole list, not per run of positions", SEARCH),
 }
 
 

@@ -8,6 +8,8 @@ with one and two missing module pairs, exact scatter ties (including one where t
 lock-step seeding picks another seed than the scalar seeding) and a ring of tracks to follow
 that overflows its 2048 slots. The same comparison on generated pile-up events, and the count of
 tracks that differ between the two seeding orders on the 200- and 1000-event workloads, close it.
+The learned scorer's pruning rule (seeds decided in groups, keep-all fallback) is compared the
+same way on generated events with random logits.
 """
 import numpy as np
 import pytest
@@ -284,3 +286,42 @@ def test_seeding_orders_agree_on_the_workloads(preset, geometry, beamline):
     for k in ["tracks_to_follow", "tracklets", "hit_used", "three_hit_tracks_output", "velo_kalman_beamline_states"]:
         assert np.array_equal(a[k], b[k]), k
     assert np.all(b["atomics_velo"].reshape(-1, 3)[:, 2] == 0)
+
+
+# ---------------------------------------------------------------------------- pruned seeding
+@pytest.mark.parametrize("block,threshold", [(256, 0.0), (16, 0.0), (16, 0.9), (64, -0.5)])
+def test_pruned_seeding_against_statement(block, threshold, geometry, beamline, zs):
+    """Random logits (one per hit, standard normal); runs of `block` positions, groups of block / K.
+    Small blocks put many group and run boundaries inside each module pair of these events."""
+    ev = synth.generate(n_events=6, pileup=1, nu=7.6, seed=21)
+    plain = oracle.run(ev, geometry, beamline, compute_f64=0)
+    off, cnt = plain["offsets_estimated_input_size"], plain["module_cluster_num"]
+    logits = np.random.default_rng(block).standard_normal(int(off[-1])).astype(np.float32)
+    r = oracle.run(ev, geometry, beamline, compute_f64=0, seed_logits=logits, seed_threshold=threshold,
+                   seed_block=block)
+    assert int(r["status"][0]) == 0
+    pruned_total = 0
+    for e in range(ev.n_events):
+        E = ts.event_from_container(plain["sorted_velo_cluster_container"], off, cnt, e, zs)
+        base = int(off[26 * e])
+        got = ts.search(E, logits=logits[base:], threshold=threshold, block=block)
+        toff = int(np.float32(off[26 * e]) * np.float32(0.34))
+        nt = int(r["number_of_velo_tracks"][e])
+        assert [t["hits"][:t["hitsNum"]].tolist() for t in r["tracks"][toff:toff + nt]] == got["tracks"]
+        n3 = int(r["atomics_velo"][3 * e])
+        assert [t["hits"].tolist() for t in r["three_hit_tracks"][toff:toff + n3]] == got["three_hit"]
+        assert int(r["atomics_velo"][3 * e + 1]) == got["ttf"]
+        pruned_total += int(plain["atomics_velo"][3 * e + 1]) - got["ttf"]
+    assert pruned_total > 0, "the logits pruned nothing"
+
+
+def test_pruned_seeding_extremes_equal_plain(geometry, beamline):
+    """Every logit above the threshold, or every one below (each group falls back to all of its
+    seeds): both are the plain search."""
+    ev = synth.generate(n_events=6, pileup=1, nu=7.6, seed=22)
+    plain = oracle.run(ev, geometry, beamline, compute_f64=0)
+    n = int(plain["offsets_estimated_input_size"][-1])
+    for value in (1.0, -1.0):
+        r = oracle.run(ev, geometry, beamline, compute_f64=0, seed_logits=np.full(n, value, np.float32))
+        for k in ["tracks", "tracklets", "tracks_to_follow", "three_hit_tracks", "hit_used", "atomics_velo"]:
+            assert np.array_equal(r[k], plain[k]), k

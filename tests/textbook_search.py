@@ -24,6 +24,11 @@ max_skipped_modules; otherwise a three-hit tracklet becomes a three-hit track. A
 step the tracklets left in the ring become three-hit tracks. The ring has 2048 slots written
 modulo 2048, and a tracklet lives in the slot of its seed.
 
+Pruned seeding (a learned scorer's rule, no reference counterpart): each seed carries a logit
+(looked up by its h1); the unused h1 hits of every run of `block` positions of a module pair, in
+hit order, are cut into groups of block / K, and a group keeps its seeds whose logit reaches the
+threshold, or all of them when none does.
+
 Floating point is numpy float32 (IEEE single, one rounding per operation, as the reference
 without contraction). The 16-bit phi of a point is the oracle's hit_phi_16, pinned on its own.
 """
@@ -87,14 +92,17 @@ class Event:
         return out
 
 
-def search(ev: Event, phi_tolerance=0.045, max_scatter=0.08, max_skipped_modules=1, vectorized=False):
+def search(ev: Event, phi_tolerance=0.045, max_scatter=0.08, max_skipped_modules=1, vectorized=False,
+           logits=None, threshold=0.0, block=256):
+    """logits: None (plain seeding) or a mapping from an event-relative h1 index to its seed's
+    logit (pruned seeding)."""
     # Overwritten ring slots can pair a hit with itself: 1/0 and 0 * inf follow IEEE rules, as in
     # the reference, and compare false.
     with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
-        return _search(ev, phi_tolerance, max_scatter, max_skipped_modules, vectorized)
+        return _search(ev, phi_tolerance, max_scatter, max_skipped_modules, vectorized, logits, threshold, block)
 
 
-def _search(ev, phi_tolerance, max_scatter, max_skipped_modules, vectorized):
+def _search(ev, phi_tolerance, max_scatter, max_skipped_modules, vectorized, logits, threshold, block):
     tol = int(oracle.lib().vo_hit_phi_float_to_16(float(phi_tolerance)))
     max_scatter = F(max_scatter)
     n_hits = ev.x.size
@@ -109,9 +117,15 @@ def _search(ev, phi_tolerance, max_scatter, max_skipped_modules, vectorized):
         ps, pn, pz = ev.pair(prev_m)
         cs, cn, _ = ev.pair(cur_m)
         dz_phi = F(ev.pair(next_m)[2][0] - pz[0])
+        groups = {}  # (run, group) -> seeds (h0, h1, h2) in h1 order; dicts keep insertion order
+        rank_in_run = {}
         for h1 in range(cs, cs + cn):
             if used[h1]:
                 continue
+            run = (h1 - cs) // block
+            rank = rank_in_run.get(run, 0)
+            rank_in_run[run] = rank + 1
+            group = groups.setdefault((run, rank // (block // k_h0)), [])
             h0s = []
             if pn:
                 idx = leftmost(ev.phi[ps:ps + pn], int(ev.phi[h1]))
@@ -142,6 +156,12 @@ def _search(ev, phi_tolerance, max_scatter, max_skipped_modules, vectorized):
                     r2 += 1
             if cands:
                 _, h0, h2 = min(cands, key=lambda c: c[0])
+                group.append((h0, h1, h2))
+        for seeds in groups.values():
+            if logits is not None:
+                kept = [t for t in seeds if logits[t[1]] >= threshold]
+                seeds = kept if kept else seeds
+            for h0, h1, h2 in seeds:
                 slot = counter % RING
                 counter += 1
                 tracklets[slot] = [h0, h1, h2]
