@@ -80,7 +80,7 @@ def velo_calculate_number_of_candidates_t(args, events, block_dim_x=256):
     p.dev_velo_raw_input, p.dev_velo_raw_input_offsets = events.raw.data_ptr(), events.offsets.data_ptr()
     p.dev_velo_raw_input_sizes, p.dev_velo_raw_input_types = events.sizes.data_ptr(), events.types.data_ptr()
     p.dev_number_of_candidates, p.dev_velo_bank_index = args.ptr("number_of_candidates"), args.ptr("velo_bank_index")
-    p.block_dim_x, p.event_start, p.mep_layout = block_dim_x, args.host["event_start"], 0
+    p.block_dim_x, p.event_start, p.mep_layout = block_dim_x, args.host["event_start"], int(events.mep)
     velo.check(velo.lib().velo_calculate_number_of_candidates(ctypes.byref(p), None),
                "velo_calculate_number_of_candidates")
 
@@ -102,7 +102,7 @@ def velo_estimate_input_size_t(args, events):
     p.dev_estimated_input_size = args.ptr("estimated_input_size")
     p.dev_module_candidate_num = args.ptr("module_candidate_num")
     p.dev_cluster_candidates = args.ptr("cluster_candidates")
-    p.event_start, p.mep_layout = args.host["event_start"], 0
+    p.event_start, p.mep_layout = args.host["event_start"], int(events.mep)
     velo.check(velo.lib().velo_estimate_input_size(ctypes.byref(p), None), "velo_estimate_input_size")
 
 
@@ -128,7 +128,7 @@ def velo_masked_clustering_t(args, events, constants):
     p.dev_module_cluster_num = args.ptr("module_cluster_num")
     p.dev_velo_cluster_container = args.ptr("velo_cluster_container")
     p.constants = constants.handle
-    p.event_start, p.mep_layout = args.host["event_start"], 0
+    p.event_start, p.mep_layout = args.host["event_start"], int(events.mep)
     args.make("velo_clusters", n * velo.CLUSTERS_VIEW_DTYPE.itemsize, torch.uint8)
     p.dev_velo_clusters = args.ptr("velo_clusters")
     velo.check(velo.lib().velo_masked_clustering(ctypes.byref(p), None), "velo_masked_clustering")

@@ -221,7 +221,6 @@ int velo_prefix_sum(const velo_prefix_sum_params* p, hipStream_t stream)
 int velo_calculate_number_of_candidates(const velo_calculate_number_of_candidates_params* p, hipStream_t stream)
 {
   if (!p || !p->dev_number_of_candidates || !p->dev_velo_bank_index) return VELO_ERR_INVALID_ARGUMENT;
-  if (p->mep_layout) return VELO_ERR_UNSUPPORTED_LAYOUT;
   if (p->host_raw_bank_version < 0) return VELO_OK; // no VP banks in the data
   if (!version_ok(p->host_raw_bank_version)) return VELO_ERR_BANK_VERSION;
   velo::CandidatesLaunch a;
@@ -230,6 +229,7 @@ int velo_calculate_number_of_candidates(const velo_calculate_number_of_candidate
   a.offsets = p->dev_velo_raw_input_offsets;
   a.sizes = p->dev_velo_raw_input_sizes;
   a.types = p->dev_velo_raw_input_types;
+  a.mep = p->mep_layout != 0;
   a.event_list = p->dev_event_list;
   a.n_list = p->event_list_size;
   a.event_start = p->event_start;
@@ -248,7 +248,6 @@ int velo_calculate_number_of_retinaclusters_each_sensor_pair(
   hipStream_t stream)
 {
   if (!p || !p->dev_each_sensor_pair_size || !p->dev_retina_bank_index) return VELO_ERR_INVALID_ARGUMENT;
-  if (p->mep_layout) return VELO_ERR_UNSUPPORTED_LAYOUT;
   const unsigned slots = velo::retina_sensor_slots(p->host_raw_bank_version);
   int rc = hip_status(
     velo::launch_fill_u32(p->dev_each_sensor_pair_size, (size_t) p->host_number_of_events * slots, 0, stream));
@@ -261,6 +260,7 @@ int velo_calculate_number_of_retinaclusters_each_sensor_pair(
   a.offsets = p->dev_velo_retina_raw_input_offsets;
   a.sizes = p->dev_velo_retina_raw_input_sizes;
   a.types = p->dev_velo_retina_raw_input_types;
+  a.mep = p->mep_layout != 0;
   a.event_list = p->dev_event_list;
   a.n_list = p->event_list_size;
   a.event_start = p->event_start;
@@ -275,7 +275,6 @@ int velo_decode_retinaclusters(const velo_decode_retinaclusters_params* p, hipSt
       !p->dev_velo_cluster_container || !p->dev_hit_permutations || !p->dev_hit_sorting_key ||
       !p->dev_number_of_events)
     return VELO_ERR_INVALID_ARGUMENT;
-  if (p->mep_layout) return VELO_ERR_UNSUPPORTED_LAYOUT;
   const size_t items = (size_t) p->host_number_of_events * velo::n_module_pairs;
   const size_t T = p->host_total_number_of_velo_clusters;
   // The reference's memsets (DecodeRetinaClusters.cu:428-447).
@@ -298,6 +297,7 @@ int velo_decode_retinaclusters(const velo_decode_retinaclusters_params* p, hipSt
   a.offsets = p->dev_velo_retina_raw_input_offsets;
   a.sizes = p->dev_velo_retina_raw_input_sizes;
   a.types = p->dev_velo_retina_raw_input_types;
+  a.mep = p->mep_layout != 0;
   a.event_list = p->dev_event_list;
   a.n_list = p->event_list_size;
   a.event_start = p->event_start;
@@ -360,7 +360,6 @@ int velo_pv_beamline(const velo_pv_beamline_params* p, hipStream_t stream)
 int velo_estimate_input_size(const velo_estimate_input_size_params* p, hipStream_t stream)
 {
   if (!p || !p->dev_estimated_input_size || !p->dev_module_candidate_num) return VELO_ERR_INVALID_ARGUMENT;
-  if (p->mep_layout) return VELO_ERR_UNSUPPORTED_LAYOUT;
   int rc = hip_status(velo::launch_fill_u32(
     p->dev_estimated_input_size, (size_t) p->host_number_of_events * velo::n_module_pairs, 0, stream));
   if (rc == VELO_OK) rc = hip_status(velo::launch_fill_u32(p->dev_module_candidate_num, p->host_number_of_events, 0, stream));
@@ -372,6 +371,7 @@ int velo_estimate_input_size(const velo_estimate_input_size_params* p, hipStream
   a.offsets = p->dev_velo_raw_input_offsets;
   a.sizes = p->dev_velo_raw_input_sizes;
   a.types = p->dev_velo_raw_input_types;
+  a.mep = p->mep_layout != 0;
   a.event_list = p->dev_event_list;
   a.n_list = p->event_list_size;
   a.event_start = p->event_start;
@@ -388,7 +388,6 @@ int velo_masked_clustering(const velo_masked_clustering_params* p, hipStream_t s
     return VELO_ERR_INVALID_ARGUMENT;
   if (p->dev_velo_clusters && (!p->dev_number_of_events || !p->dev_offsets_estimated_input_size))
     return VELO_ERR_INVALID_ARGUMENT;
-  if (p->mep_layout) return VELO_ERR_UNSUPPORTED_LAYOUT;
   int rc = hip_status(velo::launch_fill_u32(
     p->dev_module_cluster_num, (size_t) p->host_number_of_events * velo::n_module_pairs, 0, stream));
   if (rc != VELO_OK || p->host_raw_bank_version < 0) return rc;
@@ -399,6 +398,7 @@ int velo_masked_clustering(const velo_masked_clustering_params* p, hipStream_t s
   a.offsets = p->dev_velo_raw_input_offsets;
   a.sizes = p->dev_velo_raw_input_sizes;
   a.types = p->dev_velo_raw_input_types;
+  a.mep = p->mep_layout != 0;
   a.event_list = p->dev_event_list;
   a.n_list = p->event_list_size;
   a.event_start = p->event_start;
@@ -916,6 +916,7 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream
     r1.offsets = in->dev_velo_raw_input_offsets;
     r1.sizes = in->dev_velo_raw_input_sizes;
     r1.types = in->dev_velo_raw_input_types;
+    r1.mep = in->mep_layout != 0;
     r1.event_list = nullptr;
     r1.n_list = E;
     r1.event_start = 0;
@@ -935,6 +936,7 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream
     r2.offsets = r1.offsets;
     r2.sizes = r1.sizes;
     r2.types = r1.types;
+    r2.mep = r1.mep;
     r2.event_list = nullptr;
     r2.n_list = E;
     r2.event_start = 0;
@@ -959,6 +961,7 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream
     k1.offsets = in->dev_velo_raw_input_offsets;
     k1.sizes = in->dev_velo_raw_input_sizes;
     k1.types = in->dev_velo_raw_input_types;
+    k1.mep = in->mep_layout != 0;
     k1.event_list = nullptr;
     k1.n_list = E;
     k1.event_start = 0;
@@ -980,6 +983,7 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream
     k2.offsets = k1.offsets;
     k2.sizes = k1.sizes;
     k2.types = k1.types;
+    k2.mep = k1.mep;
     k2.event_list = nullptr;
     k2.n_list = E;
     k2.event_start = 0;
@@ -1035,6 +1039,7 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream
       k3.offsets = k1.offsets;
       k3.sizes = k1.sizes;
       k3.types = k1.types;
+      k3.mep = k1.mep;
       k3.event_list = nullptr;
       k3.n_list = E;
       k3.event_start = 0;

@@ -298,6 +298,7 @@ __global__ void k_number_of_candidates(
   const uint32_t* offsets,
   const uint32_t* sizes,
   const uint32_t* types,
+  unsigned mep,
   const uint32_t* event_list,
   unsigned n_list,
   unsigned event_start,
@@ -307,7 +308,7 @@ __global__ void k_number_of_candidates(
   const unsigned ev_idx = blockIdx.x * waves + threadIdx.x / wave_size;
   if (ev_idx >= n_list) return;
   const unsigned event_number = event_list ? event_list[ev_idx] : ev_idx;
-  const RawEvent ev(raw, offsets, sizes, types, event_number + event_start);
+  const RawEvent ev(raw, offsets, sizes, types, mep, event_number + event_start);
   uint32_t n = 0;
   for (unsigned b = lane_id(); b < ev.n_banks; b += wave_size) {
     const RawBank bank = raw_bank<version>(ev, b);
@@ -328,6 +329,7 @@ __global__ void k_bank_directory(
   const uint32_t* offsets,
   const uint32_t* sizes,
   const uint32_t* types,
+  unsigned mep,
   unsigned n_events,
   uint32_t* number_of_candidates,
   BankEntry* directory,
@@ -337,7 +339,7 @@ __global__ void k_bank_directory(
   const unsigned waves = blockDim.x / wave_size;
   const unsigned e = blockIdx.x * waves + threadIdx.x / wave_size;
   if (e >= n_events) return;
-  const RawEvent ev(raw, offsets, sizes, types, e);
+  const RawEvent ev(raw, offsets, sizes, types, mep, e);
   uint32_t total = 0;
   unsigned my_count = 0; // lane m: banks of module pair m so far
   for (unsigned base = 0; base < ev.n_banks; base += wave_size) {
@@ -388,6 +390,7 @@ __global__ void k_bank_index(
   const uint32_t* offsets,
   const uint32_t* sizes,
   const uint32_t* types,
+  unsigned mep,
   const uint32_t* event_list,
   unsigned n_list,
   unsigned event_start,
@@ -406,7 +409,7 @@ __global__ void k_bank_index(
     for (unsigned i = threadIdx.x; i < n_sensors; i += blockDim.x) best[i] = -1;
     __syncthreads();
     const unsigned event_number = event_list ? event_list[ev_idx] : ev_idx;
-    const RawEvent ev(raw, offsets, sizes, types, event_number + event_start);
+    const RawEvent ev(raw, offsets, sizes, types, mep, event_number + event_start);
     for (unsigned b = threadIdx.x; b < ev.n_banks; b += blockDim.x) {
       const RawBank bank = raw_bank<version>(ev, b);
       const uint32_t pair = bank.sourceID & 0x1FFu;
@@ -440,6 +443,7 @@ __global__ void k_estimate_input_size(
   const uint32_t* offsets,
   const uint32_t* sizes,
   const uint32_t* types,
+  unsigned mep,
   const uint32_t* event_list,
   unsigned event_start,
   const uint32_t* candidates_offsets,
@@ -454,7 +458,7 @@ __global__ void k_estimate_input_size(
   if (threadIdx.x == 0) n_cand = 0;
   __syncthreads();
   uint32_t* cands = cluster_candidates + candidates_offsets[event_number];
-  const RawEvent ev(raw, offsets, sizes, types, event_number + event_start);
+  const RawEvent ev(raw, offsets, sizes, types, mep, event_number + event_start);
   const unsigned waves = blockDim.x / wave_size;
   for (unsigned b = threadIdx.x / wave_size; b < ev.n_banks; b += waves) {
     const RawBank bank = raw_bank<version>(ev, b);
@@ -494,6 +498,7 @@ __global__ void k_masked_clustering(
   const uint32_t* offsets,
   const uint32_t* sizes,
   const uint32_t* types,
+  unsigned mep,
   const uint32_t* event_list,
   unsigned event_start,
   const uint32_t* number_of_events,
@@ -517,7 +522,7 @@ __global__ void k_masked_clustering(
   for (unsigned i = threadIdx.x; i < n_module_pairs; i += blockDim.x) count[i] = 0;
   __syncthreads();
   const Geometry& g = *geometry;
-  const RawEvent ev(raw, offsets, sizes, types, event_number + event_start);
+  const RawEvent ev(raw, offsets, sizes, types, mep, event_number + event_start);
   const unsigned waves = blockDim.x / wave_size;
   for (unsigned b = threadIdx.x / wave_size; b < ev.n_banks; b += waves) {
     const RawBank bank = raw_bank<version>(ev, b);
@@ -641,6 +646,7 @@ __global__ void __launch_bounds__(256) k_cluster_sorted(
   const uint32_t* offsets,
   const uint32_t* sizes,
   const uint32_t* types,
+  unsigned mep,
   unsigned n_events,
   const uint32_t* offsets_estimated_input_size,
   const Geometry* geometry,
@@ -668,7 +674,7 @@ __global__ void __launch_bounds__(256) k_cluster_sorted(
   const unsigned start = offsets_estimated_input_size[event_number * n_module_pairs + mp];
   const unsigned estimate = offsets_estimated_input_size[event_number * n_module_pairs + mp + 1] - start;
   const Geometry& g = *geometry;
-  const RawEvent ev(raw, offsets, sizes, types, event_number);
+  const RawEvent ev(raw, offsets, sizes, types, mep, event_number);
 
   // Banks of this module pair, in bank order.
   if (threadIdx.x == 0) s_nbanks = 0;
@@ -806,18 +812,18 @@ static hipError_t launch_candidates_v(const CandidatesLaunch& a, hipStream_t s)
   if (grid && a.directory) {
     if (a.event_list || a.event_start) return hipErrorInvalidValue;
     hipLaunchKernelGGL(
-      k_bank_directory<version>, dim3(grid), dim3(waves * wave_size), 0, s, a.raw, a.offsets, a.sizes, a.types,
+      k_bank_directory<version>, dim3(grid), dim3(waves * wave_size), 0, s, a.raw, a.offsets, a.sizes, a.types, unsigned(a.mep),
       a.n_list, a.number_of_candidates, static_cast<BankEntry*>(a.directory), a.directory_count);
   }
   else if (grid) {
     hipLaunchKernelGGL(
-      k_number_of_candidates<version>, dim3(grid), dim3(waves * wave_size), 0, s, a.raw, a.offsets, a.sizes, a.types,
+      k_number_of_candidates<version>, dim3(grid), dim3(waves * wave_size), 0, s, a.raw, a.offsets, a.sizes, a.types, unsigned(a.mep),
       a.event_list, a.n_list, a.event_start, a.number_of_candidates);
   }
   const unsigned cpu_grid = (a.n_list + a.block_dim_x - 1) / a.block_dim_x;
   if (a.n_list) {
     hipLaunchKernelGGL(
-      k_bank_index<version>, dim3(1), dim3(256), 0, s, a.raw, a.offsets, a.sizes, a.types, a.event_list, a.n_list,
+      k_bank_index<version>, dim3(1), dim3(256), 0, s, a.raw, a.offsets, a.sizes, a.types, unsigned(a.mep), a.event_list, a.n_list,
       a.event_start, cpu_grid, a.bank_index);
   }
   return hipGetLastError();
@@ -837,7 +843,7 @@ hipError_t launch_estimate_input_size(const EstimateLaunch& a, hipStream_t s)
   if (!a.n_list) return hipSuccess;
 #define VELO_LAUNCH_EST(V)                                                                                          \
   hipLaunchKernelGGL(                                                                                               \
-    k_estimate_input_size<V>, dim3(a.n_list), dim3(256), 0, s, a.raw, a.offsets, a.sizes, a.types, a.event_list,    \
+    k_estimate_input_size<V>, dim3(a.n_list), dim3(256), 0, s, a.raw, a.offsets, a.sizes, a.types, unsigned(a.mep), a.event_list,    \
     a.event_start, a.candidates_offsets, a.estimated_input_size, a.module_candidate_num, a.cluster_candidates)
   switch (a.version) {
   case 2: VELO_LAUNCH_EST(2); break;
@@ -853,7 +859,7 @@ hipError_t launch_masked_clustering(const MaskedClusteringLaunch& a, hipStream_t
   if (!a.n_list) return hipSuccess;
 #define VELO_LAUNCH_MC(V)                                                                                           \
   hipLaunchKernelGGL(                                                                                               \
-    k_masked_clustering<V>, dim3(a.n_list), dim3(256), 0, s, a.raw, a.offsets, a.sizes, a.types, a.event_list,      \
+    k_masked_clustering<V>, dim3(a.n_list), dim3(256), 0, s, a.raw, a.offsets, a.sizes, a.types, unsigned(a.mep), a.event_list,      \
     a.event_start, a.number_of_events, a.offsets_estimated_input_size, a.module_candidate_num, a.cluster_candidates, \
     a.candidates_offsets, a.bank_index, reinterpret_cast<const Geometry*>(a.geometry), a.sp_patterns, a.sp_fx,       \
     a.sp_fy, a.module_cluster_num, a.cluster_container)
@@ -880,7 +886,7 @@ hipError_t launch_cluster_sorted(const FusedClusterLaunch& a, hipStream_t s)
   if (!a.n_events) return hipSuccess;
 #define VELO_LAUNCH_FUSED(V)                                                                                        \
   hipLaunchKernelGGL(                                                                                               \
-    k_cluster_sorted<V>, dim3(n_module_pairs, a.n_events), dim3(256), 0, s, a.raw, a.offsets, a.sizes, a.types,     \
+    k_cluster_sorted<V>, dim3(n_module_pairs, a.n_events), dim3(256), 0, s, a.raw, a.offsets, a.sizes, a.types, unsigned(a.mep),     \
     a.n_events, a.offsets_estimated_input_size, reinterpret_cast<const Geometry*>(a.geometry), a.sp_patterns,        \
     a.sp_fx, a.sp_fy, a.module_cluster_num, a.sorted_container, a.unsorted_scratch, a.status)
   switch (a.version) {

@@ -77,26 +77,42 @@ struct Clusters {
   __device__ __forceinline__ int16_t* phis() const { return reinterpret_cast<int16_t*>(base) + 8 * (size_t) total; }
 };
 
-// Allen event layout accessors (ClusteringDefinitions.cuh:100-146; MEPTools.h:25-55).
+// Raw-event accessors: Allen's layout (ClusteringDefinitions.cuh:100-146; MEPTools.h:25-55), or
+// with mep the MEP layout (MEP::RawEvent, MEPTools.h:59-200): offsets[0] = banks, offsets[2 + b]
+// = source ID of bank b, offsets[2 + banks (1 + event) + b] = byte offset of the event's fragment
+// of bank b (a bank without its source ID word); sizes and types indexed by bank, then event.
 struct RawEvent {
   uint32_t n_banks;
-  const uint32_t* bank_offsets;
-  const char* payload;
-  const uint16_t* bank_sizes;
-  const uint8_t* bank_types;
+  const uint32_t* bank_offsets; // Allen: the event's bank offsets; MEP: the offsets array
+  const char* payload;          // Allen: the event's payload; MEP: the fragments
+  const uint16_t* bank_sizes;   // Allen: the event's sizes; MEP: the sizes array
+  const uint8_t* bank_types;    // Allen: the event's types; MEP: the types array
+  unsigned event;               // MEP: the event's index
+  bool mep;
   __device__ __forceinline__ RawEvent(
     const char* raw,
     const uint32_t* offsets,
     const uint32_t* sizes,
     const uint32_t* types,
-    unsigned event)
+    unsigned mep_layout,
+    unsigned event_number)
   {
-    const char* p = raw + offsets[event];
+    mep = mep_layout != 0;
+    event = event_number;
+    if (mep) {
+      n_banks = offsets[0];
+      bank_offsets = offsets;
+      payload = raw;
+      bank_sizes = reinterpret_cast<const uint16_t*>(sizes);
+      bank_types = reinterpret_cast<const uint8_t*>(types);
+      return;
+    }
+    const char* p = raw + offsets[event_number];
     n_banks = *reinterpret_cast<const uint32_t*>(p);
     bank_offsets = reinterpret_cast<const uint32_t*>(p + 4);
     payload = p + 4 + 4 * (size_t)(n_banks + 1);
-    bank_sizes = reinterpret_cast<const uint16_t*>(sizes) + sizes[event];
-    bank_types = reinterpret_cast<const uint8_t*>(types) + types[event];
+    bank_sizes = reinterpret_cast<const uint16_t*>(sizes) + sizes[event_number];
+    bank_types = reinterpret_cast<const uint8_t*>(types) + types[event_number];
   }
 };
 
@@ -120,18 +136,30 @@ template<int version>
 __device__ __forceinline__ RawBank raw_bank(const RawEvent& ev, unsigned i)
 {
   RawBank b;
-  const uint32_t* p = reinterpret_cast<const uint32_t*>(ev.payload + ev.bank_offsets[i]);
-  b.sourceID = p[0];
+  const uint32_t* p;
+  if (ev.mep) {
+    // MEP::raw_bank (MEPTools.h:112-126): the fragment, its source ID from the offsets.
+    const uint32_t* offs = ev.bank_offsets;
+    b.sourceID = offs[2 + i];
+    p = reinterpret_cast<const uint32_t*>(ev.payload + offs[2 + ev.n_banks * (1 + ev.event) + i]);
+    b.size = (ev.bank_sizes + reinterpret_cast<const uint32_t*>(ev.bank_sizes)[i])[ev.event];
+    b.type = (ev.bank_types + reinterpret_cast<const uint32_t*>(ev.bank_types)[i])[ev.event];
+  }
+  else {
+    p = reinterpret_cast<const uint32_t*>(ev.payload + ev.bank_offsets[i]);
+    b.sourceID = p[0];
+    p += 1;
+    b.size = ev.bank_sizes[i];
+    b.type = ev.bank_types[i];
+  }
   if constexpr (version == 2 || version == 3) {
-    b.count = p[1];
-    b.word = p + 2;
+    b.count = p[0];
+    b.word = p + 1;
   }
   else {
     b.count = 0;
-    b.word = p + 1;
+    b.word = p;
   }
-  b.size = ev.bank_sizes[i];
-  b.type = ev.bank_types[i];
   return b;
 }
 

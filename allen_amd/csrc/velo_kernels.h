@@ -13,6 +13,7 @@ struct CandidatesLaunch {
   const uint32_t* offsets;
   const uint32_t* sizes;
   const uint32_t* types;
+  bool mep = false; // MEP layout (MEPTools.h:59-200)
   const uint32_t* event_list; // NULL = identity
   unsigned n_list;
   unsigned event_start;
@@ -32,6 +33,7 @@ struct RetinaSizesLaunch {
   const uint32_t* offsets;
   const uint32_t* sizes;
   const uint32_t* types;
+  bool mep = false; // MEP layout (MEPTools.h:59-200)
   const uint32_t* event_list; // NULL = identity
   unsigned n_list;
   unsigned event_start;
@@ -44,6 +46,7 @@ struct RetinaDecodeLaunch {
   const uint32_t* offsets;
   const uint32_t* sizes;
   const uint32_t* types;
+  bool mep = false; // MEP layout (MEPTools.h:59-200)
   const uint32_t* event_list; // NULL = identity
   unsigned n_list;
   unsigned event_start;
@@ -101,6 +104,7 @@ struct EstimateLaunch {
   const uint32_t* offsets;
   const uint32_t* sizes;
   const uint32_t* types;
+  bool mep = false; // MEP layout (MEPTools.h:59-200)
   const uint32_t* event_list;
   unsigned n_list;
   unsigned event_start;
@@ -116,6 +120,7 @@ struct MaskedClusteringLaunch {
   const uint32_t* offsets;
   const uint32_t* sizes;
   const uint32_t* types;
+  bool mep = false; // MEP layout (MEPTools.h:59-200)
   const uint32_t* event_list;
   unsigned n_list;
   unsigned event_start;
@@ -150,6 +155,7 @@ struct FusedClusterLaunch {
   const uint32_t* offsets;
   const uint32_t* sizes;
   const uint32_t* types;
+  bool mep = false; // MEP layout (MEPTools.h:59-200)
   unsigned n_events;
   const uint32_t* offsets_estimated_input_size;
   const void* geometry;

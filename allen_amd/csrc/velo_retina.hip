@@ -104,6 +104,7 @@ __global__ void __launch_bounds__(wave_size) k_retina_sensor_sizes(
   const uint32_t* offsets,
   const uint32_t* sizes,
   const uint32_t* types,
+  unsigned mep,
   const uint32_t* event_list,
   unsigned event_start,
   uint32_t* each_sensor_pair_size,
@@ -114,7 +115,7 @@ __global__ void __launch_bounds__(wave_size) k_retina_sensor_sizes(
   __shared__ int s_any[S];
   const unsigned l = lane_id();
   const unsigned event_number = event_list ? event_list[blockIdx.x] : blockIdx.x;
-  const RawEvent ev(raw, offsets, sizes, types, event_number + event_start);
+  const RawEvent ev(raw, offsets, sizes, types, mep, event_number + event_start);
   for (unsigned i = l; i < S; i += wave_size) {
     s_retina[i] = -1;
     s_any[i] = -1;
@@ -143,6 +144,7 @@ __global__ void __launch_bounds__(waves_per_block* wave_size) k_retina_decode(
   const uint32_t* offsets,
   const uint32_t* sizes,
   const uint32_t* types,
+  unsigned mep,
   const uint32_t* event_list,
   unsigned n_list,
   unsigned event_start,
@@ -176,7 +178,7 @@ __global__ void __launch_bounds__(waves_per_block* wave_size) k_retina_decode(
   const unsigned n = hi - lo;
   const unsigned total = offsets_each_sensor_pair_size[static_cast<size_t>(*number_of_events) * S];
   const Geometry& g = *geometry;
-  const RawEvent ev(raw, offsets, sizes, types, event_number + event_start);
+  const RawEvent ev(raw, offsets, sizes, types, mep, event_number + event_start);
   const bool in_lds = n <= key_cap;
   int64_t* keys = s_key[wave];
 
@@ -270,7 +272,7 @@ hipError_t launch_retina_sensor_sizes(const RetinaSizesLaunch& a, hipStream_t s)
   if (!a.n_list) return hipSuccess;
 #define VELO_LAUNCH_RSS(V)                                                                                          \
   hipLaunchKernelGGL(                                                                                               \
-    k_retina_sensor_sizes<V>, dim3(a.n_list), dim3(wave_size), 0, s, a.raw, a.offsets, a.sizes, a.types,            \
+    k_retina_sensor_sizes<V>, dim3(a.n_list), dim3(wave_size), 0, s, a.raw, a.offsets, a.sizes, a.types, unsigned(a.mep),            \
     a.event_list, a.event_start, a.each_sensor_pair_size, a.retina_bank_index)
   switch (a.version) {
   case 2: VELO_LAUNCH_RSS(2); break;
@@ -288,7 +290,7 @@ hipError_t launch_retina_decode(const RetinaDecodeLaunch& a, hipStream_t s)
   const unsigned grid = (items + waves_per_block - 1) / waves_per_block;
 #define VELO_LAUNCH_RD(V)                                                                                           \
   hipLaunchKernelGGL(                                                                                               \
-    k_retina_decode<V>, dim3(grid), dim3(waves_per_block * wave_size), 0, s, a.raw, a.offsets, a.sizes, a.types,    \
+    k_retina_decode<V>, dim3(grid), dim3(waves_per_block * wave_size), 0, s, a.raw, a.offsets, a.sizes, a.types, unsigned(a.mep),    \
     a.event_list, a.n_list, a.event_start, reinterpret_cast<const Geometry*>(a.geometry),                         \
     a.offsets_each_sensor_pair_size, a.retina_bank_index, a.number_of_events, a.module_cluster_num,              \
     a.offsets_module_pair_cluster, a.cluster_container, a.hit_permutations, a.hit_sorting_key, a.populate_items)

@@ -237,13 +237,16 @@ __device__ __forceinline__ Scan3 scan3(SearchShared& sh, Counters& c, bool f0, b
 }
 
 // Seeding scratch for one chunk of h1 hits, blockDim entries each, in dynamic LDS.
+// Seeding scratch of a chunk of blockDim h1 hits: per h1, its h0 candidates (up to
+// max_h0_candidates), their number and the chosen seed (h0 | h2 << 16, or no_seed).
+static_assert(max_h0_candidates >= initial_seeding_h0_candidates, "room for every h0 candidate");
+constexpr uint32_t no_seed = 0xFFFFFFFFu;
+constexpr unsigned seed_scratch_bytes_per_lane = 2 * max_h0_candidates + 4 + 2 + 1;
 struct SeedScratch {
-  float* pair_scatter;
-  uint16_t* h1;
   uint16_t* h0;
-  uint16_t* pair_h2;
-  uint8_t* n_h0;
+  uint32_t* seed;
   uint16_t* h1_chunk; // unused hits of the current module pair, one compaction chunk
+  uint8_t* n_h0;
   uint16_t* unused;   // unused hits of the previous module pair (relative indices, ascending)
 };
 
@@ -527,11 +530,12 @@ __device__ __forceinline__ unsigned h0_candidates_from_list(
 // SearchByTriplet.cu:298-446, ordered, with the (h1, h0) pairs spread over lanes.
 //
 // The reference walks, per h1, its h0 candidates in pendulum order and, per h0, the h2 window
-// in phi order, keeping the first strictly smaller scatter. Here a chunk of C = blockDim / K
-// h1 hits is processed at once (K = h0 candidates considered): lanes first collect the h0
-// candidates of one h1 each, then every lane scores one (h1, h0) pair over its h2 window
-// (first strict minimum), and finally one lane per h1 keeps the first strict minimum over its
-// h0 candidates in pendulum order: the same pair the sequential double loop selects.
+// in phi order, keeping the first strictly smaller scatter. Here a chunk of blockDim unused h1
+// hits is processed at once (K = h0 candidates considered): every lane collects the h0
+// candidates of one h1, then in rounds of C = blockDim / K h1 hits every lane scores one
+// (h1, h0) pair over its h2 window (first strict minimum) and the K lanes of an h1 keep the
+// first strict minimum over its h0 candidates in pendulum order by cross-lane exchanges: the
+// same pair the sequential double loop selects. The chunk's seeds are then appended in h1 order.
 template<bool count, int seed_mode, typename H>
 __device__ __forceinline__ void track_seeding(
   WorkCount& wc,
@@ -583,184 +587,189 @@ __device__ __forceinline__ void track_seeding(
     __syncthreads();
     phase_mark<count>(wc, phase_compaction);
 
-    for (unsigned c = 0; c < n_h1; c += C) {
-      // 1. h0 candidates of h1 number c + t (lanes t < C), pendulum order.
-      if (threadIdx.x < C) {
-        const unsigned rel = c + threadIdx.x;
-        unsigned n_found = 0;
-        if (rel < n_h1) {
-          const unsigned h1_index = ss.h1_chunk[rel];
-          const int16_t h1_phi = hits.phi(h1_index);
-          const int start = counted_search<count>(hits.phi_begin() + prev.hit_start, prev.hit_num, h1_phi, wc);
-          phase_mark<count>(wc, phase_seed1_search);
-          uint16_t* h0_out = ss.h0 + threadIdx.x * K;
-          if (use_list) {
-            n_found = K == initial_seeding_h0_candidates
-                        ? h0_candidates_from_list<initial_seeding_h0_candidates, count>(
-                            ss.unused, n_unused, prev.hit_num, prev.hit_start, start, h0_out, wc)
-                        : h0_candidates_from_list<seeding_h0_candidates, count>(
-                            ss.unused, n_unused, prev.hit_num, prev.hit_start, start, h0_out, wc);
-          }
-          else {
-            // Module pairs beyond the list's capacity: the pendulum walk itself, start, +1, -1,
-            // +2, -2, ... (each wrapped once into the module pair).
-            const int n = static_cast<int>(prev.hit_num);
-            int phi_index = start;
-            for (unsigned i = 0; i < prev.hit_num && n_found < K; ++i) {
-              phi_index += (i & 1u) ? static_cast<int>(i) : -static_cast<int>(i);
-              const int in_bounds = phi_index < 0 ? phi_index + n : (phi_index >= n ? phi_index - n : phi_index);
-              const unsigned h0_index = prev.hit_start + static_cast<unsigned>(in_bounds);
-              if (!hits.used(h0_index)) h0_out[n_found++] = h0_index;
-            }
-          }
-          ss.h1[threadIdx.x] = h1_index;
-          phase_mark<count>(wc, phase_seed1_pendulum);
+    // 1. h0 candidates of every h1 of the chunk (lane t: the chunk's h1 number t), pendulum order.
+    {
+      const unsigned rel = threadIdx.x;
+      unsigned n_found = 0;
+      if (rel < n_h1) {
+        const unsigned h1_index = ss.h1_chunk[rel];
+        const int16_t h1_phi = hits.phi(h1_index);
+        const int start = counted_search<count>(hits.phi_begin() + prev.hit_start, prev.hit_num, h1_phi, wc);
+        phase_mark<count>(wc, phase_seed1_search);
+        uint16_t* h0_out = ss.h0 + rel * max_h0_candidates;
+        if (use_list) {
+          n_found = K == initial_seeding_h0_candidates
+                      ? h0_candidates_from_list<initial_seeding_h0_candidates, count>(
+                          ss.unused, n_unused, prev.hit_num, prev.hit_start, start, h0_out, wc)
+                      : h0_candidates_from_list<seeding_h0_candidates, count>(
+                          ss.unused, n_unused, prev.hit_num, prev.hit_start, start, h0_out, wc);
         }
-        ss.n_h0[threadIdx.x] = n_found;
-      }
-      __syncthreads();
-      phase_mark<count>(wc, phase_seed1_wait);
-
-      // 2. One (h1, h0) pair per lane: best h2 of this h0 (first strict minimum).
-      {
-        const unsigned t = threadIdx.x / K;
-        const unsigned k = threadIdx.x % K;
-        float best = max_scatter;
-        uint16_t best_h2 = 0;
-        if (t < C && c + t < n_h1 && k < ss.n_h0[t]) {
-          const unsigned h1_index = ss.h1[t];
-          const unsigned h0_index = ss.h0[t * K + k];
-          const float h1x = hits.x(h1_index), h1y = hits.y(h1_index), h1z = hits.z(h1_index);
-          const float h0x = hits.x(h0_index), h0y = hits.y(h0_index), h0z = hits.z(h0_index);
-          const float td = 1.0f / (h1z - h0z);
-          const float txn = h1x - h0x;
-          const float tyn = h1y - h0y;
-          const float tx = txn * td;
-          const float ty = tyn * td;
-          int16_t cand, ephi;
-          find_forward_candidate<count>(wc, hits, next, h0x, h0y, tx, ty, dz_next, phi_tolerance, cand, ephi);
-          // The h2 window in phi order from the candidate, four positions read per round.
-          const unsigned n2 = next.hit_num;
-          const unsigned c2 = static_cast<unsigned>(static_cast<int>(cand));
-          for (unsigned j0 = 0; j0 < n2; j0 += 4) {
-            Window4 w;
-            w.load(hits, next.hit_start, n2, c2, j0, true);
-            bool stop = false;
-#pragma unroll
-            for (unsigned k = 0; k < 4; ++k) {
-              if (stop || j0 + k >= n2) continue;
-              const int16_t phi_diff = w.phi[k] - ephi;
-              const int16_t abs_phi_diff = phi_diff < 0 ? -phi_diff : phi_diff;
-              if (abs_phi_diff > phi_tolerance) {
-                stop = true;
-                continue;
-              }
-              if (!w.used[k]) {
-                if constexpr (count) ++wc.seed_eval;
-                const float dz = w.z[k] - h0z;
-                const float predx = h0x + tx * dz;
-                const float predy = h0y + ty * dz;
-                const float dx = predx - w.x[k];
-                const float dy = predy - w.y[k];
-                const float scatter = dx * dx + dy * dy;
-                if (scatter < best) {
-                  best = scatter;
-                  best_h2 = w.index[k];
-                }
-              }
-            }
-            if (stop) break;
+        else {
+          // Module pairs beyond the list's capacity: the pendulum walk itself, start, +1, -1,
+          // +2, -2, ... (each wrapped once into the module pair).
+          const int n = static_cast<int>(prev.hit_num);
+          int phi_index = start;
+          for (unsigned i = 0; i < prev.hit_num && n_found < K; ++i) {
+            phi_index += (i & 1u) ? static_cast<int>(i) : -static_cast<int>(i);
+            const int in_bounds = phi_index < 0 ? phi_index + n : (phi_index >= n ? phi_index - n : phi_index);
+            const unsigned h0_index = prev.hit_start + static_cast<unsigned>(in_bounds);
+            if (!hits.used(h0_index)) h0_out[n_found++] = h0_index;
           }
         }
-        ss.pair_scatter[threadIdx.x] = best;
-        ss.pair_h2[threadIdx.x] = best_h2;
+        phase_mark<count>(wc, phase_seed1_pendulum);
       }
-      __syncthreads();
-      phase_mark<count>(wc, phase_seed2);
-
-      // 3. One lane per h1: first strict minimum over its h0 candidates, then ordered seeds.
-      bool found = false;
-      uint16_t best_h0 = 0, best_h2 = 0, h1_index = 0;
-      if (threadIdx.x < C && c + threadIdx.x < n_h1) {
-        const unsigned t = threadIdx.x;
-        float best_fit = max_scatter;
-        for (unsigned k = 0; k < ss.n_h0[t]; ++k) {
-          const float sc = ss.pair_scatter[t * K + k];
-          if (sc < best_fit) {
-            best_fit = sc;
-            best_h0 = ss.h0[t * K + k];
-            best_h2 = ss.pair_h2[t * K + k];
-          }
-        }
-        h1_index = ss.h1[t];
-        found = best_fit < max_scatter;
-      }
-      if constexpr (seed_mode == seed_mode_log) {
-        if (found) {
-          float f[n_seed_features];
-          seed_features(hits, best_h0, h1_index, best_h2, module_pair, f);
-          const unsigned slot = atomicAdd(o.seed_log.counter, 1u);
-          if (slot < o.seed_log.capacity) {
-            uint32_t* rec = o.seed_log.records + 5 * (size_t) slot;
-            rec[0] = o.event_number;
-            rec[1] = module_pair;
-            rec[2] = best_h0;
-            rec[3] = h1_index;
-            rec[4] = best_h2;
-            float* fo = o.seed_log.features + n_seed_features * (size_t) slot;
-#pragma unroll
-            for (unsigned k = 0; k < n_seed_features; ++k) fo[k] = f[k];
-          }
-        }
-      }
-      if constexpr (seed_mode == seed_mode_scored) {
-        // The seeds of a sub-chunk are all in wave 0 (C <= 64). A seed is kept when its logit
-        // reaches the threshold; a sub-chunk whose seeds would all go keeps them (plain search
-        // as fallback).
-        if (threadIdx.x < wave_size) {
-          float fp[n_seed_features];
-#pragma unroll
-          for (unsigned k = 0; k < n_seed_features; ++k) fp[k] = 0.f;
-          if (found) seed_features(hits, best_h0, h1_index, best_h2, module_pair, fp);
-          const uint64_t live = __ballot(found);
-          const float logit = live ? scorer_logit(fp, live, *o.scorer) : 0.f;
-          const bool keep = found && logit >= o.scorer_threshold;
-          if (found && o.seed_log.counter) {
-            const unsigned slot = atomicAdd(o.seed_log.counter, 1u);
-            if (slot < o.seed_log.capacity) {
-              uint32_t* rec = o.seed_log.records + 5 * (size_t) slot;
-              rec[0] = o.event_number;
-              rec[1] = module_pair;
-              rec[2] = best_h0;
-              rec[3] = h1_index;
-              rec[4] = best_h2;
-              float* fo = o.seed_log.features + n_seed_features * (size_t) slot;
-#pragma unroll
-              for (unsigned k = 0; k < n_seed_features; ++k) fo[k] = fp[k];
-              if (o.seed_log.logits) o.seed_log.logits[slot] = logit;
-            }
-          }
-          if (__ballot(keep)) found = keep;
-        }
-      }
-      const Scan3 s = scan3(sh, cn, found, false, false);
-      if (found) {
-        const unsigned tn = (cn.ttf + s.r[0]) % max_tracks_to_follow;
-        TrackletHits t;
-        t.hits[0] = best_h0;
-        t.hits[1] = h1_index;
-        t.hits[2] = best_h2;
-        o.tracklets[tn] = t;
-        o.ttf[tn] = bit_seed | tn;
-        const unsigned wslot = tn % ring_window;
-        o.ttf_window[wslot] = bit_seed | tn;
-        o.ttf_last2[wslot] = static_cast<uint32_t>(h1_index) | (static_cast<uint32_t>(best_h2) << 16);
-        o.ttf_first[wslot] = best_h0;
-        o.ttf_n[wslot] = 3;
-      }
-      cn.ttf += s.t[0];
-      phase_mark<count>(wc, phase_seed3);
+      ss.n_h0[rel] = n_found;
     }
+    __syncthreads();
+    phase_mark<count>(wc, phase_seed1_wait);
+
+    // 2. Rounds of C h1 hits: lane t K + k scores the pair (h1 number r C + t, its h0 number k)
+    // over the h2 window (first strict minimum), and the K lanes of an h1 (consecutive lanes of
+    // one wave) keep the least (scatter, k) by exchanging keys: the first strict minimum over
+    // the h0 candidates in pendulum order, the pair the sequential double loop selects. A
+    // scatter below max_scatter is a non-negative float, ordered as its bit pattern.
+    for (unsigned r0 = 0; r0 < n_h1; r0 += C) {
+      const unsigned t = threadIdx.x / K;
+      const unsigned k = threadIdx.x % K;
+      const unsigned q = r0 + t;
+      float best = max_scatter;
+      uint16_t best_h2 = 0;
+      if (t < C && q < n_h1 && k < ss.n_h0[q]) {
+        const unsigned h1_index = ss.h1_chunk[q];
+        const unsigned h0_index = ss.h0[q * max_h0_candidates + k];
+        const float h1x = hits.x(h1_index), h1y = hits.y(h1_index), h1z = hits.z(h1_index);
+        const float h0x = hits.x(h0_index), h0y = hits.y(h0_index), h0z = hits.z(h0_index);
+        const float td = 1.0f / (h1z - h0z);
+        const float txn = h1x - h0x;
+        const float tyn = h1y - h0y;
+        const float tx = txn * td;
+        const float ty = tyn * td;
+        int16_t cand, ephi;
+        find_forward_candidate<count>(wc, hits, next, h0x, h0y, tx, ty, dz_next, phi_tolerance, cand, ephi);
+        // The h2 window in phi order from the candidate, four positions read per round.
+        const unsigned n2 = next.hit_num;
+        const unsigned c2 = static_cast<unsigned>(static_cast<int>(cand));
+        for (unsigned j0 = 0; j0 < n2; j0 += 4) {
+          Window4 w;
+          w.load(hits, next.hit_start, n2, c2, j0, true);
+          bool stop = false;
+#pragma unroll
+          for (unsigned kk = 0; kk < 4; ++kk) {
+            if (stop || j0 + kk >= n2) continue;
+            const int16_t phi_diff = w.phi[kk] - ephi;
+            const int16_t abs_phi_diff = phi_diff < 0 ? -phi_diff : phi_diff;
+            if (abs_phi_diff > phi_tolerance) {
+              stop = true;
+              continue;
+            }
+            if (!w.used[kk]) {
+              if constexpr (count) ++wc.seed_eval;
+              const float dz = w.z[kk] - h0z;
+              const float predx = h0x + tx * dz;
+              const float predy = h0y + ty * dz;
+              const float dx = predx - w.x[kk];
+              const float dy = predy - w.y[kk];
+              const float scatter = dx * dx + dy * dy;
+              if (scatter < best) {
+                best = scatter;
+                best_h2 = w.index[kk];
+              }
+            }
+          }
+          if (stop) break;
+        }
+      }
+      // key = scatter bits, k, h2; the whole wave takes part in the exchange.
+      unsigned long long key = (static_cast<unsigned long long>(__float_as_uint(best)) << 32) | (k << 16) | best_h2;
+      for (unsigned d = K / 2; d > 0; d >>= 1) {
+        const unsigned long long other = __shfl_xor(key, d, wave_size);
+        key = other < key ? other : key;
+      }
+      if (k == 0 && t < C && q < n_h1) {
+        const bool found = __uint_as_float(static_cast<uint32_t>(key >> 32)) < max_scatter;
+        const unsigned kb = static_cast<unsigned>(key >> 16) & 0xFFu;
+        ss.seed[q] = found ? (static_cast<uint32_t>(ss.h0[q * max_h0_candidates + kb]) |
+                              (static_cast<uint32_t>(key & 0xFFFFu) << 16))
+                           : no_seed;
+      }
+    }
+    __syncthreads();
+    phase_mark<count>(wc, phase_seed2);
+
+    // 3. The chunk's seeds in h1 order (lane t: h1 number t).
+    uint32_t seed = no_seed;
+    uint16_t h1_index = 0;
+    if (threadIdx.x < n_h1) {
+      seed = ss.seed[threadIdx.x];
+      h1_index = ss.h1_chunk[threadIdx.x];
+    }
+    bool found = seed != no_seed;
+    const uint16_t best_h0 = static_cast<uint16_t>(seed & 0xFFFFu);
+    const uint16_t best_h2 = static_cast<uint16_t>(seed >> 16);
+    if constexpr (seed_mode == seed_mode_log) {
+      if (found) {
+        float f[n_seed_features];
+        seed_features(hits, best_h0, h1_index, best_h2, module_pair, f);
+        const unsigned slot = atomicAdd(o.seed_log.counter, 1u);
+        if (slot < o.seed_log.capacity) {
+          uint32_t* rec = o.seed_log.records + 5 * (size_t) slot;
+          rec[0] = o.event_number;
+          rec[1] = module_pair;
+          rec[2] = best_h0;
+          rec[3] = h1_index;
+          rec[4] = best_h2;
+          float* fo = o.seed_log.features + n_seed_features * (size_t) slot;
+#pragma unroll
+          for (unsigned kk = 0; kk < n_seed_features; ++kk) fo[kk] = f[kk];
+        }
+      }
+    }
+    if constexpr (seed_mode == seed_mode_scored) {
+      // Seeds are decided in groups of C consecutive h1 hits (a wave, or half of one when
+      // C = 32). A seed is kept when its logit reaches the threshold; a group whose seeds would
+      // all go keeps them (plain search as fallback).
+      float fp[n_seed_features];
+#pragma unroll
+      for (unsigned kk = 0; kk < n_seed_features; ++kk) fp[kk] = 0.f;
+      if (found) seed_features(hits, best_h0, h1_index, best_h2, module_pair, fp);
+      const uint64_t live = __ballot(found);
+      const float logit = live ? scorer_logit(fp, live, *o.scorer) : 0.f;
+      const bool keep = found && logit >= o.scorer_threshold;
+      if (found && o.seed_log.counter) {
+        const unsigned slot = atomicAdd(o.seed_log.counter, 1u);
+        if (slot < o.seed_log.capacity) {
+          uint32_t* rec = o.seed_log.records + 5 * (size_t) slot;
+          rec[0] = o.event_number;
+          rec[1] = module_pair;
+          rec[2] = best_h0;
+          rec[3] = h1_index;
+          rec[4] = best_h2;
+          float* fo = o.seed_log.features + n_seed_features * (size_t) slot;
+#pragma unroll
+          for (unsigned kk = 0; kk < n_seed_features; ++kk) fo[kk] = fp[kk];
+          if (o.seed_log.logits) o.seed_log.logits[slot] = logit;
+        }
+      }
+      const uint64_t group = C >= wave_size ? ~0ull : (((1ull << C) - 1) << (lane_id() / C * C));
+      if (__ballot(keep) & group) found = keep;
+    }
+    const Scan3 s = scan3(sh, cn, found, false, false);
+    if (found) {
+      const unsigned tn = (cn.ttf + s.r[0]) % max_tracks_to_follow;
+      TrackletHits tl;
+      tl.hits[0] = best_h0;
+      tl.hits[1] = h1_index;
+      tl.hits[2] = best_h2;
+      o.tracklets[tn] = tl;
+      o.ttf[tn] = bit_seed | tn;
+      const unsigned wslot = tn % ring_window;
+      o.ttf_window[wslot] = bit_seed | tn;
+      o.ttf_last2[wslot] = static_cast<uint32_t>(h1_index) | (static_cast<uint32_t>(best_h2) << 16);
+      o.ttf_first[wslot] = best_h0;
+      o.ttf_n[wslot] = 3;
+    }
+    cn.ttf += s.t[0];
+    phase_mark<count>(wc, phase_seed3);
   }
   cn.local_hits = base;
 }
@@ -1041,7 +1050,7 @@ __device__ __forceinline__ void search_event(
 
 // One workgroup per event, or (work_counter set) a resident grid whose workgroups take the
 // next event slot from the counter until the slots run out. Dynamic LDS: the seeding scratch
-// (13 B per lane), the LDS window of the ring of tracks to follow (ring_window x u32; the ring
+// (seed_scratch_bytes_per_lane per lane), the LDS window of the ring of tracks to follow (ring_window x u32; the ring
 // itself, 2048 x u32, is written through to HBM), the list of unused hits of the
 // previous module pair (1024 x u16), then `lds_hits` hits as x[], y[], z[] (f32), phi[] (i16)
 // and a flags byte (used, module parity). Events with up to 5 x lds_hits hits keep phi and the
@@ -1084,13 +1093,11 @@ __global__ void k_search_by_triplet(
   __shared__ unsigned s_slot;
   WorkCount wc;
   SeedScratch ss;
-  ss.pair_scatter = reinterpret_cast<float*>(dyn_all);
-  ss.h1 = reinterpret_cast<uint16_t*>(ss.pair_scatter + blockDim.x);
-  ss.h0 = ss.h1 + blockDim.x;
-  ss.pair_h2 = ss.h0 + blockDim.x;
-  ss.h1_chunk = ss.pair_h2 + blockDim.x;
+  ss.seed = reinterpret_cast<uint32_t*>(dyn_all);
+  ss.h0 = reinterpret_cast<uint16_t*>(ss.seed + blockDim.x);
+  ss.h1_chunk = ss.h0 + max_h0_candidates * blockDim.x;
   ss.n_h0 = reinterpret_cast<uint8_t*>(ss.h1_chunk + blockDim.x);
-  uint32_t* ring = reinterpret_cast<uint32_t*>(dyn_all + ((13 * blockDim.x + 15) & ~15u));
+  uint32_t* ring = reinterpret_cast<uint32_t*>(dyn_all + ((seed_scratch_bytes_per_lane * blockDim.x + 15) & ~15u));
   uint32_t* ring_last2 = ring + ring_window;
   uint16_t* ring_first = reinterpret_cast<uint16_t*>(ring_last2 + ring_window);
   uint8_t* ring_n = reinterpret_cast<uint8_t*>(ring_first + ring_window);
@@ -1390,7 +1397,8 @@ __global__ void k_three_hit_filter(
 // Launch geometry: the seeding scratch, the ring window, the unused-hit list, then the hits.
 size_t search_dynamic_lds(unsigned block, unsigned lds_hits)
 {
-  return ((13 * (size_t) block + 15) & ~size_t(15)) + ring_window_bytes * (size_t) ring_window + 2 * unused_list_cap +
+  return ((seed_scratch_bytes_per_lane * (size_t) block + 15) & ~size_t(15)) + ring_window_bytes * (size_t) ring_window +
+         2 * unused_list_cap +
          (size_t) lds_hits * search_bytes_per_hit;
 }
 
@@ -1401,8 +1409,8 @@ hipError_t launch_search_by_triplet(const SearchLaunch& a, hipStream_t s)
   // Dynamic LDS stays within the device's per-workgroup limit (less the static part).
   const int max_lds = device_max_lds_per_block();
   if (!max_lds) return hipErrorInvalidDevice;
-  // Scored mode: the seeds of a sub-chunk must fit wave 0 and the hidden layer the seeding
-  // scratch, which holds for 256 lanes; its weights take LDS from the hits (same residency).
+  // Scored mode: seeds are decided in groups of 256 / K h1 hits (the pruning rule is stated for
+  // 256-lane chunks); the scorer's weights take LDS from the hits (same residency).
   const bool scored = a.scorer_params != nullptr;
   if (scored && block != 256) return hipErrorInvalidConfiguration;
   const size_t extra = scored ? scorer_lds_bytes + 16 : 0;

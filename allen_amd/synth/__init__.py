@@ -96,6 +96,7 @@ class Events:
     hits: np.ndarray | None = None
     hit_offsets: np.ndarray | None = None
     retina: bool = False     # banks are retina clusters (type 73) rather than super pixels
+    mep: bool = False        # MEP layout (main/include/MEPTools.h:59-200) instead of Allen's
 
     @property
     def n_sp_words(self) -> int:
@@ -105,6 +106,64 @@ class Events:
         if self.bank_version in (2, 3):
             n -= int(self.sizes[self.n_events]) - start  # one count word per bank
         return n
+
+
+def to_mep(ev: Events) -> Events:
+    """The same events in the MEP layout that Allen's kernels read with mep_layout = true
+    (main/include/MEPTools.h:59-200): one fragment per (bank, event), grouped by bank.
+
+    - offsets (u32): [0] the number of banks, [1] the number of events, [2 + b] the source ID of
+      bank b, [2 + n_banks (1 + e) + b] the byte offset of fragment (e, b) in the blocks;
+    - blocks: for each bank, the fragments of every event in event order; a fragment is the bank
+      without its source ID word (versions 2/3 keep their count word), padded to 4 bytes;
+    - sizes (u32 + u16): [b] the position (in u16 units) of bank b's sizes, one per event;
+    - types (u32 + u8): [b] the position (in bytes) of bank b's types, one per event.
+    Every event must carry the same banks in the same order (as the fragments of one MEP do)."""
+    if ev.mep:
+        return ev
+    n = ev.n_events
+    s16 = ev.sizes.view("<u2")
+    t8 = ev.types.view(np.uint8)
+    per_event = []
+    for e in range(n):
+        base = int(ev.offsets[e])
+        nb = int(ev.raw[base:base + 4].view("<u4")[0])
+        boff = ev.raw[base + 4:base + 8 + 4 * nb].view("<u4").astype(np.int64)
+        payload = base + 8 + 4 * nb
+        sizes = s16[int(ev.sizes[e]):int(ev.sizes[e]) + nb].astype(np.int64)
+        types = t8[int(ev.types[e]):int(ev.types[e]) + nb]
+        src = [int(ev.raw[payload + o:payload + o + 4].view("<u4")[0]) for o in boff[:nb]]
+        per_event.append((nb, boff, payload, sizes, types, src))
+    nb = per_event[0][0] if n else 0
+    sources = per_event[0][5] if n else []
+    for e, (k, _, _, _, _, src) in enumerate(per_event):
+        if k != nb or src != sources:
+            raise ValueError(f"event {e} does not carry the first event's banks: no MEP layout")
+    offsets = np.zeros(2 + nb * (1 + n), np.uint32)
+    offsets[0], offsets[1] = nb, n
+    offsets[2:2 + nb] = sources
+    blocks = []
+    pos = 0
+    sizes_out = np.zeros(2 * nb + nb * n + (nb * n) % 2, np.uint16)
+    types_out = np.zeros(4 * nb + nb * n + (-(4 * nb + nb * n)) % 4, np.uint8)
+    sizes_out.view("<u4")[:nb] = 2 * nb + np.arange(nb) * n
+    types_out.view("<u4")[:nb] = 4 * nb + np.arange(nb) * n
+    for b in range(nb):
+        for e, (_, boff, payload, sizes, types, _) in enumerate(per_event):
+            start = payload + int(boff[b]) + 4
+            end = payload + int(boff[b + 1]) if b + 1 < nb else payload + int(boff[nb])
+            frag = ev.raw[start:end]
+            pad = (-frag.size) % 4
+            offsets[2 + nb * (1 + e) + b] = pos
+            blocks.append(frag)
+            if pad:
+                blocks.append(np.zeros(pad, np.uint8))
+            pos += frag.size + pad
+            sizes_out[2 * nb + b * n + e] = sizes[b]
+            types_out[4 * nb + b * n + e] = types[b]
+    raw = np.concatenate(blocks) if blocks else np.zeros(4, np.uint8)
+    return dataclasses.replace(ev, raw=np.ascontiguousarray(raw), offsets=offsets,
+                               sizes=sizes_out.view("<u4").copy(), types=types_out.view("<u4").copy(), mep=True)
 
 
 PRESETS = {

@@ -159,7 +159,7 @@ SequenceInput = _params("SequenceInput", [
     ("dev_velo_raw_input", P), ("dev_velo_raw_input_offsets", P), ("dev_velo_raw_input_sizes", P),
     ("dev_velo_raw_input_types", P), ("raw_input_bytes", ctypes.c_size_t), ("number_of_events", U),
     ("bank_version", I), ("phi_tolerance", F), ("max_scatter", F), ("max_chi2", F), ("max_skipped_modules", U),
-    ("retina", I)])
+    ("retina", I), ("mep_layout", I)])
 
 # Every function the header declares, with its ctypes signature.
 SIGNATURES = {
@@ -313,6 +313,7 @@ class DeviceEvents:
         self.n_events = events.n_events
         self.bank_version = events.bank_version
         self.retina = bool(getattr(events, "retina", False))
+        self.mep = bool(getattr(events, "mep", False))
         self.raw_bytes = int(events.raw.size)
         self.raw = torch.from_numpy(events.raw).to(device)
         self.offsets = torch.from_numpy(events.offsets.view(np.int32)).to(device)
@@ -368,6 +369,7 @@ class VeloSequence:
         inp.number_of_events = ev.n_events
         inp.bank_version = ev.bank_version
         inp.retina = int(ev.retina)
+        inp.mep_layout = int(ev.mep)
         for k, v in props.items():
             setattr(inp, k, v)
         check(lib().velo_sequence_run(self.handle, ctypes.byref(inp), stream_handle(stream)), "velo_sequence_run")
@@ -384,6 +386,7 @@ class VeloSequence:
         inp.number_of_events = ev.n_events
         inp.bank_version = ev.bank_version
         inp.retina = int(ev.retina)
+        inp.mep_layout = int(ev.mep)
         for k, v in props.items():
             setattr(inp, k, v)
         check(lib().velo_sequence_capture(self.handle, ctypes.byref(inp), stream_handle(stream)),

@@ -14,6 +14,9 @@
  *                  bank = [u32 sourceID][(v2/v3: u32 count)][u32 SP words] (Transpose.cpp:430-498)
  *   raw sizes      [u32 offsets_in_u16_units x (n+1)][u16 bank sizes]   (MEPTools.h:35-44)
  *   raw types      [u32 offsets_in_bytes x (n+1)][u8 bank types]        (MEPTools.h:46-55)
+ *   MEP layout     (mep_layout != 0, MEPTools.h:59-200) offsets = [u32 n_banks][u32 n_events]
+ *                  [u32 source IDs x n_banks][u32 fragment offsets x n_banks, per event]; a
+ *                  fragment is a bank without its source ID; sizes and types indexed by bank, then event
  *   clusters       [u32 LHCbID x T][f32 x,y,z x T][i16 phi x T], 18 B each (VeloEventModel.cuh:92-183)
  *   track hits     Velo::TrackHits (u16 hits[26], u8 hitsNum; 54 B), Velo::TrackletHits (3 x u16)
  *   states         [x,y,z,tx,ty,qop x N][c00,c20,c22,c11,c31,c33,chi2,ndof x N] (VeloConsolidated.cuh:386-640)
@@ -37,7 +40,7 @@ enum {
   VELO_ERR_BANK_VERSION = 2,    /* like StrException in EstimateInputSize.cu:265-268 */
   VELO_ERR_CAPACITY = 3,        /* a run exceeds what a velo_sequence was created for */
   VELO_ERR_HIP = 4,
-  VELO_ERR_UNSUPPORTED_LAYOUT = 5, /* mep_layout != 0, or a compressed MDF record */
+  VELO_ERR_UNSUPPORTED_LAYOUT = 5, /* a compressed MDF record */
   VELO_ERR_IO = 6                  /* a file cannot be read or is not a well-formed MDF file */
 };
 
@@ -114,7 +117,7 @@ typedef struct {
   unsigned* dev_velo_bank_index;       /* 208 */
   unsigned block_dim_x;                /* property, default 256 */
   unsigned event_start;                /* RuntimeOptions::event_interval first */
-  int mep_layout;                      /* RuntimeOptions::mep_layout (only 0 is supported) */
+  int mep_layout;                      /* RuntimeOptions::mep_layout: raw input in the MEP layout */
 } velo_calculate_number_of_candidates_params;
 int velo_calculate_number_of_candidates(const velo_calculate_number_of_candidates_params* p, hipStream_t stream);
 
@@ -439,6 +442,7 @@ typedef struct {
   unsigned max_skipped_modules;
   int retina;                      /* the banks are retina clusters (type 73): the retina decoding
                                       replaces candidates, estimate, masked clustering and sort */
+  int mep_layout;                  /* raw input in the MEP layout (RuntimeOptions::mep_layout) */
 } velo_sequence_input;
 int velo_sequence_create(const velo_constants* c, unsigned max_events, size_t max_raw_bytes, velo_sequence** out);
 int velo_sequence_destroy(velo_sequence* s);

@@ -50,6 +50,7 @@ class VoInputs(ctypes.Structure):
         ("seed_logits", ctypes.c_void_p),
         ("seed_threshold", ctypes.c_float),
         ("seed_block", ctypes.c_uint),
+        ("mep_layout", ctypes.c_int),
     ]
 
 
@@ -155,6 +156,7 @@ def make_inputs(events, geometry: bytes, beamline: np.ndarray, **props):
     a.geometry_bytes = len(geometry)
     a.beamline = _ptr(beamline)
     a.retina = int(getattr(events, "retina", False))
+    a.mep_layout = int(getattr(events, "mep", False))
     event_list = props.pop("event_list", None)
     if event_list is not None:
         event_list = np.ascontiguousarray(event_list, dtype=np.uint32)

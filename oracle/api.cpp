@@ -251,6 +251,7 @@ bool make_inputs(const vo_inputs* a, vo::RawInput& in, vo::Geometry& g, vo::SpTa
   in.offsets = a->offsets;
   in.sizes = a->sizes;
   in.types = a->types;
+  in.mep = a->mep_layout != 0;
   in.version = a->bank_version;
   if (!vo::load_geometry(a->geometry, a->geometry_bytes, g)) return false;
   vo::cache_sp_patterns(sp);

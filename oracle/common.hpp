@@ -126,13 +126,15 @@ struct Clusters {
 };
 constexpr unsigned cluster_element_size = 18;
 
-// Allen event layout (main/src/Transpose.cpp:430-498, main/include/MEPTools.h:25-55)
+// Allen event layout (main/src/Transpose.cpp:430-498, main/include/MEPTools.h:25-55), or with
+// mep the MEP layout (MEPTools.h:59-200).
 struct RawInput {
   const char* raw;
   const uint32_t* offsets;
   const uint32_t* sizes;
   const uint32_t* types;
   int version;
+  bool mep = false;
 };
 
 struct RawBank {
@@ -147,38 +149,60 @@ struct RawBank {
   uint32_t n_sp(int version) const { return (version == 2 || version == 3) ? count : size / 4u; }
 };
 
-// VeloRawEvent (ClusteringDefinitions.cuh:100-146)
+// Velo::RawEvent<version, mep_layout> (ClusteringDefinitions.cuh:100-146; MEP::RawEvent,
+// MEPTools.h:117-178). Allen layout: per event [u32 n_banks][u32 offsets[n_banks + 1]][banks],
+// a bank being [sourceID][fragment]. MEP layout: offsets[0] = n_banks, offsets[2 + b] = the
+// source ID of bank b, offsets[2 + n_banks (1 + event) + b] = the byte offset of the event's
+// fragment of bank b (MEP::offset_index, MEPTools.h:66-69); sizes and types are indexed by bank,
+// then event (MEP::bank_size / bank_type, MEPTools.h:83-104).
 struct RawEvent {
-  uint32_t n_banks;
-  const uint32_t* bank_offsets;
-  const char* payload;
-  const uint16_t* bank_sizes;
-  const uint8_t* bank_types;
-  int version;
-  RawEvent(const RawInput& in, unsigned event)
+  uint32_t n_banks = 0;
+  const uint32_t* bank_offsets = nullptr;
+  const char* payload = nullptr;
+  const uint16_t* bank_sizes = nullptr;
+  const uint8_t* bank_types = nullptr;
+  int version = 0;
+  const RawInput* mep = nullptr;
+  unsigned event = 0;
+  RawEvent(const RawInput& in, unsigned event_number)
   {
-    const char* p = in.raw + in.offsets[event];
+    version = in.version;
+    if (in.mep) {
+      mep = &in;
+      event = event_number;
+      n_banks = in.offsets[0];
+      return;
+    }
+    const char* p = in.raw + in.offsets[event_number];
     std::memcpy(&n_banks, p, 4);
     bank_offsets = reinterpret_cast<const uint32_t*>(p + 4);
     payload = p + 4 + 4 * (size_t)(n_banks + 1);
-    bank_sizes = reinterpret_cast<const uint16_t*>(in.sizes) + in.sizes[event];
-    bank_types = reinterpret_cast<const uint8_t*>(in.types) + in.types[event];
-    version = in.version;
+    bank_sizes = reinterpret_cast<const uint16_t*>(in.sizes) + in.sizes[event_number];
+    bank_types = reinterpret_cast<const uint8_t*>(in.types) + in.types[event_number];
   }
   RawBank bank(unsigned i) const
   {
     RawBank b;
-    const uint32_t* p = reinterpret_cast<const uint32_t*>(payload + bank_offsets[i]);
-    b.sourceID = p[0];
-    p += 1;
+    const uint32_t* p;
+    if (mep) {
+      b.sourceID = mep->offsets[2 + i];
+      p = reinterpret_cast<const uint32_t*>(mep->raw + mep->offsets[2 + n_banks * (1 + event) + i]);
+      b.size = (reinterpret_cast<const uint16_t*>(mep->sizes) + mep->sizes[i])[event];
+      b.type = (reinterpret_cast<const uint8_t*>(mep->types) + mep->types[i])[event];
+    }
+    else {
+      p = reinterpret_cast<const uint32_t*>(payload + bank_offsets[i]);
+      b.sourceID = p[0];
+      p += 1;
+      b.size = bank_sizes[i];
+      b.type = bank_types[i];
+    }
     b.count = 0;
     if (version == 2 || version == 3) {
       b.count = p[0];
       p += 1;
     }
     b.word = p;
-    b.size = bank_sizes[i];
-    b.type = bank_types[i];
     return b;
   }
 };

@@ -49,6 +49,7 @@ typedef struct {
   const float* seed_logits;
   float seed_threshold;
   unsigned seed_block;      /* 0: 256 */
+  int mep_layout;           /* raw, offsets, sizes and types are in the MEP layout (MEPTools.h:59-200) */
 } vo_inputs;
 
 void vo_default_inputs(vo_inputs* a);

@@ -21,6 +21,9 @@ CASES = {
     "v4_full_shifted": (dict(n_events=24, pileup=0, mean_tracks=120.0, seed=62), 20, 4, list(range(20))),
     "v2_subset": (dict(n_events=16, pileup=0, mean_tracks=100.0, bank_version=2, seed=63), 12, 3, [7, 2, 11, 5]),
     "empty_list": (dict(n_events=8, pileup=1, nu=7.6, seed=64), 6, 1, []),
+    # MEP layout (mep_layout = 1): the same kind of events as fragments grouped by bank.
+    "mep_v4_subset": (dict(n_events=30, pileup=1, nu=7.6, seed=67), 24, 5, [20, 3, 11, 7, 0]),
+    "mep_v2_subset": (dict(n_events=16, pileup=0, mean_tracks=100.0, bank_version=2, seed=68), 12, 2, [9, 4, 6]),
 }
 
 
@@ -79,6 +82,8 @@ def gpu_store(args, ref) -> dict:
 def test_sp_drop_ins_with_event_list(case, geometry, beamline, constants):
     settings, n_run, event_start, event_list = CASES[case]
     ev = synth.generate(**settings)
+    if case.startswith("mep_"):
+        ev = synth.to_mep(ev)
     print(f"[drop-in] case={case} raw_events={ev.n_events} run={n_run} start={event_start} "
           f"list={len(event_list)}", flush=True)
     assert n_run + event_start <= ev.n_events

@@ -17,6 +17,10 @@ DATASETS = {
     "v2_banks": dict(n_events=20, pileup=0, mean_tracks=100.0, bank_version=2, seed=14),
     "empty_events": dict(n_events=6, pileup=0, mean_tracks=0.0, noise_fraction=0.0, seed=15),
     "stress": dict(n_events=12, pileup=1, nu=15.0, noise_fraction=0.05, dead_asic_fraction=0.01, seed=16),
+    # The same kind of events in the MEP layout (MEPTools.h:59-200), super pixels and retina clusters.
+    "mep_pileup": dict(n_events=40, pileup=1, nu=7.6, seed=18),
+    "mep_v2_banks": dict(n_events=20, pileup=0, mean_tracks=100.0, bank_version=2, seed=19),
+    "mep_retina_v3": dict(n_events=30, pileup=1, nu=7.6, retina=1, bank_version=3, seed=20),
 }
 
 
@@ -29,6 +33,8 @@ def constants(geometry, beamline):
 
 def run_both(name, geometry, beamline, constants, fused, tuning=None, oracle_props=None, events=None):
     ev = events if events is not None else synth.generate(**DATASETS[name])
+    if name.startswith("mep_"):
+        ev = synth.to_mep(ev)
     print(f"[parity] dataset={name} events={ev.n_events} raw_bytes={ev.raw.size} fused={fused} tuning={tuning}",
           flush=True)
     ref = oracle.run(ev, geometry, beamline, **(oracle_props or {}))
@@ -66,6 +72,12 @@ def test_real_sp_banks_match_oracle(fused, geometry, beamline, constants):
     ev = textbook.load_mdf_fixture(os.path.join(os.path.dirname(__file__), "golden", "mdf_minbias_vp.npz"))
     assert ev.bank_version == 2 and ev.n_events == 10
     ref, gpu, status = run_both("mdf_minbias_vp", geometry, beamline, constants, fused, events=ev)
+    assert int(ref["status"][0]) == 0
+    assert status & velo.STATUS_OVERFLOW_MASK == 0
+    errs = parity.compare(gpu, ref, fused)
+    assert not errs, "\n".join(errs[:20])
+    # The same real banks in the MEP layout.
+    ref, gpu, status = run_both("mep_mdf_minbias_vp", geometry, beamline, constants, fused, events=ev)
     assert int(ref["status"][0]) == 0
     assert status & velo.STATUS_OVERFLOW_MASK == 0
     errs = parity.compare(gpu, ref, fused)
