@@ -161,6 +161,8 @@ int velo_constants_create(const char* geometry_bin, size_t nbytes, const float* 
   if (rc == VELO_OK) rc = hip_status(hipMemcpy(c->dev_sp_fy, c->fy, 512 * sizeof(float), hipMemcpyHostToDevice));
   if (rc == VELO_OK)
     rc = hip_status(hipMemcpy(c->dev_beamline, beamline_xy, 2 * sizeof(float), hipMemcpyHostToDevice));
+  // The device address of the allocation-free prefix sum's static scratch on this device.
+  if (rc == VELO_OK) rc = hip_status(velo::prefix_sum_static_init());
   if (rc != VELO_OK) {
     velo_constants_destroy(c);
     return rc;
@@ -210,6 +212,22 @@ int velo_prefix_sum(const velo_prefix_sum_params* p, hipStream_t stream)
   if (velo::prefix_sum_scratch_words(p->input_size) > 1 && !p->dev_scratch) return VELO_ERR_INVALID_ARGUMENT;
   int rc = hip_status(
     velo::launch_prefix_sum(p->dev_input_buffer, p->input_size, p->dev_output_buffer, p->dev_scratch, stream));
+  if (rc == VELO_OK && p->host_total_sum_holder) {
+    rc = hip_status(hipMemcpyAsync(
+      p->host_total_sum_holder, p->dev_output_buffer + p->input_size, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+    if (rc == VELO_OK) rc = hip_status(hipStreamSynchronize(stream));
+  }
+  return rc;
+}
+
+size_t velo_prefix_sum_static_max_items(void) { return velo::prefix_sum_static_max_items; }
+
+int velo_prefix_sum_static(const velo_prefix_sum_params* p, hipStream_t stream)
+{
+  if (!p || !p->dev_output_buffer || (p->input_size && !p->dev_input_buffer) || p->dev_scratch)
+    return VELO_ERR_INVALID_ARGUMENT;
+  if (p->input_size > velo::prefix_sum_static_max_items) return VELO_ERR_CAPACITY;
+  int rc = hip_status(velo::launch_prefix_sum_static(p->dev_input_buffer, p->input_size, p->dev_output_buffer, stream));
   if (rc == VELO_OK && p->host_total_sum_holder) {
     rc = hip_status(hipMemcpyAsync(
       p->host_total_sum_holder, p->dev_output_buffer + p->input_size, sizeof(unsigned), hipMemcpyDeviceToHost, stream));

@@ -128,6 +128,56 @@ hipError_t launch_prefix_sum(const uint32_t* in, unsigned n, uint32_t* out, uint
   return launch_prefix_sum_dyn(in, nullptr, n, out, scratch, s);
 }
 
+// Tile sums of the allocation-free prefix sum: one u32 per scan tile, room for
+// prefix_sum_static_max_items items (256 KiB per device, held until the process ends). Every
+// launch writes all the tile sums it reads, so nothing needs clearing between calls; calls that
+// overlap on one device share the array and race.
+constexpr unsigned prefix_sum_static_max_tiles = prefix_sum_static_max_items / scan_tile;
+static_assert(prefix_sum_static_max_items % scan_tile == 0, "whole tiles");
+static __device__ __align__(256) uint32_t scratch[prefix_sum_static_max_tiles];
+
+namespace {
+std::mutex g_scratch_mutex;
+std::map<int, uint32_t*> g_scratch_ptr; // device -> address of its copy of `scratch`
+} // namespace
+
+// The device address of the current device's `scratch`, looked up once per device (the array's
+// name is a host-side stand-in and is never passed to a kernel).
+static hipError_t static_scratch(uint32_t** ptr)
+{
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  auto it = g_scratch_ptr.find(dev);
+  if (it != g_scratch_ptr.end()) {
+    *ptr = it->second;
+    return hipSuccess;
+  }
+  void* p = nullptr;
+  e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if (!p) return hipErrorInvalidSymbol;
+  g_scratch_ptr.emplace(dev, static_cast<uint32_t*>(p));
+  *ptr = static_cast<uint32_t*>(p);
+  return hipSuccess;
+}
+
+hipError_t prefix_sum_static_init()
+{
+  uint32_t* p = nullptr;
+  return static_scratch(&p);
+}
+
+hipError_t launch_prefix_sum_static(const uint32_t* in, unsigned n, uint32_t* out, hipStream_t s)
+{
+  if (n > prefix_sum_static_max_items) return hipErrorInvalidValue;
+  uint32_t* p = nullptr;
+  const hipError_t e = static_scratch(&p);
+  if (e != hipSuccess) return e;
+  return launch_prefix_sum_dyn(in, nullptr, n, out, p, s);
+}
+
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t value)
 {
   for (size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t) gridDim.x * blockDim.x)

@@ -398,6 +398,11 @@ hipError_t launch_prefix_sum_dyn(
   uint32_t* out,
   uint32_t* scratch,
   hipStream_t s);
+// Same, with the tile sums in a static device array of the library (no allocation): at most
+// prefix_sum_static_max_items items. prefix_sum_static_init() looks the current device's copy up.
+constexpr unsigned prefix_sum_static_max_items = 1u << 28;
+hipError_t prefix_sum_static_init();
+hipError_t launch_prefix_sum_static(const uint32_t* in, unsigned n, uint32_t* out, hipStream_t s);
 // Fill `n` u32 with `value` (a kernel, so the work can be captured in a graph).
 hipError_t launch_create_velo_views(const VeloViewsLaunch& a, hipStream_t s);
 hipError_t launch_clusters_views(const ClustersViewLaunch& a, hipStream_t s);

@@ -177,6 +177,8 @@ SIGNATURES = {
     "velo_constants_sp_tables": (I, [P, P, P, P, P]),
     "velo_prefix_sum_scratch_size": (ctypes.c_size_t, [U]),
     "velo_prefix_sum": (I, [ctypes.POINTER(PrefixSumParams), P]),
+    "velo_prefix_sum_static_max_items": (ctypes.c_size_t, []),
+    "velo_prefix_sum_static": (I, [ctypes.POINTER(PrefixSumParams), P]),
     "velo_calculate_number_of_candidates": (I, [ctypes.POINTER(CandidatesParams), P]),
     "velo_estimate_input_size": (I, [ctypes.POINTER(EstimateParams), P]),
     "velo_masked_clustering": (I, [ctypes.POINTER(MaskedClusteringParams), P]),
@@ -330,6 +332,14 @@ def prefix_sum(dev_input, n: int, dev_output, stream=None, want_total=False):
     p = PrefixSumParams(_ptr(dev_input), n, _ptr(dev_output), _ptr(scratch),
                         total.ctypes.data if want_total else 0)
     check(lib().velo_prefix_sum(ctypes.byref(p), stream_handle(stream)), "velo_prefix_sum")
+    return int(total[0]) if want_total else None
+
+
+def prefix_sum_static(dev_input, n: int, dev_output, stream=None, want_total=False):
+    """The same device scan with the library's static scratch (allocates nothing)."""
+    total = np.zeros(1, np.uint32)
+    p = PrefixSumParams(_ptr(dev_input), n, _ptr(dev_output), 0, total.ctypes.data if want_total else 0)
+    check(lib().velo_prefix_sum_static(ctypes.byref(p), stream_handle(stream)), "velo_prefix_sum_static")
     return int(total[0]) if want_total else None
 
 

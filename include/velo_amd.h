@@ -103,6 +103,15 @@ typedef struct {
 size_t velo_prefix_sum_scratch_size(unsigned input_size);
 int velo_prefix_sum(const velo_prefix_sum_params* p, hipStream_t stream);
 
+/* ---- The same prefix sum without allocation or workspace. No counterpart in the reference: it is
+ * for callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). dev_scratch must be NULL: the tile sums live in a static device array of
+ * the library (256 KiB per device, held until the process ends), so input_size is at most
+ * velo_prefix_sum_static_max_items() (2^28), else VELO_ERR_CAPACITY. Calls on one device share that
+ * array: calls that overlap on one device (other streams, other threads) race. */
+size_t velo_prefix_sum_static_max_items(void);
+int velo_prefix_sum_static(const velo_prefix_sum_params* p, hipStream_t stream);
+
 /* ---- velo_calculate_number_of_candidates_t
  * (device/velo/mask_clustering/include/VeloCalculateNumberOfCandidates.cuh:10-35) */
 typedef struct {
