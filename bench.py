@@ -141,6 +141,10 @@ def main():
                          "scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
 
+    if args.graph and args.persistent:
+        print("bench.py: --graph and --persistent cannot be combined (velo_sequence_capture refuses the persistent "
+              "search)", file=sys.stderr)
+        sys.exit(2)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the VELO path shards by events "
               f"over independent GPUs, and this program measures one.", file=sys.stderr)

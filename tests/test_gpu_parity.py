@@ -86,7 +86,8 @@ def test_real_sp_banks_match_oracle(fused, geometry, beamline, constants):
 
 
 @pytest.mark.skip(reason="its first GPU run stopped answering (killed after 180 s without output) and no cause is "
-                         "shown yet; the persistent search inside a captured graph stays off the GPU until one is")
+                         "shown yet; velo_sequence_capture now refuses the persistent search, and this case stays off "
+                         "the GPU until a cause is shown")
 def test_batched_persistent_graph_replays(geometry, beamline, constants):
     """The batched workload's event mix (one vertex with Poisson(120) tracks, 5 % of the events at
     nu = 20 pile-up) through the persistent work-stealing search, the whole sequence captured as
