@@ -240,6 +240,7 @@ __device__ __forceinline__ Scan3 scan3(SearchShared& sh, Counters& c, bool f0, b
 // Seeding scratch of a chunk of blockDim h1 hits: per h1, its h0 candidates (up to
 // max_h0_candidates), their number and the chosen seed (h0 | h2 << 16, or no_seed).
 static_assert(max_h0_candidates >= initial_seeding_h0_candidates, "room for every h0 candidate");
+static_assert(initial_seeding_h0_candidates == 8 && seeding_h0_candidates == 4, "track_seeding splits lanes by K = 4 or 8");
 constexpr uint32_t no_seed = 0xFFFFFFFFu;
 constexpr unsigned seed_scratch_bytes_per_lane = 2 * max_h0_candidates + 4 + 2 + 1;
 struct SeedScratch {
@@ -283,10 +284,12 @@ struct Window4 {
   __device__ __forceinline__ void
   load(const H& hits, unsigned start, unsigned n, unsigned c, unsigned j0, bool with_used)
   {
+    // c <= n (a lower bound) and j < n, so c + j < 2 n and one conditional subtraction wraps it.
 #pragma unroll
     for (unsigned k = 0; k < 4; ++k) {
       const unsigned j = j0 + k < n ? j0 + k : j0;
-      index[k] = start + (c + j) % n;
+      const unsigned w = c + j;
+      index[k] = start + (w >= n ? w - n : w);
     }
 #pragma unroll
     for (unsigned k = 0; k < 4; ++k) {
@@ -553,15 +556,22 @@ __device__ __forceinline__ void track_seeding(
   unsigned h0_candidates_to_consider)
 {
   const unsigned K = h0_candidates_to_consider;
-  const unsigned C = blockDim.x / K;
+  const unsigned log2K = K == 8 ? 3u : 2u; // K is 4 or 8
+  const unsigned C = blockDim.x >> log2K;
   const float dz_next = next.z[0] - prev.z[0];
 
   // The list of unused hits of the previous module pair for step 1 (its writes are ordered
   // before step 1 by the barrier of the compaction scan below).
+  // The first blockDim hits of both lists are compacted by one scan (seeding marks no hit used,
+  // so both read the flags the last forwarding left).
   const bool use_list = prev.hit_num <= unused_list_cap;
-  unsigned n_unused = 0;
+  const bool free0 = use_list && threadIdx.x < prev.hit_num && !hits.used(prev.hit_start + threadIdx.x);
+  const bool act0 = threadIdx.x < cur.hit_num && !hits.used(cur.hit_start + threadIdx.x);
+  const Scan3 first = scan3(sh, cn, free0, act0, false);
+  if (free0) ss.unused[first.r[0]] = static_cast<uint16_t>(threadIdx.x);
+  unsigned n_unused = first.t[0];
   if (use_list) {
-    for (unsigned c0 = 0; c0 < prev.hit_num; c0 += blockDim.x) {
+    for (unsigned c0 = blockDim.x; c0 < prev.hit_num; c0 += blockDim.x) {
       const unsigned r = c0 + threadIdx.x;
       const bool free_hit = r < prev.hit_num && !hits.used(prev.hit_start + r);
       const Scan3 sc = scan3(sh, cn, free_hit, false, false);
@@ -576,8 +586,16 @@ __device__ __forceinline__ void track_seeding(
   uint32_t base = 0;
   for (unsigned c0 = 0; c0 < cur.hit_num; c0 += blockDim.x) {
     const unsigned r = c0 + threadIdx.x;
-    const bool act = r < cur.hit_num && !hits.used(cur.hit_start + r);
-    const Scan3 sc = scan3(sh, cn, act, false, false);
+    bool act = act0;
+    Scan3 sc;
+    if (c0 == 0) {
+      sc.r[0] = first.r[1];
+      sc.t[0] = first.t[1];
+    }
+    else {
+      act = r < cur.hit_num && !hits.used(cur.hit_start + r);
+      sc = scan3(sh, cn, act, false, false);
+    }
     if (act) {
       o.h1_indices[base + sc.r[0]] = cur.hit_start + r;
       ss.h1_chunk[sc.r[0]] = cur.hit_start + r;
@@ -629,14 +647,19 @@ __device__ __forceinline__ void track_seeding(
     // the h0 candidates in pendulum order, the pair the sequential double loop selects. A
     // scatter below max_scatter is a non-negative float, ordered as its bit pattern.
     for (unsigned r0 = 0; r0 < n_h1; r0 += C) {
-      const unsigned t = threadIdx.x / K;
-      const unsigned k = threadIdx.x % K;
+      const unsigned t = threadIdx.x >> log2K;
+      const unsigned k = threadIdx.x & (K - 1);
       const unsigned q = r0 + t;
       float best = max_scatter;
       uint16_t best_h2 = 0;
-      if (t < C && q < n_h1 && k < ss.n_h0[q]) {
-        const unsigned h1_index = ss.h1_chunk[q];
-        const unsigned h0_index = ss.h0[q * max_h0_candidates + k];
+      // The pair's h1 and h0 are read together with the candidate count (all within the
+      // chunk's scratch for q < n_h1), one LDS round trip instead of two.
+      const bool in_chunk = t < C && q < n_h1;
+      const unsigned qq = in_chunk ? q : 0u;
+      const unsigned n_h0_q = ss.n_h0[qq];
+      const unsigned h1_index = ss.h1_chunk[qq];
+      const unsigned h0_index = ss.h0[qq * max_h0_candidates + k];
+      if (in_chunk && k < n_h0_q) {
         const float h1x = hits.x(h1_index), h1y = hits.y(h1_index), h1z = hits.z(h1_index);
         const float h0x = hits.x(h0_index), h0y = hits.y(h0_index), h0z = hits.z(h0_index);
         const float td = 1.0f / (h1z - h0z);
