@@ -249,6 +249,7 @@ struct SeedScratch {
   uint16_t* h1_chunk; // unused hits of the current module pair, one compaction chunk
   uint8_t* n_h0;
   uint16_t* unused;   // unused hits of the previous module pair (relative indices, ascending)
+  uint16_t* before;   // per position of the previous module pair: the unused hits before it
 };
 
 struct ScorerLds;
@@ -483,19 +484,20 @@ seed_features(const H& hits, unsigned h0, unsigned h1, unsigned h2, unsigned mod
 // are among the K nearest unused entries on either side of start in the circular list, so the
 // 2K + 1 entries around its lower bound (all u if u <= 2K + 1) are ranked and the K smallest
 // taken. Writes hit_start + position to out[0..) and returns how many (<= K).
-template<unsigned K, bool count>
+template<unsigned K>
 __device__ __forceinline__ unsigned h0_candidates_from_list(
   const uint16_t* unused,
+  const uint16_t* before,
   unsigned u,
   unsigned n,
   unsigned hit_start,
   int start,
-  uint16_t* out,
-  WorkCount& wc)
+  uint16_t* out)
 {
   if (u == 0) return 0;
   const unsigned s0 = static_cast<unsigned>(start) >= n ? static_cast<unsigned>(start) - n : static_cast<unsigned>(start);
-  const unsigned p = static_cast<unsigned>(counted_search<count>(unused, u, static_cast<uint16_t>(s0), wc));
+  // The lower bound of s0 in the list: the number of unused positions below s0.
+  const unsigned p = before[s0];
   // key = rank << 16 | position (n <= 1024 on this path: rank < 2048, position < 1024).
   constexpr unsigned M = 2 * K + 1;
   constexpr unsigned none = 0xFFFFFFFFu;
@@ -569,6 +571,7 @@ __device__ __forceinline__ void track_seeding(
   const bool act0 = threadIdx.x < cur.hit_num && !hits.used(cur.hit_start + threadIdx.x);
   const Scan3 first = scan3(sh, cn, free0, act0, false);
   if (free0) ss.unused[first.r[0]] = static_cast<uint16_t>(threadIdx.x);
+  if (use_list && threadIdx.x < prev.hit_num) ss.before[threadIdx.x] = static_cast<uint16_t>(first.r[0]);
   unsigned n_unused = first.t[0];
   if (use_list) {
     for (unsigned c0 = blockDim.x; c0 < prev.hit_num; c0 += blockDim.x) {
@@ -576,6 +579,7 @@ __device__ __forceinline__ void track_seeding(
       const bool free_hit = r < prev.hit_num && !hits.used(prev.hit_start + r);
       const Scan3 sc = scan3(sh, cn, free_hit, false, false);
       if (free_hit) ss.unused[n_unused + sc.r[0]] = static_cast<uint16_t>(r);
+      if (r < prev.hit_num) ss.before[r] = static_cast<uint16_t>(n_unused + sc.r[0]);
       n_unused += sc.t[0];
     }
   }
@@ -617,10 +621,10 @@ __device__ __forceinline__ void track_seeding(
         uint16_t* h0_out = ss.h0 + rel * max_h0_candidates;
         if (use_list) {
           n_found = K == initial_seeding_h0_candidates
-                      ? h0_candidates_from_list<initial_seeding_h0_candidates, count>(
-                          ss.unused, n_unused, prev.hit_num, prev.hit_start, start, h0_out, wc)
-                      : h0_candidates_from_list<seeding_h0_candidates, count>(
-                          ss.unused, n_unused, prev.hit_num, prev.hit_start, start, h0_out, wc);
+                      ? h0_candidates_from_list<initial_seeding_h0_candidates>(
+                          ss.unused, ss.before, n_unused, prev.hit_num, prev.hit_start, start, h0_out)
+                      : h0_candidates_from_list<seeding_h0_candidates>(
+                          ss.unused, ss.before, n_unused, prev.hit_num, prev.hit_start, start, h0_out);
         }
         else {
           // Module pairs beyond the list's capacity: the pendulum walk itself, start, +1, -1,
@@ -1073,9 +1077,10 @@ __device__ __forceinline__ void search_event(
 
 // One workgroup per event, or (work_counter set) a resident grid whose workgroups take the
 // next event slot from the counter until the slots run out. Dynamic LDS: the seeding scratch
-// (seed_scratch_bytes_per_lane per lane), the LDS window of the ring of tracks to follow (ring_window x u32; the ring
-// itself, 2048 x u32, is written through to HBM), the list of unused hits of the
-// previous module pair (1024 x u16), then `lds_hits` hits as x[], y[], z[] (f32), phi[] (i16)
+// (seed_scratch_bytes_per_lane per lane), the LDS window of the ring of tracks to follow
+// (ring_window entries; the ring itself, 2048 x u32, is written through to HBM), the list of
+// unused hits of the previous module pair and, per position, the number of unused hits before
+// it (1024 x u16 each), then `lds_hits` hits as x[], y[], z[] (f32), phi[] (i16)
 // and a flags byte (used, module parity). Events with up to 5 x lds_hits hits keep phi and the
 // flags in LDS and read coordinates from global memory; larger events run on global memory.
 // The `count` instantiation also accumulates the work counters into work_stats[4] (one
@@ -1125,7 +1130,8 @@ __global__ void k_search_by_triplet(
   uint16_t* ring_first = reinterpret_cast<uint16_t*>(ring_last2 + ring_window);
   uint8_t* ring_n = reinterpret_cast<uint8_t*>(ring_first + ring_window);
   ss.unused = reinterpret_cast<uint16_t*>(ring_n + ring_window);
-  char* dyn = reinterpret_cast<char*>(ss.unused + unused_list_cap);
+  ss.before = ss.unused + unused_list_cap;
+  char* dyn = reinterpret_cast<char*>(ss.before + unused_list_cap);
   ScorerLds* scorer = nullptr;
   if constexpr (seed_mode == seed_mode_scored) {
     scorer = reinterpret_cast<ScorerLds*>(dyn + (((size_t) lds_hits * search_bytes_per_hit + 15) & ~size_t(15)));
@@ -1421,7 +1427,7 @@ __global__ void k_three_hit_filter(
 size_t search_dynamic_lds(unsigned block, unsigned lds_hits)
 {
   return ((seed_scratch_bytes_per_lane * (size_t) block + 15) & ~size_t(15)) + ring_window_bytes * (size_t) ring_window +
-         2 * unused_list_cap +
+         2 * 2 * unused_list_cap +
          (size_t) lds_hits * search_bytes_per_hit;
 }
 
