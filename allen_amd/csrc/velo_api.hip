@@ -640,6 +640,15 @@ struct velo_sequence {
   unsigned search_lds_hits = 0; // 0: the launcher's default for the kernel variant
   bool search_largest_first = true;
   bool search_persistent = false;
+  // The largest events (E / search_split of them, events by descending hit count) run on a
+  // second stream of the sequence in wider workgroups with more LDS, beside the other events'
+  // launch on the run's stream; 0 = one launch. The stream and its events are made when the
+  // knob is first set (an idle stream still takes one of the process's hardware queues).
+  unsigned search_split = 0;
+  unsigned search_big_block = 256;
+  unsigned search_big_lds_hits = 4096;
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   bool clustering_resident_grid = false;
   unsigned seed_log_capacity = 0; // seeds; buffers "seed_log_*" exist once the knob was set
   bool scorer_on = false;         // buffer "scorer_params" holds the scorer's parameters
@@ -757,6 +766,9 @@ int velo_sequence_destroy(velo_sequence* s)
 {
   if (!s) return VELO_ERR_INVALID_ARGUMENT;
   s->release_graph();
+  if (s->fork_ev) (void) hipEventDestroy(s->fork_ev);
+  if (s->join_ev) (void) hipEventDestroy(s->join_ev);
+  if (s->side) (void) hipStreamDestroy(s->side);
   for (auto& e : s->ev_pairs) {
     (void) hipEventDestroy(e.first);
     (void) hipEventDestroy(e.second);
@@ -856,6 +868,27 @@ int velo_sequence_set_tuning(velo_sequence* s, const char* knob, unsigned value)
   }
   if (k == "search_largest_first") {
     s->search_largest_first = value != 0;
+    return VELO_OK;
+  }
+  if (k == "search_split") {
+    // The second stream of the split search and its fork/join events (default priority).
+    if (value && !s->side) {
+      int rc = hip_status(hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
+      if (rc == VELO_OK) rc = hip_status(hipEventCreateWithFlags(&s->fork_ev, hipEventDisableTiming));
+      if (rc == VELO_OK) rc = hip_status(hipEventCreateWithFlags(&s->join_ev, hipEventDisableTiming));
+      if (rc != VELO_OK) return rc;
+    }
+    s->search_split = value;
+    return VELO_OK;
+  }
+  if (k == "search_big_block") {
+    if (!value || value % 64 || value > 1024) return VELO_ERR_INVALID_ARGUMENT;
+    s->search_big_block = value;
+    return VELO_OK;
+  }
+  if (k == "search_big_lds_hits") {
+    if (value > 8192) return VELO_ERR_INVALID_ARGUMENT;
+    s->search_big_lds_hits = value;
     return VELO_OK;
   }
   if (k == "fused_clustering") {
@@ -1134,7 +1167,16 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream
     k5.event_list = s->get<uint32_t>("search_order");
     VELO_TRY(velo::launch_order_events_by_hits(est_off, n_events_dev, const_cast<uint32_t*>(k5.event_list), stream));
   }
-  VELO_TRY(velo::launch_search_by_triplet(k5, stream));
+  // The largest events on the sequence's second stream (fork and join through events, so the
+  // run stays one stream to the caller and capturable); the persistent grid takes every event.
+  const unsigned n_big = s->search_largest_first && !s->search_persistent && s->search_split ? E / s->search_split : 0u;
+  if (n_big) {
+    VELO_TRY(velo::launch_search_split(k5, n_big, s->search_big_block, s->search_big_lds_hits, stream, s->side,
+                                       s->fork_ev, s->join_ev));
+  }
+  else {
+    VELO_TRY(velo::launch_search_by_triplet(k5, stream));
+  }
   VELO_MARK(mark_end(VELO_STAGE_SEARCH));
   auto* off_tracks = s->get<uint32_t>("offsets_velo_tracks");
   VELO_TRY(velo::launch_prefix_sum(k5.number_of_velo_tracks, E, off_tracks, scratch, stream));

@@ -340,6 +340,17 @@ hipError_t launch_cluster_sorted(const FusedClusterLaunch& a, hipStream_t s);
 hipError_t launch_estimate_mp(const EstimateMpLaunch& a, hipStream_t s);
 hipError_t launch_cluster_mp(const ClusterMpLaunch& a, hipStream_t s);
 hipError_t launch_search_by_triplet(const SearchLaunch& a, hipStream_t s);
+// The largest n_big events of an ordered list on `side` (fork/join with `s` through the two
+// events), the others on `s`; a plain launch when there is nothing to split.
+hipError_t launch_search_split(
+  const SearchLaunch& a,
+  unsigned n_big,
+  unsigned big_block,
+  unsigned big_lds_hits,
+  hipStream_t s,
+  hipStream_t side,
+  hipEvent_t fork,
+  hipEvent_t join);
 constexpr unsigned event_clock_words = 10; // start, end, ticks in 8 phases
 // Event hits staged in LDS by default: four workgroups of 256 lanes fit one CU's 160 KiB.
 constexpr unsigned search_default_lds_hits = 1760;

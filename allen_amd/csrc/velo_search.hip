@@ -1479,6 +1479,42 @@ hipError_t launch_search_by_triplet(const SearchLaunch& a, hipStream_t s)
   return hipGetLastError();
 }
 
+// The first n_big events of the list (the largest, when the list is ordered by descending hit
+// count) in workgroups of big_block lanes with big_lds_hits hits of LDS on `side`, the others
+// with the launch's own knobs on `s`. `side` waits for the work on `s` before it (fork) and `s`
+// waits for `side` after both launches (join), so later work on `s` sees every event's outputs.
+hipError_t launch_search_split(
+  const SearchLaunch& a,
+  unsigned n_big,
+  unsigned big_block,
+  unsigned big_lds_hits,
+  hipStream_t s,
+  hipStream_t side,
+  hipEvent_t fork,
+  hipEvent_t join)
+{
+  if (!a.event_list || !side || !fork || !join || n_big == 0 || n_big >= a.n_list || a.work_counter)
+    return launch_search_by_triplet(a, s);
+  hipError_t e = hipEventRecord(fork, s);
+  if (e != hipSuccess) return e;
+  e = hipStreamWaitEvent(side, fork, 0);
+  if (e != hipSuccess) return e;
+  SearchLaunch big = a;
+  big.n_list = n_big;
+  big.block_dim = big_block;
+  big.lds_hits = big_lds_hits;
+  // Launched first, so its large workgroups are placed before the many small ones.
+  e = launch_search_by_triplet(big, side);
+  SearchLaunch rest = a;
+  rest.event_list = a.event_list + n_big;
+  rest.n_list = a.n_list - n_big;
+  const hipError_t e2 = launch_search_by_triplet(rest, s);
+  // The join is recorded whatever happened above, so `side` never stays outside the run.
+  const hipError_t e3 = hipEventRecord(join, side);
+  const hipError_t e4 = e3 == hipSuccess ? hipStreamWaitEvent(s, join, 0) : e3;
+  return e != hipSuccess ? e : (e2 != hipSuccess ? e2 : e4);
+}
+
 hipError_t launch_three_hit_tracks_filter(const ThreeHitFilterLaunch& a, hipStream_t s)
 {
   if (!a.n_list) return hipSuccess;

@@ -3,9 +3,9 @@
 One step = the whole VELO path (super-pixel decoding + masked clustering, phi sort,
 search_by_triplet, three-hit filter, consolidation, simplified Kalman fit) over one slice
 of 1000 synthetic events with the raw banks already resident in HBM. Slices are dealt to
-three HIP streams in turn, one sequence per stream, which is how Allen measures GPU
-throughput (one stream per host thread, scanned over the stream count); the one-stream
-figure is reported beside it.
+six HIP streams in turn (the process runs with eight hardware queues), one sequence per
+stream, which is how Allen measures GPU throughput (one stream per host thread, scanned
+over the stream count); the one-stream figure is reported beside it.
 
 Prints ONE JSON line (rank 0). Progress goes to stderr.
 """
@@ -116,10 +116,19 @@ def main():
     ap.add_argument("--search-block", type=int, default=0)
     ap.add_argument("--search-lds-hits", type=int, default=0)
     ap.add_argument("--search-event-order", choices=["largest_first", "natural"], default="largest_first")
-    ap.add_argument("--streams", type=int, default=3,
+    ap.add_argument("--search-split", type=int, default=0,
+                    help="the largest 1/N of a slice's events run in a second search launch, on a second stream of "
+                         "the sequence, in 256-lane workgroups with 4096 LDS hits; 0: one launch (the default: the "
+                         "split shortens one slice's search but the extra stream per slice costs more throughput "
+                         "than it gains when several slices are in flight)")
+    ap.add_argument("--streams", type=int, default=6,
                     help="slices in flight: one sequence and one HIP stream per slice, as Allen runs one stream per "
                          "host thread and measures GPU throughput over the stream count; steps are dealt to the "
-                         "streams in turn (3 is the best count on one MI355X, whose process has 4 hardware queues)")
+                         "streams in turn (6 is the best count measured on one MI355X with 8 hardware queues, 3 "
+                         "with HIP's default of 4)")
+    ap.add_argument("--hw-queues", type=int, default=8,
+                    help="hardware queues of this process (HIP's GPU_MAX_HW_QUEUES, set before the runtime starts; "
+                         "HIP's default is 4): one per stream in flight plus the stream that times them")
     ap.add_argument("--persistent", action="store_true",
                     help="search_by_triplet as a persistent grid whose workgroups take events from a work counter")
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
@@ -145,6 +154,11 @@ def main():
         print("bench.py: --graph and --persistent cannot be combined (velo_sequence_capture refuses the persistent "
               "search)", file=sys.stderr)
         sys.exit(2)
+    if not 1 <= args.hw_queues <= 32:
+        print(f"bench.py: --hw-queues {args.hw_queues} is outside 1..32", file=sys.stderr)
+        sys.exit(2)
+    # Read by the HIP runtime when it starts, which is at the first GPU call below.
+    os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the VELO path shards by events "
               f"over independent GPUs, and this program measures one.", file=sys.stderr)
@@ -173,7 +187,7 @@ def main():
     constants = velo.VeloConstants(geometry, beamline)
     dev = velo.DeviceEvents(ev)
     knobs = {"search_largest_first": int(args.search_event_order == "largest_first"),
-             "search_persistent": int(args.persistent)}
+             "search_persistent": int(args.persistent), "search_split": args.search_split}
     if args.search_block:
         knobs["search_block"] = args.search_block
     if args.search_lds_hits:
@@ -462,7 +476,9 @@ def main():
         "config": {"workload": args.workload, "events_per_step": n, "decoding": args.decoding,
                    "pv_finder": bool(args.pv_finder), "path": "plain",
                    "launch": "hip_graph" if args.graph else "eager", "streams": n_streams,
+                   "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"]),
                    "search_grid": "persistent" if args.persistent else "one_workgroup_per_event",
+                   "search_split": args.search_split,
                    **({"search_knobs": tuned} if tuned else {}),
                    "raw_bytes_per_step": int(ev.raw.size), **{k: v for k, v in wl.items() if k != "n_events"}},
         "status": status,

@@ -126,6 +126,9 @@ TUNINGS = {
     "persistent": {"search_persistent": 1},                       # resident grid, work stealing
     "persistent_block64": {"search_persistent": 1, "search_block": 64, "search_largest_first": 0},
     "clustering_resident": {"clustering_resident_grid": 1},       # clustering waves loop over items
+    "split": {"search_split": 32},                                # largest 1/32 of the events on a second stream
+    # half the events in the side stream's launch, small workgroups and LDS there (mixed layout)
+    "split_half": {"search_split": 2, "search_big_block": 128, "search_big_lds_hits": 600},
 }
 
 
