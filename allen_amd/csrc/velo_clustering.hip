@@ -1186,6 +1186,7 @@ struct ClusterSlices {
   uint32_t* lw;
   uint64_t* lkey;
   float* lxyz;
+  uint8_t* lcand; // 2 x wave_size: the candidate pixels of one word round, (lane | pixel << 7)
 };
 
 // (phi, id) as one unsigned 64-bit key: phi (signed) in the high word, offset to unsigned. Key
@@ -1302,18 +1303,36 @@ __device__ __forceinline__ void cluster_item(
     mp::for_words(B, n_words, resident, lw, (f & 3) != 0, B.start[bi], B.start[bi + 1], [&](uint32_t other) {
       window_acc<version>(sp_row_lower, sp_col_lower, sensor_number, other, win);
     });
-    ClusterOut c0, c1, t;
-    unsigned k = 0;
-    if ((f & 1) && masked_cluster_final(g, sp_word, sensor_number, (f >> 2) & 7u, win, &t)) {
-      c0 = t;
-      k = 1;
+    // The round's candidate pixels as one list in SP order, an SP's first pixel before its
+    // second; each lane grows one of them with its SP's window and word fetched from the SP's
+    // lane, and the kept clusters are appended in list order (the order of the two calls per SP
+    // in SP order). A round of 64 words has at most 128 candidates.
+    const unsigned nc = __popc(f & 3u);
+    const unsigned nc_inc = wave_inclusive_scan(nc);
+    const unsigned n_cand = __shfl(nc_inc, wave_size - 1, wave_size);
+    const unsigned first = nc_inc - nc;
+    mp::wave_sync_lds(); // the previous round's list has been read
+    if (f & 1u) sl.lcand[first] = static_cast<uint8_t>(lane_id());
+    if (f & 2u) sl.lcand[first + (f & 1u)] = static_cast<uint8_t>(lane_id() | 0x80u);
+    mp::wave_sync_lds();
+    for (unsigned c0 = 0; c0 < n_cand; c0 += wave_size) {
+      const unsigned j = c0 + lane_id();
+      const unsigned e = j < n_cand ? sl.lcand[j] : 0u;
+      const int src = static_cast<int>(e & 0x3Fu);
+      Window ws;
+      ws.pa0 = __shfl(win.pa0, src, wave_size);
+      ws.pa1 = __shfl(win.pa1, src, wave_size);
+      ws.pa2 = __shfl(win.pa2, src, wave_size);
+      ws.pa3 = __shfl(win.pa3, src, wave_size);
+      ws.pa4 = __shfl(win.pa4, src, wave_size);
+      const uint32_t word_s = __shfl(sp_word, src, wave_size);
+      const uint32_t sensor_s = __shfl(sensor_number, src, wave_size);
+      const uint32_t f_s = __shfl(f, src, wave_size);
+      const uint32_t location = (e & 0x80u) ? (f_s >> 5) & 7u : (f_s >> 2) & 7u;
+      ClusterOut c, unused;
+      const bool keep = j < n_cand && masked_cluster_final(g, word_s, sensor_s, location, ws, &c);
+      emit(keep ? 1u : 0u, c, unused);
     }
-    if ((f & 2) && masked_cluster_final(g, sp_word, sensor_number, (f >> 5) & 7u, win, &t)) {
-      if (k == 0) c0 = t;
-      else c1 = t;
-      ++k;
-    }
-    emit(k, c0, c1);
   }
   n = min(n, estimate);
   mp::wave_sync_lds();
@@ -1384,6 +1403,7 @@ __global__ void __launch_bounds__(mp::waves_per_block* wave_size, 6) k_cluster_m
   __shared__ uint32_t s_words[mp::waves_per_block][mp::word_cap];
   __shared__ uint64_t s_key[mp::waves_per_block][mp::cluster_cap];
   __shared__ float s_xyz[mp::waves_per_block][3 * mp::cluster_cap];
+  __shared__ uint8_t s_cand[mp::waves_per_block][2 * wave_size];
   __shared__ mp::Banks s_banks[mp::waves_per_block];
   const unsigned wave = threadIdx.x / wave_size;
   const unsigned n_items = n_events * n_module_pairs;
@@ -1391,7 +1411,7 @@ __global__ void __launch_bounds__(mp::waves_per_block* wave_size, 6) k_cluster_m
   unsigned idx = blockIdx.x * mp::waves_per_block + wave;
   if (idx >= n_items) return;
   const unsigned total = offsets_estimated_input_size[n_items];
-  const ClusterSlices sl {s_words[wave], s_key[wave], s_xyz[wave]};
+  const ClusterSlices sl {s_words[wave], s_key[wave], s_xyz[wave], s_cand[wave]};
   mp::DirFetch cur = mp::fetch_dir(directory, directory_count, idx);
   for (; idx < n_items; idx += stride) {
     mp::DirFetch next = cur;
