@@ -638,6 +638,8 @@ struct velo_sequence {
   std::map<std::string, std::pair<void*, size_t>> bufs;
   unsigned search_block = 256;
   unsigned search_lds_hits = 0; // 0: the launcher's default for the kernel variant
+  unsigned search_ring_window = 0; // LDS window of the ring of tracks to follow; 0: 512
+  unsigned search_list_cap = 0;    // seeding's list of unused hits; 0: 1024
   bool search_largest_first = true;
   bool search_persistent = false;
   // The largest events (E / search_split of them, events by descending hit count) run on a
@@ -868,6 +870,16 @@ int velo_sequence_set_tuning(velo_sequence* s, const char* knob, unsigned value)
   }
   if (k == "search_largest_first") {
     s->search_largest_first = value != 0;
+    return VELO_OK;
+  }
+  if (k == "search_ring_window") {
+    if (value && (value < 64 || value > 512 || (value & (value - 1)))) return VELO_ERR_INVALID_ARGUMENT;
+    s->search_ring_window = value;
+    return VELO_OK;
+  }
+  if (k == "search_list_cap") {
+    if (value % 16 || value > 1024) return VELO_ERR_INVALID_ARGUMENT;
+    s->search_list_cap = value;
     return VELO_OK;
   }
   if (k == "search_split") {
@@ -1143,6 +1155,8 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream
   k5.max_skipped_modules = in->max_skipped_modules;
   k5.block_dim = s->search_block;
   k5.lds_hits = s->search_lds_hits;
+  k5.ring_window = s->search_ring_window;
+  k5.list_cap = s->search_list_cap;
   k5.work_stats = s->count_work ? s->get<unsigned long long>("search_work") : nullptr;
   k5.event_clock = s->count_work ? s->get<unsigned long long>("search_event_clock") : nullptr;
   if (k5.work_stats) VELO_TRY(velo::launch_fill_u32(reinterpret_cast<uint32_t*>(k5.work_stats), 8, 0, stream));

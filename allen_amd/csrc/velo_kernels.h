@@ -243,6 +243,10 @@ struct SearchLaunch {
   // Scored mode: the scorer's FP32 parameters (mlp::n_params) and the logit threshold.
   const float* scorer_params = nullptr;
   float scorer_threshold = 0.f;
+  // LDS window of the ring of tracks to follow (a power of two, 64..512) and capacity of the
+  // seeding's list of unused hits (a multiple of 16, up to 1024); 0 = 512 and 1024.
+  unsigned ring_window = 0;
+  unsigned list_cap = 0;
 };
 
 struct ThreeHitFilterLaunch {
@@ -354,7 +358,7 @@ hipError_t launch_search_split(
 constexpr unsigned event_clock_words = 10; // start, end, ticks in 8 phases
 // Event hits staged in LDS by default: four workgroups of 256 lanes fit one CU's 160 KiB.
 constexpr unsigned search_default_lds_hits = 1760;
-size_t search_dynamic_lds(unsigned block, unsigned lds_hits);
+size_t search_dynamic_lds(unsigned block, unsigned lds_hits, unsigned window, unsigned list_cap);
 // order[0..n) = events by descending hit count (n read on the device).
 hipError_t launch_order_events_by_hits(
   const uint32_t* offsets_estimated_input_size,

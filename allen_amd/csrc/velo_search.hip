@@ -20,10 +20,11 @@ namespace velo {
 
 // LDS bytes per hit of the full layout: x, y, z (f32), phi (i16), flags (u8).
 constexpr unsigned search_bytes_per_hit = 15;
-// Capacity of the list of unused hits of the previous module pair used by seeding step 1; a
-// larger module pair takes the pendulum walk.
+// Default capacity of the list of unused hits of the previous module pair used by seeding step 1
+// (a launch parameter, at most 1024); a larger module pair takes the pendulum walk.
 constexpr unsigned unused_list_cap = 1024;
-// Tracks to follow kept in LDS: the last ring_window positions of the ring (written through to
+// Default number of tracks to follow kept in LDS (a launch parameter: a power of two from 64 to
+// 512): the last ring_window positions of the ring (written through to
 // the ring in HBM, which holds all max_tracks_to_follow).
 constexpr unsigned ring_window = 512;
 constexpr unsigned ring_window_bytes = 4 + 4 + 2 + 1; // entry, last two hits, first hit, number of hits
@@ -250,6 +251,7 @@ struct SeedScratch {
   uint8_t* n_h0;
   uint16_t* unused;   // unused hits of the previous module pair (relative indices, ascending)
   uint16_t* before;   // per position of the previous module pair: the unused hits before it
+  unsigned list_cap;  // entries of `unused` and `before`
 };
 
 struct ScorerLds;
@@ -259,7 +261,7 @@ struct EventOutputs {
   TrackletHits* three_hit_tracks;
   TrackletHits* tracklets;
   uint32_t* ttf;        // the ring of tracks to follow in HBM (max_tracks_to_follow entries)
-  uint32_t* ttf_window; // its last ring_window positions, in LDS (position p at p % ring_window)
+  uint32_t* ttf_window; // its last `window` positions, in LDS (position p at p & window_mask)
   // Beside each window entry, what forwarding reads of its track: the last two hits (low and
   // high 16 bits), the first hit (read for a three-hit seed) and the number of hits.
   uint32_t* ttf_last2;
@@ -271,6 +273,8 @@ struct EventOutputs {
   SeedLog seed_log;
   const ScorerLds* scorer;
   float scorer_threshold;
+  unsigned window;      // LDS window positions (a power of two dividing max_tracks_to_follow)
+  unsigned window_mask; // window - 1
 };
 
 // Four consecutive positions of an h2 window, read at once: position j of the window is hit
@@ -566,7 +570,7 @@ __device__ __forceinline__ void track_seeding(
   // before step 1 by the barrier of the compaction scan below).
   // The first blockDim hits of both lists are compacted by one scan (seeding marks no hit used,
   // so both read the flags the last forwarding left).
-  const bool use_list = prev.hit_num <= unused_list_cap;
+  const bool use_list = prev.hit_num <= ss.list_cap;
   const bool free0 = use_list && threadIdx.x < prev.hit_num && !hits.used(prev.hit_start + threadIdx.x);
   const bool act0 = threadIdx.x < cur.hit_num && !hits.used(cur.hit_start + threadIdx.x);
   const Scan3 first = scan3(sh, cn, free0, act0, false);
@@ -789,7 +793,7 @@ __device__ __forceinline__ void track_seeding(
       tl.hits[2] = best_h2;
       o.tracklets[tn] = tl;
       o.ttf[tn] = bit_seed | tn;
-      const unsigned wslot = tn % ring_window;
+      const unsigned wslot = tn & o.window_mask;
       o.ttf_window[wslot] = bit_seed | tn;
       o.ttf_last2[wslot] = static_cast<uint32_t>(h1_index) | (static_cast<uint32_t>(best_h2) << 16);
       o.ttf_first[wslot] = best_h0;
@@ -831,7 +835,7 @@ __device__ __forceinline__ void track_forwarding(
       const unsigned q = (prev_ttf + e) % max_tracks_to_follow;
       if (window) {
         // The entry and its track's last hits from LDS (written with the entry).
-        const unsigned wslot = q % ring_window;
+        const unsigned wslot = q & o.window_mask;
         full = o.ttf_window[wslot];
         const uint32_t last2 = o.ttf_last2[wslot];
         h0_num = last2 & 0xFFFFu;
@@ -951,7 +955,7 @@ __device__ __forceinline__ void track_forwarding(
         const unsigned p = (cn.ttf + s.r[1]) % max_tracks_to_follow;
         // A promoted track beyond the slots of its event is skipped when read back.
         o.ttf[p] = track_number;
-        const unsigned wslot = p % ring_window;
+        const unsigned wslot = p & o.window_mask;
         o.ttf_window[wslot] = track_number;
         if (best_h2 != -1) {
           o.ttf_last2[wslot] = h1_num | (static_cast<uint32_t>(best_h2) << 16);
@@ -1018,7 +1022,7 @@ __device__ __forceinline__ void search_event(
     // writing to their reading (this forwarding reinserts at most diff_ttf entries) span at most
     // 2 diff_ttf positions: within the LDS window they are all distinct. Beyond it the ring is
     // read back from HBM, after a fence that makes this workgroup's ring writes visible to it.
-    const bool window = 2 * diff_ttf <= ring_window;
+    const bool window = 2 * diff_ttf <= o.window;
     if (!window) __threadfence_block();
     // Orders the ring slots and tracklets written by the last seeding chunk before they are
     // read (seeding chunks end right after their writes, without a barrier).
@@ -1051,7 +1055,7 @@ __device__ __forceinline__ void search_event(
   last_ttf = cn.ttf;
   const unsigned diff_ttf = last_ttf - prev_ttf;
   if (diff_ttf >= max_tracks_to_follow) cn.status |= status_ttf_ring_overflow;
-  const bool window = diff_ttf <= ring_window; // nothing is written while these are read
+  const bool window = diff_ttf <= o.window; // nothing is written while these are read
   if (!window) {
     __threadfence_block();
     __syncthreads();
@@ -1062,7 +1066,7 @@ __device__ __forceinline__ void search_event(
     uint32_t full = 0;
     if (e < diff_ttf) {
       const unsigned q = (prev_ttf + e) % max_tracks_to_follow;
-      full = window ? o.ttf_window[q % ring_window] : o.ttf[q];
+      full = window ? o.ttf_window[q & o.window_mask] : o.ttf[q];
       seed = (full & bit_seed) == bit_seed;
     }
     const Scan3 s = scan3(sh, cn, seed, false, false);
@@ -1114,7 +1118,9 @@ __global__ void k_search_by_triplet(
   uint32_t* work_counter,
   SeedLog seed_log,
   const float* scorer_params,
-  float scorer_threshold)
+  float scorer_threshold,
+  unsigned window,
+  unsigned list_cap)
 {
   extern __shared__ __align__(16) char dyn_all[];
   __shared__ SearchShared sh;
@@ -1126,12 +1132,13 @@ __global__ void k_search_by_triplet(
   ss.h1_chunk = ss.h0 + max_h0_candidates * blockDim.x;
   ss.n_h0 = reinterpret_cast<uint8_t*>(ss.h1_chunk + blockDim.x);
   uint32_t* ring = reinterpret_cast<uint32_t*>(dyn_all + ((seed_scratch_bytes_per_lane * blockDim.x + 15) & ~15u));
-  uint32_t* ring_last2 = ring + ring_window;
-  uint16_t* ring_first = reinterpret_cast<uint16_t*>(ring_last2 + ring_window);
-  uint8_t* ring_n = reinterpret_cast<uint8_t*>(ring_first + ring_window);
-  ss.unused = reinterpret_cast<uint16_t*>(ring_n + ring_window);
-  ss.before = ss.unused + unused_list_cap;
-  char* dyn = reinterpret_cast<char*>(ss.before + unused_list_cap);
+  uint32_t* ring_last2 = ring + window;
+  uint16_t* ring_first = reinterpret_cast<uint16_t*>(ring_last2 + window);
+  uint8_t* ring_n = reinterpret_cast<uint8_t*>(ring_first + window);
+  ss.unused = reinterpret_cast<uint16_t*>(ring_n + window);
+  ss.before = ss.unused + list_cap;
+  ss.list_cap = list_cap;
+  char* dyn = reinterpret_cast<char*>(ss.before + list_cap);
   ScorerLds* scorer = nullptr;
   if constexpr (seed_mode == seed_mode_scored) {
     scorer = reinterpret_cast<ScorerLds*>(dyn + (((size_t) lds_hits * search_bytes_per_hit + 15) & ~size_t(15)));
@@ -1181,6 +1188,8 @@ __global__ void k_search_by_triplet(
     o.seed_log = seed_log;
     o.scorer = scorer;
     o.scorer_threshold = scorer_threshold;
+    o.window = window;
+    o.window_mask = window - 1;
     // Slots of this event; the reference lets a busy event run into its neighbour's slots, here
     // nothing is written past the container and the overflow is reported.
     o.track_capacity = min(next_off > toff ? next_off - toff : 0u, container - min(toff, container));
@@ -1424,11 +1433,10 @@ __global__ void k_three_hit_filter(
 }
 
 // Launch geometry: the seeding scratch, the ring window, the unused-hit list, then the hits.
-size_t search_dynamic_lds(unsigned block, unsigned lds_hits)
+size_t search_dynamic_lds(unsigned block, unsigned lds_hits, unsigned window, unsigned list_cap)
 {
-  return ((seed_scratch_bytes_per_lane * (size_t) block + 15) & ~size_t(15)) + ring_window_bytes * (size_t) ring_window +
-         2 * 2 * unused_list_cap +
-         (size_t) lds_hits * search_bytes_per_hit;
+  return ((seed_scratch_bytes_per_lane * (size_t) block + 15) & ~size_t(15)) + ring_window_bytes * (size_t) window +
+         2 * 2 * (size_t) list_cap + (size_t) lds_hits * search_bytes_per_hit;
 }
 
 hipError_t launch_search_by_triplet(const SearchLaunch& a, hipStream_t s)
@@ -1442,15 +1450,21 @@ hipError_t launch_search_by_triplet(const SearchLaunch& a, hipStream_t s)
   // 256-lane chunks); the scorer's weights take LDS from the hits (same residency).
   const bool scored = a.scorer_params != nullptr;
   if (scored && block != 256) return hipErrorInvalidConfiguration;
+  // The LDS window of the ring: a power of two from 64 to 512 (it divides the ring); the list
+  // of unused hits: a multiple of 16 up to 1024 (relative positions fit 16 bits either way).
+  const unsigned window = a.ring_window ? a.ring_window : ring_window;
+  const unsigned list_cap = a.list_cap ? a.list_cap : unused_list_cap;
+  if (window < 64 || window > ring_window || (window & (window - 1)) || list_cap % 16 || list_cap > unused_list_cap)
+    return hipErrorInvalidValue;
   const size_t extra = scored ? scorer_lds_bytes + 16 : 0;
-  const size_t fixed = search_dynamic_lds(block, 0) + extra + 1024;
+  const size_t fixed = search_dynamic_lds(block, 0, window, list_cap) + extra + 1024;
   if ((size_t) max_lds <= fixed) return hipErrorInvalidConfiguration;
   const unsigned max_hits = static_cast<unsigned>(((size_t) max_lds - fixed) / search_bytes_per_hit);
   const unsigned default_hits =
     scored ? search_default_lds_hits - static_cast<unsigned>((extra + search_bytes_per_hit - 1) / search_bytes_per_hit)
            : search_default_lds_hits;
   const unsigned lds_hits = min(a.lds_hits ? a.lds_hits : default_hits, max_hits);
-  const size_t dyn = search_dynamic_lds(block, lds_hits) + extra;
+  const size_t dyn = search_dynamic_lds(block, lds_hits, window, list_cap) + extra;
   // Persistent mode: as many workgroups as stay resident (they take events from the counter).
   unsigned grid = a.n_list;
   if (a.work_counter) {
@@ -1470,7 +1484,8 @@ hipError_t launch_search_by_triplet(const SearchLaunch& a, hipStream_t s)
     reinterpret_cast<TrackletHits*>(a.tracklets), a.tracks_to_follow,                                               \
     reinterpret_cast<TrackletHits*>(a.three_hit_tracks), a.hit_used, a.atomics_velo, a.rel_indices,                \
     a.number_of_velo_tracks, a.status, a.phi_tolerance, a.max_scatter, a.max_skipped_modules, lds_hits,             \
-    a.work_stats, a.event_clock, a.n_list, a.work_counter, a.seed_log, a.scorer_params, a.scorer_threshold)
+    a.work_stats, a.event_clock, a.n_list, a.work_counter, a.seed_log, a.scorer_params, a.scorer_threshold, window,   \
+    list_cap)
   if (a.work_stats) VELO_LAUNCH_SEARCH(true, seed_mode_plain);
   else if (a.scorer_params) VELO_LAUNCH_SEARCH(false, seed_mode_scored);
   else if (a.seed_log.counter) VELO_LAUNCH_SEARCH(false, seed_mode_log);

@@ -115,6 +115,10 @@ def main():
                     help="CPU work of the all-core baseline sample (the single-thread sample takes half)")
     ap.add_argument("--search-block", type=int, default=0)
     ap.add_argument("--search-lds-hits", type=int, default=0)
+    ap.add_argument("--search-ring-window", type=int, default=0,
+                    help="LDS window of the ring of tracks to follow (power of two, 64-512; 0: 512)")
+    ap.add_argument("--search-list-cap", type=int, default=0,
+                    help="capacity of the seeding's LDS list of unused hits (multiple of 16, up to 1024; 0: 1024)")
     ap.add_argument("--search-event-order", choices=["largest_first", "natural"], default="largest_first")
     ap.add_argument("--search-split", type=int, default=0,
                     help="the largest 1/N of a slice's events run in a second search launch, on a second stream of "
@@ -192,6 +196,10 @@ def main():
         knobs["search_block"] = args.search_block
     if args.search_lds_hits:
         knobs["search_lds_hits"] = args.search_lds_hits
+    if args.search_ring_window:
+        knobs["search_ring_window"] = args.search_ring_window
+    if args.search_list_cap:
+        knobs["search_list_cap"] = args.search_list_cap
     # One sequence per stream (slices in flight), configured alike; the first is also the one the
     # diagnostics below run on.
     n_streams = max(1, args.streams)
@@ -204,6 +212,11 @@ def main():
             sq.set_tuning("pv_finder", 1)
     stream = torch.cuda.current_stream()
     streams = [stream] if n_streams == 1 and not args.graph else [torch.cuda.Stream() for _ in range(n_streams)]
+    # Set-up: every sequence runs once on its own stream. The HIP runtime makes a stream's hardware
+    # queue at the stream's first launch (milliseconds), which must not fall into the timed steps.
+    for sq, st in zip(seqs, streams):
+        sq.run(dev, stream=st)
+    torch.cuda.synchronize()
 
     def timed(steps: int, warmup: int, stage: str | None, used: int = n_streams) -> float:
         """Milliseconds of `steps` steps dealt to the first `used` streams in turn (eager runs or
@@ -408,6 +421,8 @@ def main():
         for sq in seqs:
             sq.set_tuning("search_block", 256)
             sq.set_tuning("search_lds_hits", 0)
+            sq.set_tuning("search_ring_window", 0)
+            sq.set_tuning("search_list_cap", 0)
             sq.set_scorer(params, threshold)
         nn_ms = timed(args.steps, args.warmup, None) / args.steps
         seq.run(dev)

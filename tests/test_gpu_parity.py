@@ -127,6 +127,9 @@ TUNINGS = {
     "persistent_block64": {"search_persistent": 1, "search_block": 64, "search_largest_first": 0},
     "clustering_resident": {"clustering_resident_grid": 1},       # clustering waves loop over items
     "split": {"search_split": 32},                                # largest 1/32 of the events on a second stream
+    # the smallest LDS window of the ring and list of unused hits: forwarding reads the ring back
+    # from HBM and seeding walks the pendulum in module pairs of more than 64 hits
+    "small_window_list": {"search_ring_window": 64, "search_list_cap": 64},
     # half the events in the side stream's launch, small workgroups and LDS there (mixed layout)
     "split_half": {"search_split": 2, "search_big_block": 128, "search_big_lds_hits": 600},
 }
