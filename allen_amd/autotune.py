@@ -53,8 +53,8 @@ class Result:
 THROUGHPUT_CANDIDATES = tuple({"search_block": b, "search_lds_hits": h}
                               for b in (128, 192, 256) for h in (512, 640, 800, 960, 1200, 1760, 2048)) + tuple(
     # smaller LDS window of the ring and list of unused hits: 4.8 KB less per workgroup
-    {"search_block": 128, "search_lds_hits": h, "search_ring_window": 256, "search_list_cap": 512}
-    for h in (640, 800, 960, 1200))
+    {"search_block": 128, "search_lds_hits": h, "search_ring_window": w, "search_list_cap": c}
+    for h in (640, 800, 960, 1200) for w, c in ((256, 512), (128, 512), (256, 256)))
 
 
 def default_knobs() -> dict:
