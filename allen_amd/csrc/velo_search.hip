@@ -86,7 +86,16 @@ struct ModulePairData {
   unsigned hit_start;
   unsigned hit_num;
   float z[2];
+  unsigned table; // offset of the module pair's phi bucket table, or no_phi_table
 };
+
+// Phi bucket tables of an event's module pairs (LDS, u8 entries, module pairs of at most 255
+// hits): entry b < phi_buckets is the first position whose phi lies in bucket b or above (bucket
+// of an int16 phi: (phi + 32768) >> 10), entry phi_buckets the number of hits.
+constexpr unsigned phi_buckets = 64;
+constexpr unsigned phi_table_stride = phi_buckets + 1;
+constexpr unsigned phi_tables_bytes = (n_module_pairs * phi_table_stride + 15) & ~15u;
+constexpr unsigned no_phi_table = 0xFFFFFFFFu;
 
 // Hits of one event, structure-of-arrays in LDS.
 // Per-hit flags byte of the LDS layouts: bit 0 = used (dev_hit_used), bit 1 = module parity of
@@ -100,6 +109,7 @@ struct LdsHits {
   const float* zs;
   const int16_t* ph;
   uint8_t* flags;
+  const uint8_t* tables;
   __device__ __forceinline__ float x(unsigned i) const { return xs[i]; }
   __device__ __forceinline__ float y(unsigned i) const { return ys[i]; }
   __device__ __forceinline__ float z(unsigned i) const { return zs[i]; }
@@ -119,6 +129,7 @@ struct GlobalHits {
   const int16_t* ph;
   uint8_t* used_flags; // dev_hit_used
   const uint32_t* ids;
+  const uint8_t* tables;
   __device__ __forceinline__ float x(unsigned i) const { return xyz[3 * i]; }
   __device__ __forceinline__ float y(unsigned i) const { return xyz[3 * i + 1]; }
   __device__ __forceinline__ float z(unsigned i) const { return xyz[3 * i + 2]; }
@@ -198,12 +209,50 @@ __device__ __forceinline__ int counted_search(const T* array, unsigned size, T v
   return static_cast<int>(lo);
 }
 
+// The same lower bound in a module pair's phi array through its bucket table: positions before
+// the value's bucket hold smaller phis and positions after it larger ones, so the answer is the
+// bucket's start plus the bucket's elements below the value (read directly for up to four, by
+// the 8-way search for more). Module pairs without a table take the 8-way search.
+template<bool count, typename H>
+__device__ __forceinline__ int mp_search(const H& hits, const ModulePairData& mp, int16_t value, WorkCount& wc)
+{
+  const int16_t* a = hits.phi_begin() + mp.hit_start;
+  if (mp.table == no_phi_table) return counted_search<count>(a, mp.hit_num, value, wc);
+  const unsigned b = static_cast<unsigned>(static_cast<uint16_t>(value) ^ 0x8000u) >> 10;
+  const uint8_t* t = hits.tables + mp.table;
+  const unsigned lo = t[b], hi = t[b + 1];
+  if (hi - lo > 4) return static_cast<int>(lo) + counted_search<count>(a + lo, hi - lo, value, wc);
+  unsigned c = lo;
+#pragma unroll
+  for (unsigned k = 0; k < 4; ++k)
+    if (lo + k < hi && a[lo + k] < value) ++c;
+  return static_cast<int>(c);
+}
+
+// Builds the tables of every module pair with at most 255 hits (whole block; the module-pair
+// descriptors are in LDS and visible). Entry b: the lower bound of the bucket's first phi.
+template<typename H>
+__device__ __forceinline__ void build_phi_tables(const H& hits, const ModulePairData* mp, uint8_t* tables)
+{
+  WorkCount wc;
+  for (unsigned t = threadIdx.x; t < n_module_pairs * phi_table_stride; t += blockDim.x) {
+    const unsigned m = t / phi_table_stride, b = t % phi_table_stride;
+    if (mp[m].table == no_phi_table) continue;
+    const unsigned n = mp[m].hit_num;
+    tables[t] = static_cast<uint8_t>(
+      b == phi_buckets ? n
+                       : counted_search<false>(hits.phi_begin() + mp[m].hit_start, n,
+                                               static_cast<int16_t>(static_cast<int>(b << 10) - 32768), wc));
+  }
+}
+
 // Hits of one event: phi and used flags in LDS, coordinates in global memory (events too large
 // for the full LDS layout).
 struct MixedHits {
   const float* xyz;
   const int16_t* ph;
   uint8_t* flags;
+  const uint8_t* tables;
   __device__ __forceinline__ float x(unsigned i) const { return xyz[3 * i]; }
   __device__ __forceinline__ float y(unsigned i) const { return xyz[3 * i + 1]; }
   __device__ __forceinline__ float z(unsigned i) const { return xyz[3 * i + 2]; }
@@ -328,8 +377,7 @@ __device__ __forceinline__ void find_forward_candidate(
   const float yp = h0y + predy;
   const int16_t phi = hit_phi_16(xp, yp);
   if constexpr (count) ++wc.extrap;
-  candidate = static_cast<int16_t>(counted_search<count>(
-    hits.phi_begin() + mp.hit_start, mp.hit_num, static_cast<int16_t>(phi - phi_tolerance), wc));
+  candidate = static_cast<int16_t>(mp_search<count>(hits, mp, static_cast<int16_t>(phi - phi_tolerance), wc));
   extrapolated_phi = phi;
 }
 
@@ -620,7 +668,7 @@ __device__ __forceinline__ void track_seeding(
       if (rel < n_h1) {
         const unsigned h1_index = ss.h1_chunk[rel];
         const int16_t h1_phi = hits.phi(h1_index);
-        const int start = counted_search<count>(hits.phi_begin() + prev.hit_start, prev.hit_num, h1_phi, wc);
+        const int start = mp_search<count>(hits, prev, h1_phi, wc);
         phase_mark<count>(wc, phase_seed1_search);
         uint16_t* h0_out = ss.h0 + rel * max_h0_candidates;
         if (use_list) {
@@ -1084,7 +1132,8 @@ __device__ __forceinline__ void search_event(
 // (seed_scratch_bytes_per_lane per lane), the LDS window of the ring of tracks to follow
 // (ring_window entries; the ring itself, 2048 x u32, is written through to HBM), the list of
 // unused hits of the previous module pair and, per position, the number of unused hits before
-// it (1024 x u16 each), then `lds_hits` hits as x[], y[], z[] (f32), phi[] (i16)
+// it (list_cap x u16 each), the module pairs' phi bucket tables, then `lds_hits` hits as x[],
+// y[], z[] (f32), phi[] (i16)
 // and a flags byte (used, module parity). Events with up to 5 x lds_hits hits keep phi and the
 // flags in LDS and read coordinates from global memory; larger events run on global memory.
 // The `count` instantiation also accumulates the work counters into work_stats[4] (one
@@ -1138,7 +1187,8 @@ __global__ void k_search_by_triplet(
   ss.unused = reinterpret_cast<uint16_t*>(ring_n + window);
   ss.before = ss.unused + list_cap;
   ss.list_cap = list_cap;
-  char* dyn = reinterpret_cast<char*>(ss.before + list_cap);
+  uint8_t* phi_tables = reinterpret_cast<uint8_t*>(ss.before + list_cap);
+  char* dyn = reinterpret_cast<char*>(phi_tables + phi_tables_bytes);
   ScorerLds* scorer = nullptr;
   if constexpr (seed_mode == seed_mode_scored) {
     scorer = reinterpret_cast<ScorerLds*>(dyn + (((size_t) lds_hits * search_bytes_per_hit + 15) & ~size_t(15)));
@@ -1202,6 +1252,7 @@ __global__ void k_search_by_triplet(
       sh.mp[m].hit_num = mhn[m];
       sh.mp[m].z[0] = geometry->module_zs[2 * m];
       sh.mp[m].z[1] = geometry->module_zs[2 * m + 1];
+      sh.mp[m].table = mhn[m] <= 255u ? m * phi_table_stride : no_phi_table;
     }
     const uint32_t* gids = cc.ids() + hit_offset;
     const int16_t* gph = cc.phis() + hit_offset;
@@ -1221,7 +1272,9 @@ __global__ void k_search_by_triplet(
         flags[i] = module_parity(gids[i]) ? flag_odd : 0;
       }
       __syncthreads();
-      const LdsHits hits {xs, ys, zs, ph, flags};
+      const LdsHits hits {xs, ys, zs, ph, flags, phi_tables};
+      build_phi_tables(hits, sh.mp, phi_tables);
+      __syncthreads();
       search_event<count, seed_mode>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
       __syncthreads();
       for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) gused[i] = flags[i] & flag_used;
@@ -1235,7 +1288,9 @@ __global__ void k_search_by_triplet(
       }
       cn.status |= status_lds_fallback;
       __syncthreads();
-      const MixedHits hits {gxyz, ph, flags};
+      const MixedHits hits {gxyz, ph, flags, phi_tables};
+      build_phi_tables(hits, sh.mp, phi_tables);
+      __syncthreads();
       search_event<count, seed_mode>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
       __syncthreads();
       for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) gused[i] = flags[i] & flag_used;
@@ -1245,7 +1300,9 @@ __global__ void k_search_by_triplet(
       cn.status |= status_lds_fallback;
       __threadfence_block();
       __syncthreads();
-      const GlobalHits hits {gxyz, gph, gused, gids};
+      const GlobalHits hits {gxyz, gph, gused, gids, phi_tables};
+      build_phi_tables(hits, sh.mp, phi_tables);
+      __syncthreads();
       search_event<count, seed_mode>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
       __syncthreads();
     }
@@ -1436,7 +1493,7 @@ __global__ void k_three_hit_filter(
 size_t search_dynamic_lds(unsigned block, unsigned lds_hits, unsigned window, unsigned list_cap)
 {
   return ((seed_scratch_bytes_per_lane * (size_t) block + 15) & ~size_t(15)) + ring_window_bytes * (size_t) window +
-         2 * 2 * (size_t) list_cap + (size_t) lds_hits * search_bytes_per_hit;
+         2 * 2 * (size_t) list_cap + phi_tables_bytes + (size_t) lds_hits * search_bytes_per_hit;
 }
 
 hipError_t launch_search_by_triplet(const SearchLaunch& a, hipStream_t s)
