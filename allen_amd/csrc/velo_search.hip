@@ -534,8 +534,9 @@ seed_features(const H& hits, unsigned h0, unsigned h1, unsigned h2, unsigned mod
 // start), else 2 ((start - x) mod n) (left, start itself included); ranks are a permutation of
 // 0..n-1, and the walk keeps the unused positions of the K smallest ranks, in rank order. Those
 // are among the K nearest unused entries on either side of start in the circular list, so the
-// 2K + 1 entries around its lower bound (all u if u <= 2K + 1) are ranked and the K smallest
-// taken. Writes hit_start + position to out[0..) and returns how many (<= K).
+// 2K entries around its lower bound (all u if u <= 2K) are ranked, sorted by a fixed Batcher
+// odd-even merge network on (rank, position) keys, and the first K taken. Writes hit_start +
+// position to out[0..) and returns how many (<= K).
 template<unsigned K>
 __device__ __forceinline__ unsigned h0_candidates_from_list(
   const uint16_t* unused,
@@ -551,7 +552,7 @@ __device__ __forceinline__ unsigned h0_candidates_from_list(
   // The lower bound of s0 in the list: the number of unused positions below s0.
   const unsigned p = before[s0];
   // key = rank << 16 | position (n <= 1024 on this path: rank < 2048, position < 1024).
-  constexpr unsigned M = 2 * K + 1;
+  constexpr unsigned M = 2 * K;
   constexpr unsigned none = 0xFFFFFFFFu;
   const bool all = u <= M;
   unsigned key[M];
@@ -570,16 +571,32 @@ __device__ __forceinline__ unsigned h0_candidates_from_list(
     const unsigned r = d == 0 ? 0u : (d <= n / 2 ? 2 * d - 1 : 2 * (n - d));
     key[j] = (!all || j < u) ? (r << 16 | x) : none;
   }
+  // Batcher's odd-even merge sort of the M keys (compile-time indices: a fixed network of
+  // min/max pairs on registers).
+#pragma unroll
+  for (unsigned pp = 1; pp < M; pp <<= 1) {
+#pragma unroll
+    for (unsigned k = pp; k >= 1; k >>= 1) {
+#pragma unroll
+      for (unsigned j = k % pp; j + k < M; j += 2 * k) {
+#pragma unroll
+        for (unsigned i = 0; i < k && i + j + k < M; ++i) {
+          if ((i + j) / (2 * pp) == (i + j + k) / (2 * pp)) {
+            const unsigned a = key[i + j], b = key[i + j + k];
+            key[i + j] = min(a, b);
+            key[i + j + k] = max(a, b);
+          }
+        }
+      }
+    }
+  }
   unsigned found = 0;
+#pragma unroll
   for (unsigned k = 0; k < K; ++k) {
-    unsigned best = none;
-#pragma unroll
-    for (unsigned j = 0; j < M; ++j) best = min(best, key[j]);
-    if (best == none) break;
-#pragma unroll
-    for (unsigned j = 0; j < M; ++j)
-      if (key[j] == best) key[j] = none;
-    out[found++] = static_cast<uint16_t>(hit_start + (best & 0xFFFFu));
+    if (key[k] != none) {
+      out[k] = static_cast<uint16_t>(hit_start + (key[k] & 0xFFFFu));
+      found = k + 1;
+    }
   }
   return found;
 }
