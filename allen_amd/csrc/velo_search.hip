@@ -229,22 +229,32 @@ __device__ __forceinline__ int mp_search(const H& hits, const ModulePairData& mp
   return static_cast<int>(c);
 }
 
-// Builds the tables of every module pair with at most 255 hits (whole block; the module-pair
-// descriptors are in LDS and visible). Entry b: the lower bound of the bucket's first phi.
+// Entry b of module pair mp's table: the lower bound of bucket b's first phi (b < phi_buckets),
+// or the number of hits.
 template<typename H>
-__device__ __forceinline__ void build_phi_tables(const H& hits, const ModulePairData* mp, uint8_t* tables)
+__device__ __forceinline__ uint8_t phi_table_entry(const H& hits, const ModulePairData& mp, unsigned b)
 {
   WorkCount wc;
-  for (unsigned t = threadIdx.x; t < n_module_pairs * phi_table_stride; t += blockDim.x) {
-    const unsigned m = t / phi_table_stride, b = t % phi_table_stride;
-    if (mp[m].table == no_phi_table) continue;
-    const unsigned n = mp[m].hit_num;
-    tables[t] = static_cast<uint8_t>(
-      b == phi_buckets ? n
-                       : counted_search<false>(hits.phi_begin() + mp[m].hit_start, n,
-                                               static_cast<int16_t>(static_cast<int>(b << 10) - 32768), wc));
+  return static_cast<uint8_t>(
+    b == phi_buckets ? mp.hit_num
+                     : counted_search<false>(hits.phi_begin() + mp.hit_start, mp.hit_num,
+                                             static_cast<int16_t>(static_cast<int>(b << 10) - 32768), wc));
+}
+
+// Builds the tables of module pairs [m0, n_module_pairs) that have one (whole block; the
+// module-pair descriptors are in LDS and visible).
+template<typename H>
+__device__ __forceinline__ void build_phi_tables(const H& hits, const ModulePairData* mp, uint8_t* tables, unsigned m0)
+{
+  for (unsigned t = m0 * phi_table_stride + threadIdx.x; t < n_module_pairs * phi_table_stride; t += blockDim.x) {
+    const unsigned m = t / phi_table_stride;
+    if (mp[m].table != no_phi_table) tables[t] = phi_table_entry(hits, mp[m], t % phi_table_stride);
   }
 }
+
+// Module pairs whose tables are built at the start of an event; every other table is built
+// during an earlier seeding (see track_seeding's `build`), before its first use.
+constexpr unsigned phi_tables_at_start = 4;
 
 // Hits of one event: phi and used flags in LDS, coordinates in global memory (events too large
 // for the full LDS layout).
@@ -624,7 +634,9 @@ __device__ __forceinline__ void track_seeding(
   unsigned module_pair,
   float max_scatter,
   int16_t phi_tolerance,
-  unsigned h0_candidates_to_consider)
+  unsigned h0_candidates_to_consider,
+  const ModulePairData* build,
+  uint8_t* tables)
 {
   const unsigned K = h0_candidates_to_consider;
   const unsigned log2K = K == 8 ? 3u : 2u; // K is 4 or 8
@@ -653,6 +665,21 @@ __device__ __forceinline__ void track_seeding(
     }
   }
 
+  // The phi table of module pair `build` (first searched in the forwarding after the next
+  // seeding, so any later barrier orders it), by the block's last wave: during step 1 of the
+  // first chunk, where that wave is idle when the chunk has no more h1 hits than the other waves
+  // have lanes, or here when the module pair has no hit to seed from.
+  const auto build_table = [&]() {
+    if (!build || build->table == no_phi_table) return;
+    const unsigned builders = blockDim.x > wave_size ? blockDim.x - wave_size : 0u;
+    if (threadIdx.x >= builders) {
+      const unsigned b = threadIdx.x - builders;
+      tables[build->table + b] = phi_table_entry(hits, *build, b);
+      if (b == 0) tables[build->table + phi_buckets] = static_cast<uint8_t>(build->hit_num);
+    }
+  };
+  if (cur.hit_num == 0) build_table();
+
   // Unused hits of the current module pair, compacted in index order one chunk of blockDim
   // hits at a time (dev_rel_indices in global memory, the chunk's list in LDS); the seeds of a
   // chunk are made before the next chunk, which keeps the h1 order of the reference.
@@ -678,6 +705,7 @@ __device__ __forceinline__ void track_seeding(
     __syncthreads();
     phase_mark<count>(wc, phase_compaction);
 
+    if (c0 == 0) build_table();
     // 1. h0 candidates of every h1 of the chunk (lane t: the chunk's h1 number t), pendulum order.
     {
       const unsigned rel = threadIdx.x;
@@ -1064,7 +1092,8 @@ __device__ __forceinline__ void search_event(
   const EventOutputs& o,
   float phi_tolerance_f,
   float max_scatter,
-  unsigned max_skipped_modules)
+  unsigned max_skipped_modules,
+  uint8_t* tables)
 {
   // Module pairs are visited from the last one upstream: seeding on (first, first - 1,
   // first - 2), forwarding into first - 2 (SearchByTriplet.cu:160-236).
@@ -1072,10 +1101,15 @@ __device__ __forceinline__ void search_event(
   const int16_t phi_tolerance = f32_to_i16(phi_tolerance_f * convert_factor_i16);
   unsigned first_module_pair = n_module_pairs - 1;
   unsigned last_ttf = 0;
+  // Seeding with first = F searches the tables of F and F - 2, the forwarding after it that of
+  // F - 3; seeding F builds the table of F - 4, so the first four are built at the event's start.
+  static_assert(phi_tables_at_start == 4 && phi_buckets == wave_size, "table schedule");
   const auto seed = [&](unsigned h0_candidates) {
     track_seeding<count, seed_mode>(
       wc, hits, ss, sh, cn, o, sh.mp[first_module_pair], sh.mp[first_module_pair - 1], sh.mp[first_module_pair - 2],
-      first_module_pair - 1, max_scatter, phi_tolerance, h0_candidates);
+      first_module_pair - 1, max_scatter, phi_tolerance, h0_candidates,
+      first_module_pair >= phi_tables_at_start ? &sh.mp[first_module_pair - phi_tables_at_start] : nullptr,
+      tables);
     --first_module_pair;
   };
   const auto forward = [&]() {
@@ -1290,9 +1324,9 @@ __global__ void k_search_by_triplet(
       }
       __syncthreads();
       const LdsHits hits {xs, ys, zs, ph, flags, phi_tables};
-      build_phi_tables(hits, sh.mp, phi_tables);
+      build_phi_tables(hits, sh.mp, phi_tables, n_module_pairs - phi_tables_at_start);
       __syncthreads();
-      search_event<count, seed_mode>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
+      search_event<count, seed_mode>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules, phi_tables);
       __syncthreads();
       for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) gused[i] = flags[i] & flag_used;
     }
@@ -1306,9 +1340,9 @@ __global__ void k_search_by_triplet(
       cn.status |= status_lds_fallback;
       __syncthreads();
       const MixedHits hits {gxyz, ph, flags, phi_tables};
-      build_phi_tables(hits, sh.mp, phi_tables);
+      build_phi_tables(hits, sh.mp, phi_tables, n_module_pairs - phi_tables_at_start);
       __syncthreads();
-      search_event<count, seed_mode>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
+      search_event<count, seed_mode>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules, phi_tables);
       __syncthreads();
       for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) gused[i] = flags[i] & flag_used;
     }
@@ -1318,9 +1352,9 @@ __global__ void k_search_by_triplet(
       __threadfence_block();
       __syncthreads();
       const GlobalHits hits {gxyz, gph, gused, gids, phi_tables};
-      build_phi_tables(hits, sh.mp, phi_tables);
+      build_phi_tables(hits, sh.mp, phi_tables, n_module_pairs - phi_tables_at_start);
       __syncthreads();
-      search_event<count, seed_mode>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules);
+      search_event<count, seed_mode>(wc, hits, ss, sh, cn, o, phi_tolerance, max_scatter, max_skipped_modules, phi_tables);
       __syncthreads();
     }
     if constexpr (count) {
