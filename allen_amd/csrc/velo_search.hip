@@ -1554,10 +1554,12 @@ hipError_t launch_search_by_triplet(const SearchLaunch& a, hipStream_t s)
   // Dynamic LDS stays within the device's per-workgroup limit (less the static part).
   const int max_lds = device_max_lds_per_block();
   if (!max_lds) return hipErrorInvalidDevice;
-  // Scored mode: seeds are decided in groups of 256 / K h1 hits (the pruning rule is stated for
-  // 256-lane chunks); the scorer's weights take LDS from the hits (same residency).
+  // Scored mode: seeds are decided in groups of block / K unused h1 hits of each chunk of block
+  // positions (the pruning rule with seed_block = block), and a group's seeds must lie in one
+  // wave for the in-wave scorer and decision: workgroups of 64, 128 or 256 lanes. The scorer's
+  // weights take LDS from the hits (same residency).
   const bool scored = a.scorer_params != nullptr;
-  if (scored && block != 256) return hipErrorInvalidConfiguration;
+  if (scored && block != 64 && block != 128 && block != 256) return hipErrorInvalidConfiguration;
   // The LDS window of the ring: a power of two from 64 to 512 (it divides the ring); the list
   // of unused hits: a multiple of 16 up to 1024 (relative positions fit 16 bits either way).
   const unsigned window = a.ring_window ? a.ring_window : ring_window;

@@ -408,8 +408,9 @@ def main():
     if dominant != "search":
         extra["search_alu_lds"] = search_alu_lds
 
-    # The NN-pruned leg: the same events and streams with the learned seed scorer pruning seeds
-    # (its in-search products need 256-lane workgroups), timed the same way.
+    # The NN-pruned leg: the same events, streams and search knobs with the learned seed scorer
+    # pruning seeds (its groups of seeds must lie in one wave: workgroups of 64, 128 or 256
+    # lanes, else 256), timed the same way.
     nn = None
     if args.nn_pruned:
         from allen_amd import scorer
@@ -418,18 +419,20 @@ def main():
         else:
             params, shipped_threshold = scorer.load_params(args.scorer), 0.0
         threshold = shipped_threshold if args.threshold is None else args.threshold
+        nn_knobs = {k: v for k, v in knobs.items() if k in ("search_block", "search_lds_hits", "search_ring_window",
+                                                             "search_list_cap")}
+        if nn_knobs.get("search_block", 256) not in (64, 128, 256):
+            nn_knobs = {"search_block": 256, "search_lds_hits": 0, "search_ring_window": 0, "search_list_cap": 0}
         for sq in seqs:
-            sq.set_tuning("search_block", 256)
-            sq.set_tuning("search_lds_hits", 0)
-            sq.set_tuning("search_ring_window", 0)
-            sq.set_tuning("search_list_cap", 0)
+            for k, v in nn_knobs.items():
+                sq.set_tuning(k, v)
             sq.set_scorer(params, threshold)
         nn_ms = timed(args.steps, args.warmup, None) / args.steps
         seq.run(dev)
         nn_physics = physics_of(seq, ev)
         nn = {"events_per_s": round(n / (nn_ms / 1e3), 1), "ms_per_step": round(nn_ms, 4),
               "speedup_vs_plain": round(ms_per_step / nn_ms, 4), "threshold": threshold,
-              "scorer": os.path.basename(args.scorer), "search_knobs": {"search_block": 256},
+              "scorer": os.path.basename(args.scorer), "search_knobs": nn_knobs,
               "physics": nn_physics,
               "long_efficiency_loss": round(physics["02_long_efficiency"] - nn_physics["02_long_efficiency"], 6),
               "ghost_rate_change": round(nn_physics["ghost_rate"] - physics["ghost_rate"], 6)}

@@ -86,10 +86,12 @@ def test_seed_log_matches_oracle(geometry, beamline):
     constants.close()
 
 
-def _scored_run(geometry, beamline, ev, params, threshold, log_cap=0):
+def _scored_run(geometry, beamline, ev, params, threshold, log_cap=0, block=0):
     from allen_amd import velo
     constants = velo.VeloConstants(geometry, beamline)
     seq = velo.VeloSequence(constants, ev.n_events, ev.raw.size)
+    if block:
+        seq.set_tuning("search_block", block)
     if log_cap:
         seq.set_tuning("seed_log", log_cap)
     seq.set_scorer(params, threshold)
@@ -209,12 +211,12 @@ def _logits_by_slot(seq, ref, n_events):
     return out
 
 
-@pytest.mark.parametrize("n_events,seed", [(300, 81), (120, 82)])
-def test_pruned_search_matches_oracle_at_shipped_threshold(n_events, seed, geometry, beamline):
+@pytest.mark.parametrize("n_events,seed,block", [(300, 81, 256), (120, 82, 256), (300, 83, 128), (120, 84, 64)])
+def test_pruned_search_matches_oracle_at_shipped_threshold(n_events, seed, block, geometry, beamline):
     """Config-4 events (one vertex, Poisson(120) tracks) through the NN-pruned search at the shipped
     scorer's threshold: with the logits the search computed (logged beside it), the oracle's pruning
-    rule (groups of 256 / K unused h1 hits per 256 positions, keep-all fallback) gives every
-    buffer of the sequence bit for bit."""
+    rule (groups of block / K unused h1 hits per `block` positions, keep-all fallback) gives every
+    buffer of the sequence bit for bit, for workgroups of 256, 128 and 64 lanes."""
     import oracle
     from allen_amd import scorer, synth, velo
     from tests import parity
@@ -222,10 +224,10 @@ def test_pruned_search_matches_oracle_at_shipped_threshold(n_events, seed, geome
     ev = synth.generate(n_events=n_events, pileup=0, mean_tracks=120.0, seed=seed)
     plain = oracle.run(ev, geometry, beamline, compute_f64=0)
     n_seeds = int(plain["atomics_velo"][1::3].sum())
-    seq, constants = _scored_run(geometry, beamline, ev, params, thr, log_cap=2 * n_seeds + 1000)
+    seq, constants = _scored_run(geometry, beamline, ev, params, thr, log_cap=2 * n_seeds + 1000, block=block)
     assert seq.status() & velo.STATUS_OVERFLOW_MASK == 0
     logits = _logits_by_slot(seq, plain, ev.n_events)
-    ref = oracle.run(ev, geometry, beamline, seed_logits=logits, seed_threshold=thr, seed_block=256)
+    ref = oracle.run(ev, geometry, beamline, seed_logits=logits, seed_threshold=thr, seed_block=block)
     assert int(ref["status"][0]) == 0, f"oracle status {int(ref['status'][0]):#x}"
     pruned = int(ref["atomics_velo"][1::3].sum())
     print(f"[pruned] {n_events} events: {n_seeds} plain seeds, {pruned} kept at threshold {thr}")
