@@ -1289,10 +1289,6 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream
 int velo_sequence_capture(velo_sequence* s, const velo_sequence_input* in, hipStream_t stream)
 {
   if (!s || !in || !stream) return VELO_ERR_INVALID_ARGUMENT;
-  // The persistent search inside a captured graph stopped answering in its first run on the GPU
-  // and its cause is not known, so that combination is refused; the graph captures the
-  // one-workgroup-per-event search.
-  if (s->search_persistent) return VELO_ERR_INVALID_ARGUMENT;
   s->release_graph();
   // Stage timing records events on the host's schedule; it is off inside a captured run.
   const unsigned timed = s->timed_stage;

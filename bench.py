@@ -154,10 +154,6 @@ def main():
                          "scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
 
-    if args.graph and args.persistent:
-        print("bench.py: --graph and --persistent cannot be combined (velo_sequence_capture refuses the persistent "
-              "search)", file=sys.stderr)
-        sys.exit(2)
     if not 1 <= args.hw_queues <= 32:
         print(f"bench.py: --hw-queues {args.hw_queues} is outside 1..32", file=sys.stderr)
         sys.exit(2)

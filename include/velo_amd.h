@@ -509,9 +509,7 @@ int velo_sequence_stage_ms(velo_sequence* s, float* total_ms, unsigned* runs);
 /* Captures one velo_sequence_run with these inputs on `stream` (not the null stream) into a
  * HIP graph held by the sequence, replacing any earlier one; velo_sequence_replay launches it.
  * The graph keeps the input pointers, event count and knobs of the capture; every size inside
- * the run is read on the device, and buffers are cleared by kernels, not memset nodes.
- * VELO_ERR_INVALID_ARGUMENT while the search_persistent knob is set (the persistent search is not
- * captured). */
+ * the run is read on the device, and buffers are cleared by kernels, not memset nodes. */
 int velo_sequence_capture(velo_sequence* s, const velo_sequence_input* in, hipStream_t stream);
 int velo_sequence_replay(velo_sequence* s, hipStream_t stream);
 /* NN-pruned seeding: with params (VELO_MLP_PARAMS host floats, copied) search_by_triplet scores
