@@ -219,6 +219,25 @@ window_acc(int32_t sp_row_lower, int32_t sp_col_lower, uint32_t sensor_number, u
   }
 }
 
+// window_acc() for a word already known to be non-isolated and on the candidate's sensor (the
+// sorted lists of k_cluster_mp hold only such words in a candidate's range). Bits 24-31 of
+// `other` are not read.
+__device__ __forceinline__ void window_acc_same_sensor(int32_t sp_row_lower, int32_t sp_col_lower, uint32_t other, Window& w)
+{
+  const uint32_t oaddr = (other & 0x007FFF00u) >> 8;
+  const int32_t rr = static_cast<int32_t>(oaddr & 0x3Fu) - sp_row_lower;
+  const int32_t rc = static_cast<int32_t>(oaddr >> 6) - sp_col_lower;
+  if (rr >= 0 && rr <= 3 && rc >= 0 && rc <= 4) {
+    const uint32_t osp = other & 0xFFu;
+    const uint32_t bits = ((osp & 0x0Fu) << (4 * rr)) | ((osp & 0xF0u) << (12 + 4 * rr));
+    w.pa0 |= rc == 0 ? bits : 0u;
+    w.pa1 |= rc == 1 ? bits : 0u;
+    w.pa2 |= rc == 2 ? bits : 0u;
+    w.pa3 |= rc == 3 ? bits : 0u;
+    w.pa4 |= rc == 4 ? bits : 0u;
+  }
+}
+
 __device__ __forceinline__ bool masked_cluster_final(
   const Geometry& g,
   uint32_t sp_word,
@@ -928,6 +947,7 @@ struct Banks {
   uint32_t start[max_banks_per_mp + 1];
   uint32_t source[max_banks_per_mp];
   const uint32_t* word[max_banks_per_mp];
+  uint32_t sorted[max_banks_per_mp + 1]; // k_cluster_mp: each bank's first entry in the sorted list
 };
 
 __device__ __forceinline__ void wave_sync_lds()
@@ -972,16 +992,17 @@ __device__ __forceinline__ void load_banks(const char* raw, const DirFetch& f, B
   wave_sync_lds();
 }
 
-// The bank holding word w: the starts are non-decreasing, so the last bank starting at or before
-// w is the number of banks after the first that do (all starts read at once, one LDS round trip).
-__device__ __forceinline__ unsigned bank_of(const Banks& B, unsigned w)
+// The bank holding position w of a list laid out bank after bank (starts[k]: bank k's first
+// position): the starts are non-decreasing, so the last bank starting at or before w is the
+// number of banks after the first that do (all starts read at once, one LDS round trip).
+__device__ __forceinline__ unsigned bank_at(const uint32_t* starts, unsigned n, unsigned w)
 {
-  const unsigned n = B.n;
   unsigned bi = 0;
 #pragma unroll
-  for (unsigned k = 1; k < max_banks_per_mp; ++k) bi += (k < n && B.start[k] <= w) ? 1u : 0u;
+  for (unsigned k = 1; k < max_banks_per_mp; ++k) bi += (k < n && starts[k] <= w) ? 1u : 0u;
   return bi;
 }
+__device__ __forceinline__ unsigned bank_of(const Banks& B, unsigned w) { return bank_at(B.start, B.n, w); }
 } // namespace mp
 
 namespace mp {
@@ -1266,15 +1287,18 @@ __device__ __forceinline__ void cluster_item(
     }
     n += __shfl(inc, wave_size - 1, wave_size);
   };
-  // Isolated SPs first, then the candidates, each in SP order: the CPU target's order
-  // (MaskedVeloClustering.cu:544-575).
+  // Isolated SPs first (in SP order), then the candidates. The order in which clusters are
+  // appended does not reach the output: the module pair's clusters are ranked by their distinct
+  // (phi, id) keys below.
+  uint32_t bank_nonisolated = 0; // resident: lane k counts bank k's non-isolated words
   for (unsigned base = 0; base < n_words; base += wave_size) {
     const unsigned w = base + lane_id();
     ClusterOut c0, c1;
     unsigned k = 0;
+    uint32_t sp_word = 0x80000000u;
     if (w < n_words) {
       const unsigned bi = mp::bank_of(B, w);
-      const uint32_t sp_word = resident ? lw[w] : B.word[bi][w - B.start[bi]];
+      sp_word = resident ? lw[w] : B.word[bi][w - B.start[bi]];
       if (sp_word & 0x80000000u) {
         RawBank bank;
         bank.sourceID = B.source[bi];
@@ -1282,31 +1306,22 @@ __device__ __forceinline__ void cluster_item(
       }
     }
     emit(k, c0, c1);
-  }
-  for (unsigned base = 0; base < n_words; base += wave_size) {
-    const unsigned w = base + lane_id();
-    const bool active = w < n_words;
-    const unsigned bi = active ? mp::bank_of(B, w) : 0;
-    const uint32_t sp_word = active ? (resident ? lw[w] : B.word[bi][w - B.start[bi]]) : 0x80000000u;
-    uint32_t f = 0;
-    if (active && !(sp_word & 0x80000000u)) {
-      const uint32_t* wp = B.word[bi] + (w - B.start[bi]);
-      f = sp_flags[static_cast<size_t>(reinterpret_cast<const char*>(wp) - raw) / 4];
+    if (resident) {
+      const uint64_t m = __ballot(!(sp_word & 0x80000000u));
+      const unsigned l = lane_id();
+      if (l < B.n) {
+        auto below = [](unsigned x) { return x >= wave_size ? ~0ull : ((1ull << x) - 1); };
+        const unsigned lo = B.start[l] <= base ? 0u : min(B.start[l] - base, wave_size);
+        const unsigned hi = B.start[l + 1] <= base ? 0u : min(B.start[l + 1] - base, wave_size);
+        bank_nonisolated += __popcll(m & below(hi)) - __popcll(m & below(lo));
+      }
     }
-    const uint32_t pair = B.source[bi] & 0x1FFu;
-    const uint32_t sensor_number = v23 ? pair : ((pair << 1) | ((sp_word >> 23) & 0x1u));
-    const uint32_t sp_addr = (sp_word & 0x007FFF00u) >> 8;
-    const int32_t sp_row_lower = static_cast<int32_t>(sp_addr & 0x3Fu) - 2;
-    const int32_t sp_col_lower = static_cast<int32_t>(sp_addr >> 6) - 3;
-    Window win {0, 0, 0, 0, 0};
-    // Both candidates of an SP share its window.
-    mp::for_words(B, n_words, resident, lw, (f & 3) != 0, B.start[bi], B.start[bi + 1], [&](uint32_t other) {
-      window_acc<version>(sp_row_lower, sp_col_lower, sensor_number, other, win);
-    });
-    // The round's candidate pixels as one list in SP order, an SP's first pixel before its
-    // second; each lane grows one of them with its SP's window and word fetched from the SP's
-    // lane, and the kept clusters are appended in list order (the order of the two calls per SP
-    // in SP order). A round of 64 words has at most 128 candidates.
+  }
+  // The candidates of one round of (at most 64) SPs: their pixels as one list, an SP's first
+  // pixel before its second; each lane grows one of them with its SP's window and word fetched
+  // from the SP's lane, and the kept clusters are appended in list order. A round has at most 128
+  // candidates.
+  auto candidates_round = [&](uint32_t sp_word, uint32_t sensor_number, uint32_t f, const Window& win) {
     const unsigned nc = __popc(f & 3u);
     const unsigned nc_inc = wave_inclusive_scan(nc);
     const unsigned n_cand = __shfl(nc_inc, wave_size - 1, wave_size);
@@ -1332,6 +1347,117 @@ __device__ __forceinline__ void cluster_item(
       ClusterOut c, unused;
       const bool keep = j < n_cand && masked_cluster_final(g, word_s, sensor_s, location, ws, &c);
       emit(keep ? 1u : 0u, c, unused);
+    }
+  };
+  if (resident) {
+    // The non-isolated words, sorted per bank by (sensor, SP column, SP row, position) into the
+    // front of the slice, each with its candidate byte in bits 24-31 (v2/v3: the bank is the
+    // sensor, bit 23 cleared). A candidate's window then takes only the words of its bank and
+    // sensor in SP columns col - 3 .. col + 1, a contiguous run found by binary search, instead
+    // of every word of the bank. OR-ing the window is order-free, so the windows are the scan's.
+    {
+      const unsigned l = lane_id();
+      const uint32_t cnt = l < B.n ? bank_nonisolated : 0u;
+      const uint32_t inc = wave_inclusive_scan(cnt);
+      if (l <= B.n) B.sorted[l] = inc - cnt;
+      mp::wave_sync_lds();
+    }
+    constexpr unsigned max_rounds = mp::word_cap / wave_size;
+    static_assert(mp::word_cap % wave_size == 0, "whole rounds");
+    constexpr uint32_t field = v23 ? 0x7FFFu : 0xFFFFu;
+    // Distinct for distinct positions k; an isolated word's key is above every other.
+    auto sort_key = [](uint32_t word, unsigned k) { return (word & 0x80000000u) | (((word >> 8) & field) << 9) | k; };
+    uint32_t entry[max_rounds], pos[max_rounds];
+#pragma unroll
+    for (unsigned r = 0; r < max_rounds; ++r) {
+      entry[r] = 0;
+      pos[r] = ~0u;
+      const unsigned w = r * wave_size + lane_id();
+      if (w < n_words) {
+        const uint32_t word = lw[w];
+        if (!(word & 0x80000000u)) {
+          const unsigned bi = mp::bank_of(B, w);
+          const unsigned a = B.start[bi], b = B.start[bi + 1];
+          const uint32_t key = sort_key(word, w);
+          unsigned rank = 0, k = a;
+          for (; k + 4 <= b; k += 4) {
+            uint32_t v[4];
+#pragma unroll
+            for (unsigned u = 0; u < 4; ++u) v[u] = lw[k + u];
+#pragma unroll
+            for (unsigned u = 0; u < 4; ++u) rank += sort_key(v[u], k + u) < key ? 1u : 0u;
+          }
+          for (; k < b; ++k) rank += sort_key(lw[k], k) < key ? 1u : 0u;
+          const uint32_t* wp = B.word[bi] + (w - a);
+          const uint32_t f = sp_flags[static_cast<size_t>(reinterpret_cast<const char*>(wp) - raw) / 4];
+          entry[r] = (f << 24) | (word & (v23 ? 0x007FFFFFu : 0x00FFFFFFu));
+          pos[r] = B.sorted[bi] + rank;
+        }
+      }
+    }
+    mp::wave_sync_lds(); // every word has been read
+#pragma unroll
+    for (unsigned r = 0; r < max_rounds; ++r)
+      if (pos[r] != ~0u) lw[pos[r]] = entry[r];
+    mp::wave_sync_lds();
+    const unsigned n_sorted = B.sorted[B.n];
+    for (unsigned base = 0; base < n_sorted; base += wave_size) {
+      const unsigned j = base + lane_id();
+      const bool active = j < n_sorted;
+      const uint32_t e = active ? lw[j] : 0u;
+      const uint32_t f = e >> 24;
+      const unsigned bi = active ? mp::bank_at(B.sorted, B.n, j) : 0u;
+      const uint32_t sp_word = e & 0x00FFFFFFu;
+      const uint32_t pair = B.source[bi] & 0x1FFu;
+      const uint32_t sensor_number = v23 ? pair : ((pair << 1) | ((sp_word >> 23) & 0x1u));
+      Window win {0, 0, 0, 0, 0};
+      if (f & 3u) {
+        const uint32_t key = (e >> 8) & 0xFFFFu;
+        const uint32_t sensor = key & 0x8000u;
+        const uint32_t col = (key >> 6) & 0x1FFu;
+        const uint32_t lower = sensor | ((max(col, 3u) - 3u) << 6);
+        const uint32_t upper = sensor + min((col + 2u) << 6, 0x8000u);
+        unsigned lo = B.sorted[bi], hi = B.sorted[bi + 1];
+        const unsigned end = hi;
+        while (lo < hi) {
+          const unsigned mid = (lo + hi) >> 1;
+          if (((lw[mid] >> 8) & 0xFFFFu) < lower) lo = mid + 1;
+          else hi = mid;
+        }
+        const int32_t row_lower = static_cast<int32_t>(key & 0x3Fu) - 2;
+        const int32_t col_lower = static_cast<int32_t>(col) - 3;
+        for (unsigned k = lo; k < end; ++k) {
+          const uint32_t o = lw[k];
+          if (((o >> 8) & 0xFFFFu) >= upper) break;
+          window_acc_same_sensor(row_lower, col_lower, o, win);
+        }
+      }
+      candidates_round(sp_word, sensor_number, f, win);
+    }
+  }
+  else {
+    // Larger module pairs: every candidate scans its bank, tile by tile through the slice.
+    for (unsigned base = 0; base < n_words; base += wave_size) {
+      const unsigned w = base + lane_id();
+      const bool active = w < n_words;
+      const unsigned bi = active ? mp::bank_of(B, w) : 0;
+      const uint32_t sp_word = active ? B.word[bi][w - B.start[bi]] : 0x80000000u;
+      uint32_t f = 0;
+      if (active && !(sp_word & 0x80000000u)) {
+        const uint32_t* wp = B.word[bi] + (w - B.start[bi]);
+        f = sp_flags[static_cast<size_t>(reinterpret_cast<const char*>(wp) - raw) / 4];
+      }
+      const uint32_t pair = B.source[bi] & 0x1FFu;
+      const uint32_t sensor_number = v23 ? pair : ((pair << 1) | ((sp_word >> 23) & 0x1u));
+      const uint32_t sp_addr = (sp_word & 0x007FFF00u) >> 8;
+      const int32_t sp_row_lower = static_cast<int32_t>(sp_addr & 0x3Fu) - 2;
+      const int32_t sp_col_lower = static_cast<int32_t>(sp_addr >> 6) - 3;
+      Window win {0, 0, 0, 0, 0};
+      // Both candidates of an SP share its window.
+      mp::for_words(B, n_words, resident, lw, (f & 3) != 0, B.start[bi], B.start[bi + 1], [&](uint32_t other) {
+        window_acc<version>(sp_row_lower, sp_col_lower, sensor_number, other, win);
+      });
+      candidates_round(sp_word, sensor_number, f, win);
     }
   }
   n = min(n, estimate);
