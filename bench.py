@@ -3,7 +3,7 @@
 One step = the whole VELO path (super-pixel decoding + masked clustering, phi sort,
 search_by_triplet, three-hit filter, consolidation, simplified Kalman fit) over one slice
 of 1000 synthetic events with the raw banks already resident in HBM. Slices are dealt to
-six HIP streams in turn (the process runs with eight hardware queues), one sequence per
+ten HIP streams in turn (the process runs with twelve hardware queues), one sequence per
 stream, which is how Allen measures GPU throughput (one stream per host thread, scanned
 over the stream count); the one-stream figure is reported beside it.
 
@@ -125,12 +125,12 @@ def main():
                          "the sequence, in 256-lane workgroups with 4096 LDS hits; 0: one launch (the default: the "
                          "split shortens one slice's search but the extra stream per slice costs more throughput "
                          "than it gains when several slices are in flight)")
-    ap.add_argument("--streams", type=int, default=6,
+    ap.add_argument("--streams", type=int, default=10,
                     help="slices in flight: one sequence and one HIP stream per slice, as Allen runs one stream per "
                          "host thread and measures GPU throughput over the stream count; steps are dealt to the "
-                         "streams in turn (6 is the best count measured on one MI355X with 8 hardware queues, 3 "
-                         "with HIP's default of 4)")
-    ap.add_argument("--hw-queues", type=int, default=8,
+                         "streams in turn (10 streams on 12 hardware queues measured best on one MI355X, within a "
+                         "plateau from 10 to 14 streams on 11 to 16 queues; 3 with HIP's default of 4)")
+    ap.add_argument("--hw-queues", type=int, default=12,
                     help="hardware queues of this process (HIP's GPU_MAX_HW_QUEUES, set before the runtime starts; "
                          "HIP's default is 4): one per stream in flight plus the stream that times them")
     ap.add_argument("--persistent", action="store_true",
