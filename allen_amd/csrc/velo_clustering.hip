@@ -1368,34 +1368,47 @@ __device__ __forceinline__ void cluster_item(
     // Distinct for distinct positions k; an isolated word's key is above every other.
     auto sort_key = [](uint32_t word, unsigned k) { return (word & 0x80000000u) | (((word >> 8) & field) << 9) | k; };
     uint32_t entry[max_rounds], pos[max_rounds];
+    // Each word's slot takes its key (slots are the lanes' own, so no lane reads another's slot
+    // here), then every non-isolated word counts the keys below its own in its bank's range.
 #pragma unroll
     for (unsigned r = 0; r < max_rounds; ++r) {
       entry[r] = 0;
-      pos[r] = ~0u;
       const unsigned w = r * wave_size + lane_id();
       if (w < n_words) {
         const uint32_t word = lw[w];
         if (!(word & 0x80000000u)) {
           const unsigned bi = mp::bank_of(B, w);
+          const uint32_t* wp = B.word[bi] + (w - B.start[bi]);
+          const uint32_t f = sp_flags[static_cast<size_t>(reinterpret_cast<const char*>(wp) - raw) / 4];
+          entry[r] = (f << 24) | (word & (v23 ? 0x007FFFFFu : 0x00FFFFFFu));
+        }
+        lw[w] = sort_key(word, w);
+      }
+    }
+    mp::wave_sync_lds();
+#pragma unroll
+    for (unsigned r = 0; r < max_rounds; ++r) {
+      pos[r] = ~0u;
+      const unsigned w = r * wave_size + lane_id();
+      if (w < n_words) {
+        const uint32_t key = lw[w];
+        if (!(key & 0x80000000u)) {
+          const unsigned bi = mp::bank_of(B, w);
           const unsigned a = B.start[bi], b = B.start[bi + 1];
-          const uint32_t key = sort_key(word, w);
           unsigned rank = 0, k = a;
           for (; k + 4 <= b; k += 4) {
             uint32_t v[4];
 #pragma unroll
             for (unsigned u = 0; u < 4; ++u) v[u] = lw[k + u];
 #pragma unroll
-            for (unsigned u = 0; u < 4; ++u) rank += sort_key(v[u], k + u) < key ? 1u : 0u;
+            for (unsigned u = 0; u < 4; ++u) rank += v[u] < key ? 1u : 0u;
           }
-          for (; k < b; ++k) rank += sort_key(lw[k], k) < key ? 1u : 0u;
-          const uint32_t* wp = B.word[bi] + (w - a);
-          const uint32_t f = sp_flags[static_cast<size_t>(reinterpret_cast<const char*>(wp) - raw) / 4];
-          entry[r] = (f << 24) | (word & (v23 ? 0x007FFFFFu : 0x00FFFFFFu));
+          for (; k < b; ++k) rank += lw[k] < key ? 1u : 0u;
           pos[r] = B.sorted[bi] + rank;
         }
       }
     }
-    mp::wave_sync_lds(); // every word has been read
+    mp::wave_sync_lds(); // every key has been read
 #pragma unroll
     for (unsigned r = 0; r < max_rounds; ++r)
       if (pos[r] != ~0u) lw[pos[r]] = entry[r];
