@@ -117,6 +117,16 @@ struct LdsHits {
   __device__ __forceinline__ const int16_t* phi_begin() const { return ph; }
   __device__ __forceinline__ bool used(unsigned i) const { return (flags[i] & flag_used) != 0; }
   __device__ __forceinline__ void mark_used(unsigned i) const { flags[i] |= flag_used; }
+  // Four hits at once: all four flags are read before any is written, so the four LDS round
+  // trips overlap (equal indices write equal values).
+  __device__ __forceinline__ void mark_used4(unsigned a, unsigned b, unsigned c, unsigned d) const
+  {
+    const uint8_t fa = flags[a], fb = flags[b], fc = flags[c], fd = flags[d];
+    flags[a] = fa | flag_used;
+    flags[b] = fb | flag_used;
+    flags[c] = fc | flag_used;
+    flags[d] = fd | flag_used;
+  }
   __device__ __forceinline__ bool module_odd(unsigned i) const { return (flags[i] & flag_odd) != 0; }
 };
 
@@ -137,6 +147,13 @@ struct GlobalHits {
   __device__ __forceinline__ const int16_t* phi_begin() const { return ph; }
   __device__ __forceinline__ bool used(unsigned i) const { return used_flags[i] != 0; }
   __device__ __forceinline__ void mark_used(unsigned i) const { used_flags[i] = 1; }
+  __device__ __forceinline__ void mark_used4(unsigned a, unsigned b, unsigned c, unsigned d) const
+  {
+    used_flags[a] = 1;
+    used_flags[b] = 1;
+    used_flags[c] = 1;
+    used_flags[d] = 1;
+  }
   __device__ __forceinline__ bool module_odd(unsigned i) const { return module_parity(ids[i]) != 0; }
 };
 
@@ -190,11 +207,17 @@ __device__ __forceinline__ int counted_search(const T* array, unsigned size, T v
     const unsigned span = hi - lo;
     unsigned p[7];
     bool below[7];
+    T v[7];
 #pragma unroll
     for (unsigned j = 0; j < 7; ++j) {
       p[j] = lo + ((j + 1) * span) / 8; // lo <= p[j] < hi, non-decreasing in j
-      below[j] = array[p[j]] < value;   // true for a prefix of j (sorted array)
+      v[j] = array[p[j]];
     }
+    // All seven reads are issued before the first compare waits for one (the scheduler would
+    // otherwise interleave read, wait and compare, one LDS round trip per probe).
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (unsigned j = 0; j < 7; ++j) below[j] = v[j] < value; // true for a prefix of j (sorted array)
     if constexpr (count) wc.steps += 7;
     unsigned nlo = lo, nhi = hi;
 #pragma unroll
@@ -270,6 +293,16 @@ struct MixedHits {
   __device__ __forceinline__ const int16_t* phi_begin() const { return ph; }
   __device__ __forceinline__ bool used(unsigned i) const { return (flags[i] & flag_used) != 0; }
   __device__ __forceinline__ void mark_used(unsigned i) const { flags[i] |= flag_used; }
+  // Four hits at once: all four flags are read before any is written, so the four LDS round
+  // trips overlap (equal indices write equal values).
+  __device__ __forceinline__ void mark_used4(unsigned a, unsigned b, unsigned c, unsigned d) const
+  {
+    const uint8_t fa = flags[a], fb = flags[b], fc = flags[c], fd = flags[d];
+    flags[a] = fa | flag_used;
+    flags[b] = fb | flag_used;
+    flags[c] = fc | flag_used;
+    flags[d] = fd | flag_used;
+  }
   __device__ __forceinline__ bool module_odd(unsigned i) const { return (flags[i] & flag_odd) != 0; }
 };
 
@@ -1023,11 +1056,8 @@ __device__ __forceinline__ void track_forwarding(
     const Scan3 s = scan3(sh, cn, promote, reinsert, three);
     if (valid) {
       if (best_h2 != -1) {
-        hits.mark_used(best_h2);
         if (promote) {
-          hits.mark_used(th[0]);
-          hits.mark_used(th[1]);
-          hits.mark_used(th[2]);
+          hits.mark_used4(best_h2, th[0], th[1], th[2]);
           track_number = cn.n_tracks + s.r[0];
           if (track_number < o.track_capacity) {
             TrackHits& t = o.tracks[track_number];
@@ -1039,6 +1069,7 @@ __device__ __forceinline__ void track_forwarding(
           }
         }
         else {
+          hits.mark_used(best_h2);
           TrackHits& t = o.tracks[track_number];
           t.hits[number_of_hits] = best_h2;
           t.hitsNum = number_of_hits + 1;
