@@ -70,13 +70,20 @@ struct ClusterOut {
 __device__ __forceinline__ ClusterOut
 place_cluster(const Geometry& g, const float* ltg, uint32_t lhcb_id, unsigned cx, unsigned cy, float fx, float fy)
 {
-  const float local_x = g.local_x[cx] + fx * g.x_pitch[cx];
+  // Every geometry load is issued before the first is used (the scheduler would otherwise wait
+  // out several memory round trips one after another).
+  float m[12];
+#pragma unroll
+  for (unsigned k = 0; k < 12; ++k) m[k] = ltg[k];
+  const float lx = g.local_x[cx], pitch = g.x_pitch[cx];
+  __builtin_amdgcn_sched_barrier(0);
+  const float local_x = lx + fx * pitch;
   const float local_y = (cy + 0.5f + fy) * pixel_size;
   ClusterOut c;
   c.id = lhcb_id;
-  c.x = ltg[0] * local_x + ltg[1] * local_y + ltg[9];
-  c.y = ltg[3] * local_x + ltg[4] * local_y + ltg[10];
-  c.z = ltg[6] * local_x + ltg[7] * local_y + ltg[11];
+  c.x = m[0] * local_x + m[1] * local_y + m[9];
+  c.y = m[3] * local_x + m[4] * local_y + m[10];
+  c.z = m[6] * local_x + m[7] * local_y + m[11];
   c.phi = hit_phi_16(c.x, c.y);
   return c;
 }

@@ -1346,12 +1346,25 @@ __global__ void k_search_by_triplet(
       float* zs = ys + lds_hits;
       int16_t* ph = reinterpret_cast<int16_t*>(zs + lds_hits);
       uint8_t* flags = reinterpret_cast<uint8_t*>(ph + lds_hits);
-      for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) {
-        xs[i] = gxyz[3 * i];
-        ys[i] = gxyz[3 * i + 1];
-        zs[i] = gxyz[3 * i + 2];
-        ph[i] = gph[i];
-        flags[i] = module_parity(gids[i]) ? flag_odd : 0;
+      // Two hits per lane and round, every load issued before the first store waits (the
+      // scheduler would otherwise wait out one memory round trip per field).
+      for (unsigned i0 = threadIdx.x; i0 < n_hits; i0 += 2 * blockDim.x) {
+        const unsigned i1 = i0 + blockDim.x < n_hits ? i0 + blockDim.x : i0;
+        const float x0 = gxyz[3 * i0], y0 = gxyz[3 * i0 + 1], z0 = gxyz[3 * i0 + 2];
+        const float x1 = gxyz[3 * i1], y1 = gxyz[3 * i1 + 1], z1 = gxyz[3 * i1 + 2];
+        const int16_t p0 = gph[i0], p1 = gph[i1];
+        const uint32_t id0 = gids[i0], id1 = gids[i1];
+        __builtin_amdgcn_sched_barrier(0);
+        xs[i0] = x0;
+        ys[i0] = y0;
+        zs[i0] = z0;
+        ph[i0] = p0;
+        flags[i0] = module_parity(id0) ? flag_odd : 0;
+        xs[i1] = x1;
+        ys[i1] = y1;
+        zs[i1] = z1;
+        ph[i1] = p1;
+        flags[i1] = module_parity(id1) ? flag_odd : 0;
       }
       __syncthreads();
       const LdsHits hits {xs, ys, zs, ph, flags, phi_tables};
@@ -1364,9 +1377,15 @@ __global__ void k_search_by_triplet(
     else if (3 * n_hits <= search_bytes_per_hit * lds_hits) {
       int16_t* ph = reinterpret_cast<int16_t*>(dyn);
       uint8_t* flags = reinterpret_cast<uint8_t*>(ph + n_hits);
-      for (unsigned i = threadIdx.x; i < n_hits; i += blockDim.x) {
-        ph[i] = gph[i];
-        flags[i] = module_parity(gids[i]) ? flag_odd : 0;
+      for (unsigned i0 = threadIdx.x; i0 < n_hits; i0 += 2 * blockDim.x) {
+        const unsigned i1 = i0 + blockDim.x < n_hits ? i0 + blockDim.x : i0;
+        const int16_t p0 = gph[i0], p1 = gph[i1];
+        const uint32_t id0 = gids[i0], id1 = gids[i1];
+        __builtin_amdgcn_sched_barrier(0);
+        ph[i0] = p0;
+        flags[i0] = module_parity(id0) ? flag_odd : 0;
+        ph[i1] = p1;
+        flags[i1] = module_parity(id1) ? flag_odd : 0;
       }
       cn.status |= status_lds_fallback;
       __syncthreads();
