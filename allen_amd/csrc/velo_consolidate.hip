@@ -266,6 +266,7 @@ __global__ void k_copy_track_hit_number(
 
 // K8: one thread per (track, hit); the event's tracks are [main tracks][filtered 3-hit tracks].
 constexpr unsigned consolidate_lds_tracks = 2047; // tracks of an event whose offsets are staged in LDS
+constexpr unsigned consolidate_lds_hits = 4096;    // track hits of an event mapped to their tracks in LDS
 
 __global__ void k_consolidate_tracks(
   const uint32_t* event_list,
@@ -296,29 +297,53 @@ __global__ void k_consolidate_tracks(
   const unsigned h_end = offsets_velo_track_hit_number[t0 + n_total];
   for (unsigned i = threadIdx.x; i < n_total; i += blockDim.x)
     if (accepted) accepted[t0 + i] = 1;
-  // The event's slice of the hit-number offsets, in LDS when it fits: each hit finds its track by
-  // a binary search over it.
+  // The event's slice of the hit-number offsets, in LDS when it fits. With it, every track
+  // writes (track, position in the track) for each of its hits into an LDS map (one thread per
+  // track), so a hit reads its track in one LDS access; otherwise each hit finds its track by a
+  // binary search over the offsets.
   __shared__ uint32_t s_off[consolidate_lds_tracks + 1];
+  __shared__ uint16_t s_map[consolidate_lds_hits];
   const bool in_lds = n_total <= consolidate_lds_tracks;
+  const bool mapped = in_lds && h_end - h_begin <= consolidate_lds_hits;
   if (in_lds)
     for (unsigned i = threadIdx.x; i <= n_total; i += blockDim.x) s_off[i] = offsets_velo_track_hit_number[t0 + i];
   __syncthreads();
+  if (mapped) {
+    static_assert(consolidate_lds_tracks < (1u << 11) && max_track_size <= 32, "(track, position) in 16 bits");
+    for (unsigned t = threadIdx.x; t < n_total; t += blockDim.x) {
+      const unsigned a = s_off[t] - h_begin, b = s_off[t + 1] - h_begin;
+      for (unsigned j = a; j < b; ++j) s_map[j] = static_cast<uint16_t>((t << 5) | (j - a));
+    }
+    __syncthreads();
+  }
   const uint32_t* off = in_lds ? s_off : offsets_velo_track_hit_number + t0;
   for (unsigned h = h_begin + threadIdx.x; h < h_end; h += blockDim.x) {
-    // track of hit h: binary search in the event's slice of the hit-number offsets
-    unsigned lo = 0, hi = n_total;
-    while (hi - lo > 1) {
-      const unsigned mid = (lo + hi) / 2;
-      if (off[mid] <= h) lo = mid;
-      else hi = mid;
+    unsigned lo, k;
+    if (mapped) {
+      const unsigned m = s_map[h - h_begin];
+      lo = m >> 5;
+      k = m & 31u;
     }
-    const unsigned k = h - off[lo];
+    else {
+      lo = 0;
+      unsigned hi = n_total;
+      while (hi - lo > 1) {
+        const unsigned mid = (lo + hi) / 2;
+        if (off[mid] <= h) lo = mid;
+        else hi = mid;
+      }
+      k = h - off[lo];
+    }
     const unsigned hit = lo < n_main ? tracks_all[toff + lo].hits[k] : three_hit_output_all[toff + lo - n_main].hits[k];
     const size_t src = (size_t) hit_offset + hit;
-    out.ids()[h] = in.ids()[src];
-    out.xyz()[3 * (size_t) h] = in.xyz()[3 * src];
-    out.xyz()[3 * (size_t) h + 1] = in.xyz()[3 * src + 1];
-    out.xyz()[3 * (size_t) h + 2] = in.xyz()[3 * src + 2];
+    // All four fields are read before any is written (the output may not alias the input, but
+    // the compiler cannot know that and would otherwise wait out each read in turn).
+    const uint32_t id = in.ids()[src];
+    const float x = in.xyz()[3 * src], y = in.xyz()[3 * src + 1], z = in.xyz()[3 * src + 2];
+    out.ids()[h] = id;
+    out.xyz()[3 * (size_t) h] = x;
+    out.xyz()[3 * (size_t) h + 1] = y;
+    out.xyz()[3 * (size_t) h + 2] = z;
   }
 }
 
@@ -370,25 +395,32 @@ filter_step(float z, float zhit, float xhit, float whit, float& x, float& tx, fl
   covTxTx = predcovTxTx - KTx * predcovXTx;
 }
 
+// The coordinates of a track's first two and last two hits (0, 1, nh - 2, nh - 1; nh >= 2),
+// read together before the fits.
+struct TrackEnds {
+  float f[3], f1[3], l1[3], l[3];
+};
+
 template<bool upstream, bool fp64_fallback>
 __device__ __forceinline__ KState
-simplified_fit(const float* xyz, unsigned nh, float at_tx, float at_ty, const float* beamline, bool backward)
+simplified_fit(const float* xyz, unsigned nh, const TrackEnds& te, float at_tx, float at_ty, const float* beamline, bool backward)
 {
   const int direction = (backward ? 1 : -1) * (upstream ? 1 : -1);
   const float noise2PerLayer = 1e-8f + 7e-6f * (at_tx * at_tx + at_ty * at_ty);
   int firsthit = 0;
   int lasthit = static_cast<int>(nh) - 1;
   int dhit = 1;
-  if ((xyz[3 * lasthit + 2] - xyz[3 * firsthit + 2]) * direction < 0) {
+  const bool reverse = (te.l[2] - te.f[2]) * direction < 0;
+  if (reverse) {
     const int t = firsthit;
     firsthit = lasthit;
     lasthit = t;
     dhit = -1;
   }
   KState s;
-  s.x = xyz[3 * firsthit];
-  s.y = xyz[3 * firsthit + 1];
-  s.z = xyz[3 * firsthit + 2];
+  s.x = reverse ? te.l[0] : te.f[0];
+  s.y = reverse ? te.l[1] : te.f[1];
+  s.z = reverse ? te.l[2] : te.f[2];
   s.tx = at_tx;
   s.ty = at_ty;
   s.c00 = param_w_inverted;
@@ -397,8 +429,16 @@ simplified_fit(const float* xyz, unsigned nh, float at_tx, float at_ty, const fl
   s.c31 = 0.f;
   s.c22 = 1.f;
   s.c33 = 1.f;
+  // The next hit's coordinates are read while the current step computes (each step waits for
+  // nothing but its own arithmetic).
+  float nx = reverse ? te.l1[0] : te.f1[0], ny = reverse ? te.l1[1] : te.f1[1], nz = reverse ? te.l1[2] : te.f1[2];
   for (int i = firsthit + dhit; i != lasthit + dhit; i += dhit) {
-    const float hx = xyz[3 * i], hy = xyz[3 * i + 1], hz = xyz[3 * i + 2];
+    const float hx = nx, hy = ny, hz = nz;
+    if (i + dhit != lasthit + dhit) {
+      nx = xyz[3 * (i + dhit)];
+      ny = xyz[3 * (i + dhit) + 1];
+      nz = xyz[3 * (i + dhit) + 2];
+    }
     s.c22 += noise2PerLayer;
     s.c33 += noise2PerLayer;
     filter_step<fp64_fallback>(s.z, hz, hx, param_w, s.x, s.tx, s.c00, s.c20, s.c22);
@@ -455,9 +495,18 @@ __device__ __forceinline__ void fit_track(
   float* endvelo_states,
   unsigned fp64_fallback)
 {
+  TrackEnds te;
+#pragma unroll
+  for (unsigned c = 0; c < 3; ++c) {
+    te.f[c] = xyz[c];
+    te.f1[c] = xyz[3 + c];
+    te.l1[c] = xyz[3 * (nh - 2) + c];
+    te.l[c] = xyz[3 * (nh - 1) + c];
+  }
+  __builtin_amdgcn_sched_barrier(0); // the four hits' loads are in flight together
   // linear_fit (VeloKalmanFilter.cu:335-355)
-  const float fx = xyz[0], fy = xyz[1], fz = xyz[2];
-  const float lx = xyz[3 * (nh - 1)], ly = xyz[3 * (nh - 1) + 1], lz = xyz[3 * (nh - 1) + 2];
+  const float fx = te.f[0], fy = te.f[1], fz = te.f[2];
+  const float lx = te.l[0], ly = te.l[1], lz = te.l[2];
   const float ltx = (lx - fx) / (lz - fz);
   const float lty = (ly - fy) / (lz - fz);
   const float ldz = (ltx * (bl[0] - lx) + lty * (bl[1] - ly)) / (ltx * ltx + lty * lty);
@@ -465,14 +514,14 @@ __device__ __forceinline__ void fit_track(
   const bool backward = lin_z > fz;
   is_backward[t] = backward;
   if (fp64_fallback) {
-    const KState b = simplified_fit<true, true>(xyz, nh, ltx, lty, bl, backward);
-    const KState v = simplified_fit<false, true>(xyz, nh, b.tx, b.ty, bl, backward);
+    const KState b = simplified_fit<true, true>(xyz, nh, te, ltx, lty, bl, backward);
+    const KState v = simplified_fit<false, true>(xyz, nh, te, b.tx, b.ty, bl, backward);
     store_state(beamline_states, total_tracks, t, b);
     store_state(endvelo_states, total_tracks, t, v);
   }
   else {
-    const KState b = simplified_fit<true, false>(xyz, nh, ltx, lty, bl, backward);
-    const KState v = simplified_fit<false, false>(xyz, nh, b.tx, b.ty, bl, backward);
+    const KState b = simplified_fit<true, false>(xyz, nh, te, ltx, lty, bl, backward);
+    const KState v = simplified_fit<false, false>(xyz, nh, te, b.tx, b.ty, bl, backward);
     store_state(beamline_states, total_tracks, t, b);
     store_state(endvelo_states, total_tracks, t, v);
   }
