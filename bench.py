@@ -214,6 +214,8 @@ def main():
         sq.run(dev, stream=st)
     torch.cuda.synchronize()
 
+    host = {"ms": 0.0}  # host time of the last timed() call's launch loop
+
     def timed(steps: int, warmup: int, stage: str | None, used: int = n_streams) -> float:
         """Milliseconds of `steps` steps dealt to the first `used` streams in turn (eager runs or
         graph replays), HIP events around them on the current stream; `stage` is bracketed in the
@@ -241,8 +243,10 @@ def main():
         start.record(main_stream)
         for st in streams[:used]:
             st.wait_event(start)
+        t_host = time.perf_counter()
         for i in range(steps):
             launch(i)
+        host["ms"] = (time.perf_counter() - t_host) * 1e3
         for st in streams[:used]:
             main_stream.wait_stream(st)
         end.record(main_stream)
@@ -337,6 +341,7 @@ def main():
     # Timed region: K steps; without a graph the dominant stage is bracketed by HIP events on its
     # stream in the first sequence.
     total_ms = timed(args.steps, args.warmup, dominant)
+    host_ms_per_step = host["ms"] / args.steps
     if args.graph:
         # The graph carries no stage events; the dominant kernel's launch time is the eager one.
         kern_avg_ms = stage_ms[dominant]
@@ -503,6 +508,9 @@ def main():
         **({"autotune": tuning} if tuning else {}),
         **({"single_stream": {"ms_per_step": round(single_ms, 4), "events_per_s": round(n / (single_ms / 1e3), 1)}}
            if single_ms else {}),
+        # Host time to enqueue one step (the launch loop of the timed region): a step cannot
+        # complete faster than the host issues it.
+        "host_enqueue_ms_per_step": round(host_ms_per_step, 4),
         **extra,
     }
     print(json.dumps(out), flush=True)
