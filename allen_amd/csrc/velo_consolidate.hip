@@ -354,45 +354,48 @@ struct KState {
   float c00, c20, c22, c11, c31, c33;
 };
 
-// FP64 covariance fallback (no reference counterpart): the FP32 update (1 - Kx) * predcovXX
-// loses about kappa * 2^-24 of relative accuracy, kappa = 1 + whit * predcovXX (about 1e7 on a
+// FP64 covariance fallback (no reference counterpart): the FP32 update (1 - gain_p) * pp_pred
+// loses about kappa * 2^-24 of relative accuracy, kappa = 1 + w * pp_pred (about 1e7 on a
 // track's first step). Above kappa = 100 the step's covariance prediction and update run in
-// FP64 from the stored FP32 covariance, the update as (1 - Kx) = R / whit, and the three terms
-// are rounded to FP32; the state update keeps the reference's FP32 arithmetic.
+// FP64 from the stored FP32 covariance, the update as (1 - gain_p) = inv / w, and the three
+// terms are rounded to FP32; the state update keeps the reference's FP32 arithmetic.
 constexpr float kalman_kappa_limit_minus_one = 99.f;
 
+// One coordinate's filter step to the hit at z_hit with measurement m and weight w: position
+// p, slope t and their covariance (pp, pt, tt) are predicted over dz and updated with the gain
+// (the reference's expression order, VeloKalmanFilter.cuh:25-60).
 template<bool fp64_fallback>
 __device__ __forceinline__ void
-filter_step(float z, float zhit, float xhit, float whit, float& x, float& tx, float& covXX, float& covXTx, float& covTxTx)
+filter_step(float z, float z_hit, float m, float w, float& p, float& t, float& pp, float& pt, float& tt)
 {
-  const float dz = zhit - z;
-  const float predx = x + dz * tx;
-  const float dz_t_covTxTx = dz * covTxTx;
-  const float predcovXTx = covXTx + dz_t_covTxTx;
-  const float dx_t_covXTx = dz * covXTx;
-  const float predcovXX = covXX + 2 * dx_t_covXTx + dz * dz_t_covTxTx;
-  const float predcovTxTx = covTxTx;
-  const float R = 1.0f / ((1.0f / whit) + predcovXX);
-  const float Kx = predcovXX * R;
-  const float KTx = predcovXTx * R;
-  const float r = xhit - predx;
-  x = predx + Kx * r;
-  tx = tx + KTx * r;
-  if (fp64_fallback && predcovXX * whit > kalman_kappa_limit_minus_one) {
-    const double dzd = dz, c00 = covXX, c20 = covXTx, c22 = covTxTx;
-    const double pXTx = c20 + dzd * c22;
-    const double pXX = c00 + 2.0 * (dzd * c20) + dzd * (dzd * c22);
-    const double pTxTx = c22;
-    const double iw = 1.0 / static_cast<double>(whit);
-    const double Rd = 1.0 / (iw + pXX);
-    covXX = static_cast<float>(pXX * iw * Rd);
-    covXTx = static_cast<float>(pXTx * iw * Rd);
-    covTxTx = static_cast<float>(pTxTx - pXTx * pXTx * Rd);
+  const float dz = z_hit - z;
+  const float p_pred = p + dz * t;
+  const float dz_tt = dz * tt;
+  const float pt_pred = pt + dz_tt;
+  const float dz_pt = dz * pt;
+  const float pp_pred = pp + 2 * dz_pt + dz * dz_tt;
+  const float tt_pred = tt;
+  const float inv = 1.0f / ((1.0f / w) + pp_pred);
+  const float gain_p = pp_pred * inv;
+  const float gain_t = pt_pred * inv;
+  const float res = m - p_pred;
+  p = p_pred + gain_p * res;
+  t = t + gain_t * res;
+  if (fp64_fallback && pp_pred * w > kalman_kappa_limit_minus_one) {
+    const double dzd = dz, c00 = pp, c20 = pt, c22 = tt;
+    const double pt_d = c20 + dzd * c22;
+    const double pp_d = c00 + 2.0 * (dzd * c20) + dzd * (dzd * c22);
+    const double tt_d = c22;
+    const double iw = 1.0 / static_cast<double>(w);
+    const double inv_d = 1.0 / (iw + pp_d);
+    pp = static_cast<float>(pp_d * iw * inv_d);
+    pt = static_cast<float>(pt_d * iw * inv_d);
+    tt = static_cast<float>(tt_d - pt_d * pt_d * inv_d);
     return;
   }
-  covXX = (1 - Kx) * predcovXX;
-  covXTx = (1 - Kx) * predcovXTx;
-  covTxTx = predcovTxTx - KTx * predcovXTx;
+  pp = (1 - gain_p) * pp_pred;
+  pt = (1 - gain_p) * pt_pred;
+  tt = tt_pred - gain_t * pt_pred;
 }
 
 // The coordinates of a track's first two and last two hits (0, 1, nh - 2, nh - 1; nh >= 2),
@@ -406,16 +409,16 @@ __device__ __forceinline__ KState
 simplified_fit(const float* xyz, unsigned nh, const TrackEnds& te, float at_tx, float at_ty, const float* beamline, bool backward)
 {
   const int direction = (backward ? 1 : -1) * (upstream ? 1 : -1);
-  const float noise2PerLayer = 1e-8f + 7e-6f * (at_tx * at_tx + at_ty * at_ty);
-  int firsthit = 0;
-  int lasthit = static_cast<int>(nh) - 1;
-  int dhit = 1;
+  const float layer_noise = 1e-8f + 7e-6f * (at_tx * at_tx + at_ty * at_ty);
+  int first = 0;
+  int last = static_cast<int>(nh) - 1;
+  int step = 1;
   const bool reverse = (te.l[2] - te.f[2]) * direction < 0;
   if (reverse) {
-    const int t = firsthit;
-    firsthit = lasthit;
-    lasthit = t;
-    dhit = -1;
+    const int t = first;
+    first = last;
+    last = t;
+    step = -1;
   }
   KState s;
   s.x = reverse ? te.l[0] : te.f[0];
@@ -432,21 +435,21 @@ simplified_fit(const float* xyz, unsigned nh, const TrackEnds& te, float at_tx, 
   // The next hit's coordinates are read while the current step computes (each step waits for
   // nothing but its own arithmetic).
   float nx = reverse ? te.l1[0] : te.f1[0], ny = reverse ? te.l1[1] : te.f1[1], nz = reverse ? te.l1[2] : te.f1[2];
-  for (int i = firsthit + dhit; i != lasthit + dhit; i += dhit) {
+  for (int i = first + step; i != last + step; i += step) {
     const float hx = nx, hy = ny, hz = nz;
-    if (i + dhit != lasthit + dhit) {
-      nx = xyz[3 * (i + dhit)];
-      ny = xyz[3 * (i + dhit) + 1];
-      nz = xyz[3 * (i + dhit) + 2];
+    if (i + step != last + step) {
+      nx = xyz[3 * (i + step)];
+      ny = xyz[3 * (i + step) + 1];
+      nz = xyz[3 * (i + step) + 2];
     }
-    s.c22 += noise2PerLayer;
-    s.c33 += noise2PerLayer;
+    s.c22 += layer_noise;
+    s.c33 += layer_noise;
     filter_step<fp64_fallback>(s.z, hz, hx, param_w, s.x, s.tx, s.c00, s.c20, s.c22);
     filter_step<fp64_fallback>(s.z, hz, hy, param_w, s.y, s.ty, s.c11, s.c31, s.c33);
     s.z = hz;
   }
-  s.c22 += noise2PerLayer;
-  s.c33 += noise2PerLayer;
+  s.c22 += layer_noise;
+  s.c33 += layer_noise;
   float delta_z = 0.f;
   if constexpr (upstream) {
     delta_z = (s.tx * (beamline[0] - s.x) + s.ty * (beamline[1] - s.y)) / (s.tx * s.tx + s.ty * s.ty);
