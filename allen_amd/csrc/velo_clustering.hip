@@ -932,8 +932,10 @@ hipError_t launch_cluster_sorted(const FusedClusterLaunch& a, hipStream_t s)
 //                   SP word, its candidate pixels in one byte indexed by the word's position in
 //                   the raw buffer (bit0/bit1: first/second candidate, bits 2-4 and 5-7: pixels)
 //   k_cluster_mp    K3 + K4 of one module pair: isolated SPs through the pattern table, the
-//                   recorded candidates through the masked clustering window, clusters appended
-//                   in SP order with wave ballots, then ranked by (phi, id) and written sorted
+//                   recorded candidates through the masked clustering window (each bank's
+//                   non-isolated words sorted by sensor, column and row, so a window reads one
+//                   run of them), clusters appended with wave ballots, then ranked by (phi, id)
+//                   and written sorted
 //
 // Each wave stages the SP words of its module pair's banks in its own LDS slice; a module pair
 // larger than the slice runs the same code on global memory.
