@@ -36,6 +36,7 @@ WORKLOADS = {
 
 HBM_PEAK_GBS = 8000.0                    # MI355X HBM3E
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 lanes x 2.4 GHz, non-packed lane ops/s
+VALU_ISSUE_PEAK_G = 1024 * 2.4e9 / 4 / 1e9  # 1024 SIMDs, one wave64 VALU instruction per 4 cycles each
 LDS_PEAK_GBS = 256 * 128 * 2.4e9 / 1e9    # 256 CUs x 128 B/clk x 2.4 GHz
 CPU_SHARE = 16                            # host cores that belong to one GPU of the box
 
@@ -64,6 +65,19 @@ def measured_traffic(workload: str, stage: str):
         cols = [c.strip() for c in line.strip().strip("|").split("|")]
         if len(cols) >= 3 and cols[0] == stage and cols[-1].isdigit():
             return int(cols[-1])
+    return None
+
+
+def measured_valu_instructions(workload: str):
+    """Wave-level VALU instructions per launch of search_by_triplet from the rocprofv3 shader-counter
+    summary committed for this workload (the first column of its SQ_INSTS_VALU row), or None."""
+    path = os.path.join(ROOT, "profiles", f"search_by_triplet_{workload}_counters.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cols = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cols) >= 2 and cols[0] == "SQ_INSTS_VALU" and cols[1].isdigit():
+            return int(cols[1])
     return None
 
 
@@ -392,6 +406,14 @@ def main():
                       "avg_launch_ms": round(stage_ms["search"], 4),
                       "work": {"seed_evaluations": int(work[0]), "forward_evaluations": int(work[1]),
                                "extrapolations": int(work[2]), "search_probes": int(work[3])}}
+    # The instructions the kernel actually issues (the committed shader counters of this workload,
+    # at the line's default knobs) against the chip's VALU issue rate: one launch alone on the GPU.
+    valu_instr = measured_valu_instructions(traffic_name) if args.decoding == "super_pixel" else None
+    if valu_instr:
+        issue_g = valu_instr / search_s / 1e9
+        search_alu_lds["valu_issue"] = {"instructions_per_launch": valu_instr, "achieved": round(issue_g, 2),
+                                        "peak": round(VALU_ISSUE_PEAK_G, 1), "unit": "G wave-instructions/s",
+                                        "frac": round(issue_g / VALU_ISSUE_PEAK_G, 4)}
     if dominant == "search":
         roofline["search_alu_lds"] = search_alu_lds
     # Decoding and clustering against HBM: the algorithmic bytes of K1 + K2 + K3 over the sum of
