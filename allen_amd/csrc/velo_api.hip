@@ -656,6 +656,7 @@ struct velo_sequence {
   bool scorer_on = false;         // buffer "scorer_params" holds the scorer's parameters
   float scorer_threshold = 0.f;
   unsigned kalman_fp64_fallback = 0;
+  bool consolidate_lds_map = true; // 0: consolidation takes the binary-search path for every event
   unsigned pv_finder = 0;         // 1: PV finder after the fit; 2: also the z histograms ("zhisto")
   bool fused_clustering = true;
   bool count_work = false;
@@ -835,6 +836,10 @@ int velo_sequence_set_tuning(velo_sequence* s, const char* knob, unsigned value)
   }
   if (k == "kalman_fp64_fallback") {
     s->kalman_fp64_fallback = value != 0;
+    return VELO_OK;
+  }
+  if (k == "consolidate_lds_map") {
+    s->consolidate_lds_map = value != 0;
     return VELO_OK;
   }
   if (k == "seed_log") {
@@ -1227,6 +1232,7 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, hipStream
     k7.velo_track_hit_number, k7.offsets_all_velo_tracks + E, 2 * s->track_cap, off_hits, scratch, stream));
 
   velo::ConsolidateLaunch k8;
+  k8.lds_map = s->consolidate_lds_map;
   k8.event_list = nullptr;
   k8.n_list = E;
   k8.number_of_events = n_events_dev;

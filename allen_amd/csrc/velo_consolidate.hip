@@ -279,7 +279,8 @@ __global__ void k_consolidate_tracks(
   const TrackletHits* three_hit_output_all,
   const uint32_t* offsets_three_hit,
   char* velo_track_hits,
-  uint8_t* accepted)
+  uint8_t* accepted,
+  bool lds_map)
 {
   const unsigned e = event_list ? event_list[blockIdx.x] : blockIdx.x;
   const unsigned n_events = number_of_events[0];
@@ -304,7 +305,7 @@ __global__ void k_consolidate_tracks(
   __shared__ uint32_t s_off[consolidate_lds_tracks + 1];
   __shared__ uint16_t s_map[consolidate_lds_hits];
   const bool in_lds = n_total <= consolidate_lds_tracks;
-  const bool mapped = in_lds && h_end - h_begin <= consolidate_lds_hits;
+  const bool mapped = lds_map && in_lds && h_end - h_begin <= consolidate_lds_hits;
   if (in_lds)
     for (unsigned i = threadIdx.x; i <= n_total; i += blockDim.x) s_off[i] = offsets_velo_track_hit_number[t0 + i];
   __syncthreads();
@@ -704,7 +705,7 @@ hipError_t launch_consolidate_tracks(const ConsolidateLaunch& a, hipStream_t s)
     k_consolidate_tracks, dim3(a.n_list), dim3(256), 0, s, a.event_list, a.number_of_events, a.offsets_all_velo_tracks,
     reinterpret_cast<const TrackHits*>(a.tracks), a.offsets_velo_track_hit_number, a.sorted_container,
     a.offsets_estimated_input_size, reinterpret_cast<const TrackletHits*>(a.three_hit_tracks_output),
-    a.offsets_three_hit, a.velo_track_hits, a.accepted_velo_tracks);
+    a.offsets_three_hit, a.velo_track_hits, a.accepted_velo_tracks, a.lds_map);
   return hipGetLastError();
 }
 

@@ -286,6 +286,7 @@ struct ConsolidateLaunch {
   const uint32_t* offsets_three_hit;
   char* velo_track_hits;
   uint8_t* accepted_velo_tracks; // optional
+  bool lds_map = true;           // hits find their track through an LDS map (else a binary search)
 };
 
 struct KalmanLaunch {

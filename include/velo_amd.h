@@ -484,6 +484,10 @@ int velo_sequence_status(velo_sequence* s, hipStream_t stream, unsigned* status)
  *                       reference algorithms one after the other
  *   "timed_stage"       VELO_STAGE_* to bracket with HIP events on the run's stream (0 = none)
  *   "kalman_fp64_fallback"  1: velo_kalman_filter's covariance_fp64_fallback (default 0)
+ *   "consolidate_lds_map"   1 (default): consolidation maps an event's track hits to their
+ *                       tracks in LDS (events up to 2047 tracks and 4096 track hits); 0: every
+ *                       hit finds its track by a binary search over the hit-number offsets, as
+ *                       larger events always do. Outputs do not depend on it
  *   "count_work"        1: search_by_triplet also counts its work into the buffer "search_work"
  *                       (u64 x 4: seeding evaluations, forwarding evaluations, extrapolations,
  *                       phi-search probes) and, per event, the start and end wall-clock

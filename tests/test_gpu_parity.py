@@ -160,6 +160,8 @@ TUNINGS = {
     "small_window_list": {"search_ring_window": 64, "search_list_cap": 64},
     # half the events in the side stream's launch, small workgroups and LDS there (mixed layout)
     "split_half": {"search_split": 2, "search_big_block": 128, "search_big_lds_hits": 600},
+    # consolidation's binary-search path (events beyond 2047 tracks or 4096 track hits take it)
+    "consolidate_search": {"consolidate_lds_map": 0},
 }
 
 
