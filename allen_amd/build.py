@@ -15,7 +15,7 @@ SYNTH_LIB = os.path.join(PKG, "synth", "libvelo_synth.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["velo_api.hip", "velo_clustering.hip", "velo_retina.hip", "velo_search.hip", "velo_consolidate.hip",
-               "velo_scorer.hip", "velo_pv.hip", "velo_mdf.cpp"]
+               "velo_scorer.hip", "velo_pv.hip", "velo_microbench.hip", "velo_mdf.cpp"]
 # -ffp-contract=off: every float expression is evaluated as written (bit parity with the
 # CPU target). Packed-FP32 instructions are disabled for the device code.
 HIP_FLAGS = [

@@ -125,6 +125,12 @@ int velo_wall_clock_khz(unsigned* khz)
   return rc;
 }
 
+int velo_microbench(int kind, double* result, hipStream_t stream)
+{
+  if (!result || kind < 0 || kind > 2) return VELO_ERR_INVALID_ARGUMENT;
+  return hip_status(velo::run_microbench(kind, result, stream));
+}
+
 const char* velo_error_string(int code)
 {
   switch (code) {

@@ -429,6 +429,8 @@ hipError_t launch_fill_u8(uint8_t* p, size_t n, uint8_t value, hipStream_t s);
 // every host thread (the lookups are serialised by a mutex).
 // Dynamic LDS a workgroup may use (hipDeviceAttributeMaxSharedMemoryPerBlock), 0 on failure.
 int device_max_lds_per_block();
+// Measured peaks (velo_microbench.hip): kind 0 vector-ALU lane-ops/s, 1 LDS bytes/s, 2 HBM copy bytes/s.
+hipError_t run_microbench(int kind, double* result, hipStream_t stream);
 // Workgroups of `kernel` with `block` lanes and `dyn_lds` bytes of dynamic LDS that stay resident
 // on the whole device (CUs x occupancy), 0 if the runtime cannot tell.
 unsigned device_resident_blocks(const void* kernel, unsigned block, size_t dyn_lds);

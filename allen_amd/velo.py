@@ -166,6 +166,7 @@ SIGNATURES = {
     "velo_last_hip_error": (I, []),
     "velo_error_string": (ctypes.c_char_p, [I]),
     "velo_wall_clock_khz": (I, [ctypes.POINTER(ctypes.c_uint)]),
+    "velo_microbench": (I, [ctypes.c_int, ctypes.POINTER(ctypes.c_double), P]),
     "velo_constants_create": (I, [ctypes.c_char_p, ctypes.c_size_t, P, ctypes.POINTER(P)]),
     "velo_constants_destroy": (I, [P]),
     "velo_constants_dev_geometry": (P, [P]),
@@ -239,6 +240,17 @@ def check(rc: int, what: str = ""):
         msg = lib().velo_error_string(rc).decode()
         hip = lib().velo_last_hip_error()
         raise VeloError(f"{what}: {msg} (code {rc}, hip error {hip})")
+
+
+MICROBENCH_KINDS = {"valu_lane_ops_per_s": 0, "lds_bytes_per_s": 1, "hbm_copy_bytes_per_s": 2}
+
+
+def microbench(kind: str, stream=None) -> float:
+    """A measured peak of the current device (velo_microbench): vector-ALU FMA lane operations per
+    second, LDS read bytes per second or HBM copy bytes (read + written) per second."""
+    out = ctypes.c_double(0.0)
+    check(lib().velo_microbench(MICROBENCH_KINDS[kind], ctypes.byref(out), stream), f"microbench {kind}")
+    return float(out.value)
 
 
 def wall_clock_khz() -> int:

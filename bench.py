@@ -37,7 +37,7 @@ WORKLOADS = {
 HBM_PEAK_GBS = 8000.0                    # MI355X HBM3E
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 lanes x 2.4 GHz, non-packed lane ops/s
 VALU_ISSUE_PEAK_G = 1024 * 2.4e9 / 4 / 1e9  # 1024 SIMDs, one wave64 VALU instruction per 4 cycles each
-LDS_PEAK_GBS = 256 * 128 * 2.4e9 / 1e9    # 256 CUs x 128 B/clk x 2.4 GHz
+LDS_PEAK_GBS = 256 * 256 * 2.4e9 / 1e9    # 256 CUs x 256 B/clk (CDNA4, 128-bit reads) x 2.4 GHz
 CPU_SHARE = 16                            # host cores that belong to one GPU of the box
 
 
@@ -163,6 +163,9 @@ def main():
                          "parameters (allen_amd.scorer)")
     ap.add_argument("--threshold", type=float, default=None,
                     help="logit below which a seed is pruned (default: the shipped scorer's operating threshold)")
+    ap.add_argument("--microbench", action="store_true",
+                    help="only measure this device's vector-ALU FMA issue rate, LDS read bandwidth and HBM copy "
+                         "bandwidth (velo_microbench) and print them beside the published peaks")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times (the reference kernel for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -182,6 +185,33 @@ def main():
     from allen_amd import synth, velo
 
     assert torch.cuda.is_available(), "no HIP device"
+    if args.microbench:
+        valu = velo.microbench("valu_lane_ops_per_s")
+        lds = velo.microbench("lds_bytes_per_s")
+        hbm = velo.microbench("hbm_copy_bytes_per_s")
+        # The HIP runtime's own device copy of the same size, timed the same way (five copies
+        # after one untimed copy, HIP events).
+        src = torch.ones((1 << 30) // 4, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record()
+        for _ in range(5):
+            dst.copy_(src)
+        t1.record()
+        torch.cuda.synchronize()
+        hbm_rt = 5 * 2 * (1 << 30) / (t0.elapsed_time(t1) / 1e3)
+        del src, dst
+        print(json.dumps({"microbench": {
+            "valu": {"measured": round(valu / 1e12, 2), "published": round(VALU_PEAK_TOPS, 2), "unit": "T lane-ops/s",
+                     "frac": round(valu / 1e12 / VALU_PEAK_TOPS, 3)},
+            "lds": {"measured": round(lds / 1e9, 1), "published": round(LDS_PEAK_GBS, 1), "unit": "GB/s",
+                    "frac": round(lds / 1e9 / LDS_PEAK_GBS, 3)},
+            "hbm_copy": {"measured": round(hbm / 1e9, 1), "published": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(hbm / 1e9 / HBM_PEAK_GBS, 3)},
+            "hbm_copy_runtime": {"measured": round(hbm_rt / 1e9, 1), "published": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": round(hbm_rt / 1e9 / HBM_PEAK_GBS, 3)}}}), flush=True)
+        return
     if args.copy_calibration:
         n_bytes = args.copy_calibration << 20
         src = torch.ones(n_bytes // 4, dtype=torch.int32, device="cuda")

@@ -58,6 +58,13 @@ int velo_last_hip_error(void);
 const char* velo_error_string(int code);
 /* Tick rate (kHz) of the device wall clock used by the "search_event_clock" buffer. */
 int velo_wall_clock_khz(unsigned* khz);
+/* Measured peaks of the current device for roofline figures (no counterpart in the reference).
+ * kind 0: vector-ALU FMA issue, lane operations per second (16 independent v_fma_f32 chains per
+ * lane); 1: LDS read bandwidth, bytes per second (conflict-free 128-bit reads); 2: HBM copy
+ * bandwidth, bytes read plus written per second (a 1 GiB device copy). Each times five launches
+ * after one untimed launch with HIP events on `stream` and waits for them; it allocates its own
+ * buffers (kind 2: two of 1 GiB) and frees them. */
+int velo_microbench(int kind, double* result, hipStream_t stream);
 
 /* ---- Constants: replaces Constants::dev_velo_geometry / dev_velo_sp_patterns / sp_fx / sp_fy /
  * dev_beamline filled by integration/non_event_data/src/VPGeometry.cpp:21-63 and Beamline.cpp:17-31. */
