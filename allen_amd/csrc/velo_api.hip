@@ -139,7 +139,7 @@ const char* velo_error_string(int code)
   case VELO_ERR_BANK_VERSION: return "Velo SP bank version not supported";
   case VELO_ERR_CAPACITY: return "input exceeds the capacity of the sequence";
   case VELO_ERR_HIP: return "HIP error";
-  case VELO_ERR_UNSUPPORTED_LAYOUT: return "MEP layout or compressed MDF record not supported";
+  case VELO_ERR_UNSUPPORTED_LAYOUT: return "compressed MDF record not supported";
   case VELO_ERR_IO: return "file cannot be read or is not a well-formed MDF file";
   default: return "unknown error";
   }

@@ -75,7 +75,7 @@ def test_round_trip_through_a_written_file(retina, tmp_path):
 def test_malformed_files(tmp_path):
     payload = struct.pack("<2I", 1, 0x12345678)
     cases = {
-        "compressed": (dict(compression=1), velo.VeloError, "MEP layout or compressed"),
+        "compressed": (dict(compression=1), velo.VeloError, "compressed MDF record"),
         "bad_magic": (dict(magic=0xABCD), velo.VeloError, "well-formed MDF"),
     }
     for name, (kw, exc, text) in cases.items():
