@@ -956,7 +956,8 @@ struct Banks {
   uint32_t start[max_banks_per_mp + 1];
   uint32_t source[max_banks_per_mp];
   const uint32_t* word[max_banks_per_mp];
-  uint32_t sorted[max_banks_per_mp + 1]; // k_cluster_mp: each bank's first entry in the sorted list
+  uint32_t sorted[max_banks_per_mp + 1]; // k_cluster_mp: non-isolated words of the banks before each
+  uint32_t first_isolated[max_banks_per_mp]; // k_cluster_mp: a bank's first isolated position, sorted
 };
 
 __device__ __forceinline__ void wave_sync_lds()
@@ -1296,27 +1297,32 @@ __device__ __forceinline__ void cluster_item(
     }
     n += __shfl(inc, wave_size - 1, wave_size);
   };
-  // Isolated SPs first (in SP order), then the candidates. The order in which clusters are
-  // appended does not reach the output: the module pair's clusters are ranked by their distinct
-  // (phi, id) keys below.
-  uint32_t bank_nonisolated = 0; // resident: lane k counts bank k's non-isolated words
-  for (unsigned base = 0; base < n_words; base += wave_size) {
-    const unsigned w = base + lane_id();
-    ClusterOut c0, c1;
-    unsigned k = 0;
-    uint32_t sp_word = 0x80000000u;
-    if (w < n_words) {
-      const unsigned bi = mp::bank_of(B, w);
-      sp_word = resident ? lw[w] : B.word[bi][w - B.start[bi]];
-      if (sp_word & 0x80000000u) {
-        RawBank bank;
-        bank.sourceID = B.source[bi];
-        k = isolated_sp_clusters<version>(g, sp_patterns, sp_fx, sp_fy, bank, sp_word, c0, c1);
+  // The order in which clusters are appended does not reach the output: the module pair's
+  // clusters are ranked by their distinct (phi, id) keys below.
+  constexpr uint32_t field = v23 ? 0x7FFFu : 0xFFFFu;
+  if (resident) {
+    // Every bank's words sorted in the slice by (isolated, sensor, SP column, SP row, position)
+    // (v2/v3: the bank is the sensor, bit 23 cleared), each with its candidate byte in bits 24-31:
+    // a bank's non-isolated words first, then its isolated ones. A candidate's window then takes
+    // only the words of its bank and sensor in SP columns col - 3 .. col + 1, a contiguous run
+    // found by binary search, instead of every word of the bank; OR-ing the window is order-free,
+    // so the windows are the scan's. The cluster slices are still empty here and serve as the
+    // sort's scratch: keys in the xyz slice, sorted entries in the key slice, copied back over
+    // the words at the end.
+    static_assert(3 * mp::cluster_cap >= mp::word_cap && 2 * mp::cluster_cap >= mp::word_cap, "sort scratch");
+    uint32_t* keys = reinterpret_cast<uint32_t*>(lxyz);
+    uint32_t* sorted = reinterpret_cast<uint32_t*>(lkey);
+    // Distinct for distinct positions k; isolated words above the non-isolated ones.
+    auto sort_key = [](uint32_t word, unsigned k) { return (word & 0x80000000u) | (((word >> 8) & field) << 9) | k; };
+    uint32_t bank_nonisolated = 0; // lane k counts bank k's non-isolated words
+    for (unsigned base = 0; base < n_words; base += wave_size) {
+      const unsigned w = base + lane_id();
+      uint32_t word = 0x80000000u;
+      if (w < n_words) {
+        word = lw[w];
+        keys[w] = sort_key(word, w);
       }
-    }
-    emit(k, c0, c1);
-    if (resident) {
-      const uint64_t m = __ballot(!(sp_word & 0x80000000u));
+      const uint64_t m = __ballot(!(word & 0x80000000u));
       const unsigned l = lane_id();
       if (l < B.n) {
         auto below = [](unsigned x) { return x >= wave_size ? ~0ull : ((1ull << x) - 1); };
@@ -1325,6 +1331,58 @@ __device__ __forceinline__ void cluster_item(
         bank_nonisolated += __popcll(m & below(hi)) - __popcll(m & below(lo));
       }
     }
+    {
+      const unsigned l = lane_id();
+      const uint32_t cnt = l < B.n ? bank_nonisolated : 0u;
+      const uint32_t inc = wave_inclusive_scan(cnt);
+      if (l <= B.n) B.sorted[l] = inc - cnt;
+      if (l < B.n) B.first_isolated[l] = B.start[l] + cnt;
+    }
+    mp::wave_sync_lds(); // keys and counts written
+    for (unsigned base = 0; base < n_words; base += wave_size) {
+      const unsigned w = base + lane_id();
+      if (w < n_words) {
+        const uint32_t key = keys[w];
+        const unsigned bi = mp::bank_of(B, w);
+        const unsigned a = B.start[bi], b = B.start[bi + 1];
+        unsigned rank = 0, k = a;
+        for (; k + 4 <= b; k += 4) {
+          uint32_t v[4];
+#pragma unroll
+          for (unsigned u = 0; u < 4; ++u) v[u] = keys[k + u];
+#pragma unroll
+          for (unsigned u = 0; u < 4; ++u) rank += v[u] < key ? 1u : 0u;
+        }
+        for (; k < b; ++k) rank += keys[k] < key ? 1u : 0u;
+        const uint32_t word = lw[w];
+        uint32_t f = 0;
+        if (!(word & 0x80000000u)) {
+          const uint32_t* wp = B.word[bi] + (w - a);
+          f = sp_flags[static_cast<size_t>(reinterpret_cast<const char*>(wp) - raw) / 4];
+        }
+        sorted[a + rank] = (f << 24) | (word & (v23 ? 0x007FFFFFu : 0x00FFFFFFu));
+      }
+    }
+    mp::wave_sync_lds(); // every entry written
+    for (unsigned w = lane_id(); w < n_words; w += wave_size) lw[w] = sorted[w];
+    mp::wave_sync_lds(); // the cluster slices are free again
+  }
+  // Isolated SPs (resident: each bank's isolated tail of the sorted words).
+  for (unsigned base = 0; base < n_words; base += wave_size) {
+    const unsigned w = base + lane_id();
+    ClusterOut c0, c1;
+    unsigned k = 0;
+    if (w < n_words) {
+      const unsigned bi = mp::bank_of(B, w);
+      const uint32_t sp_word = resident ? lw[w] : B.word[bi][w - B.start[bi]];
+      const bool isolated = resident ? w >= B.first_isolated[bi] : (sp_word & 0x80000000u) != 0;
+      if (isolated) {
+        RawBank bank;
+        bank.sourceID = B.source[bi];
+        k = isolated_sp_clusters<version>(g, sp_patterns, sp_fx, sp_fy, bank, sp_word, c0, c1);
+      }
+    }
+    emit(k, c0, c1);
   }
   // The candidates of one round of (at most 64) SPs: their pixels as one list, an SP's first
   // pixel before its second; each lane grows one of them with its SP's window and word fetched
@@ -1359,76 +1417,16 @@ __device__ __forceinline__ void cluster_item(
     }
   };
   if (resident) {
-    // The non-isolated words, sorted per bank by (sensor, SP column, SP row, position) into the
-    // front of the slice, each with its candidate byte in bits 24-31 (v2/v3: the bank is the
-    // sensor, bit 23 cleared). A candidate's window then takes only the words of its bank and
-    // sensor in SP columns col - 3 .. col + 1, a contiguous run found by binary search, instead
-    // of every word of the bank. OR-ing the window is order-free, so the windows are the scan's.
-    {
-      const unsigned l = lane_id();
-      const uint32_t cnt = l < B.n ? bank_nonisolated : 0u;
-      const uint32_t inc = wave_inclusive_scan(cnt);
-      if (l <= B.n) B.sorted[l] = inc - cnt;
-      mp::wave_sync_lds();
-    }
-    constexpr unsigned max_rounds = mp::word_cap / wave_size;
-    static_assert(mp::word_cap % wave_size == 0, "whole rounds");
-    constexpr uint32_t field = v23 ? 0x7FFFu : 0xFFFFu;
-    // Distinct for distinct positions k; an isolated word's key is above every other.
-    auto sort_key = [](uint32_t word, unsigned k) { return (word & 0x80000000u) | (((word >> 8) & field) << 9) | k; };
-    uint32_t entry[max_rounds], pos[max_rounds];
-    // Each word's slot takes its key (slots are the lanes' own, so no lane reads another's slot
-    // here), then every non-isolated word counts the keys below its own in its bank's range.
-#pragma unroll
-    for (unsigned r = 0; r < max_rounds; ++r) {
-      entry[r] = 0;
-      const unsigned w = r * wave_size + lane_id();
-      if (w < n_words) {
-        const uint32_t word = lw[w];
-        if (!(word & 0x80000000u)) {
-          const unsigned bi = mp::bank_of(B, w);
-          const uint32_t* wp = B.word[bi] + (w - B.start[bi]);
-          const uint32_t f = sp_flags[static_cast<size_t>(reinterpret_cast<const char*>(wp) - raw) / 4];
-          entry[r] = (f << 24) | (word & (v23 ? 0x007FFFFFu : 0x00FFFFFFu));
-        }
-        lw[w] = sort_key(word, w);
-      }
-    }
-    mp::wave_sync_lds();
-#pragma unroll
-    for (unsigned r = 0; r < max_rounds; ++r) {
-      pos[r] = ~0u;
-      const unsigned w = r * wave_size + lane_id();
-      if (w < n_words) {
-        const uint32_t key = lw[w];
-        if (!(key & 0x80000000u)) {
-          const unsigned bi = mp::bank_of(B, w);
-          const unsigned a = B.start[bi], b = B.start[bi + 1];
-          unsigned rank = 0, k = a;
-          for (; k + 4 <= b; k += 4) {
-            uint32_t v[4];
-#pragma unroll
-            for (unsigned u = 0; u < 4; ++u) v[u] = lw[k + u];
-#pragma unroll
-            for (unsigned u = 0; u < 4; ++u) rank += v[u] < key ? 1u : 0u;
-          }
-          for (; k < b; ++k) rank += lw[k] < key ? 1u : 0u;
-          pos[r] = B.sorted[bi] + rank;
-        }
-      }
-    }
-    mp::wave_sync_lds(); // every key has been read
-#pragma unroll
-    for (unsigned r = 0; r < max_rounds; ++r)
-      if (pos[r] != ~0u) lw[pos[r]] = entry[r];
-    mp::wave_sync_lds();
+    // The non-isolated words, bank after bank: word t of the list is position
+    // start[bank] + t - sorted[bank] of the slice.
     const unsigned n_sorted = B.sorted[B.n];
     for (unsigned base = 0; base < n_sorted; base += wave_size) {
-      const unsigned j = base + lane_id();
-      const bool active = j < n_sorted;
+      const unsigned t = base + lane_id();
+      const bool active = t < n_sorted;
+      const unsigned bi = active ? mp::bank_at(B.sorted, B.n, t) : 0u;
+      const unsigned j = B.start[bi] + (t - B.sorted[bi]);
       const uint32_t e = active ? lw[j] : 0u;
       const uint32_t f = e >> 24;
-      const unsigned bi = active ? mp::bank_at(B.sorted, B.n, j) : 0u;
       const uint32_t sp_word = e & 0x00FFFFFFu;
       const uint32_t pair = B.source[bi] & 0x1FFu;
       const uint32_t sensor_number = v23 ? pair : ((pair << 1) | ((sp_word >> 23) & 0x1u));
@@ -1439,7 +1437,7 @@ __device__ __forceinline__ void cluster_item(
         const uint32_t col = (key >> 6) & 0x1FFu;
         const uint32_t lower = sensor | ((max(col, 3u) - 3u) << 6);
         const uint32_t upper = sensor + min((col + 2u) << 6, 0x8000u);
-        unsigned lo = B.sorted[bi], hi = B.sorted[bi + 1];
+        unsigned lo = B.start[bi], hi = B.first_isolated[bi];
         const unsigned end = hi;
         while (lo < hi) {
           const unsigned mid = (lo + hi) >> 1;
@@ -1530,12 +1528,12 @@ __device__ __forceinline__ void cluster_item(
   }
 }
 
-// Resident grid: each wave takes items idx, idx + stride, ... and fetches the next item's
-// bank directory record before working on the current one. Five workgroups per CU: the kernel
-// needs 88 VGPRs; capped at 80 for six, it spilled four to scratch memory, which added 10 MB of
-// HBM writes per 1000 events.
-template<int version>
-__global__ void __launch_bounds__(mp::waves_per_block* wave_size, 5) k_cluster_mp(
+// One wave per item (loop = false, the default), or a resident grid (loop = true) whose waves
+// take items idx, idx + stride, ... and fetch the next item's bank directory record before
+// working on the current one. Six workgroups per CU (80 VGPRs, no spills for one wave per item:
+// the sort stages its keys and entries in the cluster slices rather than in registers).
+template<int version, bool loop>
+__global__ void __launch_bounds__(mp::waves_per_block* wave_size, 6) k_cluster_mp(
   const char* raw,
   const BankEntry* directory,
   const uint32_t* directory_count,
@@ -1562,6 +1560,12 @@ __global__ void __launch_bounds__(mp::waves_per_block* wave_size, 5) k_cluster_m
   if (idx >= n_items) return;
   const unsigned total = offsets_estimated_input_size[n_items];
   const ClusterSlices sl {s_words[wave], s_key[wave], s_xyz[wave], s_cand[wave]};
+  if constexpr (!loop) {
+    cluster_item<version>(
+      raw, mp::fetch_dir(directory, directory_count, idx), idx, total, offsets_estimated_input_size, sp_flags,
+      *geometry, sp_patterns, sp_fx, sp_fy, module_cluster_num, sorted_container, unsorted_scratch, s_banks[wave], sl);
+    return;
+  }
   mp::DirFetch cur = mp::fetch_dir(directory, directory_count, idx);
   for (; idx < n_items; idx += stride) {
     mp::DirFetch next = cur;
@@ -1609,19 +1613,24 @@ hipError_t launch_cluster_mp(const ClusterMpLaunch& a, hipStream_t s)
 {
   if (!a.n_events) return hipSuccess;
   const unsigned v = a.version == 2 || a.version == 3 ? a.version : 4;
-  const void* kernel = v == 2 ? reinterpret_cast<const void*>(&k_cluster_mp<2>)
-                     : v == 3 ? reinterpret_cast<const void*>(&k_cluster_mp<3>)
-                              : reinterpret_cast<const void*>(&k_cluster_mp<4>);
+  const void* kernel = v == 2 ? reinterpret_cast<const void*>(&k_cluster_mp<2, true>)
+                     : v == 3 ? reinterpret_cast<const void*>(&k_cluster_mp<3, true>)
+                              : reinterpret_cast<const void*>(&k_cluster_mp<4, true>);
   const unsigned grid = mp_grid(kernel, a.n_events * n_module_pairs, a.resident_grid);
-#define VELO_LAUNCH_CMP(V)                                                                                          \
+#define VELO_LAUNCH_CMP(V, L)                                                                                       \
   hipLaunchKernelGGL(                                                                                               \
-    k_cluster_mp<V>, dim3(grid), dim3(mp::waves_per_block * wave_size), 0, s, a.raw,                                \
+    (k_cluster_mp<V, L>), dim3(grid), dim3(mp::waves_per_block * wave_size), 0, s, a.raw,                            \
     static_cast<const BankEntry*>(a.directory), a.directory_count, a.n_events, a.offsets_estimated_input_size, a.sp_flags, reinterpret_cast<const Geometry*>(a.geometry),           \
     a.sp_patterns, a.sp_fx, a.sp_fy, a.module_cluster_num, a.sorted_container, a.unsorted_scratch)
-  switch (a.version) {
-  case 2: VELO_LAUNCH_CMP(2); break;
-  case 3: VELO_LAUNCH_CMP(3); break;
-  default: VELO_LAUNCH_CMP(4); break;
+  switch (a.version * 2 + (a.resident_grid ? 1 : 0)) {
+  case 4: VELO_LAUNCH_CMP(2, false); break;
+  case 5: VELO_LAUNCH_CMP(2, true); break;
+  case 6: VELO_LAUNCH_CMP(3, false); break;
+  case 7: VELO_LAUNCH_CMP(3, true); break;
+  default:
+    if (a.resident_grid) VELO_LAUNCH_CMP(4, true);
+    else VELO_LAUNCH_CMP(4, false);
+    break;
   }
 #undef VELO_LAUNCH_CMP
   return hipGetLastError();
