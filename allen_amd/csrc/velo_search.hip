@@ -1209,8 +1209,9 @@ __device__ __forceinline__ void search_event(
   }
 }
 
-// One workgroup per event, or (work_counter set) a resident grid whose workgroups take the
-// next event slot from the counter until the slots run out. Dynamic LDS: the seeding scratch
+// One workgroup per event, or (persistent, launched with a work counter) a resident grid whose
+// workgroups take the next event slot from the counter until the slots run out; the persistent
+// grid is an instantiation of its own, so the one-event kernel keeps no loop state live. Dynamic LDS: the seeding scratch
 // (seed_scratch_bytes_per_lane per lane), the LDS window of the ring of tracks to follow
 // (ring_window entries; the ring itself, 2048 x u32, is written through to HBM), the list of
 // unused hits of the previous module pair and, per position, the number of unused hits before
@@ -1222,8 +1223,9 @@ __device__ __forceinline__ void search_event(
 // atomic per wave). With event_clock, each workgroup stores at event_clock[10 * event_number]
 // its event's start and end time (wall clock ticks) and, in the counting instantiation, the
 // ticks spent in each Phase.
-template<bool count, int seed_mode>
-__global__ void k_search_by_triplet(
+template<bool count, int seed_mode, bool persistent>
+__global__ void __launch_bounds__(seed_mode == seed_mode_scored ? 256 : 1024, seed_mode == seed_mode_scored ? 3 : 1)
+  k_search_by_triplet(
   const uint32_t* event_list,
   const uint32_t* number_of_events,
   const char* sorted_container,
@@ -1279,18 +1281,8 @@ __global__ void k_search_by_triplet(
   }
   // One event per workgroup (work_counter == NULL), or a resident grid whose workgroups take
   // event slots from the counter until they run out (the counter starts at 0).
-  for (unsigned round = 0;; ++round) {
-    unsigned slot = blockIdx.x;
-    if (work_counter) {
-      if (threadIdx.x == 0) s_slot = atomicAdd(work_counter, 1u);
-      __syncthreads();
-      slot = s_slot;
-      __syncthreads();
-    }
-    else if (round > 0) {
-      break;
-    }
-    if (slot >= n_list) break;
+  // One event: slot `slot` of the event list (the whole workgroup, uniform).
+  const auto process_event = [&](unsigned slot) {
     const unsigned long long t_start = event_clock ? wall_clock64() : 0ull;
     wc.last = t_start;
     const unsigned event_number = event_list ? event_list[slot] : slot;
@@ -1442,6 +1434,19 @@ __global__ void k_search_by_triplet(
       }
     }
     __syncthreads(); // the next event reuses the LDS
+  };
+  if constexpr (persistent) {
+    for (;;) {
+      if (threadIdx.x == 0) s_slot = atomicAdd(work_counter, 1u);
+      __syncthreads();
+      const unsigned slot = s_slot;
+      __syncthreads();
+      if (slot >= n_list) break;
+      process_event(slot);
+    }
+  }
+  else {
+    if (blockIdx.x < n_list) process_event(blockIdx.x);
   }
 }
 
@@ -1628,17 +1633,18 @@ hipError_t launch_search_by_triplet(const SearchLaunch& a, hipStream_t s)
   // Persistent mode: as many workgroups as stay resident (they take events from the counter).
   unsigned grid = a.n_list;
   if (a.work_counter) {
-    const void* kernel = a.work_stats      ? reinterpret_cast<const void*>(&k_search_by_triplet<true, seed_mode_plain>)
-                         : a.scorer_params ? reinterpret_cast<const void*>(&k_search_by_triplet<false, seed_mode_scored>)
+    const void* kernel = a.work_stats ? reinterpret_cast<const void*>(&k_search_by_triplet<true, seed_mode_plain, true>)
+                         : a.scorer_params
+                           ? reinterpret_cast<const void*>(&k_search_by_triplet<false, seed_mode_scored, true>)
                          : a.seed_log.counter
-                           ? reinterpret_cast<const void*>(&k_search_by_triplet<false, seed_mode_log>)
-                           : reinterpret_cast<const void*>(&k_search_by_triplet<false, seed_mode_plain>);
+                           ? reinterpret_cast<const void*>(&k_search_by_triplet<false, seed_mode_log, true>)
+                           : reinterpret_cast<const void*>(&k_search_by_triplet<false, seed_mode_plain, true>);
     const unsigned resident = device_resident_blocks(kernel, block, dyn);
     if (resident) grid = min(grid, resident);
   }
-#define VELO_LAUNCH_SEARCH(C, L)                                                                                    \
+#define VELO_LAUNCH_SEARCH_P(C, L, P)                                                                               \
   hipLaunchKernelGGL(                                                                                               \
-    (k_search_by_triplet<C, L>), dim3(grid), dim3(block), dyn, s, a.event_list, a.number_of_events,                 \
+    (k_search_by_triplet<C, L, P>), dim3(grid), dim3(block), dyn, s, a.event_list, a.number_of_events,              \
     a.sorted_container, a.offsets_estimated_input_size, a.module_cluster_num,                                       \
     reinterpret_cast<const Geometry*>(a.geometry), reinterpret_cast<TrackHits*>(a.tracks),                          \
     reinterpret_cast<TrackletHits*>(a.tracklets), a.tracks_to_follow,                                               \
@@ -1646,11 +1652,17 @@ hipError_t launch_search_by_triplet(const SearchLaunch& a, hipStream_t s)
     a.number_of_velo_tracks, a.status, a.phi_tolerance, a.max_scatter, a.max_skipped_modules, lds_hits,             \
     a.work_stats, a.event_clock, a.n_list, a.work_counter, a.seed_log, a.scorer_params, a.scorer_threshold, window,   \
     list_cap)
+#define VELO_LAUNCH_SEARCH(C, L)                                                                                    \
+  do {                                                                                                              \
+    if (a.work_counter) VELO_LAUNCH_SEARCH_P(C, L, true);                                                           \
+    else VELO_LAUNCH_SEARCH_P(C, L, false);                                                                         \
+  } while (0)
   if (a.work_stats) VELO_LAUNCH_SEARCH(true, seed_mode_plain);
   else if (a.scorer_params) VELO_LAUNCH_SEARCH(false, seed_mode_scored);
   else if (a.seed_log.counter) VELO_LAUNCH_SEARCH(false, seed_mode_log);
   else VELO_LAUNCH_SEARCH(false, seed_mode_plain);
 #undef VELO_LAUNCH_SEARCH
+#undef VELO_LAUNCH_SEARCH_P
   return hipGetLastError();
 }
 
