@@ -958,6 +958,7 @@ struct Banks {
   const uint32_t* word[max_banks_per_mp];
   uint32_t sorted[max_banks_per_mp + 1]; // k_cluster_mp: non-isolated words of the banks before each
   uint32_t first_isolated[max_banks_per_mp]; // k_cluster_mp: a bank's first isolated position, sorted
+  uint32_t isolated_before[max_banks_per_mp + 1]; // k_cluster_mp: isolated words of the banks before each
 };
 
 __device__ __forceinline__ void wave_sync_lds()
@@ -1335,7 +1336,10 @@ __device__ __forceinline__ void cluster_item(
       const unsigned l = lane_id();
       const uint32_t cnt = l < B.n ? bank_nonisolated : 0u;
       const uint32_t inc = wave_inclusive_scan(cnt);
-      if (l <= B.n) B.sorted[l] = inc - cnt;
+      if (l <= B.n) {
+        B.sorted[l] = inc - cnt;
+        B.isolated_before[l] = B.start[l] - (inc - cnt);
+      }
       if (l < B.n) B.first_isolated[l] = B.start[l] + cnt;
     }
     mp::wave_sync_lds(); // keys and counts written
@@ -1367,22 +1371,41 @@ __device__ __forceinline__ void cluster_item(
     for (unsigned w = lane_id(); w < n_words; w += wave_size) lw[w] = sorted[w];
     mp::wave_sync_lds(); // the cluster slices are free again
   }
-  // Isolated SPs (resident: each bank's isolated tail of the sorted words).
-  for (unsigned base = 0; base < n_words; base += wave_size) {
-    const unsigned w = base + lane_id();
-    ClusterOut c0, c1;
-    unsigned k = 0;
-    if (w < n_words) {
-      const unsigned bi = mp::bank_of(B, w);
-      const uint32_t sp_word = resident ? lw[w] : B.word[bi][w - B.start[bi]];
-      const bool isolated = resident ? w >= B.first_isolated[bi] : (sp_word & 0x80000000u) != 0;
-      if (isolated) {
+  // Isolated SPs. Resident: the isolated tails of the sorted banks, one after another, so every
+  // lane of a round takes an isolated word (positions in the same order as a walk over all words
+  // would visit them). Otherwise every word, in place, by its isolated bit.
+  if (resident) {
+    const unsigned n_isolated = B.isolated_before[B.n];
+    for (unsigned base = 0; base < n_isolated; base += wave_size) {
+      const unsigned t = base + lane_id();
+      ClusterOut c0, c1;
+      unsigned k = 0;
+      if (t < n_isolated) {
+        const unsigned bi = mp::bank_at(B.isolated_before, B.n, t);
+        const uint32_t sp_word = lw[B.first_isolated[bi] + (t - B.isolated_before[bi])];
         RawBank bank;
         bank.sourceID = B.source[bi];
         k = isolated_sp_clusters<version>(g, sp_patterns, sp_fx, sp_fy, bank, sp_word, c0, c1);
       }
+      emit(k, c0, c1);
     }
-    emit(k, c0, c1);
+  }
+  else {
+    for (unsigned base = 0; base < n_words; base += wave_size) {
+      const unsigned w = base + lane_id();
+      ClusterOut c0, c1;
+      unsigned k = 0;
+      if (w < n_words) {
+        const unsigned bi = mp::bank_of(B, w);
+        const uint32_t sp_word = B.word[bi][w - B.start[bi]];
+        if (sp_word & 0x80000000u) {
+          RawBank bank;
+          bank.sourceID = B.source[bi];
+          k = isolated_sp_clusters<version>(g, sp_patterns, sp_fx, sp_fy, bank, sp_word, c0, c1);
+        }
+      }
+      emit(k, c0, c1);
+    }
   }
   // The candidates of one round of (at most 64) SPs: their pixels as one list, an SP's first
   // pixel before its second; each lane grows one of them with its SP's window and word fetched
