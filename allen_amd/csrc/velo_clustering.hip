@@ -950,15 +950,23 @@ constexpr uint32_t hash_no_key = 0x01000000u; // a lookup key that matches no sl
 constexpr unsigned hash_min_bank = 16; // largest bank (words) above which the table beats the scan (measured)
 static_assert(hash_slots > word_cap, "hash table load");
 
-// Banks of one module pair (at most 8), kept in the wave's LDS slice.
+// Banks of one module pair (at most 8), kept in the wave's LDS slice. Bank k's words start at
+// base + word_offset[k] (32-bit offsets in place of pointers: 112 B per wave, so that seven
+// workgroups of k_estimate_mp fit a CU's LDS).
 struct Banks {
+  const uint32_t* base;
   unsigned n;
   uint32_t start[max_banks_per_mp + 1];
   uint32_t source[max_banks_per_mp];
-  const uint32_t* word[max_banks_per_mp];
-  uint32_t sorted[max_banks_per_mp + 1]; // k_cluster_mp: non-isolated words of the banks before each
-  uint32_t first_isolated[max_banks_per_mp]; // k_cluster_mp: a bank's first isolated position, sorted
-  uint32_t isolated_before[max_banks_per_mp + 1]; // k_cluster_mp: isolated words of the banks before each
+  uint32_t word_offset[max_banks_per_mp];
+  __device__ __forceinline__ const uint32_t* word(unsigned k) const { return base + word_offset[k]; }
+};
+
+// k_cluster_mp's sorted banks: each bank's non-isolated words first, then its isolated ones.
+struct SortedBanks {
+  uint32_t sorted[max_banks_per_mp + 1];          // non-isolated words of the banks before each
+  uint32_t first_isolated[max_banks_per_mp];      // a bank's first isolated position
+  uint32_t isolated_before[max_banks_per_mp + 1]; // isolated words of the banks before each
 };
 
 __device__ __forceinline__ void wave_sync_lds()
@@ -996,9 +1004,12 @@ __device__ __forceinline__ void load_banks(const char* raw, const DirFetch& f, B
   if (l < nb) {
     B.start[l] = inc - en.n_sp;
     B.source[l] = en.source;
-    B.word[l] = reinterpret_cast<const uint32_t*>(raw) + en.word_offset;
+    B.word_offset[l] = en.word_offset;
   }
-  if (l == 0) B.n = nb;
+  if (l == 0) {
+    B.n = nb;
+    B.base = reinterpret_cast<const uint32_t*>(raw);
+  }
   if (l == nb) B.start[nb] = inc - (l < nb ? en.n_sp : 0u);
   wave_sync_lds();
 }
@@ -1021,7 +1032,7 @@ namespace mp {
 __device__ __forceinline__ uint32_t word_at(const Banks& B, unsigned w)
 {
   const unsigned bi = bank_of(B, w);
-  return B.word[bi][w - B.start[bi]];
+  return B.word(bi)[w - B.start[bi]];
 }
 
 // Copies words [t0, t0 + n) of the module pair into the wave's LDS slice.
@@ -1123,7 +1134,7 @@ __device__ __forceinline__ void estimate_item(
     const unsigned w = base + lane_id();
     const bool active = w < n_words;
     const unsigned bi = active ? mp::bank_of(B, w) : 0;
-    const uint32_t sp_word = active ? (resident ? lw[w] : B.word[bi][w - B.start[bi]]) : 0u;
+    const uint32_t sp_word = active ? (resident ? lw[w] : B.word(bi)[w - B.start[bi]]) : 0u;
     const bool isolated = (sp_word & 0x80000000u) != 0;
     const uint32_t sp_addr = (sp_word & 0x007FFF00u) >> 8;
     const uint32_t sp_row = sp_addr & 0x3Fu, sp_col = sp_addr >> 6;
@@ -1173,7 +1184,7 @@ __device__ __forceinline__ void estimate_item(
         unsigned p0 = 0, p1 = 0;
         const unsigned mk = candidates_final(pixels, &p0, &p1);
         count += __popc(mk);
-        const uint32_t* wp = B.word[bi] + (w - B.start[bi]);
+        const uint32_t* wp = B.word(bi) + (w - B.start[bi]);
         sp_flags[static_cast<size_t>(reinterpret_cast<const char*>(wp) - raw) / 4] =
           static_cast<uint8_t>(mk | (p0 << 2) | (p1 << 5));
       }
@@ -1262,6 +1273,7 @@ __device__ __forceinline__ void cluster_item(
   char* sorted_container,
   char* unsorted_scratch,
   mp::Banks& B,
+  mp::SortedBanks& S,
   const ClusterSlices& sl)
 {
   constexpr bool v23 = (version == 2 || version == 3);
@@ -1337,10 +1349,10 @@ __device__ __forceinline__ void cluster_item(
       const uint32_t cnt = l < B.n ? bank_nonisolated : 0u;
       const uint32_t inc = wave_inclusive_scan(cnt);
       if (l <= B.n) {
-        B.sorted[l] = inc - cnt;
-        B.isolated_before[l] = B.start[l] - (inc - cnt);
+        S.sorted[l] = inc - cnt;
+        S.isolated_before[l] = B.start[l] - (inc - cnt);
       }
-      if (l < B.n) B.first_isolated[l] = B.start[l] + cnt;
+      if (l < B.n) S.first_isolated[l] = B.start[l] + cnt;
     }
     mp::wave_sync_lds(); // keys and counts written
     for (unsigned base = 0; base < n_words; base += wave_size) {
@@ -1361,7 +1373,7 @@ __device__ __forceinline__ void cluster_item(
         const uint32_t word = lw[w];
         uint32_t f = 0;
         if (!(word & 0x80000000u)) {
-          const uint32_t* wp = B.word[bi] + (w - a);
+          const uint32_t* wp = B.word(bi) + (w - a);
           f = sp_flags[static_cast<size_t>(reinterpret_cast<const char*>(wp) - raw) / 4];
         }
         sorted[a + rank] = (f << 24) | (word & (v23 ? 0x007FFFFFu : 0x00FFFFFFu));
@@ -1375,14 +1387,14 @@ __device__ __forceinline__ void cluster_item(
   // lane of a round takes an isolated word (positions in the same order as a walk over all words
   // would visit them). Otherwise every word, in place, by its isolated bit.
   if (resident) {
-    const unsigned n_isolated = B.isolated_before[B.n];
+    const unsigned n_isolated = S.isolated_before[B.n];
     for (unsigned base = 0; base < n_isolated; base += wave_size) {
       const unsigned t = base + lane_id();
       ClusterOut c0, c1;
       unsigned k = 0;
       if (t < n_isolated) {
-        const unsigned bi = mp::bank_at(B.isolated_before, B.n, t);
-        const uint32_t sp_word = lw[B.first_isolated[bi] + (t - B.isolated_before[bi])];
+        const unsigned bi = mp::bank_at(S.isolated_before, B.n, t);
+        const uint32_t sp_word = lw[S.first_isolated[bi] + (t - S.isolated_before[bi])];
         RawBank bank;
         bank.sourceID = B.source[bi];
         k = isolated_sp_clusters<version>(g, sp_patterns, sp_fx, sp_fy, bank, sp_word, c0, c1);
@@ -1397,7 +1409,7 @@ __device__ __forceinline__ void cluster_item(
       unsigned k = 0;
       if (w < n_words) {
         const unsigned bi = mp::bank_of(B, w);
-        const uint32_t sp_word = B.word[bi][w - B.start[bi]];
+        const uint32_t sp_word = B.word(bi)[w - B.start[bi]];
         if (sp_word & 0x80000000u) {
           RawBank bank;
           bank.sourceID = B.source[bi];
@@ -1442,12 +1454,12 @@ __device__ __forceinline__ void cluster_item(
   if (resident) {
     // The non-isolated words, bank after bank: word t of the list is position
     // start[bank] + t - sorted[bank] of the slice.
-    const unsigned n_sorted = B.sorted[B.n];
+    const unsigned n_sorted = S.sorted[B.n];
     for (unsigned base = 0; base < n_sorted; base += wave_size) {
       const unsigned t = base + lane_id();
       const bool active = t < n_sorted;
-      const unsigned bi = active ? mp::bank_at(B.sorted, B.n, t) : 0u;
-      const unsigned j = B.start[bi] + (t - B.sorted[bi]);
+      const unsigned bi = active ? mp::bank_at(S.sorted, B.n, t) : 0u;
+      const unsigned j = B.start[bi] + (t - S.sorted[bi]);
       const uint32_t e = active ? lw[j] : 0u;
       const uint32_t f = e >> 24;
       const uint32_t sp_word = e & 0x00FFFFFFu;
@@ -1460,7 +1472,7 @@ __device__ __forceinline__ void cluster_item(
         const uint32_t col = (key >> 6) & 0x1FFu;
         const uint32_t lower = sensor | ((max(col, 3u) - 3u) << 6);
         const uint32_t upper = sensor + min((col + 2u) << 6, 0x8000u);
-        unsigned lo = B.start[bi], hi = B.first_isolated[bi];
+        unsigned lo = B.start[bi], hi = S.first_isolated[bi];
         const unsigned end = hi;
         while (lo < hi) {
           const unsigned mid = (lo + hi) >> 1;
@@ -1484,10 +1496,10 @@ __device__ __forceinline__ void cluster_item(
       const unsigned w = base + lane_id();
       const bool active = w < n_words;
       const unsigned bi = active ? mp::bank_of(B, w) : 0;
-      const uint32_t sp_word = active ? B.word[bi][w - B.start[bi]] : 0x80000000u;
+      const uint32_t sp_word = active ? B.word(bi)[w - B.start[bi]] : 0x80000000u;
       uint32_t f = 0;
       if (active && !(sp_word & 0x80000000u)) {
-        const uint32_t* wp = B.word[bi] + (w - B.start[bi]);
+        const uint32_t* wp = B.word(bi) + (w - B.start[bi]);
         f = sp_flags[static_cast<size_t>(reinterpret_cast<const char*>(wp) - raw) / 4];
       }
       const uint32_t pair = B.source[bi] & 0x1FFu;
@@ -1576,6 +1588,7 @@ __global__ void __launch_bounds__(mp::waves_per_block* wave_size, 6) k_cluster_m
   __shared__ float s_xyz[mp::waves_per_block][3 * mp::cluster_cap];
   __shared__ uint8_t s_cand[mp::waves_per_block][2 * wave_size];
   __shared__ mp::Banks s_banks[mp::waves_per_block];
+  __shared__ mp::SortedBanks s_sorted[mp::waves_per_block];
   const unsigned wave = threadIdx.x / wave_size;
   const unsigned n_items = n_events * n_module_pairs;
   const unsigned stride = gridDim.x * mp::waves_per_block;
@@ -1586,7 +1599,7 @@ __global__ void __launch_bounds__(mp::waves_per_block* wave_size, 6) k_cluster_m
   if constexpr (!loop) {
     cluster_item<version>(
       raw, mp::fetch_dir(directory, directory_count, idx), idx, total, offsets_estimated_input_size, sp_flags,
-      *geometry, sp_patterns, sp_fx, sp_fy, module_cluster_num, sorted_container, unsorted_scratch, s_banks[wave], sl);
+      *geometry, sp_patterns, sp_fx, sp_fy, module_cluster_num, sorted_container, unsorted_scratch, s_banks[wave], s_sorted[wave], sl);
     return;
   }
   mp::DirFetch cur = mp::fetch_dir(directory, directory_count, idx);
@@ -1595,7 +1608,7 @@ __global__ void __launch_bounds__(mp::waves_per_block* wave_size, 6) k_cluster_m
     if (idx + stride < n_items) next = mp::fetch_dir(directory, directory_count, idx + stride);
     cluster_item<version>(
       raw, cur, idx, total, offsets_estimated_input_size, sp_flags, *geometry, sp_patterns, sp_fx, sp_fy,
-      module_cluster_num, sorted_container, unsorted_scratch, s_banks[wave], sl);
+      module_cluster_num, sorted_container, unsorted_scratch, s_banks[wave], s_sorted[wave], sl);
     cur = next;
   }
 }
